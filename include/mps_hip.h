@@ -1,0 +1,235 @@
+/*
+ * mps_hip.h -- C-ABI of libmps_hip: the MPS per-timestep hot path on one
+ * MI355X (gfx950).  Plain C types and pointers only.
+ *
+ * Part 1 declares the reference's own entry points with the reference's
+ * prototypes; a program written against mps_fluid3D_parallel (or, built with
+ * MPS_DIM=2, mps_fluid2D) links this library in place of the reference's
+ * object files.  Each declaration names the reference interface it replaces.
+ * Part 2 declares what the device-resident design adds: explicit
+ * upload/download, a resident multi-step call, options, statistics, exports
+ * used by parity tests, kernel timing, the batched ensemble and the adjoint.
+ *
+ * Semantics of the drop-in entry points: every call leaves the caller's
+ * host structures exactly as the reference would (positions, velocities,
+ * accelerations, pressure, number density, flags, neighbour lists), unless
+ * the ParticleSystem has been made resident with mps_hip_set_resident(ps,1);
+ * then state stays in HBM between calls and the host copy is refreshed only
+ * by mps_hip_download().  Errors the reference reports with exit() (a
+ * neighbour list overflow) still exit with the reference's message unless
+ * mps_hip_set_option("exit_on_error", 0) is set, in which case the call
+ * returns and mps_hip_last_error() reports the code.
+ */
+#ifndef MPS_HIP_H
+#define MPS_HIP_H
+
+#include "mps_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ===================================================================== */
+/* Part 1: reference entry points                                         */
+/* ===================================================================== */
+
+/* --- configuration (host C, same keys, defaults and derivations) ------ */
+/* mps_fluid3D_parallel/include/sim_config.h:86  config_set_defaults      */
+void config_set_defaults(SimConfig *config);
+/* mps_fluid3D_parallel/include/sim_config.h:89  config_load_cal          */
+int  config_load_cal(const char *cal_path, SimConfig *config);
+/* mps_fluid3D_parallel/include/sim_config.h:92  config_load_params       */
+int  config_load_params(const char *param_path, SimConfig *config);
+/* mps_fluid3D_parallel/include/sim_config.h:95  config_print             */
+void config_print(const SimConfig *config);
+
+/* --- particle container ------------------------------------------------ */
+/* mps_fluid3D_parallel/include/particle.h:31-32 */
+ParticleSystem *particle_system_create(int capacity);
+void            particle_system_free(ParticleSystem *ps);
+/* mps_fluid3D_parallel/include/particle.h:35 */
+int  particle_system_add(ParticleSystem *ps, double pos[], double vel[], int type);
+/* mps_fluid3D_parallel/include/particle.h:38 -- n0, lambda (+ N0, C_LL,
+ * C_SL) computed on the GPU with a cell list, summing in the reference's
+ * ascending-index order so the values are bit-identical to its O(N^2) pass */
+void particle_system_calc_initial_params(ParticleSystem *ps);
+
+/* --- kernel function --------------------------------------------------- */
+/* mps_fluid3D_parallel/include/kernel.h:5,8 */
+double kernel_weight(double r, double re);
+double kernel_weight_derivative(double r, double re);
+
+/* --- neighbour search -------------------------------------------------- */
+/* mps_fluid3D_parallel/include/neighbor_search.h:33-34 */
+NeighborList *neighbor_list_create(int num_particles, int max_neighbors);
+void          neighbor_list_free(NeighborList *nl);
+/* mps_fluid3D_parallel/include/neighbor_search.h:37-40 */
+CellList *cell_list_create(int num_particles, double re,
+                           const double domain_min[MPS_DIM],
+                           const double domain_max[MPS_DIM]);
+void      cell_list_free(CellList *cl);
+/* mps_fluid3D_parallel/include/neighbor_search.h:43 (O(N^2) list, same set
+ * and index order as the reference's unused brute-force search) */
+void neighbor_search_brute_force(NeighborList *nl, ParticleSystem *ps, double re);
+/* mps_fluid3D_parallel/include/neighbor_search.h:46-47 -- counting sort by
+ * cell on the GPU; list order is the reference's (cells z->y->x, descending
+ * particle index inside a cell).  cl->head/next are filled as the
+ * reference's linked lists would be. */
+void neighbor_search_cell_linked_list(NeighborList *nl, ParticleSystem *ps,
+                                      CellList *cl, double re);
+/* mps_fluid3D_parallel/include/neighbor_search.h:50 */
+int  neighbor_get(NeighborList *nl, int i, int j);
+
+/* --- particle operators (mps_fluid3D_parallel/include/operators.h:8-37;
+ *     mps_fluid2D/include/operators.h:8-35 for the 2-D collision) ------- */
+void calc_particle_number_density(ParticleSystem *ps, NeighborList *nl);
+void calc_viscosity_term(ParticleSystem *ps, NeighborList *nl);
+void calc_pressure_gradient(ParticleSystem *ps, NeighborList *nl);
+void judge_free_surface(ParticleSystem *ps, double threshold);
+void calc_neighbor_count(ParticleSystem *ps, NeighborList *nl);
+void judge_free_surface_by_count(ParticleSystem *ps, double beta);
+void clamp_pressure(ParticleSystem *ps);
+#if MPS_DIM == 3
+void collision(ParticleSystem *ps, NeighborList *nl);
+#else
+void collision(ParticleSystem *ps);
+#endif
+void calc_surface_tension(ParticleSystem *ps, NeighborList *nl);
+void calc_surface_tension_SL(ParticleSystem *ps, NeighborList *nl);
+
+/* --- boundary (mps_fluid3D_parallel/include/boundary.h:8,12-14) -------- */
+void apply_wall_boundary(ParticleSystem *ps);
+#if MPS_DIM == 3
+void remove_out_of_bounds(ParticleSystem *ps, double xmin, double xmax,
+                          double ymin, double ymax, double zmin, double zmax);
+#else
+void remove_out_of_bounds(ParticleSystem *ps, double xmin, double xmax,
+                          double ymin, double ymax);
+#endif
+
+/* --- pressure (mps_fluid3D_parallel/include/pressure_solver.h:8) ------- */
+void solve_pressure(ParticleSystem *ps, NeighborList *nl);
+
+/* --- driver (mps_fluid3D_parallel/include/simulation.h:8,17) ----------- */
+void simulation_run(ParticleSystem *ps);
+void simulation_step(ParticleSystem *ps, NeighborList *nl, CellList *cl, int step);
+
+/* --- output (mps_fluid3D_parallel/include/io.h:8,11) -------------------- */
+void output_csv(const ParticleSystem *ps, int step, const char *output_dir);
+void output_vtk(const ParticleSystem *ps, int step, const char *output_dir);
+
+/* ===================================================================== */
+/* Part 2: additions of the device-resident design (no reference          */
+/* counterpart; each is needed because state lives in HBM)               */
+/* ===================================================================== */
+
+/* Error codes returned by the mps_hip_* calls and mps_hip_last_error(). */
+#define MPS_OK                 0
+#define MPS_ERR_HIP            1   /* a HIP runtime call failed            */
+#define MPS_ERR_NO_DEVICE      2   /* no usable gfx950 device               */
+#define MPS_ERR_ALLOC          3   /* device allocation failed              */
+#define MPS_ERR_NEIGHBOR_OVERFLOW 4 /* a particle exceeded max_neighbors    */
+#define MPS_ERR_ARG            5   /* invalid argument / state              */
+#define MPS_ERR_NO_CONFIG      6   /* g_config is NULL                      */
+#define MPS_ERR_UNSUPPORTED    7   /* option combination not built          */
+#define MPS_ERR_SOLVER         8   /* solver breakdown / level overflow     */
+
+/* Reads the loader format of mps_fluid3D_parallel/src/main.c:20-85
+ * (mps_fluid2D/src/main.c for 2-D): a count line, then "x y [z] vx vy [vz]
+ * type" rows; comment lines start with '#'.  Returns 0 or -1. */
+int  mps_load_particles(const char *path, ParticleSystem *ps);
+/* Counts the particle rows announced by the file's header (for sizing). */
+int  mps_count_particles(const char *path);
+
+/* Writes a dam-break scene on the lattice rules of
+ * mps_fluid3D_parallel/tools/gen_dambreak.py:99-166 (2-D:
+ * mps_fluid2D/tools/gen_dambreak.py:107-150): particles in i->j->k order,
+ * wall then dummy layers, fluid inside the column.  If ps is non-NULL the
+ * particles are appended to it (with the %.8e rounding the text file would
+ * carry); if path is non-NULL the particles.txt text is written there.
+ * dims = {domain_x, domain_y, domain_z, water_x, water_y, water_z}.
+ * Returns the particle count, or -1. */
+int  mps_gen_dambreak(const double dims[6], double l0, int wall_layers,
+                      int dummy_layers, ParticleSystem *ps, const char *path);
+
+int  mps_hip_init(int device);          /* optional; first call does it   */
+void mps_hip_finalize(void);            /* frees every device context     */
+int  mps_hip_last_error(void);
+const char *mps_hip_error_string(int code);
+/* gfx arch name of the active device, e.g. "gfx950" ("" if none). */
+const char *mps_hip_device_arch(void);
+
+/* Host <-> device state.  upload copies ps (and nl, if given, as the stale
+ * list the next step's viscosity and collision read) into the device
+ * context of ps; download copies device state back into ps (and nl). */
+int  mps_hip_upload(ParticleSystem *ps, const NeighborList *nl);
+int  mps_hip_download(ParticleSystem *ps, NeighborList *nl);
+int  mps_hip_set_resident(ParticleSystem *ps, int resident);
+void mps_hip_release(ParticleSystem *ps);
+
+/* Runs nsteps timesteps on the resident device state of ps (uploading
+ * first if ps has no device state).  No host copies inside. */
+int  mps_hip_run_steps(ParticleSystem *ps, int nsteps);
+
+/* Options (process-wide): keys and values
+ *   "tol_mode"       0 absolute sqrt(r.r) < tol (reference), 1 relative
+ *                      sqrt(r.r) < tol*sqrt(b.b)
+ *   "precision"      0 FP64 throughout, 1 mixed: FP32 inner CG inside FP64
+ *                      iterative refinement
+ *   "inner_tol"      relative tolerance of the FP32 inner solve (1e-5)
+ *   "exit_on_error"  1 (reference behaviour) or 0
+ *   "check_every"    CG iterations between host convergence polls (16)
+ *   "spmv"           0 stored CSR values, 1 matrix-free (recomputes w)
+ *   "profile"        1 times the main kernels with HIP events
+ * Returns 0, or MPS_ERR_ARG for an unknown key. */
+int    mps_hip_set_option(const char *key, double value);
+double mps_hip_get_option(const char *key);
+
+typedef struct {
+    int    steps;           /* steps run on this context                  */
+    int    last_iters;      /* CG/ICCG iterations of the last solve       */
+    long long total_iters;  /* summed over all solves                     */
+    int    last_outer;      /* iterative-refinement sweeps (mixed)        */
+    int    n_eq;            /* unknowns of the last PPE                   */
+    long long nnz;          /* nonzeros of the last PPE matrix (with diag)*/
+    long long n_pairs;      /* directed neighbour pairs of the last search*/
+    double last_residual;   /* sqrt(r.r) at exit (recursive residual)     */
+    double last_bnorm;      /* sqrt(b.b)                                  */
+    int    ic_levels;       /* dependency levels of the last IC(0) factor */
+    int    max_count;       /* largest neighbour count of the last search */
+} mps_step_stats;
+int  mps_hip_get_stats(ParticleSystem *ps, mps_step_stats *out);
+
+/* Exports of the last assembled PPE in the reference's layout
+ * (mps_fluid3D_parallel/src/pressure_solver.c:411-551): rows in eq order
+ * (ascending particle index over unknowns), columns sorted ascending.
+ * eq_idx has ps->num entries; row_ptr n_eq+1; col/val nnz; diag and b
+ * n_eq.  Pass NULL for arrays not wanted.  Counts via mps_hip_get_stats. */
+int  mps_hip_export_ppe(ParticleSystem *ps, int *eq_idx, int *row_ptr,
+                        int *col, double *val, double *diag, double *b);
+/* Last PPE solution x in eq order (n_eq entries). */
+int  mps_hip_export_solution(ParticleSystem *ps, double *x);
+/* IC(0) factor of the last ICCG solve in the reference's layout
+ * (pressure_solver.c:156-232): strictly-lower CSR + 1/sqrt(d). */
+int  mps_hip_export_ic(ParticleSystem *ps, int *row_ptr, int *col,
+                       double *val, double *diag_inv);
+
+/* Kernel timing ("profile" option): accumulated milliseconds and launch
+ * count of a named kernel class ("spmv", "search", ...).  Returns -1 if the
+ * name is unknown. */
+int  mps_hip_profile_get(const char *name, double *total_ms, long long *count);
+void mps_hip_profile_reset(void);
+
+/* Solves A x = b on the GPU for a caller-supplied CSR matrix (rows sorted
+ * by column) with the library's CG/ICCG -- the linear-algebra half of
+ * solve_pressure, exposed for testing against textbook solvers. */
+int  mps_hip_solve_csr(int n, const int *row_ptr, const int *col,
+                       const double *val, const double *b, double *x,
+                       int solver_type, int max_iter, double tol,
+                       int *iters_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPS_HIP_H */
