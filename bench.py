@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Throughput of the MPS timestep on one MI355X.
+
+Workload (default): the plain 3-D dam break -- tank 1.0 (x) x 0.6 (y) x 0.5
+(z) m, water column 0.3 x 0.5 x 0.5 m on a cubic lattice at l0 = 0.005 m with
+3 wall + 2 dummy layers (1,077,505 particles, 594,000 fluid), rho = 1000,
+nu = 1e-6, g = -9.81, dt = 5e-4, re = 3.1 l0 (Laplacian/gradient/PPE) and
+2.1 l0 (number density), density-deviation PPE source, CG to relative
+residual 1e-9, FP64.  Synthetic input: the lattice generator's particles, at
+rest.
+
+A step is one full timestep (viscosity/predictor, collision, counting sort
+and neighbour search, density and flags, PPE assembly, CG solve, gradient
+corrector, boundary) with all state resident in HBM.  W untimed warm-up
+steps, then K timed steps between two device synchronisations.
+
+Prints one JSON line.  `roofline` is the dominant kernel (the CG SpMV):
+algorithmic bytes per launch / its mean duration measured with HIP events on
+its stream during the timed steps.  `cpu_baseline` times the CPU oracle (the
+C/OpenMP restatement of the reference, same stopping rule) on a bounded
+sample of the same workload on this host.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+import mps_amd  # noqa: E402
+from mps_amd import abi as A  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "dam3d_l0.005": dict(dims=(1.0, 0.6, 0.5, 0.3, 0.5, 0.5), l0=0.005, dt=5e-4),
+    "dam3d_l0.0035": dict(dims=(1.0, 0.6, 0.5, 0.3, 0.5, 0.5), l0=0.0035, dt=5e-4),
+    "dam3d_l0.01": dict(dims=(1.0, 0.6, 0.5, 0.3, 0.5, 0.5), l0=0.01, dt=5e-4),
+}
+
+
+def make_config(lib, w, solver, tol, max_iter=20000):
+    from mps_amd.params import bench_config
+    return bench_config(lib, w, solver, tol, max_iter)
+
+
+def build_scene(lib, w):
+    dims = (C.c_double * 6)(*w["dims"])
+    n = lib.mps_gen_dambreak(dims, w["l0"], 3, 2, None, None)
+    ps = lib.particle_system_create(n)
+    got = lib.mps_gen_dambreak(dims, w["l0"], 3, 2, ps, None)
+    if got != n:
+        raise RuntimeError("scene generation failed")
+    return ps, n
+
+
+def spmv_bytes(st, value_bytes=8):
+    """Algorithmic bytes of one CSR SpMV y = A x: values and column indices
+    of every nonzero (diagonal included), row pointers, x read once, y
+    written once."""
+    z, ne = st.nnz, st.n_eq
+    return z * (value_bytes + 4) + 4 * (ne + 1) + 2 * value_bytes * ne
+
+
+def cpu_baseline(w, tol, seconds_hint=30.0):
+    """Oracle (C/OpenMP restatement of the reference) on this host: initial
+    parameters, the initial search, then as many steps of the same workload
+    as fit about 20 s (at least one)."""
+    from oracle.oracle import Oracle
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import scenes  # the reference generator's lattice rules
+    abi = A.types(3)
+    orc = Oracle(3)
+    cfg = make_config(mps_amd.load(3), w, 0, tol)
+    d = w["dims"]
+    pos, vel, typ = scenes.dambreak(3, d[:3], d[3:], w["l0"], 3, 2)
+    sc = A.HostScene(abi, pos, vel, typ)
+    orc.init_params(sc, cfg)
+    sim = orc.sim(tol_mode=1)
+    sim.begin(sc, cfg)
+    steps, iters = 0, 0
+    t0 = time.perf_counter()
+    while True:
+        sim.step(sc, cfg)
+        steps += 1
+        iters += sim.ppe(0)["iters"]
+        el = time.perf_counter() - t0
+        if el > 20.0 or steps >= 50:
+            break
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=len(typ) * steps / el, unit="particle-steps/s", cores=threads, kind="port",
+                sample="%d step(s) from rest of the %d-particle workload, CG relative 1e-9, "
+                       "mean %.0f iterations/step, %.1f s" % (steps, len(typ), iters / steps, el))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="dam3d_l0.005", choices=sorted(WORKLOADS))
+    ap.add_argument("--solver", default="cg", choices=("cg", "iccg"))
+    ap.add_argument("--precision", default="fp64", choices=("fp64", "mixed"))
+    ap.add_argument("--tol", type=float, default=1e-9)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py: this build runs on one GPU only (each box has one MI355X); "
+              "--gpus %d is not supported" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    rank = int(os.environ.get("RANK", "0"))
+    w = WORKLOADS[a.workload]
+    lib = mps_amd.load(3)
+    lib.check(lib.mps_hip_init(0), "mps_hip_init")
+    cfg = make_config(lib, w, 1 if a.solver == "iccg" else 0, a.tol)
+    lib.set_config(cfg)
+    lib.set_option("tol_mode", 1)
+    lib.set_option("precision", 1 if a.precision == "mixed" else 0)
+    lib.set_option("exit_on_error", 0)
+    ps, n = build_scene(lib, w)
+    print("bench: %s, %d particles, solver %s, %s" % (a.workload, n, a.solver, a.precision),
+          file=sys.stderr, flush=True)
+    lib.particle_system_calc_initial_params(ps)
+    lib.check(lib.mps_hip_run_steps(ps, a.warmup), "warm-up steps")
+    lib.set_option("profile", 0 if a.no_profile else 1)
+    lib.mps_hip_profile_reset()
+    st0 = mps_amd.StepStats()
+    lib.mps_hip_get_stats(ps, C.byref(st0))
+    t0 = time.perf_counter()
+    lib.check(lib.mps_hip_run_steps(ps, a.steps), "timed steps")   # returns after a device sync
+    t1 = time.perf_counter()
+    lib.set_option("profile", 0)
+    st = mps_amd.StepStats()
+    lib.mps_hip_get_stats(ps, C.byref(st))
+    el = t1 - t0
+    iters = (st.total_iters - st0.total_iters) / max(a.steps, 1)
+    ms = C.c_double()
+    cnt = C.c_longlong()
+    lib.mps_hip_profile_get(b"spmv", C.byref(ms), C.byref(cnt))
+    spmv_ms = ms.value / max(cnt.value, 1)
+    roof = None
+    if cnt.value > 0:
+        vb = 8 if a.precision == "fp64" else 4
+        byt = spmv_bytes(st, vb)
+        ach = byt / (spmv_ms * 1e-3) / 1e9
+        roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
+                    kernel="k_spmv_dot", bytes_per_launch=int(byt),
+                    mean_launch_ms=round(spmv_ms, 5), launches=cnt.value)
+    kernels = {}
+    for name in ("visc_predict", "collide_bin", "sort", "search", "assemble", "spmv", "axpy",
+                 "pupdate", "grad_correct", "ic"):
+        if lib.mps_hip_profile_get(name.encode(), C.byref(ms), C.byref(cnt)) == 0 and cnt.value:
+            kernels[name] = dict(total_ms=round(ms.value, 3), launches=cnt.value)
+    cpu = None
+    if not a.no_cpu_baseline and rank == 0:
+        try:
+            cpu = cpu_baseline(w, a.tol)
+        except Exception as e:  # baseline is reported, never required
+            cpu = dict(error=str(e))
+    value = n * a.steps / el
+    line = {
+        "metric": "particle-steps/s, 3D dam break 0.8M: FP64 vs mixed at <=1e-6 rel p error",
+        "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(1e3 * el / a.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if a.precision == "fp64" else "f32+f64",
+        "data": "synthetic (reference dam-break lattice generator, fluid at rest)",
+        "config": {"workload": a.workload, "particles": n, "fluid": None, "l0": w["l0"],
+                   "dt": w["dt"], "solver": a.solver, "precision": a.precision,
+                   "tolerance": "relative %g" % a.tol, "mean_iters_per_step": round(iters, 1),
+                   "n_eq": st.n_eq, "nnz": st.nnz, "pairs": st.n_pairs, "parallelism": "1 GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernels_ms": kernels,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -29,6 +29,10 @@
 extern "C" {
 #endif
 
+/* Only the declarations below are exported; the library is compiled with
+ * hidden default visibility. */
+#pragma GCC visibility push(default)
+
 /* ===================================================================== */
 /* Part 1: reference entry points                                         */
 /* ===================================================================== */
@@ -227,6 +231,13 @@ int  mps_hip_solve_csr(int n, const int *row_ptr, const int *col,
                        const double *val, const double *b, double *x,
                        int solver_type, int max_iter, double tol,
                        int *iters_out);
+
+/* Diagnostics: computes sqrt(a[i]) and a[i]/b[i] on the GPU with the FP64
+ * operations the kernels use (to check they round exactly as the host). */
+int  mps_hip_fp64_probe(int n, const double *a, const double *b, double *sqrt_out,
+                        double *div_out);
+
+#pragma GCC visibility pop
 
 #ifdef __cplusplus
 }

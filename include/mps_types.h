@@ -141,7 +141,7 @@ typedef struct {
 
 /* Process-wide configuration read by every step function, as in the
  * reference (mps_fluid3D_parallel/include/sim_config.h:83). */
-extern SimConfig *g_config;
+extern __attribute__((visibility("default"))) SimConfig *g_config;
 
 #ifdef __cplusplus
 }

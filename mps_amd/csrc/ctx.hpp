@@ -1,0 +1,175 @@
+// ctx.hpp -- device-resident state of one ParticleSystem and the host-side
+// launchers of the step kernels.
+//
+// HBM layout (slot s = position in the current cell-sorted order):
+//   X[2], V[2]   dvec per slot, ping-ponged by the kernels that move
+//                particles (predictor, collision, corrector)
+//   T[2], O[2]   particle type and original index (ps->particles[] index),
+//                ping-ponged by the counting sort
+//   P[2]         pressure (ping-ponged by the corrector, which reads the
+//                pressures of neighbours while it ghosts particles)
+//   ACC, ND, SURF, NCOUNT [2]  per-slot fields, permuted with T/O
+//   unk          1 = PPE unknown (interior fluid or wall)
+//   nbr          neighbour list, sliced-ELL (64-slot slices, stride =
+//                max_neighbors), entries are slots in reference order
+//   ncnt         neighbour counts
+//   Aval, Acol, Alen   off-diagonal PPE entries per unknown slot, same
+//                sliced-ELL layout, plus diag[] and b[]
+//   x, r, pv, Ap       CG vectors over all slots (0 at non-unknown slots)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "dev_common.hpp"
+#include "mps_hip.h"
+
+#define HIP_OK(call)                                                               \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            mps_set_error(MPS_ERR_HIP, __FILE__, __LINE__, hipGetErrorString(e_)); \
+            return MPS_ERR_HIP;                                                    \
+        }                                                                          \
+    } while (0)
+
+void mps_set_error(int code, const char *file, int line, const char *msg);
+
+// Step constants computed on the host in the reference's association order
+// (host and device FP64 arithmetic are both IEEE, so precomputed products
+// are bit-identical to the reference computing them inline).
+struct StepConsts {
+    Grid grid;
+    double l0, r_min;
+    double re_lap, re_n, re2;        // re2 = search radius squared
+    double re_n2;                    // re_n*re_n (count-based surface test)
+    double re_n2_hi;                 // r2 above this => w(re_n) = 0
+    double n0, coeff;                // coeff = 2*DIM/(n0*lambda)
+    double neg_coeff;                // -coeff
+    double visc_coeff;               // viscosity*coeff
+    double diag_coeff;               // coeff or c_ppe*coeff
+    double g[3], dt;
+    double cd2, col_tiny, e_half;    // collision: cd^2, 1e-12*l0, (1+e)*0.5
+    double thr_n0;                   // surface_threshold*n0
+    double beta_n0c;                 // surface_count_threshold*n0_count
+    double src_k;                    // relax*(rho/dt^2) or gamma*(rho/dt^2)
+    double neg_rho_dt;               // -rho/dt (weakly compressible PPE)
+    double hs_k;                     // -(rho/n0/dt)
+    double dim_n0;                   // DIM/n0
+    double grad_min_r2;              // l0*l0*1e-12
+    double neg_gc, rho;              // -(DIM/n0), density
+    double bnd_lo[3], bnd_hi[3];     // domain -/+ l0*(wall_layers+1)
+    double re_st, st_ll, st_sl, st_tiny;   // surface tension (operators.c:281-354)
+    int stride;                      // max_neighbors
+    int surf_method, ppe_type, hs_mode, cmps, clamp, collide;
+    int st_enabled, st_sl_on;
+};
+
+// Counters written by the search and assembly kernels.
+struct DevCounters {
+    int overflow_orig;               // smallest original index that overflowed, or INT_MAX
+    int n_eq;                        // unknowns of the last search
+    int max_count;
+    int pad;
+    unsigned long long n_pairs;
+    unsigned long long nnz;
+};
+
+struct CgCtrl;   // device control block of the solvers (k_cg.hip)
+struct IcData;   // IC(0) factor and schedules (k_ic.hip)
+struct MixedBufs;
+
+struct Ctx {
+    ParticleSystem *ps = nullptr;
+    int n = 0, cap = 0, stride = 0, grid_cap = 0;
+    hipStream_t stream = nullptr;
+    int cx = 0, ct = 0, cp = 0;      // current index of X/V, T/O/ACC/ND/SURF/NCOUNT, P
+    dvec *X[2] = {nullptr, nullptr}, *V[2] = {nullptr, nullptr};
+    dvec *ACC[2] = {nullptr, nullptr};
+    int *T[2] = {nullptr, nullptr}, *O[2] = {nullptr, nullptr};
+    int *SURF[2] = {nullptr, nullptr}, *NCOUNT[2] = {nullptr, nullptr};
+    double *P[2] = {nullptr, nullptr}, *ND[2] = {nullptr, nullptr};
+    int *unk = nullptr;
+    double *pmin = nullptr;
+    // counting sort
+    int *cell = nullptr, *ccount = nullptr, *cstart = nullptr, *cfill = nullptr;
+    int *tmp_key = nullptr, *sflag = nullptr, *sscan = nullptr;
+    int *scan_blk = nullptr;
+    // neighbour list
+    int *nbr = nullptr, *ncnt = nullptr;
+    // PPE + CG
+    double *diag = nullptr, *b = nullptr, *Aval = nullptr;
+    int *Acol = nullptr, *Alen = nullptr;
+    double *x = nullptr, *r = nullptr, *pv = nullptr, *Ap = nullptr;
+    double *partA = nullptr, *partB = nullptr;
+    float *fAval = nullptr, *fdiag = nullptr, *fr = nullptr, *fp = nullptr, *fAp = nullptr, *fz = nullptr;
+    CgCtrl *ctrl = nullptr;          // device
+    CgCtrl *hctrl = nullptr;         // pinned host mirror
+    DevCounters *counters = nullptr, *hcounters = nullptr;
+    // staging for host <-> device copies
+    void *stage = nullptr;
+    size_t stage_bytes = 0;
+    int *nl_stage = nullptr;         // compact list staging (device)
+    size_t nl_stage_elems = 0;
+    int *cnt_orig = nullptr, *off_orig = nullptr;   // per original index
+    // state flags
+    bool has_state = false, resident = false, list_valid = false, ppe_valid = false;
+    Grid grid{};
+    mps_step_stats stats{};
+    IcData *ic = nullptr;
+    MixedBufs *mixed = nullptr;
+};
+
+// ---- launchers (k_particles.hip) -------------------------------------
+int  ctx_alloc(Ctx *c, int n, int stride, int grid_cells);
+void ctx_free(Ctx *c);
+int  launch_fill_int(int *p, int v, size_t n, hipStream_t s);
+int  launch_fill_double(double *p, double v, size_t n, hipStream_t s);
+int  launch_scan_excl(const int *in, int *out, int n, int *blk, hipStream_t s);
+int  launch_unpack(Ctx *c, const void *stage_dev, int with_list);
+int  launch_pack(Ctx *c, void *stage_dev);
+int  launch_visc_predict(Ctx *c, const StepConsts &k);
+int  launch_collide_bin(Ctx *c, const StepConsts &k, int collide);
+int  launch_bin(Ctx *c, const StepConsts &k);
+int  launch_sort(Ctx *c, const StepConsts &k, int full);
+int  launch_search(Ctx *c, const StepConsts &k, int with_density, DevCounters *ctr);
+int  launch_assemble(Ctx *c, const StepConsts &k, DevCounters *ctr);
+int  launch_pressure_out(Ctx *c, const StepConsts &k, int n_eq_zero);
+int  launch_grad_correct(Ctx *c, const StepConsts &k);
+int  launch_density_only(Ctx *c, const StepConsts &k);
+// neighbour list <-> compact CSR over original indices (host interchange)
+int  launch_list_export(Ctx *c, long long *total_out);
+int  launch_list_import(Ctx *c, const int *counts_dev, const int *offsets_dev);
+
+// ---- solvers (k_cg.hip, k_ic.hip) ------------------------------------
+struct SolveOpts {
+    int solver;        // 0 CG, 1 ICCG
+    int max_iter;
+    double tol;
+    int tol_mode;      // 0 absolute, 1 relative
+    int precision;     // 0 FP64, 1 mixed
+    double inner_tol;
+    int check_every;
+    int dense_ic;      // 2-D dense-formula IC(0)
+};
+struct SolveResult {
+    int iters, outer, breakdown;
+    double res, bnorm;
+};
+int cg_alloc_ctrl(Ctx *c);
+int solve_ppe(Ctx *c, const SolveOpts &o, SolveResult *out);
+
+// ---- runtime accessors (runtime.hip) -----------------------------------
+Ctx *mps_ctx_for(ParticleSystem *ps, int create);
+int mps_ctx_upload(Ctx *c, ParticleSystem *ps, const NeighborList *nl, const Grid &g);
+int mps_ctx_download(Ctx *c, ParticleSystem *ps, NeighborList *nl);
+int mps_ctx_search(Ctx *c, const StepConsts &k);
+StepConsts mps_consts(const SimConfig *cf, const ParticleSystem *ps, const Grid &g, double re);
+Grid mps_grid_for(double re, const double *dmin, const double *dmax);
+Grid mps_grid_from_cells(const CellList *cl);
+double mps_search_radius(const SimConfig *cf);
+
+// ---- timing (runtime.hip) ----------------------------------------------
+enum ProfKernel { PK_VISC, PK_COLLIDE, PK_SORT, PK_SEARCH, PK_ASSEMBLE, PK_SPMV, PK_AXPY,
+                  PK_PUPDATE, PK_GRAD, PK_IC, PK_COUNT };
+void prof_begin(int k, hipStream_t s);
+void prof_end(int k, hipStream_t s);

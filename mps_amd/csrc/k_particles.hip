@@ -1,0 +1,1016 @@
+// k_particles.hip -- particle-side kernels of one MPS timestep on gfx950.
+//
+// Design (one thread per particle slot, 256-thread blocks = 4 wave64):
+//   * particles live in cell-sorted order: a counting sort by cell each step
+//     (histogram fused into the collision kernel, exclusive scan, atomic
+//     scatter, then a deterministic rank inside each cell by descending
+//     original index -- the order the reference's prepend-linked cell lists
+//     are walked in, neighbor_search.c:180-182, :211-233);
+//   * the neighbour search walks the clamped 3x3(x3) cell block z->y->x and
+//     writes each list in exactly the reference's order, so every per-
+//     particle sum below accumulates in the reference's order;
+//   * number density, free-surface flag, unknown flag, PPE diagonal
+//     (coeff*sum of non-dummy weights) and the density-deviation source are
+//     fused into the search: they need nothing but the list being built;
+//   * lists and matrix rows are sliced-ELL (64-slot slices) so a wavefront's
+//     k-th entries are one contiguous load.
+// FP64 everywhere here, association order of the reference, no contraction.
+#include <limits.h>
+
+#include "ctx.hpp"
+
+#define BLK 256
+
+static inline unsigned nblk(size_t n, unsigned b = BLK) { return (unsigned)((n + b - 1) / b); }
+
+#define LAUNCH_CHECK()                                                       \
+    do {                                                                     \
+        hipError_t e_ = hipGetLastError();                                   \
+        if (e_ != hipSuccess) {                                              \
+            mps_set_error(MPS_ERR_HIP, __FILE__, __LINE__, hipGetErrorString(e_)); \
+            return MPS_ERR_HIP;                                              \
+        }                                                                    \
+    } while (0)
+
+// ---------------------------------------------------------------------
+// fills and scans
+// ---------------------------------------------------------------------
+
+__global__ void k_fill_int(int *p, int v, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_fill_double(double *p, double v, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+int launch_fill_int(int *p, int v, size_t n, hipStream_t s)
+{
+    if (n == 0) return 0;
+    unsigned g = nblk(n);
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(k_fill_int, dim3(g), dim3(BLK), 0, s, p, v, n);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_fill_double(double *p, double v, size_t n, hipStream_t s)
+{
+    if (n == 0) return 0;
+    unsigned g = nblk(n);
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(k_fill_double, dim3(g), dim3(BLK), 0, s, p, v, n);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+// Exclusive scan: out[0..n] with out[n] = total.  Three passes over
+// 2048-element tiles (256 threads x 8 items): tile sums, a one-block scan of
+// the tile sums (up to 8192 tiles = 16.7M elements), tile scans + offset.
+#define SCAN_ITEMS 8
+#define SCAN_TILE (BLK * SCAN_ITEMS)
+
+__global__ void __launch_bounds__(BLK) k_scan_tiles(const int *in, int n, int *tile_sum)
+{
+    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; q++)
+        if (base + q < n) s += in[base + q];
+    int tot;
+    (void)block_scan_excl<BLK>(s, &tot);
+    if (threadIdx.x == 0) tile_sum[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(1024) k_scan_tile_sums(int *tile_sum, int ntiles)
+{
+    // 1024 threads x 8 tiles each
+    const int base = threadIdx.x * 8;
+    int v[8];
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        v[q] = (base + q < ntiles) ? tile_sum[base + q] : 0;
+        s += v[q];
+    }
+    int tot;
+    int run = block_scan_excl<1024>(s, &tot);
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        if (base + q < ntiles) tile_sum[base + q] = run;
+        run += v[q];
+    }
+    if (threadIdx.x == 0) tile_sum[ntiles] = tot;
+}
+
+__global__ void __launch_bounds__(BLK) k_scan_apply(const int *in, int *out, int n,
+                                                    const int *tile_sum, int ntiles)
+{
+    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    int v[SCAN_ITEMS];
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; q++) {
+        v[q] = (base + q < n) ? in[base + q] : 0;
+        s += v[q];
+    }
+    int tot;
+    int run = block_scan_excl<BLK>(s, &tot) + tile_sum[blockIdx.x];
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; q++) {
+        if (base + q < n) out[base + q] = run;
+        run += v[q];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = tile_sum[ntiles];
+}
+
+int launch_scan_excl(const int *in, int *out, int n, int *blk, hipStream_t s)
+{
+    const int ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (ntiles > 8192) {
+        mps_set_error(MPS_ERR_UNSUPPORTED, __FILE__, __LINE__, "scan longer than 16.7M elements");
+        return MPS_ERR_UNSUPPORTED;
+    }
+    if (n == 0) {
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, out, 0, (size_t)1);
+        LAUNCH_CHECK();
+        return 0;
+    }
+    hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(BLK), 0, s, in, n, blk);
+    hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(1024), 0, s, blk, ntiles);
+    hipLaunchKernelGGL(k_scan_apply, dim3(ntiles), dim3(BLK), 0, s, in, out, n, blk, ntiles);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+// ---------------------------------------------------------------------
+// host AoS <-> device SoA
+// ---------------------------------------------------------------------
+
+struct SlotArrays {
+    dvec *X, *V, *A;
+    double *P, *ND;
+    int *T, *O, *S, *NC;
+};
+
+static SlotArrays cur_arrays(Ctx *c)
+{
+    SlotArrays a;
+    a.X = c->X[c->cx]; a.V = c->V[c->cx]; a.A = c->ACC[c->ct];
+    a.P = c->P[c->cp]; a.ND = c->ND[c->ct];
+    a.T = c->T[c->ct]; a.O = c->O[c->ct]; a.S = c->SURF[c->ct]; a.NC = c->NCOUNT[c->ct];
+    return a;
+}
+
+__global__ void __launch_bounds__(BLK) k_unpack(const Particle *st, int n, SlotArrays a)
+{
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    const Particle q = st[i];
+    dvec x, v, ac;
+#pragma unroll
+    for (int d = 0; d < (D_ == 3 ? 4 : 2); d++) x.c[d] = v.c[d] = ac.c[d] = 0.0;
+#pragma unroll
+    for (int d = 0; d < D_; d++) {
+        x.c[d] = q.pos[d];
+        v.c[d] = q.vel[d];
+        ac.c[d] = q.acc[d];
+    }
+    a.X[i] = x; a.V[i] = v; a.A[i] = ac;
+    a.P[i] = q.pressure; a.ND[i] = q.n;
+    a.T[i] = q.type; a.O[i] = i; a.S[i] = q.on_surface; a.NC[i] = q.neighbor_count;
+}
+
+__global__ void __launch_bounds__(BLK) k_pack(Particle *st, int n, SlotArrays a)
+{
+    const int s = blockIdx.x * BLK + threadIdx.x;
+    if (s >= n) return;
+    Particle q;
+    const dvec x = a.X[s], v = a.V[s], ac = a.A[s];
+#pragma unroll
+    for (int d = 0; d < D_; d++) {
+        q.pos[d] = x.c[d];
+        q.vel[d] = v.c[d];
+        q.acc[d] = ac.c[d];
+    }
+    q.pressure = a.P[s];
+    q.n = a.ND[s];
+    q.neighbor_count = a.NC[s];
+    q.type = a.T[s];
+    q.on_surface = a.S[s];
+    st[a.O[s]] = q;
+}
+
+int launch_unpack(Ctx *c, const void *stage_dev, int)
+{
+    c->cx = c->ct = c->cp = 0;
+    hipLaunchKernelGGL(k_unpack, dim3(nblk(c->n)), dim3(BLK), 0, c->stream,
+                       (const Particle *)stage_dev, c->n, cur_arrays(c));
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int launch_pack(Ctx *c, void *stage_dev)
+{
+    hipLaunchKernelGGL(k_pack, dim3(nblk(c->n)), dim3(BLK), 0, c->stream,
+                       (Particle *)stage_dev, c->n, cur_arrays(c));
+    LAUNCH_CHECK();
+    return 0;
+}
+
+// ---------------------------------------------------------------------
+// neighbour list <-> compact CSR over original indices
+// ---------------------------------------------------------------------
+
+__global__ void __launch_bounds__(BLK) k_counts_by_orig(int n, const int *O, const int *ncnt, int *cnt_orig)
+{
+    const int s = blockIdx.x * BLK + threadIdx.x;
+    if (s < n) cnt_orig[O[s]] = ncnt[s];
+}
+
+__global__ void __launch_bounds__(BLK) k_list_compact(int n, int stride, const int *O, const int *nbr,
+                                                      const int *ncnt, const int *off_orig, int *out)
+{
+    const int s = blockIdx.x * BLK + threadIdx.x;
+    if (s >= n) return;
+    const int m = ncnt[s];
+    int *dst = out + off_orig[O[s]];
+    for (int k = 0; k < m; k++) dst[k] = O[nbr[sell_off(s, k, stride)]];
+}
+
+// The imported list refers to original indices; the device order is the
+// identity after an upload, so slots equal original indices.
+__global__ void __launch_bounds__(BLK) k_list_expand(int n, int stride, const int *src, const int *counts,
+                                                     const int *offsets, int *nbr, int *ncnt)
+{
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    const int m = counts[i] < stride ? counts[i] : stride;
+    const int *row = src + offsets[i];
+    for (int k = 0; k < m; k++) nbr[sell_off(i, k, stride)] = row[k];
+    ncnt[i] = m;
+}
+
+int launch_list_export(Ctx *c, long long *total_out)
+{
+    hipLaunchKernelGGL(k_counts_by_orig, dim3(nblk(c->n)), dim3(BLK), 0, c->stream, c->n,
+                       c->O[c->ct], c->ncnt, c->cnt_orig);
+    LAUNCH_CHECK();
+    int rc = launch_scan_excl(c->cnt_orig, c->off_orig, c->n, c->scan_blk, c->stream);
+    if (rc) return rc;
+    int total = 0;
+    HIP_OK(hipMemcpyAsync(&total, c->off_orig + c->n, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    if ((size_t)total > c->nl_stage_elems) {
+        if (c->nl_stage) HIP_OK(hipFree(c->nl_stage));
+        c->nl_stage = nullptr;
+        HIP_OK(hipMalloc(&c->nl_stage, sizeof(int) * (size_t)(total > 0 ? total : 1)));
+        c->nl_stage_elems = (size_t)(total > 0 ? total : 1);
+    }
+    hipLaunchKernelGGL(k_list_compact, dim3(nblk(c->n)), dim3(BLK), 0, c->stream, c->n, c->stride,
+                       c->O[c->ct], c->nbr, c->ncnt, c->off_orig, c->nl_stage);
+    LAUNCH_CHECK();
+    *total_out = total;
+    return 0;
+}
+
+int launch_list_import(Ctx *c, const int *counts_dev, const int *offsets_dev)
+{
+    hipLaunchKernelGGL(k_list_expand, dim3(nblk(c->n)), dim3(BLK), 0, c->stream, c->n, c->stride,
+                       c->nl_stage, counts_dev, offsets_dev, c->nbr, c->ncnt);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+// ---------------------------------------------------------------------
+// 1. acceleration = gravity + viscosity (stale list), predictor
+//    simulation.c:31-67, operators.c:50-85
+// ---------------------------------------------------------------------
+
+__global__ void __launch_bounds__(BLK)
+k_visc_predict(int n, const dvec *__restrict__ X, const dvec *__restrict__ V, const int *__restrict__ T,
+               const int *__restrict__ nbr, const int *__restrict__ ncnt, dvec *__restrict__ Xo,
+               dvec *__restrict__ Vo, StepConsts k)
+{
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    const dvec xi = X[i], vi = V[i];
+    if (T[i] != FLUID_PARTICLE) {
+        Xo[i] = xi;
+        Vo[i] = vi;
+        return;
+    }
+    double lap[D_];
+#pragma unroll
+    for (int d = 0; d < D_; d++) lap[d] = 0.0;
+    const int m = ncnt[i];
+    for (int q = 0; q < m; q++) {
+        const int j = nbr[sell_off(i, q, k.stride)];
+        const dvec xj = X[j], vj = V[j];
+        double dr[D_];
+        const double w = wfun(sqrt(dist2(xi, xj, dr)), k.re_lap, k.r_min);
+#pragma unroll
+        for (int d = 0; d < D_; d++) lap[d] += (vj.c[d] - vi.c[d]) * w;
+    }
+    double acc[D_];
+#pragma unroll
+    for (int d = 0; d < D_; d++) acc[d] = k.g[d] + k.visc_coeff * lap[d];
+    if (k.st_enabled) {
+        // potential-type surface tension, liquid-liquid then solid-liquid
+        // (operators.c:281-354), added partner by partner as the reference does
+        for (int pass = 0; pass < 2; pass++) {
+            if (pass == 1 && !k.st_sl_on) break;
+            const int partner = pass == 0 ? FLUID_PARTICLE : WALL_PARTICLE;
+            const double cst = pass == 0 ? k.st_ll : k.st_sl;
+            for (int q = 0; q < m; q++) {
+                const int j = nbr[sell_off(i, q, k.stride)];
+                if (T[j] != partner) continue;
+                double dr[D_];
+                const double r = sqrt(dist2(xi, X[j], dr));
+                if (r < k.st_tiny || r >= k.re_st) continue;
+                const double f = -(r - k.l0) * (r - k.re_st);
+#pragma unroll
+                for (int d = 0; d < D_; d++) acc[d] += cst * f * (dr[d] / r);
+            }
+        }
+    }
+    dvec xo = xi, vo = vi;
+#pragma unroll
+    for (int d = 0; d < D_; d++) {
+        vo.c[d] = vi.c[d] + k.dt * acc[d];
+        xo.c[d] = xi.c[d] + k.dt * vo.c[d];
+    }
+    Xo[i] = xo;
+    Vo[i] = vo;
+}
+
+int launch_visc_predict(Ctx *c, const StepConsts &k)
+{
+    prof_begin(PK_VISC, c->stream);
+    hipLaunchKernelGGL(k_visc_predict, dim3(nblk(c->n)), dim3(BLK), 0, c->stream, c->n,
+                       c->X[c->cx], c->V[c->cx], c->T[c->ct], c->nbr, c->ncnt,
+                       c->X[1 - c->cx], c->V[1 - c->cx], k);
+    prof_end(PK_VISC, c->stream);
+    LAUNCH_CHECK();
+    c->cx ^= 1;
+    return 0;
+}
+
+// ---------------------------------------------------------------------
+// 2. collision over the stale list (operators.c:371-446), then the cell of
+//    every particle and the per-cell histogram of the counting sort
+// ---------------------------------------------------------------------
+
+__global__ void __launch_bounds__(BLK)
+k_collide_bin(int n, const dvec *__restrict__ X, const dvec *__restrict__ V, const int *__restrict__ T,
+              const int *__restrict__ nbr, const int *__restrict__ ncnt, dvec *__restrict__ Xo,
+              dvec *__restrict__ Vo, int *__restrict__ cell, int *__restrict__ ccount, StepConsts k,
+              int collide)
+{
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    const int t = T[i];
+    const dvec xi = X[i], vi = V[i];
+    dvec xo = xi, vo = vi;
+    if (collide && t == FLUID_PARTICLE) {
+        double w[D_];
+#pragma unroll
+        for (int d = 0; d < D_; d++) w[d] = vi.c[d];
+        const int m = ncnt[i];
+        for (int q = 0; q < m; q++) {
+            const int j = nbr[sell_off(i, q, k.stride)];
+            if (T[j] == GHOST_PARTICLE) continue;
+            double dr[D_];
+            const double r2 = dist2(xi, X[j], dr);
+            if (r2 >= k.cd2) continue;
+            const double r = sqrt(r2);
+            if (r < k.col_tiny) continue;
+            const dvec vj = V[j];
+            double vn = 0.0;
+#pragma unroll
+            for (int d = 0; d < D_; d++) vn += (w[d] - vj.c[d]) * (dr[d] / r);
+            if (vn <= 0.0) continue;
+            const double imp = k.e_half * vn;
+#pragma unroll
+            for (int d = 0; d < D_; d++) w[d] -= imp * (dr[d] / r);
+        }
+#pragma unroll
+        for (int d = 0; d < D_; d++) {
+            const double dv = w[d] - vi.c[d];
+            xo.c[d] = xi.c[d] + k.dt * dv;
+            vo.c[d] = w[d];
+        }
+    }
+    Xo[i] = xo;
+    Vo[i] = vo;
+    int cc[3];
+    int ci = (t == GHOST_PARTICLE) ? -1 : cell_coords(k.grid, xo, cc);
+    if (ci < 0) ci = k.grid.total;
+    cell[i] = ci;
+    atomicAdd(&ccount[ci], 1);
+}
+
+int launch_collide_bin(Ctx *c, const StepConsts &k, int collide)
+{
+    int rc = launch_fill_int(c->ccount, 0, (size_t)k.grid.total + 1, c->stream);
+    if (rc) return rc;
+    prof_begin(PK_COLLIDE, c->stream);
+    hipLaunchKernelGGL(k_collide_bin, dim3(nblk(c->n)), dim3(BLK), 0, c->stream, c->n,
+                       c->X[c->cx], c->V[c->cx], c->T[c->ct], c->nbr, c->ncnt,
+                       c->X[1 - c->cx], c->V[1 - c->cx], c->cell, c->ccount, k, collide);
+    prof_end(PK_COLLIDE, c->stream);
+    LAUNCH_CHECK();
+    c->cx ^= 1;
+    return 0;
+}
+
+int launch_bin(Ctx *c, const StepConsts &k) { return launch_collide_bin(c, k, 0); }
+
+// ---------------------------------------------------------------------
+// 2-D collision (mps_fluid2D/src/operators.c:324-392): every j in ascending
+// index order, not a neighbour list.  Only partners closer than the
+// collision distance (< cell size) act, so each fluid particle visits them
+// in ascending original index by repeated selection over its 3x3 cell block
+// plus every non-ghost particle outside the grid (or over all particles if
+// it is itself outside).  The sequence of velocity updates is the
+// reference's.
+// ---------------------------------------------------------------------
+
+__global__ void __launch_bounds__(BLK)
+k_bin_cells(int n, const dvec *__restrict__ X, const int *__restrict__ T, int *__restrict__ cell,
+            int *__restrict__ ccount, Grid g)
+{
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    int cc[3];
+    int ci;
+    if (T[i] == GHOST_PARTICLE) ci = g.total + 1;
+    else {
+        ci = cell_coords(g, X[i], cc);
+        if (ci < 0) ci = g.total;
+    }
+    cell[i] = ci;
+    atomicAdd(&ccount[ci], 1);
+}
+
+__global__ void __launch_bounds__(BLK)
+k_bucket_slots(int n, const int *__restrict__ cell, const int *__restrict__ cstart,
+               int *__restrict__ cfill, int *__restrict__ bucket)
+{
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    const int ci = cell[i];
+    bucket[cstart[ci] + atomicAdd(&cfill[ci], 1)] = i;
+}
+
+__device__ __forceinline__ void collide_one(const dvec &xi, const dvec &xj, const dvec &vj,
+                                            double w[D_], const StepConsts &k)
+{
+    double dr[D_];
+    const double r2 = dist2(xi, xj, dr);
+    if (r2 >= k.cd2) return;
+    const double r = sqrt(r2);
+    if (r < k.col_tiny) return;
+    double vn = 0.0;
+#pragma unroll
+    for (int d = 0; d < D_; d++) vn += (w[d] - vj.c[d]) * (dr[d] / r);
+    if (vn <= 0.0) return;
+    const double imp = k.e_half * vn;
+#pragma unroll
+    for (int d = 0; d < D_; d++) w[d] -= imp * (dr[d] / r);
+}
+
+__global__ void __launch_bounds__(BLK)
+k_collide2d(int n, const dvec *__restrict__ X, const dvec *__restrict__ V, const int *__restrict__ T,
+            const int *__restrict__ O, const int *__restrict__ cstart, const int *__restrict__ bucket,
+            dvec *__restrict__ Xo, dvec *__restrict__ Vo, StepConsts k)
+{
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    const dvec xi = X[i], vi = V[i];
+    if (T[i] != FLUID_PARTICLE) {
+        Xo[i] = xi;
+        Vo[i] = vi;
+        return;
+    }
+    const Grid &g = k.grid;
+    int cc[3];
+    const bool inside = cell_coords(g, xi, cc) >= 0;
+    int lo[3] = {0, 0, 0}, hi[3] = {-1, -1, 0};
+    if (inside) {
+        for (int d = 0; d < 2; d++) {
+            lo[d] = cc[d] > 0 ? cc[d] - 1 : 0;
+            hi[d] = cc[d] < g.nc[d] - 1 ? cc[d] + 1 : g.nc[d] - 1;
+        }
+    }
+    double w[D_];
+#pragma unroll
+    for (int d = 0; d < D_; d++) w[d] = vi.c[d];
+    int last = -1;
+    for (;;) {
+        int best = INT_MAX, best_slot = -1;
+        // a candidate acts in the reference iff it is not a ghost and
+        // !(r^2 >= cd^2) (NaN distances included)
+#define CONSIDER(jj)                                                         \
+        {                                                                    \
+            const int j_ = (jj);                                             \
+            if (j_ != i && T[j_] != GHOST_PARTICLE) {                        \
+                double dr_[D_];                                              \
+                if (!(dist2(xi, X[j_], dr_) >= k.cd2)) {                     \
+                    const int o_ = O[j_];                                    \
+                    if (o_ > last && o_ < best) { best = o_; best_slot = j_; } \
+                }                                                            \
+            }                                                                \
+        }
+        if (inside) {
+            for (int cy = lo[1]; cy <= hi[1]; cy++)
+                for (int cxx = lo[0]; cxx <= hi[0]; cxx++) {
+                    const int cid = cy * g.nc[0] + cxx;
+                    for (int u = cstart[cid]; u < cstart[cid + 1]; u++) CONSIDER(bucket[u]);
+                }
+            for (int u = cstart[g.total]; u < cstart[g.total + 1]; u++) CONSIDER(bucket[u]);
+        } else {
+            for (int j = 0; j < n; j++) CONSIDER(j);
+        }
+#undef CONSIDER
+        if (best_slot < 0) break;
+        collide_one(xi, X[best_slot], V[best_slot], w, k);
+        last = best;
+    }
+    dvec xo = xi, vo = vi;
+#pragma unroll
+    for (int d = 0; d < D_; d++) {
+        const double dv = w[d] - vi.c[d];
+        xo.c[d] = xi.c[d] + k.dt * dv;
+        vo.c[d] = w[d];
+    }
+    Xo[i] = xo;
+    Vo[i] = vo;
+}
+
+int collide_2d(Ctx *c, const StepConsts &k)
+{
+    const int total = k.grid.total;
+    int rc = launch_fill_int(c->ccount, 0, (size_t)total + 2, c->stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_bin_cells, dim3(nblk(c->n)), dim3(BLK), 0, c->stream, c->n, c->X[c->cx],
+                       c->T[c->ct], c->cell, c->ccount, k.grid);
+    LAUNCH_CHECK();
+    rc = launch_scan_excl(c->ccount, c->cstart, total + 2, c->scan_blk, c->stream);
+    if (!rc) rc = launch_fill_int(c->cfill, 0, (size_t)total + 2, c->stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_bucket_slots, dim3(nblk(c->n)), dim3(BLK), 0, c->stream, c->n, c->cell,
+                       c->cstart, c->cfill, c->tmp_key);
+    prof_begin(PK_COLLIDE, c->stream);
+    hipLaunchKernelGGL(k_collide2d, dim3(nblk(c->n)), dim3(BLK), 0, c->stream, c->n, c->X[c->cx],
+                       c->V[c->cx], c->T[c->ct], c->O[c->ct], c->cstart, c->tmp_key,
+                       c->X[1 - c->cx], c->V[1 - c->cx], k);
+    prof_end(PK_COLLIDE, c->stream);
+    LAUNCH_CHECK();
+    c->cx ^= 1;
+    return 0;
+}
+
+// ---------------------------------------------------------------------
+// 3. counting sort by (cell, descending original index)
+// ---------------------------------------------------------------------
+
+__global__ void __launch_bounds__(BLK)
+k_scatter(int n, const int *__restrict__ cell, const int *__restrict__ O, const int *__restrict__ cstart,
+          int *__restrict__ cfill, int *__restrict__ tmp_key, int *__restrict__ sflag, int total)
+{
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    const int ci = cell[i];
+    sflag[i] = (ci == total);
+    if (ci == total) return;
+    const int at = atomicAdd(&cfill[ci], 1);
+    tmp_key[cstart[ci] + at] = O[i];
+}
+
+struct SortMove {
+    SlotArrays src, dst;
+    int full;
+};
+
+__global__ void __launch_bounds__(BLK)
+k_rank_move(int n, const int *__restrict__ cell, const int *__restrict__ cstart,
+            const int *__restrict__ tmp_key, const int *__restrict__ sscan, int total, SortMove mv)
+{
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    const int ci = cell[i];
+    const int key = mv.src.O[i];
+    int dst;
+    if (ci < total) {
+        const int s0 = cstart[ci], s1 = cstart[ci + 1];
+        int rank = 0;
+        for (int u = s0; u < s1; u++) rank += (tmp_key[u] > key);
+        dst = s0 + rank;
+    } else {
+        dst = cstart[total] + sscan[i];   // outside the grid: previous slot order
+    }
+    mv.dst.X[dst] = mv.src.X[i];
+    mv.dst.V[dst] = mv.src.V[i];
+    mv.dst.T[dst] = mv.src.T[i];
+    mv.dst.O[dst] = key;
+    mv.dst.NC[dst] = mv.src.NC[i];
+    if (mv.full) {
+        mv.dst.A[dst] = mv.src.A[i];
+        mv.dst.P[dst] = mv.src.P[i];
+        mv.dst.ND[dst] = mv.src.ND[i];
+        mv.dst.S[dst] = mv.src.S[i];
+    }
+}
+
+int launch_sort(Ctx *c, const StepConsts &k, int full)
+{
+    const int total = k.grid.total;
+    prof_begin(PK_SORT, c->stream);
+    int rc = launch_scan_excl(c->ccount, c->cstart, total + 1, c->scan_blk, c->stream);
+    if (rc) return rc;
+    rc = launch_fill_int(c->cfill, 0, (size_t)total + 1, c->stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_scatter, dim3(nblk(c->n)), dim3(BLK), 0, c->stream, c->n, c->cell,
+                       c->O[c->ct], c->cstart, c->cfill, c->tmp_key, c->sflag, total);
+    LAUNCH_CHECK();
+    rc = launch_scan_excl(c->sflag, c->sscan, c->n, c->scan_blk, c->stream);
+    if (rc) return rc;
+    SortMove mv;
+    mv.src = cur_arrays(c);
+    mv.dst.X = c->X[1 - c->cx]; mv.dst.V = c->V[1 - c->cx];
+    mv.dst.A = c->ACC[1 - c->ct]; mv.dst.ND = c->ND[1 - c->ct];
+    mv.dst.T = c->T[1 - c->ct]; mv.dst.O = c->O[1 - c->ct];
+    mv.dst.S = c->SURF[1 - c->ct]; mv.dst.NC = c->NCOUNT[1 - c->ct];
+    mv.dst.P = c->P[1 - c->cp];
+    mv.full = full;
+    hipLaunchKernelGGL(k_rank_move, dim3(nblk(c->n)), dim3(BLK), 0, c->stream, c->n, c->cell,
+                       c->cstart, c->tmp_key, c->sscan, total, mv);
+    prof_end(PK_SORT, c->stream);
+    LAUNCH_CHECK();
+    c->cx ^= 1;
+    c->ct ^= 1;
+    if (full) c->cp ^= 1;
+    return 0;
+}
+
+// ---------------------------------------------------------------------
+// 4. neighbour search (neighbor_search.c:185-239) fused with number
+//    density (operators.c:17-38), free-surface test (:206-219 / :226-267),
+//    unknown flag, PPE diagonal and density-deviation source
+//    (pressure_solver.c:411-422, :495-529, :328-340)
+// ---------------------------------------------------------------------
+
+struct SearchOut {
+    int *nbr, *ncnt, *surf, *unk, *ncount;
+    double *nd, *diag, *b;
+    DevCounters *ctr;
+};
+
+__global__ void __launch_bounds__(BLK)
+k_search(int n, const dvec *__restrict__ X, const int *__restrict__ T, const int *__restrict__ O,
+         const int *__restrict__ cstart, SearchOut out, StepConsts k, int with_density)
+{
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    int cnt = 0, cnt_n = 0, u = 0;
+    if (i < n) {
+        const int t = T[i];
+        const dvec xi = X[i];
+        double nsum = 0.0, sumw = 0.0;
+        int cc[3];
+        if (t != GHOST_PARTICLE && cell_coords(k.grid, xi, cc) >= 0) {
+            int lo[3], hi[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                lo[d] = cc[d] > 0 ? cc[d] - 1 : 0;
+                hi[d] = cc[d] < k.grid.nc[d] - 1 ? cc[d] + 1 : k.grid.nc[d] - 1;
+            }
+            for (int cz = lo[2]; cz <= hi[2]; cz++)
+                for (int cy = lo[1]; cy <= hi[1]; cy++)
+                    for (int cxx = lo[0]; cxx <= hi[0]; cxx++) {
+                        const int cid = (cz * k.grid.nc[1] + cy) * k.grid.nc[0] + cxx;
+                        const int j1 = cstart[cid + 1];
+                        for (int j = cstart[cid]; j < j1; j++) {
+                            if (j == i) continue;
+                            double dr[D_];
+                            const double r2 = dist2(xi, X[j], dr);
+                            if (!(r2 < k.re2)) continue;
+                            if (cnt < k.stride) out.nbr[sell_off(i, cnt, k.stride)] = j;
+                            cnt++;
+                            if (with_density) {
+                                const int tj = T[j];
+                                const bool need_n = r2 <= k.re_n2_hi;
+                                const bool need_l = tj != DUMMY_PARTICLE;
+                                if (need_n || need_l) {
+                                    const double r = sqrt(r2);
+                                    if (need_n) nsum += wfun(r, k.re_n, k.r_min);
+                                    if (need_l) sumw += wfun(r, k.re_lap, k.r_min);
+                                }
+                                cnt_n += (r2 < k.re_n2);
+                            }
+                        }
+                    }
+        }
+        out.ncnt[i] = cnt < k.stride ? cnt : k.stride;
+        if (cnt > k.stride) atomicMin(&out.ctr->overflow_orig, O[i]);
+        if (with_density) {
+            int s;
+            if (k.surf_method == 1) {
+                out.ncount[i] = cnt_n;
+                s = (t == FLUID_PARTICLE) ? ((double)cnt_n < k.beta_n0c) : 0;
+            } else {
+                s = (t == FLUID_PARTICLE) ? (nsum < k.thr_n0) : 0;
+            }
+            u = (t == FLUID_PARTICLE && !s) || t == WALL_PARTICLE;
+            out.nd[i] = nsum;
+            out.surf[i] = s;
+            out.unk[i] = u;
+            out.diag[i] = u ? k.diag_coeff * sumw : 0.0;
+            double bi = 0.0;
+            if (u && t == FLUID_PARTICLE && k.hs_mode == 0 && k.ppe_type == 0)
+                bi = k.src_k * (nsum - k.n0) / k.n0;
+            out.b[i] = bi;
+        }
+    }
+    // counters: unknowns, directed pairs, largest list
+    const long long pairs = block_sum<long long, BLK>((long long)cnt);
+    const long long nu = block_sum<long long, BLK>((long long)u);
+    if (threadIdx.x == 0) {
+        atomicAdd(&out.ctr->n_pairs, (unsigned long long)pairs);
+        atomicAdd(&out.ctr->n_eq, (int)nu);
+    }
+    if (i < n && cnt > 0) atomicMax(&out.ctr->max_count, cnt);
+}
+
+int launch_search(Ctx *c, const StepConsts &k, int with_density, DevCounters *ctr)
+{
+    SearchOut o;
+    o.nbr = c->nbr; o.ncnt = c->ncnt; o.surf = c->SURF[c->ct]; o.unk = c->unk;
+    o.ncount = c->NCOUNT[c->ct]; o.nd = c->ND[c->ct]; o.diag = c->diag; o.b = c->b; o.ctr = ctr;
+    prof_begin(PK_SEARCH, c->stream);
+    hipLaunchKernelGGL(k_search, dim3(nblk(c->n)), dim3(BLK), 0, c->stream, c->n, c->X[c->cx],
+                       c->T[c->ct], c->O[c->ct], c->cstart, o, k, with_density);
+    prof_end(PK_SEARCH, c->stream);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+// ---------------------------------------------------------------------
+// 5. PPE off-diagonals (pressure_solver.c:490-531) and the sources that
+//    need velocities (HS :342-362, weakly compressible :381-409)
+// ---------------------------------------------------------------------
+
+__global__ void __launch_bounds__(BLK)
+k_assemble(int n, const dvec *__restrict__ X, const dvec *__restrict__ V, const int *__restrict__ T,
+           const double *__restrict__ ND, const int *__restrict__ unk, const int *__restrict__ nbr,
+           const int *__restrict__ ncnt, double *__restrict__ Aval, int *__restrict__ Acol,
+           int *__restrict__ Alen, double *__restrict__ b, DevCounters *ctr, StepConsts k)
+{
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    long long row_nnz = 0;
+    if (i < n) {
+        if (!unk[i]) {
+            Alen[i] = 0;
+        } else {
+            const dvec xi = X[i];
+            const int t = T[i];
+            const bool need_v = (t == FLUID_PARTICLE) && (k.hs_mode == 1 || k.ppe_type == 1);
+            dvec vi;
+            if (need_v) vi = V[i];
+            double hs = 0.0, div = 0.0;
+            int m = 0;
+            const int cnt = ncnt[i];
+            for (int q = 0; q < cnt; q++) {
+                const int j = nbr[sell_off(i, q, k.stride)];
+                const int uj = unk[j];
+                if (!uj && !need_v) continue;
+                if (T[j] == DUMMY_PARTICLE) continue;
+                double dr[D_];
+                const double r2 = dist2(xi, X[j], dr);
+                if (uj) {
+                    const double w = wfun(sqrt(r2), k.re_lap, k.r_min);
+                    Aval[sell_off(i, m, k.stride)] = k.neg_coeff * w;
+                    Acol[sell_off(i, m, k.stride)] = j;
+                    m++;
+                }
+                if (need_v && !(r2 < 1.0e-24)) {
+                    const dvec vj = V[j];
+                    if (k.hs_mode == 1) {
+                        const double r = sqrt(r2);
+                        double dv = 0.0;
+#pragma unroll
+                        for (int d = 0; d < D_; d++) dv += dr[d] * (vj.c[d] - vi.c[d]);
+                        hs += k.re_lap / (r * r * r) * dv;
+                    } else {
+                        double dv = 0.0;
+#pragma unroll
+                        for (int d = 0; d < D_; d++) dv += (vj.c[d] - vi.c[d]) * dr[d];
+                        div += dv / r2 * wfun(sqrt(r2), k.re_lap, k.r_min);
+                    }
+                }
+            }
+            Alen[i] = m;
+            row_nnz = m + 1;
+            if (need_v) {
+                if (k.hs_mode == 1) {
+                    b[i] = k.hs_k * hs;
+                } else {
+                    const double vel_div = k.dim_n0 * div;
+                    b[i] = k.neg_rho_dt * vel_div + k.src_k * (ND[i] - k.n0) / k.n0;
+                }
+            }
+        }
+    }
+    const long long tot = block_sum<long long, BLK>(row_nnz);
+    if (threadIdx.x == 0) atomicAdd(&ctr->nnz, (unsigned long long)tot);
+}
+
+int launch_assemble(Ctx *c, const StepConsts &k, DevCounters *ctr)
+{
+    prof_begin(PK_ASSEMBLE, c->stream);
+    hipLaunchKernelGGL(k_assemble, dim3(nblk(c->n)), dim3(BLK), 0, c->stream, c->n, c->X[c->cx],
+                       c->V[c->cx], c->T[c->ct], c->ND[c->ct], c->unk, c->nbr, c->ncnt, c->Aval,
+                       c->Acol, c->Alen, c->b, ctr, k);
+    prof_end(PK_ASSEMBLE, c->stream);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+// ---------------------------------------------------------------------
+// 6. solution -> pressure (pressure_solver.c:563-566), negative clamp
+//    (operators.c:188-199)
+// ---------------------------------------------------------------------
+
+__global__ void __launch_bounds__(BLK)
+k_pressure_out(int n, const double *__restrict__ x, const int *__restrict__ unk, const int *__restrict__ T,
+               double *__restrict__ P, int clamp)
+{
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    double p = unk[i] ? x[i] : 0.0;
+    if (clamp && T[i] == FLUID_PARTICLE && p < 0.0) p = 0.0;
+    P[i] = p;
+}
+
+int launch_pressure_out(Ctx *c, const StepConsts &k, int n_eq_zero)
+{
+    (void)n_eq_zero;
+    hipLaunchKernelGGL(k_pressure_out, dim3(nblk(c->n)), dim3(BLK), 0, c->stream, c->n, c->x,
+                       c->unk, c->T[c->ct], c->P[c->cp], k.clamp);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+// ---------------------------------------------------------------------
+// 7. pressure gradient (operators.c:102-181), corrector (simulation.c:
+//    100-108), wall re-fix and out-of-domain ghosting (boundary.c:12-61)
+// ---------------------------------------------------------------------
+
+__global__ void __launch_bounds__(BLK)
+k_pmin(int n, const double *__restrict__ P, const int *__restrict__ nbr, const int *__restrict__ ncnt,
+       double *__restrict__ pmin, int stride)
+{
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    double m = P[i];
+    const int cnt = ncnt[i];
+    for (int q = 0; q < cnt; q++) {
+        const double pj = P[nbr[sell_off(i, q, stride)]];
+        if (pj < m) m = pj;
+    }
+    pmin[i] = m;
+}
+
+struct GradIO {
+    const dvec *X, *V;
+    const double *P, *pmin;
+    dvec *Xo, *Vo, *A;
+    double *Po, *ND;
+    int *T;
+};
+
+__global__ void __launch_bounds__(BLK)
+k_grad_correct(int n, GradIO io, const int *__restrict__ nbr, const int *__restrict__ ncnt, StepConsts k)
+{
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    int t = io.T[i];
+    const dvec xi = io.X[i], vi = io.V[i];
+    const double pi = io.P[i];
+    dvec xo = xi, vo = vi, ao;
+#pragma unroll
+    for (int d = 0; d < (D_ == 3 ? 4 : 2); d++) ao.c[d] = 0.0;
+    double po = pi;
+    if (t == FLUID_PARTICLE) {
+        const int cnt = ncnt[i];
+        double pimin = 0.0;
+        if (k.cmps == 1) {
+            pimin = io.pmin[i];
+        } else if (k.cmps == 0) {
+            pimin = pi;
+            for (int q = 0; q < cnt; q++) {
+                const double pj = io.P[nbr[sell_off(i, q, k.stride)]];
+                if (pj < pimin) pimin = pj;
+            }
+        }
+        double grad[D_];
+#pragma unroll
+        for (int d = 0; d < D_; d++) grad[d] = 0.0;
+        for (int q = 0; q < cnt; q++) {
+            const int j = nbr[sell_off(i, q, k.stride)];
+            double dr[D_];
+            const double r2 = dist2(xi, io.X[j], dr);
+            if (r2 < k.grad_min_r2) continue;
+            const double w = wfun(sqrt(r2), k.re_lap, k.r_min);
+            const double pj = io.P[j];
+            double dp;
+            if (k.cmps == 1) dp = pi + pj - pimin - io.pmin[j];
+            else if (k.cmps == 2) dp = pi + pj;
+            else dp = pj - pimin;
+#pragma unroll
+            for (int d = 0; d < D_; d++) grad[d] += dp / r2 * dr[d] * w;
+        }
+#pragma unroll
+        for (int d = 0; d < D_; d++) {
+            ao.c[d] = k.neg_gc * grad[d] / k.rho;
+            const double du = k.dt * ao.c[d];
+            vo.c[d] = vi.c[d] + du;
+            xo.c[d] = xi.c[d] + k.dt * du;
+        }
+        bool out = false;
+#pragma unroll
+        for (int d = 0; d < D_; d++)
+            if (xo.c[d] < k.bnd_lo[d] || xo.c[d] > k.bnd_hi[d] || isnan(xo.c[d])) out = true;
+        if (out) {
+            t = GHOST_PARTICLE;
+            io.T[i] = GHOST_PARTICLE;
+            po = 0.0;
+            io.ND[i] = 0.0;
+#pragma unroll
+            for (int d = 0; d < D_; d++) vo.c[d] = ao.c[d] = 0.0;
+        }
+    } else if (t == WALL_PARTICLE || t == GHOST_PARTICLE) {
+#pragma unroll
+        for (int d = 0; d < D_; d++) vo.c[d] = 0.0;
+    }
+    io.Xo[i] = xo;
+    io.Vo[i] = vo;
+    io.A[i] = ao;
+    io.Po[i] = po;
+}
+
+int launch_grad_correct(Ctx *c, const StepConsts &k)
+{
+    if (k.cmps == 1) {
+        hipLaunchKernelGGL(k_pmin, dim3(nblk(c->n)), dim3(BLK), 0, c->stream, c->n, c->P[c->cp],
+                           c->nbr, c->ncnt, c->pmin, k.stride);
+        LAUNCH_CHECK();
+    }
+    GradIO io;
+    io.X = c->X[c->cx]; io.V = c->V[c->cx]; io.P = c->P[c->cp]; io.pmin = c->pmin;
+    io.Xo = c->X[1 - c->cx]; io.Vo = c->V[1 - c->cx]; io.A = c->ACC[c->ct];
+    io.Po = c->P[1 - c->cp]; io.ND = c->ND[c->ct]; io.T = c->T[c->ct];
+    prof_begin(PK_GRAD, c->stream);
+    hipLaunchKernelGGL(k_grad_correct, dim3(nblk(c->n)), dim3(BLK), 0, c->stream, c->n, io, c->nbr,
+                       c->ncnt, k);
+    prof_end(PK_GRAD, c->stream);
+    LAUNCH_CHECK();
+    c->cx ^= 1;
+    c->cp ^= 1;
+    return 0;
+}
+
+// ---------------------------------------------------------------------
+// stand-alone operators over the current list (drop-in entry points)
+// ---------------------------------------------------------------------
+
+// calc_particle_number_density (operators.c:17-38)
+__global__ void __launch_bounds__(BLK)
+k_density(int n, const dvec *__restrict__ X, const int *__restrict__ nbr, const int *__restrict__ ncnt,
+          double *__restrict__ ND, StepConsts k)
+{
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    const dvec xi = X[i];
+    double s = 0.0;
+    const int cnt = ncnt[i];
+    for (int q = 0; q < cnt; q++) {
+        double dr[D_];
+        s += wfun(sqrt(dist2(xi, X[nbr[sell_off(i, q, k.stride)]], dr)), k.re_n, k.r_min);
+    }
+    ND[i] = s;
+}
+
+int launch_density_only(Ctx *c, const StepConsts &k)
+{
+    hipLaunchKernelGGL(k_density, dim3(nblk(c->n)), dim3(BLK), 0, c->stream, c->n, c->X[c->cx], c->nbr,
+                       c->ncnt, c->ND[c->ct], k);
+    LAUNCH_CHECK();
+    return 0;
+}

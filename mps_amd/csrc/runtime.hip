@@ -1,0 +1,803 @@
+// runtime.hip -- device contexts, host<->device transfers, the timestep
+// orchestration and the C-ABI entry points of libmps_hip.
+//
+// One HIP stream per ParticleSystem.  State stays in HBM between calls when
+// the system is resident (simulation_run, mps_hip_run_steps); otherwise each
+// drop-in call uploads the caller's structures, runs, and downloads, so the
+// caller sees exactly what the reference would have left behind.
+#include <limits.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "ctx.hpp"
+
+// ---------------------------------------------------------------------
+// errors, options, profiling
+// ---------------------------------------------------------------------
+
+static int g_last_error = MPS_OK;
+static char g_last_msg[512] = "";
+
+void mps_set_error(int code, const char *file, int line, const char *msg)
+{
+    g_last_error = code;
+    snprintf(g_last_msg, sizeof g_last_msg, "%s:%d: %s", file, line, msg ? msg : "");
+}
+
+struct Options {
+    double tol_mode = 0, precision = 0, inner_tol = 1e-5, exit_on_error = 1, check_every = 16;
+    double spmv = 0, profile = 0, verbose = 0;
+};
+static Options g_opt;
+
+// Kernel timing: HIP events recorded on the launching stream around each
+// launch of a kernel class, drawn from a pool so recording costs no
+// allocation; pairs are folded into totals after the stream synchronises.
+struct ProfSlot {
+    const char *name;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    double total_ms = 0;
+    long long count = 0;
+};
+static ProfSlot g_prof[PK_COUNT] = {{"visc_predict"}, {"collide_bin"}, {"sort"}, {"search"},
+                                    {"assemble"}, {"spmv"}, {"axpy"}, {"pupdate"},
+                                    {"grad_correct"}, {"ic"}};
+static std::vector<hipEvent_t> g_event_pool;
+static hipEvent_t g_open[PK_COUNT];
+
+static hipEvent_t pool_get()
+{
+    if (g_event_pool.empty()) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    hipEvent_t e = g_event_pool.back();
+    g_event_pool.pop_back();
+    return e;
+}
+
+void prof_begin(int k, hipStream_t s)
+{
+    if (g_opt.profile < 0.5) return;
+    hipEvent_t e = pool_get();
+    if (!e) return;
+    hipEventRecord(e, s);
+    g_open[k] = e;
+}
+
+void prof_end(int k, hipStream_t s)
+{
+    if (g_opt.profile < 0.5 || !g_open[k]) return;
+    hipEvent_t e = pool_get();
+    if (!e) return;
+    hipEventRecord(e, s);
+    g_prof[k].pending.push_back({g_open[k], e});
+    g_open[k] = nullptr;
+}
+
+// Folds completed event pairs into the totals (called after a sync).
+static void prof_collect()
+{
+    for (int k = 0; k < PK_COUNT; k++) {
+        ProfSlot &p = g_prof[k];
+        for (auto &pr : p.pending) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+                p.total_ms += ms;
+                p.count++;
+            }
+            g_event_pool.push_back(pr.first);
+            g_event_pool.push_back(pr.second);
+        }
+        p.pending.clear();
+    }
+}
+
+// ---------------------------------------------------------------------
+// device and contexts
+// ---------------------------------------------------------------------
+
+static int g_device = -1;
+static char g_arch[64] = "";
+static std::map<ParticleSystem *, Ctx *> g_ctx;
+
+static int ensure_device()
+{
+    if (g_device >= 0) return MPS_OK;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        mps_set_error(MPS_ERR_NO_DEVICE, __FILE__, __LINE__, "no HIP device visible");
+        return MPS_ERR_NO_DEVICE;
+    }
+    HIP_OK(hipSetDevice(0));
+    hipDeviceProp_t prop;
+    HIP_OK(hipGetDeviceProperties(&prop, 0));
+    snprintf(g_arch, sizeof g_arch, "%s", prop.gcnArchName);
+    if (strncmp(g_arch, "gfx950", 6) != 0) {
+        mps_set_error(MPS_ERR_NO_DEVICE, __FILE__, __LINE__, "library is built for gfx950 only");
+        return MPS_ERR_NO_DEVICE;
+    }
+    g_device = 0;
+    return MPS_OK;
+}
+
+template <typename T>
+static int dalloc(T **p, size_t n)
+{
+    if (*p) return 0;
+    HIP_OK(hipMalloc((void **)p, sizeof(T) * (n > 0 ? n : 1)));
+    return 0;
+}
+
+#define ALLOC(p, n)                   \
+    do {                              \
+        int rc_ = dalloc(&(p), (n));  \
+        if (rc_) return rc_;          \
+    } while (0)
+
+#define DFREE(p)                      \
+    do {                              \
+        if (p) hipFree(p);            \
+        (p) = nullptr;                \
+    } while (0)
+
+void ic_free(IcData *d);   // k_ic.hip
+
+void ctx_free(Ctx *c)
+{
+    for (int q = 0; q < 2; q++) {
+        DFREE(c->X[q]); DFREE(c->V[q]); DFREE(c->ACC[q]); DFREE(c->T[q]); DFREE(c->O[q]);
+        DFREE(c->SURF[q]); DFREE(c->NCOUNT[q]); DFREE(c->P[q]); DFREE(c->ND[q]);
+    }
+    DFREE(c->unk); DFREE(c->pmin); DFREE(c->cell); DFREE(c->ccount); DFREE(c->cstart);
+    DFREE(c->cfill); DFREE(c->tmp_key); DFREE(c->sflag); DFREE(c->sscan); DFREE(c->scan_blk);
+    DFREE(c->nbr); DFREE(c->ncnt); DFREE(c->diag); DFREE(c->b); DFREE(c->Aval); DFREE(c->Acol);
+    DFREE(c->Alen); DFREE(c->x); DFREE(c->r); DFREE(c->pv); DFREE(c->Ap); DFREE(c->partA);
+    DFREE(c->partB); DFREE(c->fAval); DFREE(c->fdiag); DFREE(c->fr); DFREE(c->fp); DFREE(c->fAp);
+    DFREE(c->fz); DFREE(c->stage); DFREE(c->nl_stage); DFREE(c->cnt_orig); DFREE(c->off_orig);
+    DFREE(c->counters);
+    if (c->ctrl) { hipFree(c->ctrl); c->ctrl = nullptr; }
+    if (c->hctrl) { hipHostFree(c->hctrl); c->hctrl = nullptr; }
+    if (c->hcounters) { hipHostFree(c->hcounters); c->hcounters = nullptr; }
+    if (c->ic) { ic_free(c->ic); c->ic = nullptr; }
+    c->n = c->cap = c->stride = c->grid_cap = 0;
+    c->stage_bytes = c->nl_stage_elems = 0;
+    c->has_state = c->list_valid = false;
+}
+
+int ctx_alloc(Ctx *c, int n, int stride, int grid_cells)
+{
+    if (n > c->cap || stride != c->stride) {
+        // particle-sized buffers
+        Grid keep = c->grid;
+        int gcap = c->grid_cap;
+        int *cc = c->ccount, *cs = c->cstart, *cf = c->cfill;
+        c->ccount = c->cstart = c->cfill = nullptr;
+        ctx_free(c);
+        c->grid = keep;
+        c->ccount = cc; c->cstart = cs; c->cfill = cf; c->grid_cap = gcap;
+        const size_t N = (size_t)(n > 0 ? n : 1);
+        const size_t NS = ((N + 63) / 64) * 64;   // whole slices
+        for (int q = 0; q < 2; q++) {
+            ALLOC(c->X[q], N); ALLOC(c->V[q], N); ALLOC(c->ACC[q], N); ALLOC(c->T[q], N);
+            ALLOC(c->O[q], N); ALLOC(c->SURF[q], N); ALLOC(c->NCOUNT[q], N); ALLOC(c->P[q], N);
+            ALLOC(c->ND[q], N);
+        }
+        ALLOC(c->unk, N); ALLOC(c->pmin, N); ALLOC(c->cell, N); ALLOC(c->tmp_key, N);
+        ALLOC(c->sflag, N + 1); ALLOC(c->sscan, N + 1); ALLOC(c->scan_blk, 8200);
+        ALLOC(c->nbr, NS * (size_t)stride); ALLOC(c->ncnt, N);
+        ALLOC(c->diag, N); ALLOC(c->b, N); ALLOC(c->Aval, NS * (size_t)stride);
+        ALLOC(c->Acol, NS * (size_t)stride); ALLOC(c->Alen, N);
+        ALLOC(c->x, N); ALLOC(c->r, N); ALLOC(c->pv, N); ALLOC(c->Ap, N);
+        ALLOC(c->partA, 65536); ALLOC(c->partB, 65536);
+        ALLOC(c->cnt_orig, N + 1); ALLOC(c->off_orig, N + 1);
+        ALLOC(c->counters, 1);
+        HIP_OK(hipHostMalloc(&c->hcounters, sizeof(DevCounters), hipHostMallocDefault));
+        c->stage_bytes = sizeof(Particle) * N;
+        HIP_OK(hipMalloc(&c->stage, c->stage_bytes));
+        c->cap = n;
+        c->stride = stride;
+        // zero the matrix so unused entries of the last slice are benign
+        HIP_OK(hipMemsetAsync(c->Alen, 0, sizeof(int) * N, c->stream));
+        HIP_OK(hipMemsetAsync(c->ncnt, 0, sizeof(int) * N, c->stream));
+    }
+    if (grid_cells + 4 > c->grid_cap) {
+        DFREE(c->ccount); DFREE(c->cstart); DFREE(c->cfill);
+        ALLOC(c->ccount, (size_t)grid_cells + 4);
+        ALLOC(c->cstart, (size_t)grid_cells + 4);
+        ALLOC(c->cfill, (size_t)grid_cells + 4);
+        c->grid_cap = grid_cells + 4;
+    }
+    c->n = n;
+    return 0;
+}
+
+static Ctx *find_ctx(ParticleSystem *ps)
+{
+    auto it = g_ctx.find(ps);
+    return it == g_ctx.end() ? nullptr : it->second;
+}
+
+static Ctx *get_ctx(ParticleSystem *ps)
+{
+    Ctx *c = find_ctx(ps);
+    if (c) return c;
+    if (ensure_device() != MPS_OK) return nullptr;
+    c = new Ctx();
+    c->ps = ps;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        mps_set_error(MPS_ERR_HIP, __FILE__, __LINE__, "stream creation failed");
+        delete c;
+        return nullptr;
+    }
+    g_ctx[ps] = c;
+    return c;
+}
+
+// ---------------------------------------------------------------------
+// constants and geometry
+// ---------------------------------------------------------------------
+
+static double search_radius(const SimConfig *cf)
+{
+    // simulation.c:22-24
+    double re = fmax(cf->influence_radius_lap, cf->influence_radius_n);
+    if (cf->surface_tension_enabled) re = fmax(re, cf->influence_radius_st);
+    return re;
+}
+
+// cell_list_create (neighbor_search.c:92-122)
+static Grid grid_for(double re, const double *dmin, const double *dmax)
+{
+    Grid g{};
+    const double margin = 4.0 * re;
+    g.total = 1;
+    for (int d = 0; d < 3; d++) {
+        if (d < D_) {
+            g.origin[d] = dmin[d] - margin;
+            const double span = (dmax[d] + margin) - g.origin[d];
+            g.nc[d] = (int)(span / re) + 2;
+        } else {
+            g.origin[d] = 0.0;
+            g.nc[d] = 1;
+        }
+        g.total *= g.nc[d];
+    }
+    g.inv = 1.0 / re;
+    return g;
+}
+
+static Grid grid_from_cells(const CellList *cl)
+{
+    Grid g{};
+    g.nc[0] = cl->nx;
+    g.nc[1] = cl->ny;
+#if MPS_DIM == 3
+    g.nc[2] = cl->nz;
+#else
+    g.nc[2] = 1;
+#endif
+    g.total = g.nc[0] * g.nc[1] * g.nc[2];
+    for (int d = 0; d < 3; d++) g.origin[d] = d < D_ ? cl->origin[d] : 0.0;
+    g.inv = 1.0 / cl->cell_size;
+    return g;
+}
+
+static StepConsts make_consts(const SimConfig *cf, const ParticleSystem *ps, const Grid &g, double re)
+{
+    StepConsts k{};
+    k.grid = g;
+    k.l0 = cf->particle_distance;
+    k.r_min = 0.01 * cf->particle_distance;
+    k.re_lap = cf->influence_radius_lap;
+    k.re_n = cf->influence_radius_n;
+    k.re2 = re * re;
+    k.re_n2 = cf->influence_radius_n * cf->influence_radius_n;
+    k.re_n2_hi = k.re_n2 * (1.0 + 1.0e-14);
+    k.n0 = ps->n0;
+    k.coeff = 2.0 * D_ / (ps->n0 * ps->lambda);
+    k.neg_coeff = -k.coeff;
+    k.visc_coeff = cf->viscosity * k.coeff;
+    k.diag_coeff = (cf->ppe_type == 1) ? cf->c_ppe * k.coeff : k.coeff;
+    for (int d = 0; d < 3; d++) k.g[d] = d < D_ ? cf->gravity[d] : 0.0;
+    k.dt = cf->dt;
+    const double cd = cf->collision_distance_ratio * cf->particle_distance;
+    k.cd2 = cd * cd;
+    k.col_tiny = 1.0e-12 * cf->particle_distance;
+    k.e_half = (1.0 + cf->restitution_coeff) * 0.5;
+    k.thr_n0 = cf->surface_threshold * ps->n0;
+    k.beta_n0c = cf->surface_count_threshold * ps->n0_count;
+    const double dt2 = cf->dt * cf->dt;
+    k.src_k = (cf->ppe_type == 1) ? cf->gamma_ppe * (cf->density / dt2)
+                                  : cf->relaxation_coeff * (cf->density / dt2);
+    k.neg_rho_dt = -cf->density / cf->dt;
+    k.hs_k = -(cf->density / ps->n0 / cf->dt);
+    k.dim_n0 = (double)D_ / ps->n0;
+    k.grad_min_r2 = cf->particle_distance * cf->particle_distance * 1.0e-12;
+    k.neg_gc = -((double)D_ / ps->n0);
+    k.rho = cf->density;
+    const double margin = cf->particle_distance * (cf->wall_layers + 1);
+    for (int d = 0; d < 3; d++) {
+        k.bnd_lo[d] = d < D_ ? cf->domain_min[d] - margin : -INFINITY;
+        k.bnd_hi[d] = d < D_ ? cf->domain_max[d] + margin : INFINITY;
+    }
+    k.stride = 0;
+    k.surf_method = cf->surface_detection_method;
+    k.ppe_type = cf->ppe_type;
+    k.hs_mode = cf->hs_mode;
+    k.cmps = cf->cmps_gradient;
+    k.clamp = cf->clamp_negative_pressure;
+    k.collide = (D_ == 2) ? 1 : (cf->collision_distance_ratio > 0.0);
+    k.st_enabled = cf->surface_tension_enabled;
+    k.re_st = cf->influence_radius_st;
+    k.st_ll = ps->C_LL / (cf->density * cf->particle_distance * cf->particle_distance *
+                          cf->particle_distance);
+    k.st_sl = ps->C_SL / (cf->density * cf->particle_distance * cf->particle_distance *
+                          cf->particle_distance);
+    k.st_sl_on = ps->C_SL != 0.0;
+    k.st_tiny = 1.0e-9 * cf->particle_distance;
+    return k;
+}
+
+// ---------------------------------------------------------------------
+// host <-> device
+// ---------------------------------------------------------------------
+
+static int stride_for(const SimConfig *cf, const NeighborList *nl)
+{
+    int s = cf ? cf->max_neighbors : 512;
+    if (nl && nl->max_neighbors > s) s = nl->max_neighbors;
+    if (s < 1) s = 1;
+    return s;
+}
+
+static int upload(Ctx *c, ParticleSystem *ps, const NeighborList *nl, const Grid &g)
+{
+    int rc = ctx_alloc(c, ps->num, stride_for(g_config, nl), g.total);
+    if (rc) return rc;
+    c->grid = g;
+    if (ps->num > 0) {
+        HIP_OK(hipMemcpyAsync(c->stage, ps->particles, sizeof(Particle) * (size_t)ps->num,
+                              hipMemcpyHostToDevice, c->stream));
+        rc = launch_unpack(c, c->stage, 0);
+        if (rc) return rc;
+    } else {
+        c->cx = c->ct = c->cp = 0;
+    }
+    c->list_valid = false;
+    if (nl && nl->count && nl->neighbors && ps->num > 0) {
+        // compact the caller's padded list to CSR and expand it on the device
+        std::vector<int> cnt(ps->num), off(ps->num + 1);
+        long long tot = 0;
+        for (int i = 0; i < ps->num; i++) {
+            int m = nl->count[i];
+            if (m < 0) m = 0;
+            if (m > nl->max_neighbors) m = nl->max_neighbors;
+            cnt[i] = m;
+            off[i] = (int)tot;
+            tot += m;
+        }
+        off[ps->num] = (int)tot;
+        std::vector<int> flat((size_t)(tot > 0 ? tot : 1));
+        for (int i = 0; i < ps->num; i++)
+            memcpy(&flat[off[i]], nl->neighbors + (size_t)i * nl->max_neighbors, sizeof(int) * cnt[i]);
+        if ((size_t)tot > c->nl_stage_elems) {
+            DFREE(c->nl_stage);
+            HIP_OK(hipMalloc(&c->nl_stage, sizeof(int) * (size_t)(tot > 0 ? tot : 1)));
+            c->nl_stage_elems = (size_t)(tot > 0 ? tot : 1);
+        }
+        HIP_OK(hipMemcpyAsync(c->nl_stage, flat.data(), sizeof(int) * (size_t)tot, hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipMemcpyAsync(c->cnt_orig, cnt.data(), sizeof(int) * ps->num, hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipMemcpyAsync(c->off_orig, off.data(), sizeof(int) * (ps->num + 1), hipMemcpyHostToDevice, c->stream));
+        rc = launch_list_import(c, c->cnt_orig, c->off_orig);
+        if (rc) return rc;
+        HIP_OK(hipStreamSynchronize(c->stream));   // host vectors go out of scope
+        c->list_valid = true;
+    }
+    c->has_state = true;
+    return 0;
+}
+
+static int download(Ctx *c, ParticleSystem *ps, NeighborList *nl)
+{
+    if (!c->has_state || ps->num != c->n) {
+        mps_set_error(MPS_ERR_ARG, __FILE__, __LINE__, "no device state matching this ParticleSystem");
+        return MPS_ERR_ARG;
+    }
+    if (ps->num > 0) {
+        int rc = launch_pack(c, c->stage);
+        if (rc) return rc;
+        HIP_OK(hipMemcpyAsync(ps->particles, c->stage, sizeof(Particle) * (size_t)ps->num,
+                              hipMemcpyDeviceToHost, c->stream));
+    }
+    if (nl && nl->count && nl->neighbors && c->list_valid && ps->num > 0) {
+        long long tot = 0;
+        int rc = launch_list_export(c, &tot);
+        if (rc) return rc;
+        std::vector<int> flat((size_t)(tot > 0 ? tot : 1)), cnt(ps->num);
+        HIP_OK(hipMemcpyAsync(flat.data(), c->nl_stage, sizeof(int) * (size_t)tot, hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipMemcpyAsync(cnt.data(), c->cnt_orig, sizeof(int) * ps->num, hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        size_t at = 0;
+        for (int i = 0; i < ps->num; i++) {
+            int m = cnt[i] < nl->max_neighbors ? cnt[i] : nl->max_neighbors;
+            memcpy(nl->neighbors + (size_t)i * nl->max_neighbors, &flat[at], sizeof(int) * m);
+            nl->count[i] = m;
+            at += cnt[i];
+        }
+    }
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------
+// the timestep
+// ---------------------------------------------------------------------
+
+static int reset_counters(Ctx *c)
+{
+    DevCounters z{};
+    z.overflow_orig = INT_MAX;
+    *c->hcounters = z;
+    HIP_OK(hipMemcpyAsync(c->counters, c->hcounters, sizeof(DevCounters), hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+
+// Neighbour-overflow handling: the reference prints and exits
+// (neighbor_search.c:221-227).
+static int check_overflow(Ctx *c, int stride)
+{
+    if (c->hcounters->overflow_orig == INT_MAX) return 0;
+    if (g_opt.exit_on_error > 0.5) {
+        fprintf(stderr, "Error: neighbor count exceeded max_neighbors (%d) for particle %d. "
+                        "Increase max_neighbors in params.txt.\n",
+                stride, c->hcounters->overflow_orig);
+        exit(EXIT_FAILURE);
+    }
+    char msg[128];
+    snprintf(msg, sizeof msg, "neighbour list overflow at particle %d", c->hcounters->overflow_orig);
+    mps_set_error(MPS_ERR_NEIGHBOR_OVERFLOW, __FILE__, __LINE__, msg);
+    return MPS_ERR_NEIGHBOR_OVERFLOW;
+}
+
+static int fetch_counters(Ctx *c)
+{
+    HIP_OK(hipMemcpyAsync(c->hcounters, c->counters, sizeof(DevCounters), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// Sorts by cell and builds the list (the initial search of simulation_run,
+// simulation.c:153, and the neighbor_search_cell_linked_list drop-in).
+static int search_only(Ctx *c, const StepConsts &k)
+{
+    int rc = launch_bin(c, k);
+    if (!rc) rc = launch_sort(c, k, 1);
+    if (!rc) rc = reset_counters(c);
+    if (!rc) rc = launch_search(c, k, 0, c->counters);
+    if (!rc) rc = fetch_counters(c);
+    if (rc) return rc;
+    c->list_valid = true;
+    c->stats.n_pairs = (long long)c->hcounters->n_pairs;
+    c->stats.max_count = c->hcounters->max_count;
+    return check_overflow(c, k.stride);
+}
+
+int collide_2d(Ctx *c, const StepConsts &k);   // k_particles.hip
+
+static int run_step(Ctx *c, const SimConfig *cf)
+{
+    StepConsts k = make_consts(cf, c->ps, c->grid, search_radius(cf));
+    k.stride = c->stride;
+    int rc = launch_visc_predict(c, k);
+    if (rc) return rc;
+#if MPS_DIM == 2
+    rc = collide_2d(c, k);
+    if (!rc) rc = launch_collide_bin(c, k, 0);
+#else
+    rc = launch_collide_bin(c, k, k.collide);
+#endif
+    if (!rc) rc = launch_sort(c, k, 0);
+    if (!rc) rc = reset_counters(c);
+    if (!rc) rc = launch_search(c, k, 1, c->counters);
+    if (!rc) rc = launch_assemble(c, k, c->counters);
+    if (rc) return rc;
+    c->list_valid = true;
+    SolveOpts so;
+    so.solver = cf->solver_type;
+    so.max_iter = cf->cg_max_iter;
+    so.tol = cf->cg_tolerance;
+    so.tol_mode = (int)g_opt.tol_mode;
+    so.precision = (int)g_opt.precision;
+    so.inner_tol = g_opt.inner_tol;
+    so.check_every = (int)g_opt.check_every;
+    so.dense_ic = (D_ == 2);
+    SolveResult sr{};
+    rc = solve_ppe(c, so, &sr);
+    if (rc) return rc;
+    rc = launch_pressure_out(c, k, 0);
+    if (!rc) rc = launch_grad_correct(c, k);
+    if (!rc) rc = fetch_counters(c);
+    if (rc) return rc;
+    c->ppe_valid = true;
+    c->stats.steps++;
+    c->stats.last_iters = sr.iters;
+    c->stats.total_iters += sr.iters;
+    c->stats.last_outer = sr.outer;
+    c->stats.last_residual = sr.res;
+    c->stats.last_bnorm = sr.bnorm;
+    c->stats.n_eq = c->hcounters->n_eq;
+    c->stats.nnz = (long long)c->hcounters->nnz;
+    c->stats.n_pairs = (long long)c->hcounters->n_pairs;
+    c->stats.max_count = c->hcounters->max_count;
+    if (g_opt.profile > 0.5) prof_collect();
+    return check_overflow(c, k.stride);
+}
+
+// ---------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------
+
+extern "C" {
+
+int mps_hip_init(int device)
+{
+    (void)device;
+    return ensure_device();
+}
+
+void mps_hip_finalize(void)
+{
+    for (auto &kv : g_ctx) {
+        ctx_free(kv.second);
+        if (kv.second->stream) hipStreamDestroy(kv.second->stream);
+        delete kv.second;
+    }
+    g_ctx.clear();
+}
+
+int mps_hip_last_error(void) { return g_last_error; }
+
+const char *mps_hip_error_string(int code)
+{
+    switch (code) {
+    case MPS_OK: return "ok";
+    case MPS_ERR_HIP: return g_last_msg[0] ? g_last_msg : "HIP runtime error";
+    case MPS_ERR_NO_DEVICE: return "no gfx950 device";
+    case MPS_ERR_ALLOC: return "device allocation failed";
+    case MPS_ERR_NEIGHBOR_OVERFLOW: return "neighbour list overflow";
+    case MPS_ERR_ARG: return "invalid argument";
+    case MPS_ERR_NO_CONFIG: return "g_config is NULL";
+    case MPS_ERR_UNSUPPORTED: return g_last_msg[0] ? g_last_msg : "unsupported";
+    case MPS_ERR_SOLVER: return "solver failure";
+    default: return "unknown error";
+    }
+}
+
+const char *mps_hip_device_arch(void)
+{
+    if (ensure_device() != MPS_OK) return "";
+    return g_arch;
+}
+
+int mps_hip_set_option(const char *key, double value)
+{
+    if (!key) return MPS_ERR_ARG;
+    std::string k(key);
+    if (k == "tol_mode") g_opt.tol_mode = value;
+    else if (k == "precision") g_opt.precision = value;
+    else if (k == "inner_tol") g_opt.inner_tol = value;
+    else if (k == "exit_on_error") g_opt.exit_on_error = value;
+    else if (k == "check_every") g_opt.check_every = value;
+    else if (k == "spmv") g_opt.spmv = value;
+    else if (k == "profile") g_opt.profile = value;
+    else if (k == "verbose") g_opt.verbose = value;
+    else return MPS_ERR_ARG;
+    return MPS_OK;
+}
+
+double mps_hip_get_option(const char *key)
+{
+    if (!key) return NAN;
+    std::string k(key);
+    if (k == "tol_mode") return g_opt.tol_mode;
+    if (k == "precision") return g_opt.precision;
+    if (k == "inner_tol") return g_opt.inner_tol;
+    if (k == "exit_on_error") return g_opt.exit_on_error;
+    if (k == "check_every") return g_opt.check_every;
+    if (k == "spmv") return g_opt.spmv;
+    if (k == "profile") return g_opt.profile;
+    if (k == "verbose") return g_opt.verbose;
+    return NAN;
+}
+
+int mps_hip_upload(ParticleSystem *ps, const NeighborList *nl)
+{
+    if (!ps) return MPS_ERR_ARG;
+    if (!g_config) return g_last_error = MPS_ERR_NO_CONFIG;
+    Ctx *c = get_ctx(ps);
+    if (!c) return g_last_error;
+    const double re = search_radius(g_config);
+    return upload(c, ps, nl, grid_for(re, g_config->domain_min, g_config->domain_max));
+}
+
+int mps_hip_download(ParticleSystem *ps, NeighborList *nl)
+{
+    Ctx *c = ps ? find_ctx(ps) : nullptr;
+    if (!c) return MPS_ERR_ARG;
+    return download(c, ps, nl);
+}
+
+int mps_hip_set_resident(ParticleSystem *ps, int resident)
+{
+    Ctx *c = ps ? get_ctx(ps) : nullptr;
+    if (!c) return MPS_ERR_ARG;
+    c->resident = resident != 0;
+    return MPS_OK;
+}
+
+void mps_hip_release(ParticleSystem *ps)
+{
+    auto it = g_ctx.find(ps);
+    if (it == g_ctx.end()) return;
+    ctx_free(it->second);
+    if (it->second->stream) hipStreamDestroy(it->second->stream);
+    delete it->second;
+    g_ctx.erase(it);
+}
+
+// Resident multi-step path: upload once (with the initial search when no
+// list is held), then steps with no host copies.
+int mps_hip_run_steps(ParticleSystem *ps, int nsteps)
+{
+    if (!ps) return MPS_ERR_ARG;
+    if (!g_config) return g_last_error = MPS_ERR_NO_CONFIG;
+    Ctx *c = get_ctx(ps);
+    if (!c) return g_last_error;
+    int rc;
+    const double re = search_radius(g_config);
+    if (!c->has_state || c->n != ps->num) {
+        rc = upload(c, ps, nullptr, grid_for(re, g_config->domain_min, g_config->domain_max));
+        if (rc) return rc;
+    }
+    c->resident = true;
+    if (!c->list_valid) {
+        StepConsts k = make_consts(g_config, ps, c->grid, re);
+        k.stride = c->stride;
+        rc = search_only(c, k);
+        if (rc) return rc;
+    }
+    for (int s = 0; s < nsteps; s++) {
+        rc = run_step(c, g_config);
+        if (rc) return rc;
+    }
+    return MPS_OK;
+}
+
+int mps_hip_get_stats(ParticleSystem *ps, mps_step_stats *out)
+{
+    Ctx *c = ps ? find_ctx(ps) : nullptr;
+    if (!c || !out) return MPS_ERR_ARG;
+    *out = c->stats;
+    return MPS_OK;
+}
+
+int mps_hip_profile_get(const char *name, double *total_ms, long long *count)
+{
+    prof_collect();
+    for (int k = 0; k < PK_COUNT; k++)
+        if (name && strcmp(name, g_prof[k].name) == 0) {
+            if (total_ms) *total_ms = g_prof[k].total_ms;
+            if (count) *count = g_prof[k].count;
+            return 0;
+        }
+    return -1;
+}
+
+void mps_hip_profile_reset(void)
+{
+    prof_collect();
+    for (int k = 0; k < PK_COUNT; k++) {
+        g_prof[k].total_ms = 0;
+        g_prof[k].count = 0;
+    }
+}
+
+// --- reference drop-ins: driver ----------------------------------------
+
+// simulation_step (simulation.c:19-118).  The CellList supplies the grid
+// geometry; nl is the stale list this step's viscosity and collision read
+// and receives the new list.
+void simulation_step(ParticleSystem *ps, NeighborList *nl, CellList *cl, int step)
+{
+    (void)step;
+    if (!g_config) { g_last_error = MPS_ERR_NO_CONFIG; return; }
+    Ctx *c = get_ctx(ps);
+    if (!c) return;
+    const double re = search_radius(g_config);
+    Grid g = cl ? grid_from_cells(cl) : grid_for(re, g_config->domain_min, g_config->domain_max);
+    int rc = 0;
+    if (!c->resident || !c->has_state || c->n != ps->num) {
+        rc = upload(c, ps, nl, g);
+        if (rc) return;
+    }
+    c->grid = g;
+    rc = run_step(c, g_config);
+    if (rc) return;
+    if (!c->resident) download(c, ps, nl);
+}
+
+// simulation_run (simulation.c:123-181): resident state, one download per
+// output step.
+void simulation_run(ParticleSystem *ps)
+{
+    if (!g_config) { g_last_error = MPS_ERR_NO_CONFIG; return; }
+    const double re = search_radius(g_config);
+    const double dt = g_config->dt;
+    const int total_steps = (int)(g_config->t_end / dt);
+    const int out_interval = g_config->output_interval;
+    const char *out_dir = g_config->output_dir;
+    printf("HIP device: %s\n", mps_hip_device_arch());
+    NeighborList *nl = neighbor_list_create(ps->num, g_config->max_neighbors);
+    CellList *cl = cell_list_create(ps->num, re, g_config->domain_min, g_config->domain_max);
+    if (!nl || !cl) {
+        fprintf(stderr, "Error: failed to create neighbor/cell list\n");
+        neighbor_list_free(nl);
+        cell_list_free(cl);
+        return;
+    }
+    Ctx *c = get_ctx(ps);
+    if (!c) { fprintf(stderr, "Error: %s\n", mps_hip_error_string(g_last_error)); return; }
+    Grid g = grid_from_cells(cl);
+    if (upload(c, ps, nullptr, g)) { fprintf(stderr, "Error: upload failed: %s\n", g_last_msg); return; }
+    c->resident = true;
+    StepConsts k = make_consts(g_config, ps, g, re);
+    k.stride = c->stride;
+    if (search_only(c, k)) { fprintf(stderr, "Error: %s\n", g_last_msg); return; }
+    output_csv(ps, 0, out_dir);
+    output_vtk(ps, 0, out_dir);
+    printf("Starting simulation (%dD, HIP %s): %d steps, dt = %.2e\n", D_, g_arch, total_steps, dt);
+    for (int step = 1; step <= total_steps; step++) {
+        if (run_step(c, g_config)) {
+            fprintf(stderr, "Error at step %d: %s\n", step, g_last_msg);
+            break;
+        }
+        if (out_interval > 0 && step % out_interval == 0) {
+            download(c, ps, nullptr);
+            int fluid = 0;
+            for (int i = 0; i < ps->num; i++) fluid += ps->particles[i].type == FLUID_PARTICLE;
+            printf("Step %6d / %d  (t = %.4f s)  fluid particles: %d\n", step, total_steps,
+                   step * dt, fluid);
+            output_csv(ps, step, out_dir);
+            output_vtk(ps, step, out_dir);
+        }
+    }
+    download(c, ps, nl);
+    c->resident = false;
+    printf("Simulation complete.\n");
+    cell_list_free(cl);
+    neighbor_list_free(nl);
+}
+
+}  // extern "C"
+
+// accessors for the other translation units
+Ctx *mps_ctx_for(ParticleSystem *ps, int create) { return create ? get_ctx(ps) : find_ctx(ps); }
+int mps_ctx_upload(Ctx *c, ParticleSystem *ps, const NeighborList *nl, const Grid &g) { return upload(c, ps, nl, g); }
+int mps_ctx_download(Ctx *c, ParticleSystem *ps, NeighborList *nl) { return download(c, ps, nl); }
+int mps_ctx_search(Ctx *c, const StepConsts &k) { return search_only(c, k); }
+StepConsts mps_consts(const SimConfig *cf, const ParticleSystem *ps, const Grid &g, double re)
+{
+    return make_consts(cf, ps, g, re);
+}
+Grid mps_grid_for(double re, const double *dmin, const double *dmax) { return grid_for(re, dmin, dmax); }
+Grid mps_grid_from_cells(const CellList *cl) { return grid_from_cells(cl); }
+double mps_search_radius(const SimConfig *cf) { return search_radius(cf); }

@@ -1,0 +1,100 @@
+"""Runs the same scene through the GPU library and the CPU oracle.
+
+Both see identical inputs (positions from the reference generator's lattice,
+the same SimConfig field values, the same n0/lambda) and the same solver
+stopping rule, so their outputs can be compared field by field.
+"""
+import ctypes as C
+
+import numpy as np
+
+import scenes
+from mps_amd import abi as A
+from oracle.oracle import Oracle, make_config
+
+FIELDS = ("pos", "vel", "acc", "pressure", "n", "type", "on_surface")
+
+
+def build(name, **over):
+    sc = scenes.CONFIGS[name]
+    dim = sc["dim"]
+    p = scenes.params_for(name, **over)
+    cfg = make_config(A.types(dim), **p)
+    pos, vel, typ = scenes.dambreak(**sc)
+    return dim, cfg, pos, vel, typ
+
+
+class GpuRun:
+    """Drives libmps_hip through its reference entry points (sync mode)."""
+
+    def __init__(self, lib, cfg, pos, vel, typ, n0=None, lam=None):
+        self.lib = lib
+        self.cfg = cfg
+        lib.set_config(cfg)
+        self.scene = A.HostScene(lib.abi, pos, vel, typ)
+        if n0 is None:
+            lib.particle_system_calc_initial_params(self.scene.ptr())
+        else:
+            self.scene.ps.n0 = n0
+            self.scene.ps.lambda_ = lam
+        n = self.scene.num
+        dim = lib.dim
+        re = max(cfg.influence_radius_lap, cfg.influence_radius_n)
+        dmin = (C.c_double * dim)(*[cfg.domain_min[d] for d in range(dim)])
+        dmax = (C.c_double * dim)(*[cfg.domain_max[d] for d in range(dim)])
+        self.nl = lib.neighbor_list_create(n, cfg.max_neighbors)
+        self.cl = lib.cell_list_create(n, re, dmin, dmax)
+        lib.neighbor_search_cell_linked_list(self.nl, self.scene.ptr(), self.cl, re)
+        self.step_no = 0
+
+    def step(self):
+        self.lib.set_config(self.cfg)
+        self.step_no += 1
+        self.lib.simulation_step(self.scene.ptr(), self.nl, self.cl, self.step_no)
+        rc = self.lib.mps_hip_last_error()
+        return rc
+
+    def lists(self):
+        return A.padded_lists(self.nl.contents, self.scene.num)
+
+    def close(self):
+        self.lib.mps_hip_release(self.scene.ptr())
+        self.lib.neighbor_list_free(self.nl)
+        self.lib.cell_list_free(self.cl)
+
+
+class OracleRun:
+    def __init__(self, cfg, pos, vel, typ, dim, tol_mode=0, dot_order=0, n0=None, lam=None):
+        self.orc = Oracle(dim)
+        self.cfg = cfg
+        self.scene = A.HostScene(A.types(dim), pos, vel, typ)
+        if n0 is None:
+            self.orc.init_params(self.scene, cfg)
+        else:
+            self.scene.ps.n0 = n0
+            self.scene.ps.lambda_ = lam
+        self.sim = self.orc.sim(tol_mode, dot_order)
+        self.sim.begin(self.scene, cfg)
+
+    def step(self):
+        return self.sim.step(self.scene, self.cfg)
+
+    def lists(self):
+        st, ix = self.sim.neighbor_lists()
+        return [ix[st[i]:st[i + 1]] for i in range(len(st) - 1)]
+
+    def ppe(self):
+        return self.sim.ppe(self.scene.num)
+
+
+def rel_err(a, b):
+    a = np.asarray(a, dtype=float)
+    b = np.asarray(b, dtype=float)
+    den = max(np.linalg.norm(b), 1e-300)
+    return float(np.linalg.norm(a - b) / den)
+
+
+def lists_equal(la, lb):
+    if len(la) != len(lb):
+        return False
+    return all(np.array_equal(x, y) for x, y in zip(la, lb))

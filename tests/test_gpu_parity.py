@@ -1,0 +1,231 @@
+"""GPU parity: libmps_hip (through its C-ABI) against the CPU oracle on the
+same inputs.
+
+Bit-exact: neighbour lists (set and order), number density, free-surface
+flags, unknown numbering, PPE pattern, values and right-hand side, and the
+initial n0/lambda.  Tolerance: pressure, velocity and position after the
+CG solve, whose dot products sum in a different order than the reference's
+serial loop.  Each tolerance is stated against the spread two oracle runs
+show when only their dot-product order differs (forward vs backward serial
+sums), which is the disagreement the reference itself would show.
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import harness as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _p(a, t=C.c_double):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_fp64_sqrt_div_match_host(lib3):
+    """The kernels' FP64 sqrt and division round exactly like the host's."""
+    rng = np.random.default_rng(7)
+    n = 200000
+    a = 10.0 ** rng.uniform(-12, 6, n)
+    b = (10.0 ** rng.uniform(-12, 6, n)) * rng.choice([-1.0, 1.0], n)
+    s = np.empty(n)
+    q = np.empty(n)
+    say("fp64 probe n=%d" % n)
+    assert lib3.mps_hip_fp64_probe(n, _p(a), _p(b), _p(s), _p(q)) == 0
+    assert np.array_equal(s.view(np.uint64), np.sqrt(a).view(np.uint64))
+    assert np.array_equal(q.view(np.uint64), (a / b).view(np.uint64))
+
+
+@pytest.mark.parametrize("name", ["shipped3d", "dam3d_small"])
+def test_initial_params_bitexact(lib3, name):
+    dim, cfg, pos, vel, typ = H.build(name)
+    say("init params", name, len(typ))
+    g = H.GpuRun(lib3, cfg, pos, vel, typ)
+    o = H.OracleRun(cfg, pos, vel, typ, dim)
+    assert g.scene.ps.n0 == o.scene.ps.n0
+    assert g.scene.ps.lambda_ == o.scene.ps.lambda_
+    # initial search: identical lists, in order
+    assert H.lists_equal(g.lists(), o.lists())
+    g.close()
+
+
+def _step_compare(lib, name, steps, tol=1e-12, **over):
+    dim, cfg, pos, vel, typ = H.build(name, cg_tolerance=tol, **over)
+    lib.set_option("tol_mode", 1)
+    say("case", name, "N=%d" % len(typ), "steps", steps, over)
+    o = H.OracleRun(cfg, pos, vel, typ, dim, tol_mode=1)
+    ob = H.OracleRun(cfg, pos, vel, typ, dim, tol_mode=1, dot_order=1)
+    g = H.GpuRun(lib, cfg, pos, vel, typ, n0=o.scene.ps.n0, lam=o.scene.ps.lambda_)
+    out = []
+    for s in range(1, steps + 1):
+        assert g.step() == 0
+        o.step()
+        ob.step()
+        a, b, c = g.scene.arr, o.scene.arr, ob.scene.arr
+        rec = dict(step=s,
+                   p_gpu=H.rel_err(a["pressure"], b["pressure"]),
+                   p_spread=H.rel_err(c["pressure"], b["pressure"]),
+                   x_gpu=H.rel_err(a["pos"], b["pos"]),
+                   x_spread=H.rel_err(c["pos"], b["pos"]),
+                   v_gpu=H.rel_err(a["vel"], b["vel"]),
+                   v_spread=H.rel_err(c["vel"], b["vel"]),
+                   n_exact=bool(np.array_equal(a["n"].view(np.uint64), b["n"].view(np.uint64))),
+                   flags_exact=bool(np.array_equal(a["on_surface"], b["on_surface"])),
+                   type_exact=bool(np.array_equal(a["type"], b["type"])))
+        out.append(rec)
+        say(rec)
+    return g, o, out
+
+
+def test_one_step_3d_bitexact_parts(lib3):
+    g, o, recs = _step_compare(lib3, "shipped3d", 1)
+    r = recs[0]
+    assert r["n_exact"] and r["flags_exact"] and r["type_exact"]
+    assert H.lists_equal(g.lists(), o.lists())
+    # PPE in the reference's layout
+    q = lib3.export_ppe(g.scene)
+    oq = o.ppe()
+    assert q["n_eq"] == oq["n_eq"]
+    assert np.array_equal(q["eq_idx"], oq["eq_idx"])
+    assert np.array_equal(q["row_ptr"], oq["row_ptr"].astype(np.int32))
+    assert np.array_equal(q["col"], oq["col"])
+    assert np.array_equal(q["val"].view(np.uint64), oq["val"].view(np.uint64))
+    assert np.array_equal(q["b"].view(np.uint64), oq["b"].view(np.uint64))
+    # FP64 solve: within 100x the reference's own dot-order spread (or 1e-10)
+    assert r["p_gpu"] <= max(100 * r["p_spread"], 1e-10)
+    assert r["x_gpu"] <= max(100 * r["x_spread"], 1e-14)
+    assert r["v_gpu"] <= max(100 * r["v_spread"], 1e-10)
+    g.close()
+
+
+def test_steps_3d_small(lib3):
+    g, o, recs = _step_compare(lib3, "dam3d_small", 10)
+    for r in recs:
+        assert r["type_exact"]
+        assert r["p_gpu"] <= max(100 * r["p_spread"], 1e-9)
+        assert r["x_gpu"] <= max(100 * r["x_spread"], 1e-13)
+    g.close()
+
+
+def test_steps_2d(lib2):
+    g, o, recs = _step_compare(lib2, "dam2d_l0.01", 10)
+    assert recs[0]["n_exact"] and recs[0]["flags_exact"]
+    for r in recs:
+        assert r["type_exact"]
+        assert r["p_gpu"] <= max(100 * r["p_spread"], 1e-9)
+        assert r["x_gpu"] <= max(100 * r["x_spread"], 1e-13)
+    g.close()
+
+
+def test_hs_source_step(lib3):
+    """High-order source: bit-exact right-hand side at step 1 (its inputs,
+    the predicted velocities, are bit-exact then; from step 2 on they carry
+    the CG's rounding)."""
+    g, o, recs = _step_compare(lib3, "shipped3d", 1, hs_mode=1, influence_ratio_lap=2.1)
+    assert recs[0]["n_exact"] and recs[0]["flags_exact"]
+    q = lib3.export_ppe(g.scene)
+    oq = o.ppe()
+    assert np.array_equal(q["col"], oq["col"])
+    assert np.array_equal(q["b"].view(np.uint64), oq["b"].view(np.uint64))
+    assert recs[0]["p_gpu"] <= max(100 * recs[0]["p_spread"], 1e-10)
+    g.close()
+
+
+def test_resident_equals_sync(lib3):
+    """mps_hip_run_steps (state resident in HBM) gives the same bits as the
+    reference-style per-step calls that copy state in and out."""
+    dim, cfg, pos, vel, typ = H.build("dam3d_small", cg_tolerance=1e-10)
+    lib3.set_option("tol_mode", 1)
+    g = H.GpuRun(lib3, cfg, pos, vel, typ)
+    for _ in range(4):
+        assert g.step() == 0
+    from mps_amd import abi as A
+    lib3.set_config(cfg)
+    s2 = A.HostScene(lib3.abi, pos, vel, typ)
+    s2.ps.n0 = g.scene.ps.n0
+    s2.ps.lambda_ = g.scene.ps.lambda_
+    say("resident run 4 steps")
+    lib3.check(lib3.mps_hip_run_steps(s2.ptr(), 4), "run_steps")
+    lib3.check(lib3.mps_hip_download(s2.ptr(), None), "download")
+    for f in H.FIELDS:
+        assert np.ascontiguousarray(g.scene.arr[f]).tobytes() == \
+            np.ascontiguousarray(s2.arr[f]).tobytes(), f
+    lib3.mps_hip_release(s2.ptr())
+    g.close()
+
+
+def _export_ic(lib, scene):
+    st = lib.stats(scene)
+    ne = st.n_eq
+    lnnz = (st.nnz - ne) // 2
+    rp = np.zeros(ne + 1, np.int32)
+    col = np.zeros(max(lnnz, 1), np.int32)
+    val = np.zeros(max(lnnz, 1))
+    dinv = np.zeros(max(ne, 1))
+    lib.check(lib.mps_hip_export_ic(scene.ptr(), _p(rp, C.c_int), _p(col, C.c_int), _p(val), _p(dinv)),
+              "export_ic")
+    return rp, col[:lnnz], val[:lnnz], dinv[:ne]
+
+
+@pytest.mark.parametrize("dim,name", [(3, "shipped3d"), (2, "dam2d_l0.01")])
+def test_iccg_exact_factor(lib3, lib2, dim, name):
+    """ICCG: the IC(0) factor is the reference's, bit for bit (3-D sparse
+    formulas; 2-D dense formulas), and the solve agrees to the dot-order
+    spread."""
+    lib = lib3 if dim == 3 else lib2
+    g, o, recs = _step_compare(lib, name, 1, solver_type=1)
+    r = recs[0]
+    assert r["n_exact"] and r["flags_exact"]
+    rp, col, val, dinv = _export_ic(lib, g.scene)
+    oq = o.ppe()
+    assert np.array_equal(rp, oq["l_ptr"].astype(np.int32))
+    assert np.array_equal(col, oq["l_col"])
+    assert np.array_equal(val.view(np.uint64), oq["l_val"].view(np.uint64))
+    assert np.array_equal(dinv.view(np.uint64), oq["l_diag"].view(np.uint64))
+    assert r["p_gpu"] <= max(100 * r["p_spread"], 1e-10)
+    g.close()
+
+
+def _laplacian(nx, ny, shift=0.05):
+    n = nx * ny
+    rows, cols, vals = [], [], []
+    for i in range(nx):
+        for j in range(ny):
+            k = i * ny + j
+            for di, dj in ((-1, 0), (0, -1), (0, 0), (0, 1), (1, 0)):
+                a, b = i + di, j + dj
+                if 0 <= a < nx and 0 <= b < ny:
+                    rows.append(k)
+                    cols.append(a * ny + b)
+                    vals.append(4.0 + shift if (di, dj) == (0, 0) else -1.0)
+    A = np.zeros((n, n))
+    A[rows, cols] = vals
+    rp = np.zeros(n + 1, np.int32)
+    np.add.at(rp, np.array(rows) + 1, 1)
+    rp = np.cumsum(rp).astype(np.int32)
+    order = np.lexsort((cols, rows))
+    return A, rp, np.array(cols, np.int32)[order], np.array(vals)[order]
+
+
+@pytest.mark.parametrize("solver", [0, 1])
+def test_solve_csr_vs_numpy(lib3, solver):
+    A, rp, col, val = _laplacian(24, 17)
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(A.shape[0])
+    x = np.zeros_like(b)
+    it = C.c_int()
+    lib3.set_option("tol_mode", 1)
+    rc = lib3.mps_hip_solve_csr(A.shape[0], _p(rp, C.c_int), _p(col, C.c_int), _p(val), _p(b), _p(x),
+                                solver, 2000, 1e-13, C.byref(it))
+    assert rc == 0
+    xs = np.linalg.solve(A, b)
+    say("solver", solver, "iters", it.value, "err", H.rel_err(x, xs))
+    assert H.rel_err(x, xs) < 1e-11
+    assert 0 < it.value < 2000
