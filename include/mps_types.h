@@ -120,19 +120,19 @@ typedef struct {
 
     double wetting_angle_SL;
 
-#ifdef MPS_FRICTION_FIELDS
-    int    friction_enabled;
-    double friction_delta_theta;
-    double dynamic_friction_alpha;
-    double friction_velocity_threshold;
-#endif
-
     double domain_min[MPS_DIM];
     double domain_max[MPS_DIM];
 
     int    cmps_gradient;          /* 0 P_j - P_min, 1 CMPS, 2 Oochi */
     int    use_analytical_lambda;
     int    hs_mode;                /* 0 standard source, 1 high-order source */
+
+#ifdef MPS_FRICTION_FIELDS
+    int    friction_enabled;
+    double friction_delta_theta;
+    double dynamic_friction_alpha;
+    double friction_velocity_threshold;
+#endif
 
     char   output_dir[256];
     char   particle_file[256];

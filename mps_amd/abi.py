@@ -3,7 +3,7 @@
 The layouts are those of the reference programs (mps_fluid3D_parallel/include/
 particle.h:7-28, neighbor_search.h:8-30, sim_config.h:10-80; 2-D:
 mps_fluid2D/include/*.h; with friction fields: mps_fluid3D/include/
-sim_config.h:72-77), so one Python-side ParticleSystem can be handed to this
+sim_config.h:72-77, after hs_mode), so one Python-side ParticleSystem can be handed to this
 library and to the reference compiled from its sources alike.
 """
 import ctypes as C
@@ -92,13 +92,14 @@ def _config_fields(dim, friction):
            ("collision_distance_ratio", C.c_double), ("surface_tension_enabled", C.c_int),
            ("surface_tension_coeff", C.c_double), ("surface_tension_re_ratio", C.c_double),
            ("influence_radius_st", C.c_double), ("wetting_angle_SL", C.c_double)]
+    cfg += [("domain_min", C.c_double * dim), ("domain_max", C.c_double * dim),
+            ("cmps_gradient", C.c_int), ("use_analytical_lambda", C.c_int),
+            ("hs_mode", C.c_int)]
     if friction:
         cfg += [("friction_enabled", C.c_int), ("friction_delta_theta", C.c_double),
                 ("dynamic_friction_alpha", C.c_double),
                 ("friction_velocity_threshold", C.c_double)]
-    cfg += [("domain_min", C.c_double * dim), ("domain_max", C.c_double * dim),
-            ("cmps_gradient", C.c_int), ("use_analytical_lambda", C.c_int),
-            ("hs_mode", C.c_int), ("output_dir", C.c_char * 256),
+    cfg += [("output_dir", C.c_char * 256),
             ("particle_file", C.c_char * 256), ("param_file", C.c_char * 256)]
     return cfg
 

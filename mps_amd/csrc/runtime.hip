@@ -338,10 +338,16 @@ static StepConsts make_consts(const SimConfig *cf, const ParticleSystem *ps, con
     k.collide = (D_ == 2) ? 1 : (cf->collision_distance_ratio > 0.0);
     k.st_enabled = cf->surface_tension_enabled;
     k.re_st = cf->influence_radius_st;
+#if MPS_DIM == 3
     k.st_ll = ps->C_LL / (cf->density * cf->particle_distance * cf->particle_distance *
                           cf->particle_distance);
     k.st_sl = ps->C_SL / (cf->density * cf->particle_distance * cf->particle_distance *
                           cf->particle_distance);
+#else
+    // the 2-D program divides by rho*l0^2 (mps_fluid2D/src/operators.c)
+    k.st_ll = ps->C_LL / (cf->density * cf->particle_distance * cf->particle_distance);
+    k.st_sl = ps->C_SL / (cf->density * cf->particle_distance * cf->particle_distance);
+#endif
     k.st_sl_on = ps->C_SL != 0.0;
     k.st_tiny = 1.0e-9 * cf->particle_distance;
     return k;

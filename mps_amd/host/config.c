@@ -2,7 +2,8 @@
  * config.c -- cal.txt / params.txt reader.
  *
  * Same keys, defaults, path resolution and derived radii as the reference
- * (mps_fluid3D_parallel/src/sim_config.c:20-79 defaults, :86-109 path
+ * (mps_fluid3D_parallel/src/sim_config.c:20-79 defaults -- the 2-D
+ * program's differ in max_neighbors and domain_max, :86-109 path
  * resolution, :120-159 cal.txt, :166-270 params.txt, :272-274 radii), so an
  * existing case directory runs unchanged.  The parser is table-driven: each
  * key maps to a typed field offset.  Keys added by this library (accepted
@@ -27,7 +28,11 @@ void config_set_defaults(SimConfig *c)
     c->influence_radius_lap = 2.1 * 0.025;
     c->influence_ratio_n = 2.1;
     c->influence_radius_n = 2.1 * 0.025;
+#if MPS_DIM == 3
     c->max_neighbors = 512;
+#else
+    c->max_neighbors = 256;     /* mps_fluid2D/src/sim_config.c defaults */
+#endif
     c->wall_layers = 2;
     c->dummy_layers = 2;
     c->density = 1000.0;
@@ -54,10 +59,18 @@ void config_set_defaults(SimConfig *c)
     c->surface_tension_re_ratio = 3.2;
     c->influence_radius_st = 3.2 * 0.025;
     c->wetting_angle_SL = 90.0;
+#if MPS_DIM == 3
     c->domain_max[0] = 0.4;
     c->domain_max[1] = 0.3;
-#if MPS_DIM == 3
     c->domain_max[2] = 0.2;
+#else
+    c->domain_max[0] = 1.0;
+    c->domain_max[1] = 0.6;
+#endif
+#ifdef MPS_FRICTION_FIELDS
+    c->friction_delta_theta = 30.0;   /* mps_fluid3D/src/sim_config.c:76-79 */
+    c->dynamic_friction_alpha = 600.0;
+    c->friction_velocity_threshold = 0.01;
 #endif
     c->cmps_gradient = 0;
     c->use_analytical_lambda = 0;

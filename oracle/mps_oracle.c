@@ -332,7 +332,13 @@ static void surface_tension(ParticleSystem *ps, const SimConfig *c, const orc_nl
                             double cst, int partner)
 {
     const double re_st = c->influence_radius_st, l0 = c->particle_distance;
+#if MPS_DIM == 3
     const double coeff = cst / (c->density * l0 * l0 * l0);
+#else
+    /* the 2-D program scales by rho*l0^2 (mps_fluid2D/src/operators.c:
+     * calc_surface_tension, calc_surface_tension_SL) */
+    const double coeff = cst / (c->density * l0 * l0);
+#endif
     for (int i = 0; i < ps->num; i++) {
         Particle *pi = &ps->particles[i];
         if (pi->type != FLUID_PARTICLE) continue;
