@@ -98,6 +98,23 @@ def cpu_baseline(w, tol, seconds_hint=30.0):
                        "mean %.0f iterations/step, %.1f s" % (steps, len(typ), iters / steps, el))
 
 
+def calibrate_copy(nbytes=1 << 30, reps=5):
+    """A plain copy of known size: each launch reads and writes exactly
+    nbytes.  The HBM byte counters of its dispatches give the factor that
+    turns the same counters on the solver's kernels into bytes."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps,
+                      "GBps_read_plus_write": round(2 * nbytes * reps / el / 1e9, 1)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,11 +122,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="dam3d_l0.005", choices=sorted(WORKLOADS))
     ap.add_argument("--solver", default="cg", choices=("cg", "iccg"))
-    ap.add_argument("--precision", default="fp64", choices=("fp64", "mixed"))
+    ap.add_argument("--precision", default="fp64", choices=("fp64", "mixed", "mixed32"),
+                    help="fp64; mixed = FP32 matrix, FP64 vectors; mixed32 = FP32 matrix and vectors")
     ap.add_argument("--tol", type=float, default=1e-9)
+    ap.add_argument("--spmv", type=int, default=0, help="SpMV kernel variant of the FP64 CG (0-4)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy 5 times (the reference "
+                         "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
     a = ap.parse_args()
+    if a.calibrate_copy:
+        calibrate_copy()
+        return
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU only (each box has one MI355X); "
               "--gpus %d is not supported" % a.gpus, file=sys.stderr)
@@ -121,8 +146,9 @@ def main():
     cfg = make_config(lib, w, 1 if a.solver == "iccg" else 0, a.tol)
     lib.set_config(cfg)
     lib.set_option("tol_mode", 1)
-    lib.set_option("precision", 1 if a.precision == "mixed" else 0)
+    lib.set_option("precision", {"fp64": 0, "mixed": 1, "mixed32": 2}[a.precision])
     lib.set_option("exit_on_error", 0)
+    lib.set_option("spmv", a.spmv)
     ps, n = build_scene(lib, w)
     print("bench: %s, %d particles, solver %s, %s" % (a.workload, n, a.solver, a.precision),
           file=sys.stderr, flush=True)
@@ -151,7 +177,7 @@ def main():
         ach = byt / (spmv_ms * 1e-3) / 1e9
         roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel="k_spmv_dot", bytes_per_launch=int(byt),
+                    kernel="k_spmv_dot" if a.precision == "fp64" else "k_mx_spmv", bytes_per_launch=int(byt),
                     mean_launch_ms=round(spmv_ms, 5), launches=cnt.value)
     kernels = {}
     for name in ("visc_predict", "collide_bin", "sort", "search", "assemble", "spmv", "axpy",

@@ -178,12 +178,18 @@ int  mps_hip_run_steps(ParticleSystem *ps, int nsteps);
 /* Options (process-wide): keys and values
  *   "tol_mode"       0 absolute sqrt(r.r) < tol (reference), 1 relative
  *                      sqrt(r.r) < tol*sqrt(b.b)
- *   "precision"      0 FP64 throughout, 1 mixed: FP32 inner CG inside FP64
- *                      iterative refinement
- *   "inner_tol"      relative tolerance of the FP32 inner solve (1e-5)
+ *   "precision"      0 FP64 throughout; mixed forms iterate on an FP32 copy
+ *                      of the PPE matrix with FP64 dot products and FP64
+ *                      solution, and make reliable updates (true residual
+ *                      b - A x recomputed with the FP64 matrix), testing
+ *                      convergence on that FP64 residual: 1 FP64 CG
+ *                      vectors, 2 FP32 CG vectors and FP32 SpMV sums
+ *   "inner_tol"      residual reduction between reliable updates (1e-3)
  *   "exit_on_error"  1 (reference behaviour) or 0
  *   "check_every"    CG iterations between host convergence polls (16)
- *   "spmv"           0 stored CSR values, 1 matrix-free (recomputes w)
+ *   "spmv"           SpMV kernel variant of the FP64 CG: 0 unroll 8,
+ *                      1 unroll 16, 2 unroll 8 + streaming matrix loads,
+ *                      3 unroll 16 + streaming, 4 unroll 4
  *   "profile"        1 times the main kernels with HIP events
  * Returns 0, or MPS_ERR_ARG for an unknown key. */
 int    mps_hip_set_option(const char *key, double value);
@@ -193,7 +199,7 @@ typedef struct {
     int    steps;           /* steps run on this context                  */
     int    last_iters;      /* CG/ICCG iterations of the last solve       */
     long long total_iters;  /* summed over all solves                     */
-    int    last_outer;      /* iterative-refinement sweeps (mixed)        */
+    int    last_outer;      /* reliable updates of the last solve (mixed) */
     int    n_eq;            /* unknowns of the last PPE                   */
     long long nnz;          /* nonzeros of the last PPE matrix (with diag)*/
     long long n_pairs;      /* directed neighbour pairs of the last search*/

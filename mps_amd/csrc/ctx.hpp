@@ -101,7 +101,8 @@ struct Ctx {
     int *Acol = nullptr, *Alen = nullptr;
     double *x = nullptr, *r = nullptr, *pv = nullptr, *Ap = nullptr;
     double *partA = nullptr, *partB = nullptr;
-    float *fAval = nullptr, *fdiag = nullptr, *fr = nullptr, *fp = nullptr, *fAp = nullptr, *fz = nullptr;
+    float *fAval = nullptr, *fr = nullptr, *fp = nullptr, *fAp = nullptr;
+    double *ddir = nullptr;          // mixed solve: Dirichlet part of each row
     CgCtrl *ctrl = nullptr;          // device
     CgCtrl *hctrl = nullptr;         // pinned host mirror
     DevCounters *counters = nullptr, *hcounters = nullptr;
@@ -146,10 +147,11 @@ struct SolveOpts {
     int max_iter;
     double tol;
     int tol_mode;      // 0 absolute, 1 relative
-    int precision;     // 0 FP64, 1 mixed
+    int precision;     // 0 FP64; 1 FP32 matrix, FP64 vectors; 2 FP32 matrix and vectors
     double inner_tol;
     int check_every;
     int dense_ic;      // 2-D dense-formula IC(0)
+    int spmv_variant;  // SpMV kernel variant (unroll / streaming loads)
 };
 struct SolveResult {
     int iters, outer, breakdown;

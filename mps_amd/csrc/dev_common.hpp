@@ -84,6 +84,23 @@ __device__ __forceinline__ int cell_coords(const Grid &g, const dvec &p, int c[3
 }
 
 // ---------------------------------------------------------------------
+// XCD-aware block order.  The dispatcher hands workgroups to the 8 XCDs
+// round-robin (block b runs on XCD b % 8) and each XCD has its own L2.
+// xcd_block() renumbers the blocks so that XCD k runs one contiguous range
+// of logical blocks: rows that are neighbours in the cell-sorted order, and
+// the vector entries they gather, then share one L2.  A bijection on
+// [0, nb) for any nb.
+// ---------------------------------------------------------------------
+#define N_XCD 8
+
+__device__ __forceinline__ int xcd_block(int b, int nb)
+{
+    const int q = nb / N_XCD, rem = nb % N_XCD;
+    const int x = b % N_XCD, k = b / N_XCD;
+    return x * q + (x < rem ? x : rem) + k;
+}
+
+// ---------------------------------------------------------------------
 // wavefront / block primitives (wave64)
 // ---------------------------------------------------------------------
 

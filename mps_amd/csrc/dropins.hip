@@ -353,7 +353,7 @@ void solve_pressure(ParticleSystem *ps, NeighborList *nl)
     z.overflow_orig = INT_MAX;
     if (hipMemcpy(c->counters, &z, sizeof z, hipMemcpyHostToDevice) != hipSuccess) return;
     if (extra_ppe_rows(c, cl.k) || launch_assemble(c, cl.k, c->counters)) return;
-    SolveOpts so;
+    SolveOpts so{};
     so.solver = g_config->solver_type;
     so.max_iter = g_config->cg_max_iter;
     so.tol = g_config->cg_tolerance;
@@ -362,6 +362,7 @@ void solve_pressure(ParticleSystem *ps, NeighborList *nl)
     so.inner_tol = mps_hip_get_option("inner_tol");
     so.check_every = (int)mps_hip_get_option("check_every");
     so.dense_ic = MPS_DIM == 2;
+    so.spmv_variant = (int)mps_hip_get_option("spmv");
     SolveResult sr{};
     if (solve_ppe(c, so, &sr)) return;
     StepConsts k = cl.k;

@@ -32,7 +32,7 @@ void mps_set_error(int code, const char *file, int line, const char *msg)
 }
 
 struct Options {
-    double tol_mode = 0, precision = 0, inner_tol = 1e-5, exit_on_error = 1, check_every = 16;
+    double tol_mode = 0, precision = 0, inner_tol = 1e-3, exit_on_error = 1, check_every = 16;
     double spmv = 0, profile = 0, verbose = 0;
 };
 static Options g_opt;
@@ -150,6 +150,7 @@ static int dalloc(T **p, size_t n)
     } while (0)
 
 void ic_free(IcData *d);   // k_ic.hip
+void mixed_free(MixedBufs *m);   // k_cg.hip
 
 void ctx_free(Ctx *c)
 {
@@ -161,13 +162,14 @@ void ctx_free(Ctx *c)
     DFREE(c->cfill); DFREE(c->tmp_key); DFREE(c->sflag); DFREE(c->sscan); DFREE(c->scan_blk);
     DFREE(c->nbr); DFREE(c->ncnt); DFREE(c->diag); DFREE(c->b); DFREE(c->Aval); DFREE(c->Acol);
     DFREE(c->Alen); DFREE(c->x); DFREE(c->r); DFREE(c->pv); DFREE(c->Ap); DFREE(c->partA);
-    DFREE(c->partB); DFREE(c->fAval); DFREE(c->fdiag); DFREE(c->fr); DFREE(c->fp); DFREE(c->fAp);
-    DFREE(c->fz); DFREE(c->stage); DFREE(c->nl_stage); DFREE(c->cnt_orig); DFREE(c->off_orig);
+    DFREE(c->partB); DFREE(c->fAval); DFREE(c->ddir); DFREE(c->fr); DFREE(c->fp); DFREE(c->fAp);
+    DFREE(c->stage); DFREE(c->nl_stage); DFREE(c->cnt_orig); DFREE(c->off_orig);
     DFREE(c->counters);
     if (c->ctrl) { hipFree(c->ctrl); c->ctrl = nullptr; }
     if (c->hctrl) { hipHostFree(c->hctrl); c->hctrl = nullptr; }
     if (c->hcounters) { hipHostFree(c->hcounters); c->hcounters = nullptr; }
     if (c->ic) { ic_free(c->ic); c->ic = nullptr; }
+    if (c->mixed) { mixed_free(c->mixed); c->mixed = nullptr; }
     c->n = c->cap = c->stride = c->grid_cap = 0;
     c->stage_bytes = c->nl_stage_elems = 0;
     c->has_state = c->list_valid = false;
@@ -517,7 +519,7 @@ static int run_step(Ctx *c, const SimConfig *cf)
     if (!rc) rc = launch_assemble(c, k, c->counters);
     if (rc) return rc;
     c->list_valid = true;
-    SolveOpts so;
+    SolveOpts so{};
     so.solver = cf->solver_type;
     so.max_iter = cf->cg_max_iter;
     so.tol = cf->cg_tolerance;
@@ -526,6 +528,7 @@ static int run_step(Ctx *c, const SimConfig *cf)
     so.inner_tol = g_opt.inner_tol;
     so.check_every = (int)g_opt.check_every;
     so.dense_ic = (D_ == 2);
+    so.spmv_variant = (int)g_opt.spmv;
     SolveResult sr{};
     rc = solve_ppe(c, so, &sr);
     if (rc) return rc;

@@ -229,3 +229,24 @@ def test_solve_csr_vs_numpy(lib3, solver):
     say("solver", solver, "iters", it.value, "err", H.rel_err(x, xs))
     assert H.rel_err(x, xs) < 1e-11
     assert 0 < it.value < 2000
+
+
+def test_spmv_variants_same_bits(lib3):
+    """Every SpMV variant (unroll depth, streaming loads, XCD block order)
+    sums each row in the same order and the dot-product partials in the
+    same block order, so the solves agree bit for bit."""
+    dim, cfg, pos, vel, typ = H.build("dam3d_small", cg_tolerance=1e-10)
+    lib3.set_option("tol_mode", 1)
+    out = []
+    try:
+        for v in range(8):
+            lib3.set_option("spmv", v)
+            say("spmv variant", v)
+            g = H.GpuRun(lib3, cfg, pos, vel, typ)
+            for _ in range(2):
+                assert g.step() == 0
+            out.append(np.ascontiguousarray(g.scene.arr["pressure"]).tobytes())
+            g.close()
+    finally:
+        lib3.set_option("spmv", 0)
+    assert all(o == out[0] for o in out)
