@@ -59,12 +59,13 @@ def build_scene(lib, w):
     return ps, n
 
 
-def spmv_bytes(st, value_bytes=8):
-    """Algorithmic bytes of one CSR SpMV y = A x: values and column indices
-    of every nonzero (diagonal included), row pointers, x read once, y
-    written once."""
+def spmv_bytes(st, s_val=8, s_vec=8):
+    """Algorithmic bytes of one SpMV y = A x over the reference's CSR:
+    Z (value + 4-byte column) + 4 (n_eq + 1) row pointers + x read once and
+    y written once (Z = nonzeros with the diagonal).  The packed matrix the
+    kernel reads stores 16-bit columns, so it moves fewer bytes than this."""
     z, ne = st.nnz, st.n_eq
-    return z * (value_bytes + 4) + 4 * (ne + 1) + 2 * value_bytes * ne
+    return z * (s_val + 4) + 4 * (ne + 1) + 2 * s_vec * ne
 
 
 def cpu_baseline(w, tol, seconds_hint=30.0):
@@ -96,6 +97,20 @@ def cpu_baseline(w, tol, seconds_hint=30.0):
     return dict(value=len(typ) * steps / el, unit="particle-steps/s", cores=threads, kind="port",
                 sample="%d step(s) from rest of the %d-particle workload, CG relative 1e-9, "
                        "mean %.0f iterations/step, %.1f s" % (steps, len(typ), iters / steps, el))
+
+
+def profiled_traffic(precision, workload):
+    """HBM bytes per launch of the solver's SpMV, as measured by rocprofv3
+    (FETCH_SIZE + WRITE_SIZE scaled by the plain-copy calibration) and
+    recorded in profiles/cg_<precision>_<workload>.md; None if absent."""
+    path = os.path.join(ROOT, "profiles", "cg_%s_%s.md" % (precision, workload))
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch"):
+                return int(line.split("=")[1]), os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
 
 
 def calibrate_copy(nbytes=1 << 30, reps=5):
@@ -154,7 +169,8 @@ def main():
           file=sys.stderr, flush=True)
     lib.particle_system_calc_initial_params(ps)
     lib.check(lib.mps_hip_run_steps(ps, a.warmup), "warm-up steps")
-    lib.set_option("profile", 0 if a.no_profile else 1)
+    # the timed steps time only the SpMV launches (two HIP events each)
+    lib.set_option("profile", 0 if a.no_profile else 2)
     lib.mps_hip_profile_reset()
     st0 = mps_amd.StepStats()
     lib.mps_hip_get_stats(ps, C.byref(st0))
@@ -170,20 +186,29 @@ def main():
     cnt = C.c_longlong()
     lib.mps_hip_profile_get(b"spmv", C.byref(ms), C.byref(cnt))
     spmv_ms = ms.value / max(cnt.value, 1)
+    spmv_launches = cnt.value
     roof = None
     if cnt.value > 0:
-        vb = 8 if a.precision == "fp64" else 4
-        byt = spmv_bytes(st, vb)
+        byt = {"fp64": spmv_bytes(st, 8, 8), "mixed": spmv_bytes(st, 4, 8),
+               "mixed32": spmv_bytes(st, 4, 4)}[a.precision]
         ach = byt / (spmv_ms * 1e-3) / 1e9
+        traffic, tsrc = profiled_traffic(a.precision, a.workload)
         roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel="k_spmv_dot" if a.precision == "fp64" else "k_mx_spmv", bytes_per_launch=int(byt),
-                    mean_launch_ms=round(spmv_ms, 5), launches=cnt.value)
+                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=traffic, traffic_source=tsrc,
+                    kernel="k_spmv_pk" if a.precision == "fp64" else "k_mx_spmv_pk", bytes_per_launch=int(byt),
+                    mean_launch_ms=round(spmv_ms, 5), launches=spmv_launches)
+    # per-kernel breakdown: one more step, every kernel class timed
     kernels = {}
+    if not a.no_profile:
+        lib.set_option("profile", 1)
+        lib.mps_hip_profile_reset()
+        lib.check(lib.mps_hip_run_steps(ps, 1), "profiled step")
+        lib.set_option("profile", 0)
     for name in ("visc_predict", "collide_bin", "sort", "search", "assemble", "spmv", "axpy",
                  "pupdate", "grad_correct", "ic"):
         if lib.mps_hip_profile_get(name.encode(), C.byref(ms), C.byref(cnt)) == 0 and cnt.value:
             kernels[name] = dict(total_ms=round(ms.value, 3), launches=cnt.value)
+    lib.mps_hip_release(ps)
     cpu = None
     if not a.no_cpu_baseline and rank == 0:
         try:
@@ -204,7 +229,7 @@ def main():
                    "n_eq": st.n_eq, "nnz": st.nnz, "pairs": st.n_pairs, "parallelism": "1 GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "kernels_ms": kernels,
+        "kernels_ms_one_step": kernels,
     }
     print(json.dumps(line), flush=True)
 

@@ -187,10 +187,10 @@ int  mps_hip_run_steps(ParticleSystem *ps, int nsteps);
  *   "inner_tol"      residual reduction between reliable updates (1e-3)
  *   "exit_on_error"  1 (reference behaviour) or 0
  *   "check_every"    CG iterations between host convergence polls (16)
- *   "spmv"           SpMV kernel variant of the FP64 CG: 0 unroll 8,
- *                      1 unroll 16, 2 unroll 8 + streaming matrix loads,
- *                      3 unroll 16 + streaming, 4 unroll 4
- *   "profile"        1 times the main kernels with HIP events
+ *   "spmv"           entries per load batch in the FP64 CG's SpMV:
+ *                      0 eight, 1 sixteen, 2 four
+ *   "profile"        1 times the main kernels with HIP events, 2 only the
+ *                      CG SpMV
  * Returns 0, or MPS_ERR_ARG for an unknown key. */
 int    mps_hip_set_option(const char *key, double value);
 double mps_hip_get_option(const char *key);

@@ -74,6 +74,43 @@ struct DevCounters {
     unsigned long long nnz;
 };
 
+// The CG's copy of the PPE (k_pack.hip), rebuilt after each assembly.
+// Rows are the unknowns in slot order (row k <-> slot slot_of_row[k]);
+// columns are row numbers.  Rows are sliced 64 at a time (entry q of the
+// rows of slice s at 64 * (smeta[s].x + q) + lane, a slice as wide as its
+// longest row) and grouped 256 at a time into blocks, one per SpMV
+// workgroup.  A block's halo is the sorted set of rows its entries
+// reference (halo[bmeta[b].x ...], bmeta[b].y of them); in halo mode
+// (smeta[s].z = 1) an entry's column is a 16-bit index into that halo, two
+// entries per 32-bit word (word (smeta[s].y + q/2) * 64 + lane of cw16),
+// else a 32-bit row number (c32[(smeta[s].y + q) * 64 + lane]).  Entries of
+// a row keep the assembly's order, so every row sums as before.
+#define PK_ROWS 256                  // rows per block = SpMV workgroup size
+#define PK_HALO_CAP 4096             // largest halo staged in LDS
+struct Packed {
+    int n_eq = 0, n_slices = 0, n_blocks = 0, hmax = 0;
+    int cap_rows = 0, cap_slots = 0;
+    size_t cap_val = 0, cap_w16 = 0, cap_c32 = 0, cap_fval = 0, cap_halo = 0;
+    size_t tot_val = 0, tot_w16 = 0, tot_c32 = 0, tot_halo = 0;
+    int *row_of_slot = nullptr;      // exclusive scan of unk (n + 1)
+    int *slot_of_row = nullptr;
+    int *len = nullptr;              // off-diagonal entries of the row
+    double *diag = nullptr, *b = nullptr, *x = nullptr;
+    int *sw = nullptr, *w16 = nullptr, *w32 = nullptr;       // per slice
+    int *bh = nullptr, *bmode = nullptr;                     // per block
+    int *voff = nullptr, *c16off = nullptr, *c32off = nullptr, *hoff = nullptr;   // scans
+    int4 *smeta = nullptr;           // per slice {value offset, column offset, halo mode, 0}
+    int2 *bmeta = nullptr;           // per block {halo offset, halo length}
+    int *halo = nullptr;
+    double *val = nullptr;
+    unsigned *cw16 = nullptr;
+    int *c32 = nullptr;
+    float *fval = nullptr;           // mixed precision: FP32 values
+    double *ddir = nullptr;          // mixed precision: Dirichlet part of each row
+    int *dmax = nullptr;             // device: largest halo
+    int *htot = nullptr;             // pinned: scan totals
+};
+
 struct CgCtrl;   // device control block of the solvers (k_cg.hip)
 struct IcData;   // IC(0) factor and schedules (k_ic.hip)
 struct MixedBufs;
@@ -118,6 +155,7 @@ struct Ctx {
     mps_step_stats stats{};
     IcData *ic = nullptr;
     MixedBufs *mixed = nullptr;
+    Packed *pk = nullptr;
 };
 
 // ---- launchers (k_particles.hip) -------------------------------------
@@ -158,6 +196,9 @@ struct SolveResult {
     double res, bnorm;
 };
 int cg_alloc_ctrl(Ctx *c);
+int pack_ppe(Ctx *c, int with_f32);        // k_pack.hip
+int unpack_solution(Ctx *c);               // packed x -> c->x (slot order)
+void packed_free(Packed *p);
 int solve_ppe(Ctx *c, const SolveOpts &o, SolveResult *out);
 
 // ---- runtime accessors (runtime.hip) -----------------------------------
@@ -172,6 +213,6 @@ double mps_search_radius(const SimConfig *cf);
 
 // ---- timing (runtime.hip) ----------------------------------------------
 enum ProfKernel { PK_VISC, PK_COLLIDE, PK_SORT, PK_SEARCH, PK_ASSEMBLE, PK_SPMV, PK_AXPY,
-                  PK_PUPDATE, PK_GRAD, PK_IC, PK_COUNT };
+                  PK_PUPDATE, PK_GRAD, PK_IC, PK_PACK, PK_COUNT };
 void prof_begin(int k, hipStream_t s);
 void prof_end(int k, hipStream_t s);

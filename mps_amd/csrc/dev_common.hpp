@@ -71,6 +71,18 @@ struct Grid {
     int total;          // nx*ny*nz (the sentinel bucket is index `total`)
 };
 
+// Storage order of the cells (the counting sort's key): y, the vertical
+// axis of the dam-break scenes, slowest, then z, then x.  Only where cells
+// sit in memory follows it; every search still visits the neighbour cells
+// z -> y -> x as neighbor_search.c:185-239 does, so the lists and every sum
+// over them are unchanged.  With y slowest the slots of three consecutive
+// cell layers span few enough positions that the columns of a 64-row slice
+// of the PPE matrix mostly fit 16-bit offsets.  (2-D: y * nx + x.)
+__host__ __device__ __forceinline__ int cell_key(const Grid &g, int cx, int cy, int cz)
+{
+    return (cy * g.nc[2] + cz) * g.nc[0] + cx;
+}
+
 // Cell of a position or -1 outside (neighbor_search.c:171-178).
 __device__ __forceinline__ int cell_coords(const Grid &g, const dvec &p, int c[3])
 {
@@ -80,7 +92,7 @@ __device__ __forceinline__ int cell_coords(const Grid &g, const dvec &p, int c[3
         c[d] = to_int_x86((p.c[d] - g.origin[d]) * g.inv);
         if (c[d] < 0 || c[d] >= g.nc[d]) return -1;
     }
-    return (c[2] * g.nc[1] + c[1]) * g.nc[0] + c[0];
+    return cell_key(g, c[0], c[1], c[2]);
 }
 
 // ---------------------------------------------------------------------

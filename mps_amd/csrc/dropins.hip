@@ -180,10 +180,17 @@ void neighbor_search_cell_linked_list(NeighborList *nl, ParticleSystem *ps, Cell
         std::vector<int> O(c->n), cs((size_t)g.total + 2);
         if (hipMemcpy(O.data(), c->O[c->ct], sizeof(int) * c->n, hipMemcpyDeviceToHost) != hipSuccess) return;
         if (hipMemcpy(cs.data(), c->cstart, sizeof(int) * ((size_t)g.total + 2), hipMemcpyDeviceToHost) != hipSuccess) return;
-        for (int q = 0; q < g.total; q++) cl->head[q] = cs[q] < cs[q + 1] ? O[cs[q]] : -1;
+        // head[] is indexed in the reference's cell order (z slowest); the
+        // device stores cells in cell_key order
         for (int s = 0; s < c->n; s++) cl->next[O[s]] = -1;
-        for (int q = 0; q < g.total; q++)
-            for (int s = cs[q]; s + 1 < cs[q + 1]; s++) cl->next[O[s]] = O[s + 1];
+        for (int cz = 0; cz < g.nc[2]; cz++)
+            for (int cy = 0; cy < g.nc[1]; cy++)
+                for (int cx = 0; cx < g.nc[0]; cx++) {
+                    const int q = (cz * g.nc[1] + cy) * g.nc[0] + cx;
+                    const int key = cell_key(g, cx, cy, cz);
+                    cl->head[q] = cs[key] < cs[key + 1] ? O[cs[key]] : -1;
+                    for (int s = cs[key]; s + 1 < cs[key + 1]; s++) cl->next[O[s]] = O[s + 1];
+                }
     }
 }
 

@@ -371,7 +371,7 @@ __global__ void k_init_sums(int n, const dvec *X, const int *T, const int *cstar
     for (int cz = lo[2]; cz <= hi[2]; cz++)
         for (int cy = lo[1]; cy <= hi[1]; cy++)
             for (int cxx = lo[0]; cxx <= hi[0]; cxx++) {
-                const int cid = (cz * ig.g.nc[1] + cy) * ig.g.nc[0] + cxx;
+                const int cid = cell_key(ig.g, cxx, cy, cz);
                 for (int u = cstart[cid]; u < cstart[cid + 1]; u++) {
                     const int j = bucket[u];
                     if (j == i) continue;

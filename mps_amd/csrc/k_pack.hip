@@ -1,0 +1,417 @@
+// k_pack.hip -- the CG's copy of the PPE matrix.
+//
+// The assembly writes each unknown's row into the neighbour-list layout
+// (64-slot slices, stride max_neighbors), which suits the assembly but
+// leaves idle lanes at every non-unknown slot and gathers the CG vector
+// through the texture path once per nonzero.  The CG reads the matrix
+// hundreds of times per step, so after each assembly the rows are repacked
+// (about the cost of two SpMVs):
+//   * rows = the unknowns in slot order, columns = row numbers, so the CG
+//     vectors hold only unknowns (n_eq, not n) and every lane has a row;
+//   * slices of 64 rows, each as wide as its longest row;
+//   * blocks of 256 rows (one SpMV workgroup).  The rows a block's entries
+//     reference -- its halo, about 1,900 rows for a 256-row block of the
+//     1.08M-particle dam break, each used ~15 times -- are listed once; the
+//     SpMV stages the vector at the halo in LDS and gathers from there, and
+//     an entry's column is a 16-bit index into the halo, two per 32-bit
+//     word.  A block whose halo is larger than PK_HALO_CAP, or whose
+//     columns span more than 2^17 rows, keeps 32-bit row numbers and
+//     gathers from global memory.
+// Entries keep their order inside a row, so each row's sum is unchanged.
+// The halo of a block is found with an LDS bitmap over its column span:
+// set a bit per referenced row, then an entry's halo index is the number
+// of set bits below its own.
+#include <limits.h>
+
+#include "ctx.hpp"
+
+#define PKB 256
+#define BM_WORDS 4096     // bitmap of 2^17 rows
+static_assert(PKB == PK_ROWS, "pack block = SpMV block");
+
+static inline unsigned pk_blocks(size_t n) { return (unsigned)((n + PKB - 1) / PKB); }
+
+#define LAUNCH_CHECK()                                                       \
+    do {                                                                     \
+        hipError_t e_ = hipGetLastError();                                   \
+        if (e_ != hipSuccess) {                                              \
+            mps_set_error(MPS_ERR_HIP, __FILE__, __LINE__, hipGetErrorString(e_)); \
+            return MPS_ERR_HIP;                                              \
+        }                                                                    \
+    } while (0)
+
+__global__ void k_pk_slots(int n, const int *__restrict__ unk, const int *__restrict__ row_of_slot,
+                           int *__restrict__ slot_of_row)
+{
+    const int s = blockIdx.x * PKB + threadIdx.x;
+    if (s < n && unk[s]) slot_of_row[row_of_slot[s]] = s;
+}
+
+__device__ __forceinline__ int wave_max_i(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const int u = __shfl_xor(v, o, WAVE);
+        v = u > v ? u : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int wave_min_i(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const int u = __shfl_xor(v, o, WAVE);
+        v = u < v ? u : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int wave_sum_i(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+
+// The row of this thread and the column range of the whole block (every
+// thread gets the block's min and max).
+struct BlockRows {
+    int k, slot, m, bmn, bmx;
+};
+
+__device__ __forceinline__ BlockRows block_rows(int n_eq, int stride, const int *__restrict__ slot_of_row,
+                                                const int *__restrict__ row_of_slot, const int *__restrict__ Alen,
+                                                const int *__restrict__ Acol, int *red)
+{
+    BlockRows br;
+    br.k = blockIdx.x * PKB + threadIdx.x;
+    br.slot = -1;
+    br.m = 0;
+    int mn = INT_MAX, mx = -1;
+    if (br.k < n_eq) {
+        br.slot = slot_of_row[br.k];
+        br.m = Alen[br.slot];
+        for (int q = 0; q < br.m; q++) {
+            const int cr = row_of_slot[Acol[sell_off(br.slot, q, stride)]];
+            mn = cr < mn ? cr : mn;
+            mx = cr > mx ? cr : mx;
+        }
+    }
+    mn = wave_min_i(mn);
+    mx = wave_max_i(mx);
+    const int w = threadIdx.x / WAVE;
+    if ((threadIdx.x & (WAVE - 1)) == 0) {
+        red[w] = mn;
+        red[PKB / WAVE + w] = mx;
+    }
+    __syncthreads();
+    br.bmn = INT_MAX;
+    br.bmx = -1;
+#pragma unroll
+    for (int j = 0; j < PKB / WAVE; j++) {
+        br.bmn = red[j] < br.bmn ? red[j] : br.bmn;
+        br.bmx = red[PKB / WAVE + j] > br.bmx ? red[PKB / WAVE + j] : br.bmx;
+    }
+    __syncthreads();
+    return br;
+}
+
+// Sets the bitmap of the block's referenced rows (bit c - bmn).
+__device__ __forceinline__ void block_bitmap(const BlockRows &br, int stride, const int *__restrict__ row_of_slot,
+                                             const int *__restrict__ Acol, unsigned *bits)
+{
+    for (int w = threadIdx.x; w < BM_WORDS; w += PKB) bits[w] = 0u;
+    __syncthreads();
+    for (int q = 0; q < br.m; q++) {
+        const int c = row_of_slot[Acol[sell_off(br.slot, q, stride)]] - br.bmn;
+        atomicOr(&bits[c >> 5], 1u << (c & 31));
+    }
+    __syncthreads();
+}
+
+// Pass 1, one workgroup per block: slice widths, halo size and mode.
+__global__ void __launch_bounds__(PKB)
+k_pk_pass1(int n_eq, int n_slices, int stride, const int *__restrict__ slot_of_row,
+           const int *__restrict__ row_of_slot, const int *__restrict__ Alen, const int *__restrict__ Acol,
+           int *__restrict__ sw, int *__restrict__ w16, int *__restrict__ w32, int *__restrict__ bh,
+           int *__restrict__ bmode, int *dmax)
+{
+    __shared__ unsigned bits[BM_WORDS];
+    __shared__ int red[2 * PKB / WAVE];
+    const BlockRows br = block_rows(n_eq, stride, slot_of_row, row_of_slot, Alen, Acol, red);
+    const bool span_ok = br.bmx < 0 || (br.bmx - br.bmn) < BM_WORDS * 32;
+    int H = 0;
+    if (span_ok && br.bmx >= 0) {           // uniform over the block
+        block_bitmap(br, stride, row_of_slot, Acol, bits);
+        int cnt = 0;
+        for (int w = threadIdx.x; w < BM_WORDS; w += PKB) cnt += __popc(bits[w]);
+        cnt = wave_sum_i(cnt);
+        if ((threadIdx.x & (WAVE - 1)) == 0) red[threadIdx.x / WAVE] = cnt;
+        __syncthreads();
+        for (int j = 0; j < PKB / WAVE; j++) H += red[j];
+    }
+    const int mode = span_ok && H <= PK_HALO_CAP;
+    const int wmax = wave_max_i(br.m);
+    const int sl = blockIdx.x * (PKB / WAVE) + threadIdx.x / WAVE;
+    if (sl < n_slices && (threadIdx.x & (WAVE - 1)) == 0) {
+        sw[sl] = wmax;
+        w16[sl] = mode ? (wmax + 1) / 2 : 0;
+        w32[sl] = mode ? 0 : wmax;
+    }
+    if (threadIdx.x == 0) {
+        bh[blockIdx.x] = mode ? H : 0;
+        bmode[blockIdx.x] = mode;
+        if (mode) atomicMax(dmax, H);
+    }
+}
+
+// Pass 2, one workgroup per block: halo list, values, columns, slice and
+// block descriptors.
+__global__ void __launch_bounds__(PKB)
+k_pk_pass2(int n_eq, int n_slices, int stride, const int *__restrict__ slot_of_row,
+           const int *__restrict__ row_of_slot, const int *__restrict__ Alen, const double *__restrict__ Aval,
+           const int *__restrict__ Acol, const double *__restrict__ diag, const double *__restrict__ b,
+           const int *__restrict__ bmode, const int *__restrict__ bh, const int *__restrict__ hoff,
+           const int *__restrict__ voff, const int *__restrict__ c16off, const int *__restrict__ c32off,
+           int *__restrict__ halo, double *__restrict__ val, unsigned short *__restrict__ c16,
+           int *__restrict__ c32, int *__restrict__ len, double *__restrict__ pdiag, double *__restrict__ pb,
+           int4 *__restrict__ smeta, int2 *__restrict__ bmeta)
+{
+    __shared__ unsigned bits[BM_WORDS];
+    __shared__ int pre[BM_WORDS];
+    __shared__ int red[2 * PKB / WAVE];
+    const BlockRows br = block_rows(n_eq, stride, slot_of_row, row_of_slot, Alen, Acol, red);
+    const int mode = bmode[blockIdx.x];
+    if (mode && br.bmx >= 0) {              // uniform over the block
+        block_bitmap(br, stride, row_of_slot, Acol, bits);
+        const int w0 = threadIdx.x * (BM_WORDS / PKB);
+        int cnt[BM_WORDS / PKB];
+        int tot = 0;
+#pragma unroll
+        for (int j = 0; j < BM_WORDS / PKB; j++) {
+            cnt[j] = __popc(bits[w0 + j]);
+            tot += cnt[j];
+        }
+        int all;
+        int run = block_scan_excl<PKB>(tot, &all);
+#pragma unroll
+        for (int j = 0; j < BM_WORDS / PKB; j++) {
+            pre[w0 + j] = run;
+            run += cnt[j];
+        }
+        const int ho = hoff[blockIdx.x];
+#pragma unroll
+        for (int j = 0; j < BM_WORDS / PKB; j++) {
+            unsigned word = bits[w0 + j];
+            int at = ho + pre[w0 + j];
+            while (word) {
+                const int bit = __ffs(word) - 1;
+                halo[at++] = br.bmn + (w0 + j) * 32 + bit;
+                word &= word - 1u;
+            }
+        }
+        __syncthreads();
+    }
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int sl = blockIdx.x * (PKB / WAVE) + threadIdx.x / WAVE;
+    if (sl < n_slices && lane == 0)
+        smeta[sl] = make_int4(voff[sl], mode ? c16off[sl] : c32off[sl], mode, 0);
+    if (threadIdx.x == 0) bmeta[blockIdx.x] = make_int2(mode ? hoff[blockIdx.x] : 0, bh[blockIdx.x]);
+    if (br.slot < 0) return;
+    const int vo = voff[sl];
+    const int co = mode ? c16off[sl] : c32off[sl];
+    const int m = br.m;
+    for (int q = 0; q < m; q++) {
+        const size_t o = sell_off(br.slot, q, stride);
+        const int cr = row_of_slot[Acol[o]];
+        val[(size_t)(vo + q) * 64u + lane] = Aval[o];
+        if (mode) {
+            const int c = cr - br.bmn;
+            const unsigned below = bits[c >> 5] & ((1u << (c & 31)) - 1u);
+            const int li = pre[c >> 5] + __popc(below);
+            c16[((size_t)(co + q / 2) * 64u + lane) * 2u + (q & 1)] = (unsigned short)li;
+        } else {
+            c32[(size_t)(co + q) * 64u + lane] = cr;
+        }
+    }
+    if (mode && (m & 1)) c16[((size_t)(co + m / 2) * 64u + lane) * 2u + 1u] = 0;   // unused half
+    len[br.k] = m;
+    pdiag[br.k] = diag[br.slot];
+    pb[br.k] = b[br.slot];
+}
+
+// FP32 copy of the off-diagonals and each row's Dirichlet part
+// d_k = A_kk + sum_j A_kj in FP64 (see the mixed solver in k_cg.hip).
+__global__ void k_pk_f32(int n_eq, const int *__restrict__ len, const int4 *__restrict__ smeta,
+                         const double *__restrict__ val, const double *__restrict__ pdiag,
+                         float *__restrict__ fval, double *__restrict__ ddir)
+{
+    const int k = blockIdx.x * PKB + threadIdx.x;
+    if (k >= n_eq) return;
+    const int m = len[k];
+    const size_t base = (size_t)smeta[k >> 6].x * 64u + (size_t)(k & 63);
+    double d = pdiag[k];
+    for (int q = 0; q < m; q++) {
+        const double a = val[base + (size_t)q * 64u];
+        fval[base + (size_t)q * 64u] = (float)a;
+        d += a;
+    }
+    ddir[k] = d;
+}
+
+__global__ void k_pk_unpack_x(int n, const int *__restrict__ unk, const int *__restrict__ row_of_slot,
+                              const double *__restrict__ xp, double *__restrict__ x)
+{
+    const int s = blockIdx.x * PKB + threadIdx.x;
+    if (s >= n) return;
+    x[s] = unk[s] ? xp[row_of_slot[s]] : 0.0;
+}
+
+template <typename T>
+static int grow(T **p, size_t *cap, size_t need)
+{
+    if (*p && *cap >= need) return 0;
+    if (*p) hipFree(*p);
+    *p = nullptr;
+    size_t n = need + need / 8 + 64;
+    HIP_OK(hipMalloc((void **)p, sizeof(T) * n));
+    *cap = n;
+    return 0;
+}
+
+#define GROW(p, cap, need)                     \
+    do {                                       \
+        size_t c_ = (cap);                     \
+        int rc_ = grow(&(p), &c_, (need));     \
+        if (rc_) return rc_;                   \
+        (cap) = c_;                            \
+    } while (0)
+
+void packed_free(Packed *p)
+{
+    if (!p) return;
+    void *bufs[] = {p->row_of_slot, p->slot_of_row, p->len, p->diag, p->b, p->x, p->sw, p->w16, p->w32,
+                    p->bh, p->bmode, p->voff, p->c16off, p->c32off, p->hoff, p->smeta, p->bmeta, p->halo,
+                    p->val, p->cw16, p->c32, p->fval, p->ddir, p->dmax};
+    for (void *b : bufs)
+        if (b) hipFree(b);
+    if (p->htot) hipHostFree(p->htot);
+    delete p;
+}
+
+// Row-, slice- and block-sized buffers for up to n rows.
+static int pk_reserve_rows(Packed *p, int n)
+{
+    if (p->cap_slots < n + 1) {
+        if (p->row_of_slot) hipFree(p->row_of_slot);
+        p->row_of_slot = nullptr;
+        HIP_OK(hipMalloc(&p->row_of_slot, sizeof(int) * ((size_t)n + 1)));
+        p->cap_slots = n + 1;
+    }
+    const int rows = (n + PKB - 1) / PKB * PKB;
+    if (p->cap_rows < rows) {
+        void *old[] = {p->slot_of_row, p->len, p->diag, p->b, p->x, p->ddir, p->sw, p->w16, p->w32, p->bh,
+                       p->bmode, p->voff, p->c16off, p->c32off, p->hoff, p->smeta, p->bmeta};
+        for (void *b : old)
+            if (b) hipFree(b);
+        const size_t ns = (size_t)rows / 64 + 1, nb = (size_t)rows / PKB + 1;
+        HIP_OK(hipMalloc(&p->slot_of_row, sizeof(int) * rows));
+        HIP_OK(hipMalloc(&p->len, sizeof(int) * rows));
+        HIP_OK(hipMalloc(&p->diag, sizeof(double) * rows));
+        HIP_OK(hipMalloc(&p->b, sizeof(double) * rows));
+        HIP_OK(hipMalloc(&p->x, sizeof(double) * rows));
+        HIP_OK(hipMalloc(&p->ddir, sizeof(double) * rows));
+        HIP_OK(hipMalloc(&p->sw, sizeof(int) * ns));
+        HIP_OK(hipMalloc(&p->w16, sizeof(int) * ns));
+        HIP_OK(hipMalloc(&p->w32, sizeof(int) * ns));
+        HIP_OK(hipMalloc(&p->voff, sizeof(int) * ns));
+        HIP_OK(hipMalloc(&p->c16off, sizeof(int) * ns));
+        HIP_OK(hipMalloc(&p->c32off, sizeof(int) * ns));
+        HIP_OK(hipMalloc(&p->smeta, sizeof(int4) * ns));
+        HIP_OK(hipMalloc(&p->bh, sizeof(int) * nb));
+        HIP_OK(hipMalloc(&p->bmode, sizeof(int) * nb));
+        HIP_OK(hipMalloc(&p->hoff, sizeof(int) * nb));
+        HIP_OK(hipMalloc(&p->bmeta, sizeof(int2) * nb));
+        p->cap_rows = rows;
+    }
+    if (!p->dmax) HIP_OK(hipMalloc(&p->dmax, sizeof(int)));
+    if (!p->htot) HIP_OK(hipHostMalloc(&p->htot, sizeof(int) * 8, hipHostMallocDefault));
+    return 0;
+}
+
+int pack_ppe(Ctx *c, int with_f32)
+{
+    if (!c->pk) c->pk = new Packed();
+    Packed *p = c->pk;
+    const int n = c->n;
+    hipStream_t s = c->stream;
+    int rc = pk_reserve_rows(p, n);
+    if (rc) return rc;
+    prof_begin(PK_PACK, s);
+    rc = launch_scan_excl(c->unk, p->row_of_slot, n, c->scan_blk, s);
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(&p->htot[0], p->row_of_slot + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    const int n_eq = p->htot[0];
+    p->n_eq = n_eq;
+    p->n_slices = (n_eq + 63) / 64;
+    p->n_blocks = (n_eq + PKB - 1) / PKB;
+    p->hmax = 0;
+    if (n_eq == 0) {
+        prof_end(PK_PACK, s);
+        return 0;
+    }
+    const int ns = p->n_slices, nb = p->n_blocks;
+    hipLaunchKernelGGL(k_pk_slots, dim3(pk_blocks(n)), dim3(PKB), 0, s, n, c->unk, p->row_of_slot,
+                       p->slot_of_row);
+    rc = launch_fill_int(p->dmax, 0, 1, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_pk_pass1, dim3(nb), dim3(PKB), 0, s, n_eq, ns, c->stride, p->slot_of_row,
+                       p->row_of_slot, c->Alen, c->Acol, p->sw, p->w16, p->w32, p->bh, p->bmode, p->dmax);
+    LAUNCH_CHECK();
+    rc = launch_scan_excl(p->sw, p->voff, ns, c->scan_blk, s);
+    if (!rc) rc = launch_scan_excl(p->w16, p->c16off, ns, c->scan_blk, s);
+    if (!rc) rc = launch_scan_excl(p->w32, p->c32off, ns, c->scan_blk, s);
+    if (!rc) rc = launch_scan_excl(p->bh, p->hoff, nb, c->scan_blk, s);
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(&p->htot[1], p->voff + ns, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(&p->htot[2], p->c16off + ns, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(&p->htot[3], p->c32off + ns, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(&p->htot[4], p->hoff + nb, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(&p->htot[5], p->dmax, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    p->tot_val = (size_t)p->htot[1];
+    p->tot_w16 = (size_t)p->htot[2];
+    p->tot_c32 = (size_t)p->htot[3];
+    p->tot_halo = (size_t)p->htot[4];
+    p->hmax = p->htot[5];
+    GROW(p->val, p->cap_val, p->tot_val * 64);
+    GROW(p->cw16, p->cap_w16, p->tot_w16 * 64);
+    GROW(p->c32, p->cap_c32, p->tot_c32 * 64);
+    GROW(p->halo, p->cap_halo, p->tot_halo);
+    hipLaunchKernelGGL(k_pk_pass2, dim3(nb), dim3(PKB), 0, s, n_eq, ns, c->stride, p->slot_of_row, p->row_of_slot,
+                       c->Alen, c->Aval, c->Acol, c->diag, c->b, p->bmode, p->bh, p->hoff, p->voff, p->c16off,
+                       p->c32off, p->halo, p->val, (unsigned short *)p->cw16, p->c32, p->len, p->diag, p->b,
+                       p->smeta, p->bmeta);
+    LAUNCH_CHECK();
+    if (with_f32) {
+        GROW(p->fval, p->cap_fval, p->tot_val * 64);
+        hipLaunchKernelGGL(k_pk_f32, dim3(pk_blocks(n_eq)), dim3(PKB), 0, s, n_eq, p->len, p->smeta, p->val,
+                           p->diag, p->fval, p->ddir);
+        LAUNCH_CHECK();
+    }
+    prof_end(PK_PACK, s);
+    return 0;
+}
+
+int unpack_solution(Ctx *c)
+{
+    Packed *p = c->pk;
+    const int n = c->n;
+    if (!p || p->n_eq == 0) return launch_fill_double(c->x, 0.0, (size_t)n, c->stream);
+    hipLaunchKernelGGL(k_pk_unpack_x, dim3(pk_blocks(n)), dim3(PKB), 0, c->stream, n, c->unk, p->row_of_slot,
+                       p->x, c->x);
+    LAUNCH_CHECK();
+    return 0;
+}

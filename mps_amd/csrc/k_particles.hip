@@ -531,7 +531,7 @@ k_collide2d(int n, const dvec *__restrict__ X, const dvec *__restrict__ V, const
         if (inside) {
             for (int cy = lo[1]; cy <= hi[1]; cy++)
                 for (int cxx = lo[0]; cxx <= hi[0]; cxx++) {
-                    const int cid = cy * g.nc[0] + cxx;
+                    const int cid = cell_key(g, cxx, cy, 0);
                     for (int u = cstart[cid]; u < cstart[cid + 1]; u++) CONSIDER(bucket[u]);
                 }
             for (int u = cstart[g.total]; u < cstart[g.total + 1]; u++) CONSIDER(bucket[u]);
@@ -694,7 +694,7 @@ k_search(int n, const dvec *__restrict__ X, const int *__restrict__ T, const int
             for (int cz = lo[2]; cz <= hi[2]; cz++)
                 for (int cy = lo[1]; cy <= hi[1]; cy++)
                     for (int cxx = lo[0]; cxx <= hi[0]; cxx++) {
-                        const int cid = (cz * k.grid.nc[1] + cy) * k.grid.nc[0] + cxx;
+                        const int cid = cell_key(k.grid, cxx, cy, cz);
                         const int j1 = cstart[cid + 1];
                         for (int j = cstart[cid]; j < j1; j++) {
                             if (j == i) continue;
