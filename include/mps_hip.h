@@ -187,8 +187,12 @@ int  mps_hip_run_steps(ParticleSystem *ps, int nsteps);
  *   "inner_tol"      residual reduction between reliable updates (1e-3)
  *   "exit_on_error"  1 (reference behaviour) or 0
  *   "check_every"    CG iterations between host convergence polls (16)
- *   "spmv"           entries per load batch in the FP64 CG's SpMV:
+ *   "spmv"           entries per load batch in the CG's SpMV:
  *                      0 eight, 1 sixteen, 2 four
+ *   "fused"          0 (default): FP64 CG in five kernels per iteration;
+ *                      1: two (the previous iteration's reduction done
+ *                      redundantly by every block of the next kernel),
+ *                      same results bit for bit, measured 2 % slower
  *   "profile"        1 times the main kernels with HIP events, 2 only the
  *                      CG SpMV
  * Returns 0, or MPS_ERR_ARG for an unknown key. */

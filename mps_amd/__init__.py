@@ -137,8 +137,10 @@ class Lib:
             raise RuntimeError("%s failed: %s (%d)" % (what, msg, rc))
 
     def stats(self, scene):
+        """Step statistics of a HostScene or a ParticleSystem pointer."""
         st = StepStats()
-        self.check(self.L.mps_hip_get_stats(scene.ptr(), C.byref(st)), "get_stats")
+        ps = scene.ptr() if hasattr(scene, "ptr") else scene
+        self.check(self.L.mps_hip_get_stats(ps, C.byref(st)), "get_stats")
         return st
 
     def export_ppe(self, scene):

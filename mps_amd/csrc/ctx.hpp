@@ -137,6 +137,7 @@ struct Ctx {
     double *diag = nullptr, *b = nullptr, *Aval = nullptr;
     int *Acol = nullptr, *Alen = nullptr;
     double *x = nullptr, *r = nullptr, *pv = nullptr, *Ap = nullptr;
+    double *pv2 = nullptr;           // second search-direction buffer (fused CG)
     double *partA = nullptr, *partB = nullptr;
     float *fAval = nullptr, *fr = nullptr, *fp = nullptr, *fAp = nullptr;
     double *ddir = nullptr;          // mixed solve: Dirichlet part of each row
@@ -189,7 +190,8 @@ struct SolveOpts {
     double inner_tol;
     int check_every;
     int dense_ic;      // 2-D dense-formula IC(0)
-    int spmv_variant;  // SpMV kernel variant (unroll / streaming loads)
+    int spmv_variant;  // entries per load batch in the SpMV
+    int fused;         // FP64 CG in two kernels per iteration
 };
 struct SolveResult {
     int iters, outer, breakdown;

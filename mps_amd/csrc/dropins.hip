@@ -370,6 +370,7 @@ void solve_pressure(ParticleSystem *ps, NeighborList *nl)
     so.check_every = (int)mps_hip_get_option("check_every");
     so.dense_ic = MPS_DIM == 2;
     so.spmv_variant = (int)mps_hip_get_option("spmv");
+    so.fused = (int)mps_hip_get_option("fused");
     SolveResult sr{};
     if (solve_ppe(c, so, &sr)) return;
     StepConsts k = cl.k;

@@ -61,11 +61,13 @@ __device__ __forceinline__ double fin_sum(const double *part, unsigned nparts)
     return block_sum<double, CG_BLK>(s);
 }
 
-// Grid for the streaming vector kernels (grid-stride, 4 waves per block).
+// Grid for the streaming vector kernels (grid-stride, at most 512 blocks
+// of 4 waves: enough to stream the vectors, and few partials for the
+// blocks of the next kernel to sum).
 static inline unsigned vec_grid(int n)
 {
     unsigned g = (unsigned)((n + CG_BLK * 4 - 1) / (CG_BLK * 4));
-    return g < 1 ? 1 : (g > 2048 ? 2048 : g);
+    return g < 1 ? 1 : (g > 512 ? 512 : g);
 }
 
 // ---------------------------------------------------------------------
@@ -99,6 +101,19 @@ static PkView pk_view(const Packed *p)
     return v;
 }
 
+// Vector accessors: the value at row c of the vector a row is applied to.
+template <typename T>
+struct XPlain {
+    const T *x;
+    __device__ __forceinline__ T operator()(int c) const { return x[c]; }
+};
+// p_t = r + beta p_{t-1}, formed where it is read (fused CG iteration)
+struct XFused {
+    const double *r, *p;
+    double beta;
+    __device__ __forceinline__ double operator()(int c) const { return r[c] + beta * p[c]; }
+};
+
 template <bool DIFF, typename ACC, typename VT, typename XT>
 __device__ __forceinline__ ACC madd(ACC s, VT a, XT xj, XT xi)
 {
@@ -109,9 +124,9 @@ __device__ __forceinline__ ACC madd(ACC s, VT a, XT xj, XT xi)
 // s + sum over the row's m entries, in order, of a_q * x[c_q] (or, with
 // DIFF, a_q * (x[c_q] - xi)); 32-bit row numbers, x gathered from global
 // memory.  U column and value loads, then U gathers, per round.
-template <int U, bool DIFF, typename ACC, typename VT, typename XT>
+template <int U, bool DIFF, typename ACC, typename VT, typename XT, typename XA>
 __device__ __forceinline__ ACC row_sum_global(int m, const VT *__restrict__ vrow, const int *__restrict__ crow,
-                                              const XT *__restrict__ x, XT xi, int self, ACC s)
+                                              XA x, XT xi, int self, ACC s)
 {
     for (int q0 = 0; q0 < m; q0 += U) {
         int c[U];
@@ -125,7 +140,7 @@ __device__ __forceinline__ ACC row_sum_global(int m, const VT *__restrict__ vrow
             a[u] = ok ? vrow[o] : VT(0);
         }
 #pragma unroll
-        for (int u = 0; u < U; u++) xv[u] = x[c[u]];
+        for (int u = 0; u < U; u++) xv[u] = x(c[u]);
 #pragma unroll
         for (int u = 0; u < U; u++)
             if (q0 + u < m) s = madd<DIFF>(s, a[u], xv[u], xi);
@@ -163,9 +178,8 @@ __device__ __forceinline__ ACC row_sum_halo(int m, const VT *__restrict__ vrow, 
 // Row i of the packed matrix applied to a vector: `vals` is the FP64 or
 // FP32 copy of the values; in halo mode the vector comes from `hx` (the
 // block's halo in LDS), otherwise from global `x`.
-template <int U, bool DIFF, typename ACC, typename VT, typename XT>
-__device__ __forceinline__ ACC packed_row(const PkView &P, const VT *vals, int i, const XT *x, const XT *hx, XT xi,
-                                          ACC s)
+template <int U, bool DIFF, typename ACC, typename VT, typename XT, typename XA>
+__device__ __forceinline__ ACC packed_row(const PkView &P, const VT *vals, int i, XA x, const XT *hx, XT xi, ACC s)
 {
     const int4 mt = P.smeta[i >> 6];
     const int lane = i & 63;
@@ -188,16 +202,17 @@ __device__ __forceinline__ double packed_row_global(const PkView &P, const VT *v
         const int *hidx = P.halo + P.bmeta[i / PK_ROWS].x;
         return row_sum_halo<U, false, true, double>(m, vrow, P.cw16 + (size_t)mt.y * 64u + lane, x, hidx, 0.0, s);
     }
-    return row_sum_global<U, false, double>(m, vrow, P.c32 + (size_t)mt.y * 64u + lane, x, 0.0, i, s);
+    return row_sum_global<U, false, double>(m, vrow, P.c32 + (size_t)mt.y * 64u + lane, XPlain<double>{x}, 0.0,
+                                            i, s);
 }
 
 // Stages the block's halo of x in LDS (hx); every thread of the block
 // takes part and the values are visible to all on return.
-template <typename XT>
-__device__ __forceinline__ void stage_halo(const PkView &P, const XT *__restrict__ x, XT *hx)
+template <typename XT, typename XA>
+__device__ __forceinline__ void stage_halo(const PkView &P, XA x, XT *hx)
 {
     const int2 bm = P.bmeta[blockIdx.x];
-    for (int h = threadIdx.x; h < bm.y; h += CG_BLK) hx[h] = x[P.halo[bm.x + h]];
+    for (int h = threadIdx.x; h < bm.y; h += CG_BLK) hx[h] = x(P.halo[bm.x + h]);
     __syncthreads();
 }
 
@@ -247,12 +262,12 @@ k_spmv_pk(int n, PkView P, const double *__restrict__ p, double *__restrict__ Ap
     extern __shared__ unsigned char smem_raw[];
     double *hx = reinterpret_cast<double *>(smem_raw);
     if (ctrl->done) return;
-    stage_halo(P, p, hx);
+    stage_halo(P, XPlain<double>{p}, hx);
     const int i = blockIdx.x * CG_BLK + threadIdx.x;
     double v = 0.0;
     if (i < n) {
         const double pi = p[i];
-        const double s = packed_row<U, false, double>(P, P.val, i, p, hx, pi, P.diag[i] * pi);
+        const double s = packed_row<U, false, double>(P, P.val, i, XPlain<double>{p}, hx, pi, P.diag[i] * pi);
         Ap[i] = s;
         v = pi * s;
     }
@@ -340,6 +355,125 @@ k_pupdate(int n, const double *__restrict__ r, double *__restrict__ p, const CgC
         p[i] = r[i] + beta * p[i];
 }
 
+// ---- fused form: two kernels per iteration ---------------------------
+// k_cg_iter_a(t): closes iteration t-1 -- every block sums the axpy
+// partials in the order k_axpy_fin does and takes the same decisions;
+// block 0 records them -- then forms p_t = r + beta p_{t-1} where it is
+// read (the block's halo as it is staged in LDS, its own rows, which it
+// stores in the other p buffer, and the gathers of 32-bit blocks) and
+// computes Ap_t = A p_t with the partial p.Ap.
+// k_cg_iter_b(t): every block sums the SpMV partials as k_spmv_fin does
+// (breakdown test, alpha; block 0 records), then x += alpha p_t,
+// r -= alpha Ap_t, partial r.r.
+// Every element is computed with the arithmetic of the five-kernel form,
+// so the two forms give the same bits.
+template <int U>
+__global__ void __launch_bounds__(CG_BLK)
+k_cg_iter_a(int n, PkView P, int t, int max_iter, const double *__restrict__ r, const double *__restrict__ p_prev,
+            double *__restrict__ p_cur, double *__restrict__ Ap, const double *partB, unsigned nb, double *partA,
+            CgCtrl *ctrl)
+{
+    extern __shared__ unsigned char smem_raw[];
+    double *hx = reinterpret_cast<double *>(smem_raw);
+    __shared__ double sh_beta;
+    __shared__ int sh_stop;
+    if (ctrl->done) return;
+    double beta = 0.0;
+    if (t > 0) {
+        const double rr_new = fin_sum(partB, nb);
+        if (threadIdx.x == 0) {
+            const double rr = ctrl->rr[(t - 1) & 1];
+            int stop = 0, iters = 0;
+            double bt = 0.0;
+            if (rr < 1.0e-60) {
+                stop = 1;
+                iters = t - 1;
+            } else {
+                bt = rr_new / rr;
+                if (t >= max_iter) {
+                    stop = 1;
+                    iters = max_iter;
+                } else if (sqrt(rr_new) < ctrl->stop) {
+                    stop = 1;
+                    iters = t;
+                }
+            }
+            if (blockIdx.x == 0) {
+                ctrl->rr[t & 1] = rr_new;
+                ctrl->res = sqrt(rr_new);
+                ctrl->beta = bt;
+                if (stop) {
+                    ctrl->iters = iters;
+                    ctrl->done = 1;
+                }
+            }
+            sh_beta = bt;
+            sh_stop = stop;
+        }
+        __syncthreads();
+        if (sh_stop) return;
+        beta = sh_beta;
+    }
+    const XFused pt{r, p_prev, beta};
+    const XPlain<double> p0{p_prev};
+    if (t > 0)
+        stage_halo(P, pt, hx);
+    else
+        stage_halo(P, p0, hx);
+    const int i = blockIdx.x * CG_BLK + threadIdx.x;
+    double v = 0.0;
+    if (i < n) {
+        const double pi = t > 0 ? pt(i) : p0(i);
+        p_cur[i] = pi;
+        const double s = t > 0 ? packed_row<U, false, double>(P, P.val, i, pt, hx, pi, P.diag[i] * pi)
+                               : packed_row<U, false, double>(P, P.val, i, p0, hx, pi, P.diag[i] * pi);
+        Ap[i] = s;
+        v = pi * s;
+    }
+    v = block_sum<double, CG_BLK>(v);
+    if (threadIdx.x == 0) partA[blockIdx.x] = v;
+}
+
+__global__ void __launch_bounds__(CG_BLK)
+k_cg_iter_b(int n, int t, const double *__restrict__ p, const double *__restrict__ Ap, double *__restrict__ x,
+            double *__restrict__ r, const double *partA, unsigned na, double *partB, CgCtrl *ctrl)
+{
+    __shared__ double sh_alpha;
+    __shared__ int sh_stop;
+    if (ctrl->done) return;
+    const double pap = fin_sum(partA, na);
+    if (threadIdx.x == 0) {
+        const double rr = ctrl->rr[t & 1];
+        const int stop = fabs(pap) < 1.0e-30;
+        const double alpha = stop ? 0.0 : rr / pap;
+        if (blockIdx.x == 0) {
+            ctrl->pap = pap;
+            if (stop) {
+                ctrl->iters = t;
+                ctrl->breakdown = 1;
+                ctrl->res = sqrt(rr);
+                ctrl->done = 1;
+            } else {
+                ctrl->alpha = alpha;
+            }
+        }
+        sh_alpha = alpha;
+        sh_stop = stop;
+    }
+    __syncthreads();
+    if (sh_stop) return;
+    const double alpha = sh_alpha;
+    double v = 0.0;
+    for (int i = blockIdx.x * CG_BLK + threadIdx.x; i < n; i += gridDim.x * CG_BLK) {
+        x[i] += alpha * p[i];
+        const double ri = r[i] - alpha * Ap[i];
+        r[i] = ri;
+        v += ri * ri;
+    }
+    v = block_sum<double, CG_BLK>(v);
+    if (threadIdx.x == 0) partB[blockIdx.x] = v;
+}
+
 int cg_alloc_ctrl(Ctx *c)
 {
     if (!c->ctrl) {
@@ -399,9 +533,33 @@ static int cg_fp64(Ctx *c, const SolveOpts &o, SolveResult *res)
     hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(CG_BLK), 0, s, c->partB, gv, c->ctrl, o.tol,
                        o.tol_mode, o.max_iter);
     LAUNCH_CHECK();
-    const spmv_fn spmv = pick_spmv(o.spmv_variant);
     const size_t lds = sizeof(double) * (size_t)pk->hmax;
-    int rc = run_chunks(s, o.max_iter, o.check_every, &c->ctrl->done, &c->hctrl->done, [&](int t) {
+    int rc = 0;
+    if (o.fused) {
+        // p ping-pongs: iteration t reads pb[t & 1] (p_{t-1}; b at t = 0)
+        // and writes p_t to pb[(t + 1) & 1]
+        if (!c->pv2) HIP_OK(hipMalloc(&c->pv2, sizeof(double) * (((size_t)c->cap + 63) / 64 * 64)));
+        double *pb[2] = {p, c->pv2};
+        auto iter_a = o.spmv_variant == 1 ? k_cg_iter_a<16> : (o.spmv_variant == 2 ? k_cg_iter_a<4> : k_cg_iter_a<8>);
+        rc = run_chunks(s, o.max_iter, o.check_every, &c->ctrl->done, &c->hctrl->done, [&](int t) {
+            double *prev = pb[t & 1], *cur = pb[(t + 1) & 1];
+            prof_begin(PK_SPMV, s);
+            hipLaunchKernelGGL(iter_a, dim3(g), dim3(CG_BLK), lds, s, n, P, t, o.max_iter, r, prev, cur, Ap,
+                               c->partB, gv, c->partA, c->ctrl);
+            prof_end(PK_SPMV, s);
+            prof_begin(PK_AXPY, s);
+            hipLaunchKernelGGL(k_cg_iter_b, dim3(gv), dim3(CG_BLK), 0, s, n, t, cur, Ap, x, r, c->partA, g,
+                               c->partB, c->ctrl);
+            prof_end(PK_AXPY, s);
+        });
+        if (rc) return rc;
+        // the last iteration's stop tests when the loop ran to max_iter
+        hipLaunchKernelGGL(k_axpy_fin, dim3(1), dim3(CG_BLK), 0, s, c->partB, gv, c->ctrl, o.max_iter - 1,
+                           o.max_iter);
+        LAUNCH_CHECK();
+    } else {
+    const spmv_fn spmv = pick_spmv(o.spmv_variant);
+    rc = run_chunks(s, o.max_iter, o.check_every, &c->ctrl->done, &c->hctrl->done, [&](int t) {
         prof_begin(PK_SPMV, s);
         hipLaunchKernelGGL(spmv, dim3(g), dim3(CG_BLK), lds, s, n, P, p, Ap, c->partA, c->ctrl);
         prof_end(PK_SPMV, s);
@@ -415,6 +573,7 @@ static int cg_fp64(Ctx *c, const SolveOpts &o, SolveResult *res)
         prof_end(PK_PUPDATE, s);
     });
     if (rc) return rc;
+    }
     HIP_OK(hipStreamSynchronize(s));
     HIP_OK(hipMemcpy(c->hctrl, c->ctrl, sizeof(CgCtrl), hipMemcpyDeviceToHost));
     res->iters = c->hctrl->done ? c->hctrl->iters : o.max_iter;
@@ -507,12 +666,12 @@ k_mx_spmv_pk(int n, PkView P, const V *__restrict__ p, V *__restrict__ Ap, doubl
     extern __shared__ unsigned char smem_raw[];
     V *hx = reinterpret_cast<V *>(smem_raw);
     if (c->done) return;
-    stage_halo(P, p, hx);
+    stage_halo(P, XPlain<V>{p}, hx);
     const int i = blockIdx.x * CG_BLK + threadIdx.x;
     double v = 0.0;
     if (i < n) {
         const V pi = p[i];
-        const V s = packed_row<U, true, V>(P, P.fval, i, p, hx, pi, (V)P.ddir[i] * pi);
+        const V s = packed_row<U, true, V>(P, P.fval, i, XPlain<V>{p}, hx, pi, (V)P.ddir[i] * pi);
         Ap[i] = s;
         v = (double)pi * (double)s;
     }

@@ -33,7 +33,7 @@ void mps_set_error(int code, const char *file, int line, const char *msg)
 
 struct Options {
     double tol_mode = 0, precision = 0, inner_tol = 1e-3, exit_on_error = 1, check_every = 16;
-    double spmv = 0, profile = 0, verbose = 0;
+    double spmv = 0, profile = 0, verbose = 0, fused = 0;
 };
 static Options g_opt;
 
@@ -169,7 +169,7 @@ void ctx_free(Ctx *c)
     DFREE(c->unk); DFREE(c->pmin); DFREE(c->cell); DFREE(c->ccount); DFREE(c->cstart);
     DFREE(c->cfill); DFREE(c->tmp_key); DFREE(c->sflag); DFREE(c->sscan); DFREE(c->scan_blk);
     DFREE(c->nbr); DFREE(c->ncnt); DFREE(c->diag); DFREE(c->b); DFREE(c->Aval); DFREE(c->Acol);
-    DFREE(c->Alen); DFREE(c->x); DFREE(c->r); DFREE(c->pv); DFREE(c->Ap); DFREE(c->partA);
+    DFREE(c->Alen); DFREE(c->x); DFREE(c->r); DFREE(c->pv); DFREE(c->pv2); DFREE(c->Ap); DFREE(c->partA);
     DFREE(c->partB); DFREE(c->fAval); DFREE(c->ddir); DFREE(c->fr); DFREE(c->fp); DFREE(c->fAp);
     DFREE(c->stage); DFREE(c->nl_stage); DFREE(c->cnt_orig); DFREE(c->off_orig);
     DFREE(c->counters);
@@ -538,6 +538,7 @@ static int run_step(Ctx *c, const SimConfig *cf)
     so.check_every = (int)g_opt.check_every;
     so.dense_ic = (D_ == 2);
     so.spmv_variant = (int)g_opt.spmv;
+    so.fused = (int)g_opt.fused;
     SolveResult sr{};
     rc = solve_ppe(c, so, &sr);
     if (rc) return rc;
@@ -616,6 +617,7 @@ int mps_hip_set_option(const char *key, double value)
     else if (k == "exit_on_error") g_opt.exit_on_error = value;
     else if (k == "check_every") g_opt.check_every = value;
     else if (k == "spmv") g_opt.spmv = value;
+    else if (k == "fused") g_opt.fused = value;
     else if (k == "profile") g_opt.profile = value;
     else if (k == "verbose") g_opt.verbose = value;
     else return MPS_ERR_ARG;
@@ -632,6 +634,7 @@ double mps_hip_get_option(const char *key)
     if (k == "exit_on_error") return g_opt.exit_on_error;
     if (k == "check_every") return g_opt.check_every;
     if (k == "spmv") return g_opt.spmv;
+    if (k == "fused") return g_opt.fused;
     if (k == "profile") return g_opt.profile;
     if (k == "verbose") return g_opt.verbose;
     return NAN;

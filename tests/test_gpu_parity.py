@@ -250,3 +250,25 @@ def test_spmv_variants_same_bits(lib3):
     finally:
         lib3.set_option("spmv", 0)
     assert all(o == out[0] for o in out)
+
+
+def test_fused_cg_same_bits(lib3):
+    """The two-kernel CG iteration (previous iteration closed inside the
+    SpMV, p formed where it is read) gives the five-kernel form's bits."""
+    dim, cfg, pos, vel, typ = H.build("dam3d_mid", cg_tolerance=1e-10)
+    lib3.set_option("tol_mode", 1)
+    out = []
+    try:
+        for f in (0, 1):
+            lib3.set_option("fused", f)
+            say("fused", f)
+            g = H.GpuRun(lib3, cfg, pos, vel, typ)
+            for _ in range(2):
+                assert g.step() == 0
+            st = lib3.stats(g.scene)
+            out.append((np.ascontiguousarray(g.scene.arr["pressure"]).tobytes(), st.last_iters,
+                        st.last_residual))
+            g.close()
+    finally:
+        lib3.set_option("fused", 0)
+    assert out[0] == out[1]
