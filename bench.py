@@ -99,18 +99,95 @@ def cpu_baseline(w, tol, seconds_hint=30.0):
                        "mean %.0f iterations/step, %.1f s" % (steps, len(typ), iters / steps, el))
 
 
-def profiled_traffic(precision, workload):
+def profiled_traffic(precision, workload, nnz):
     """HBM bytes per launch of the solver's SpMV, as measured by rocprofv3
     (FETCH_SIZE + WRITE_SIZE scaled by the plain-copy calibration) and
-    recorded in profiles/cg_<precision>_<workload>.md; None if absent."""
+    recorded in profiles/cg_<precision>_<workload>.md for a matrix of
+    traffic_nnz nonzeros; scaled to this run's matrix by its nonzero count
+    (the traffic is proportional to it).  None if no profile is recorded."""
     path = os.path.join(ROOT, "profiles", "cg_%s_%s.md" % (precision, workload))
+    t = z = None
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch"):
-                return int(line.split("=")[1]), os.path.relpath(path, ROOT)
+                t = int(line.split("=")[1])
+            elif line.startswith("traffic_nnz"):
+                z = int(line.split("=")[1])
     except OSError:
         pass
-    return None, None
+    if t is None or not z:
+        return None, None
+    return int(round(t * nnz / z)), os.path.relpath(path, ROOT)
+
+
+def _fields(lib, ps, n):
+    arr = np.ctypeslib.as_array(C.cast(ps.contents.particles, C.POINTER(lib.abi.Particle)), shape=(n,))
+    return np.array(arr["pressure"]), np.array(arr["pos"])
+
+
+def compare_forms(lib, ps, n, w, a):
+    """The same scene, from the same initial parameters, resident for
+    --steps steps in FP64 and in the mixed form: particle-steps/s of both
+    and the relative L2 differences of pressure and positions at the end."""
+    mixed = {"fp64": 1, "mixed": 1, "mixed32": 2}[a.precision]
+    init = {f: getattr(ps.contents, f) for f in ("n0", "lambda_", "C_LL", "C_SL", "n0_count")}
+    out = {}
+    for name, prec in (("fp64", 0), (a.precision if a.precision != "fp64" else "mixed", mixed)):
+        q, _ = build_scene(lib, w)
+        for f, v in init.items():
+            setattr(q.contents, f, v)
+        lib.set_option("precision", prec)
+        lib.check(lib.mps_hip_run_steps(q, 1), "first step")
+        t0 = time.perf_counter()
+        lib.check(lib.mps_hip_run_steps(q, a.steps - 1), "steps")
+        el = time.perf_counter() - t0
+        lib.check(lib.mps_hip_download(q, None), "download")
+        st = lib.stats(q)
+        out[name] = dict(rate=n * (a.steps - 1) / el, ms=1e3 * el / (a.steps - 1),
+                         iters=st.total_iters / max(st.steps, 1), fields=_fields(lib, q, n))
+        lib.mps_hip_release(q)
+        lib.particle_system_free(q)
+    names = list(out)
+    (p0, x0), (p1, x1) = out[names[0]]["fields"], out[names[1]]["fields"]
+    rel = lambda u, v: float(np.linalg.norm(u - v) / max(np.linalg.norm(v), 1e-300))
+    print(json.dumps({"metric": "mixed vs FP64, same scene", "workload": a.workload, "particles": n,
+                      "steps": a.steps, "forms": {k: {"particle_steps_per_s": round(v["rate"], 1),
+                                                      "ms_per_step": round(v["ms"], 3),
+                                                      "mean_iters_per_step": round(v["iters"], 1)}
+                                                  for k, v in out.items()},
+                      "speedup": round(out[names[1]]["rate"] / out[names[0]]["rate"], 3),
+                      "rel_l2_pressure": rel(p1, p0), "rel_l2_position": rel(x1, x0)}), flush=True)
+
+
+def sync_rate(lib, ps, n, cfg, a):
+    """Throughput of the drop-in simulation_step (simulation.c:19-118) as
+    the reference's own loop calls it: host ParticleSystem and padded
+    NeighborList in, the step on the GPU, both back out."""
+    dmin = (C.c_double * 3)(*[cfg.domain_min[d] for d in range(3)])
+    dmax = (C.c_double * 3)(*[cfg.domain_max[d] for d in range(3)])
+    re = max(cfg.influence_radius_lap, cfg.influence_radius_n)
+    nl = lib.neighbor_list_create(n, cfg.max_neighbors)
+    cl = lib.cell_list_create(n, re, dmin, dmax)
+    lib.neighbor_search_cell_linked_list(nl, ps, cl, re)
+    step = 0
+    for _ in range(a.warmup):
+        step += 1
+        lib.simulation_step(ps, nl, cl, step)
+    lib.check(lib.mps_hip_last_error(), "warm-up steps")
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step += 1
+        lib.simulation_step(ps, nl, cl, step)
+    el = time.perf_counter() - t0
+    lib.check(lib.mps_hip_last_error(), "timed steps")
+    st = lib.stats(ps)
+    print(json.dumps({"metric": "particle-steps/s, drop-in simulation_step with host copies each step",
+                      "value": round(n * a.steps / el, 1), "unit": "particle-steps/s",
+                      "ms_per_step": round(1e3 * el / a.steps, 3), "steps": a.steps, "warmup": a.warmup,
+                      "config": {"workload": a.workload, "particles": n, "max_neighbors": cfg.max_neighbors,
+                                 "precision": a.precision, "last_iters": st.last_iters}}), flush=True)
+    lib.neighbor_list_free(nl)
+    lib.cell_list_free(cl)
 
 
 def calibrate_copy(nbytes=1 << 30, reps=5):
@@ -143,6 +220,12 @@ def main():
     ap.add_argument("--spmv", type=int, default=0, help="SpMV kernel variant of the FP64 CG (0-4)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--sync", action="store_true",
+                    help="time the reference-style per-step drop-in calls (simulation_step with the "
+                         "state and list copied in and out each step), PCIe included; not the headline")
+    ap.add_argument("--compare", action="store_true",
+                    help="run the scene in FP64 and in the chosen mixed form for --steps steps each and "
+                         "report both rates and the relative L2 error of pressure and positions")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy 5 times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
@@ -168,6 +251,12 @@ def main():
     print("bench: %s, %d particles, solver %s, %s" % (a.workload, n, a.solver, a.precision),
           file=sys.stderr, flush=True)
     lib.particle_system_calc_initial_params(ps)
+    if a.compare:
+        compare_forms(lib, ps, n, w, a)
+        return
+    if a.sync:
+        sync_rate(lib, ps, n, cfg, a)
+        return
     lib.check(lib.mps_hip_run_steps(ps, a.warmup), "warm-up steps")
     # the timed steps time only the SpMV launches (two HIP events each)
     lib.set_option("profile", 0 if a.no_profile else 2)
@@ -192,7 +281,7 @@ def main():
         byt = {"fp64": spmv_bytes(st, 8, 8), "mixed": spmv_bytes(st, 4, 8),
                "mixed32": spmv_bytes(st, 4, 4)}[a.precision]
         ach = byt / (spmv_ms * 1e-3) / 1e9
-        traffic, tsrc = profiled_traffic(a.precision, a.workload)
+        traffic, tsrc = profiled_traffic(a.precision, a.workload, st.nnz)
         roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(ach / HBM_PEAK_GBS, 4), traffic=traffic, traffic_source=tsrc,
                     kernel="k_spmv_pk" if a.precision == "fp64" else "k_mx_spmv_pk", bytes_per_launch=int(byt),
