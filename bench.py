@@ -41,6 +41,8 @@ WORKLOADS = {
     "dam3d_l0.005": dict(dims=(1.0, 0.6, 0.5, 0.3, 0.5, 0.5), l0=0.005, dt=5e-4),
     "dam3d_l0.0035": dict(dims=(1.0, 0.6, 0.5, 0.3, 0.5, 0.5), l0=0.0035, dt=5e-4),
     "dam3d_l0.01": dict(dims=(1.0, 0.6, 0.5, 0.3, 0.5, 0.5), l0=0.01, dt=5e-4),
+    # 64 seeded dam breaks of uneven size and viscosity (mps_amd.params.ensemble)
+    "ensemble64": dict(ensemble=64),
 }
 
 
@@ -118,6 +120,81 @@ def profiled_traffic(precision, workload, nnz):
     if t is None or not z:
         return None, None
     return int(round(t * nnz / z)), os.path.relpath(path, ROOT)
+
+
+def run_ensemble(lib, a):
+    """The ensemble workload: every scene resident, mps_hip_run_batch for
+    W + K steps (K timed).  With --sequential the same scenes also run one
+    after another through mps_hip_run_steps (the single-scene path), timed
+    the same way, and the final states of the two runs are compared bit for
+    bit."""
+    from mps_amd import params
+    ws = params.ensemble(WORKLOADS[a.workload]["ensemble"])
+
+    def make():
+        out = []
+        for w in ws:
+            cfg = params.bench_config(lib, w, 1 if a.solver == "iccg" else 0, a.tol)
+            ps, n = build_scene(lib, w)
+            lib.set_config(cfg)
+            lib.particle_system_calc_initial_params(ps)
+            out.append((cfg, ps, n))
+        return out
+
+    scenes = make()
+    total = sum(m[2] for m in scenes)
+    print("bench: %d scenes, %d particles (%d-%d per scene)" % (len(scenes), total, min(m[2] for m in scenes),
+                                                                  max(m[2] for m in scenes)), file=sys.stderr, flush=True)
+    PSP = C.POINTER(lib.abi.ParticleSystem)
+    ps_arr = (PSP * len(scenes))(*[m[1] for m in scenes])
+    cf_arr = (C.POINTER(lib.abi.SimConfig) * len(scenes))(*[C.pointer(m[0]) for m in scenes])
+    lib.check(lib.mps_hip_run_batch(ps_arr, cf_arr, len(scenes), a.warmup), "warm-up steps")
+    t0 = time.perf_counter()
+    lib.check(lib.mps_hip_run_batch(ps_arr, cf_arr, len(scenes), a.steps), "timed steps")
+    el = time.perf_counter() - t0
+    iters = [lib.stats(m[1]).total_iters / max(lib.stats(m[1]).steps, 1) for m in scenes]
+    seq = None
+    if a.sequential:
+        final = []
+        for cfg, ps, n in scenes:
+            lib.check(lib.mps_hip_download(ps, None), "download")
+            final.append(_fields(lib, ps, n))
+        seq_t = 0.0
+        equal = True
+        for q, (cfg, ps, n) in enumerate(make()):
+            lib.set_config(cfg)
+            lib.check(lib.mps_hip_run_steps(ps, a.warmup), "warm-up steps")
+            t1 = time.perf_counter()
+            lib.check(lib.mps_hip_run_steps(ps, a.steps), "timed steps")
+            seq_t += time.perf_counter() - t1
+            lib.check(lib.mps_hip_download(ps, None), "download")
+            p1, x1 = _fields(lib, ps, n)
+            equal = equal and p1.tobytes() == final[q][0].tobytes() and x1.tobytes() == final[q][1].tobytes()
+            lib.mps_hip_release(ps)
+            lib.particle_system_free(ps)
+        seq = {"value": round(total * a.steps / seq_t, 1), "ms_per_step": round(1e3 * seq_t / a.steps, 3),
+               "batch_speedup": round(seq_t / el, 3), "bitwise_equal_to_batch": bool(equal)}
+    cpu = None
+    if not a.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(ws[0], a.tol)
+            cpu["sample"] = "scene 0 of the ensemble: " + cpu["sample"]
+        except Exception as e:  # baseline is reported, never required
+            cpu = dict(error=str(e))
+    print(json.dumps({
+        "metric": "particle-steps/s, ensemble of independent 3D dam breaks",
+        "value": round(total * a.steps / el, 1), "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(1e3 * el / a.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (reference dam-break lattice generator, seeded column sizes and viscosities)",
+        "config": {"workload": a.workload, "scenes": len(scenes), "particles": total, "l0": ws[0]["l0"],
+                   "dt": ws[0]["dt"], "seed": params.ENSEMBLE_SEED,
+                   "tolerance": "relative %g" % a.tol, "mean_iters_per_step": round(sum(iters) / len(iters), 1),
+                   "max_iters_per_step": round(max(iters), 1), "parallelism": "1 GPU, batched CG"},
+        "sequential": seq, "cpu_baseline": cpu}), flush=True)
+    for cfg, ps, n in scenes:
+        lib.mps_hip_release(ps)
+        lib.particle_system_free(ps)
 
 
 def _fields(lib, ps, n):
@@ -226,6 +303,8 @@ def main():
     ap.add_argument("--compare", action="store_true",
                     help="run the scene in FP64 and in the chosen mixed form for --steps steps each and "
                          "report both rates and the relative L2 error of pressure and positions")
+    ap.add_argument("--sequential", action="store_true",
+                    help="ensemble workloads: also time the scenes one after another and compare the bits")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy 5 times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
@@ -241,6 +320,11 @@ def main():
     w = WORKLOADS[a.workload]
     lib = mps_amd.load(3)
     lib.check(lib.mps_hip_init(0), "mps_hip_init")
+    if "ensemble" in w:
+        lib.set_option("tol_mode", 1)
+        lib.set_option("exit_on_error", 0)
+        run_ensemble(lib, a)
+        return
     cfg = make_config(lib, w, 1 if a.solver == "iccg" else 0, a.tol)
     lib.set_config(cfg)
     lib.set_option("tol_mode", 1)

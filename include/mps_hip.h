@@ -175,6 +175,17 @@ void mps_hip_release(ParticleSystem *ps);
  * first if ps has no device state).  No host copies inside. */
 int  mps_hip_run_steps(ParticleSystem *ps, int nsteps);
 
+/* Ensemble of independent scenes (no counterpart in the reference, which
+ * runs one scene per process): nsteps timesteps of every ps[i] under its
+ * own configuration cfgs[i] (viscosity, geometry, ... may differ; same
+ * dimensionality), state resident on the GPU.  The particle operators run
+ * per scene; the FP64 CG runs batched -- one launch per CG phase for all
+ * scenes, per-scene reductions and stop tests -- and gives every scene
+ * the bits of its own single-scene solve.  Scenes that use ICCG or a mixed
+ * precision are solved one by one.  ps[i] stays resident: read it back with
+ * mps_hip_download.  g_config is left as it was. */
+int  mps_hip_run_batch(ParticleSystem *const *ps, SimConfig *const *cfgs, int nscenes, int nsteps);
+
 /* Options (process-wide): keys and values
  *   "tol_mode"       0 absolute sqrt(r.r) < tol (reference), 1 relative
  *                      sqrt(r.r) < tol*sqrt(b.b)

@@ -209,11 +209,17 @@ __device__ __forceinline__ double packed_row_global(const PkView &P, const VT *v
 // Stages the block's halo of x in LDS (hx); every thread of the block
 // takes part and the values are visible to all on return.
 template <typename XT, typename XA>
-__device__ __forceinline__ void stage_halo(const PkView &P, XA x, XT *hx)
+__device__ __forceinline__ void stage_halo_of(const PkView &P, int blk, XA x, XT *hx)
 {
-    const int2 bm = P.bmeta[blockIdx.x];
+    const int2 bm = P.bmeta[blk];
     for (int h = threadIdx.x; h < bm.y; h += CG_BLK) hx[h] = x(P.halo[bm.x + h]);
     __syncthreads();
+}
+
+template <typename XT, typename XA>
+__device__ __forceinline__ void stage_halo(const PkView &P, XA x, XT *hx)
+{
+    stage_halo_of(P, (int)blockIdx.x, x, hx);
 }
 
 // ---------------------------------------------------------------------
@@ -885,4 +891,317 @@ int solve_ppe(Ctx *c, const SolveOpts &o, SolveResult *res)
     rc = mixed ? cg_mixed(c, o, res) : cg_fp64(c, o, res);
     if (rc) return rc;
     return unpack_solution(c);
+}
+
+// ---------------------------------------------------------------------
+// Batched FP64 CG: many independent systems (one per scene of an
+// ensemble) iterated together, one launch per phase for all of them.
+// Block b of a phase belongs to scene bs[b] and is that scene's local
+// block b - off; each scene keeps exactly the block partition of its own
+// single-scene solve (SpMV: one block per 256 rows; vector kernels:
+// vec_grid(n_eq) blocks, grid-stride) and its finalisers sum the same
+// partials in the same order, so every scene's iterates are the bits of
+// its single-scene CG.  A scene whose solve has stopped masks its blocks
+// out; ndone counts the stopped scenes for the host.
+// ---------------------------------------------------------------------
+
+struct BScene;
+typedef void (*spmv_fn_b)(const BScene *, const int *, double *, const CgCtrl *);
+
+__global__ void k_fill_zero_int(int *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+struct BScene {
+    PkView P;
+    const double *b;
+    double *x, *r, *p, *Ap;
+    int n_eq;
+    int blk_off, nblk;       // SpMV blocks
+    int ax_off, nax;         // vector-kernel blocks
+    int max_iter, tol_mode;
+    double tol;
+};
+
+__global__ void __launch_bounds__(CG_BLK)
+k_b_init(const BScene *__restrict__ sc, const int *__restrict__ axs, double *part2)
+{
+    const BScene &S = sc[axs[blockIdx.x]];
+    const int j = blockIdx.x - S.ax_off;
+    double v = 0.0;
+    for (int i = j * CG_BLK + threadIdx.x; i < S.n_eq; i += S.nax * CG_BLK) {
+        const double bi = S.b[i];
+        S.x[i] = 0.0;
+        S.r[i] = bi;
+        S.p[i] = bi;
+        v += bi * bi;
+    }
+    v = block_sum<double, CG_BLK>(v);
+    if (threadIdx.x == 0) part2[blockIdx.x] = v;
+}
+
+__global__ void __launch_bounds__(CG_BLK)
+k_b_init_fin(const BScene *__restrict__ sc, const double *part2, CgCtrl *ctrl, int *ndone)
+{
+    const BScene &S = sc[blockIdx.x];
+    CgCtrl *c = ctrl + blockIdx.x;
+    const double bb = fin_sum(part2 + S.ax_off, S.nax);
+    if (threadIdx.x == 0) {
+        c->rr[0] = bb;
+        c->bnorm = sqrt(bb);
+        c->stop = S.tol_mode ? S.tol * sqrt(bb) : S.tol;
+        c->done = 0;
+        c->iters = 0;
+        c->breakdown = 0;
+        c->res = sqrt(bb);
+        if (S.max_iter <= 0 || sqrt(bb) < c->stop) {
+            c->done = 1;
+            atomicAdd(ndone, 1);
+        }
+    }
+}
+
+template <int U>
+__global__ void __launch_bounds__(CG_BLK)
+k_b_spmv(const BScene *__restrict__ sc, const int *__restrict__ bs, double *part, const CgCtrl *ctrl)
+{
+    extern __shared__ unsigned char smem_raw[];
+    double *hx = reinterpret_cast<double *>(smem_raw);
+    const int si = bs[blockIdx.x];
+    if (ctrl[si].done) return;
+    const BScene &S = sc[si];
+    const int lb = blockIdx.x - S.blk_off;
+    const double *p = S.p;
+    stage_halo_of(S.P, lb, XPlain<double>{p}, hx);
+    const int i = lb * CG_BLK + threadIdx.x;
+    double v = 0.0;
+    if (i < S.n_eq) {
+        const double pi = p[i];
+        const double s = packed_row<U, false, double>(S.P, S.P.val, i, XPlain<double>{p}, hx, pi, S.P.diag[i] * pi);
+        S.Ap[i] = s;
+        v = pi * s;
+    }
+    v = block_sum<double, CG_BLK>(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+__global__ void __launch_bounds__(CG_BLK)
+k_b_spmv_fin(const BScene *__restrict__ sc, const double *part, CgCtrl *ctrl, int t, int *ndone)
+{
+    CgCtrl *c = ctrl + blockIdx.x;
+    if (c->done) return;
+    const BScene &S = sc[blockIdx.x];
+    const double pap = fin_sum(part + S.blk_off, S.nblk);
+    if (threadIdx.x == 0) {
+        c->pap = pap;
+        if (fabs(pap) < 1.0e-30) {
+            c->done = 1;
+            c->iters = t;
+            c->breakdown = 1;
+            c->res = sqrt(c->rr[t & 1]);
+            atomicAdd(ndone, 1);
+        } else {
+            c->alpha = c->rr[t & 1] / pap;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(CG_BLK)
+k_b_axpy(const BScene *__restrict__ sc, const int *__restrict__ axs, double *part2, const CgCtrl *ctrl)
+{
+    const int si = axs[blockIdx.x];
+    if (ctrl[si].done) return;
+    const BScene &S = sc[si];
+    const int j = blockIdx.x - S.ax_off;
+    const double alpha = ctrl[si].alpha;
+    double v = 0.0;
+    for (int i = j * CG_BLK + threadIdx.x; i < S.n_eq; i += S.nax * CG_BLK) {
+        S.x[i] += alpha * S.p[i];
+        const double ri = S.r[i] - alpha * S.Ap[i];
+        S.r[i] = ri;
+        v += ri * ri;
+    }
+    v = block_sum<double, CG_BLK>(v);
+    if (threadIdx.x == 0) part2[blockIdx.x] = v;
+}
+
+__global__ void __launch_bounds__(CG_BLK)
+k_b_axpy_fin(const BScene *__restrict__ sc, const double *part2, CgCtrl *ctrl, int t, int *ndone)
+{
+    CgCtrl *c = ctrl + blockIdx.x;
+    if (c->done) return;
+    const BScene &S = sc[blockIdx.x];
+    const double rr_new = fin_sum(part2 + S.ax_off, S.nax);
+    if (threadIdx.x == 0) {
+        const double rr = c->rr[t & 1];
+        c->rr[(t + 1) & 1] = rr_new;
+        c->res = sqrt(rr_new);
+        int stop = 0;
+        if (rr < 1.0e-60) {
+            stop = 1;
+            c->iters = t;
+        } else {
+            c->beta = rr_new / rr;
+            if (t + 1 >= S.max_iter) {
+                stop = 1;
+                c->iters = S.max_iter;
+            } else if (sqrt(rr_new) < c->stop) {
+                stop = 1;
+                c->iters = t + 1;
+            }
+        }
+        if (stop) {
+            c->done = 1;
+            atomicAdd(ndone, 1);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(CG_BLK)
+k_b_pupdate(const BScene *__restrict__ sc, const int *__restrict__ axs, const CgCtrl *ctrl)
+{
+    const int si = axs[blockIdx.x];
+    if (ctrl[si].done) return;
+    const BScene &S = sc[si];
+    const int j = blockIdx.x - S.ax_off;
+    const double beta = ctrl[si].beta;
+    for (int i = j * CG_BLK + threadIdx.x; i < S.n_eq; i += S.nax * CG_BLK) S.p[i] = S.r[i] + beta * S.p[i];
+}
+
+struct BatchBufs {
+    int cap_sc = 0, cap_blk = 0, cap_ax = 0;
+    int cap_ctrl = 0, cap_part = 0, cap_part2 = 0;
+    BScene *sc = nullptr, *hsc = nullptr;
+    int *bs = nullptr, *hbs = nullptr, *axs = nullptr, *haxs = nullptr;
+    double *part = nullptr, *part2 = nullptr;
+    CgCtrl *ctrl = nullptr, *hctrl = nullptr;
+    int *ndone = nullptr, *hndone = nullptr;
+};
+static BatchBufs g_batch;
+
+template <typename T>
+static int reserve_pair(T **d, T **h, int *cap, int need)
+{
+    if (*cap >= need) return 0;
+    if (*d) hipFree(*d);
+    if (*h) hipHostFree(*h);
+    *d = nullptr;
+    *h = nullptr;
+    const int n = need + need / 4 + 16;
+    HIP_OK(hipMalloc((void **)d, sizeof(T) * n));
+    HIP_OK(hipHostMalloc((void **)h, sizeof(T) * n, hipHostMallocDefault));
+    *cap = n;
+    return 0;
+}
+
+// Solves the packed systems of cs[0..ns) (each already packed by pack_ppe)
+// with FP64 CG on stream s; solutions stay in each scene's packed x.
+int batch_cg(Ctx *const *cs, int ns, const SolveOpts *opts, SolveResult *res, hipStream_t s)
+{
+    BatchBufs &B = g_batch;
+    int tot_blk = 0, tot_ax = 0, hmax = 0, max_iter = 0, check_every = 16;
+    for (int k = 0; k < ns; k++) {
+        const int n = cs[k]->pk->n_eq;
+        tot_blk += (int)cg_grid(n);
+        tot_ax += n > 0 ? (int)vec_grid(n) : 0;
+        if (cs[k]->pk->hmax > hmax) hmax = cs[k]->pk->hmax;
+        if (opts[k].max_iter > max_iter) max_iter = opts[k].max_iter;
+        if (opts[k].check_every > 0) check_every = opts[k].check_every;
+    }
+    int rc = reserve_pair(&B.sc, &B.hsc, &B.cap_sc, ns);
+    if (!rc) rc = reserve_pair(&B.bs, &B.hbs, &B.cap_blk, tot_blk + 1);
+    if (!rc) rc = reserve_pair(&B.axs, &B.haxs, &B.cap_ax, tot_ax + 1);
+    if (rc) return rc;
+    if (!rc) rc = reserve_pair(&B.ctrl, &B.hctrl, &B.cap_ctrl, ns);
+    if (rc) return rc;
+    if (B.cap_part < B.cap_blk) {
+        if (B.part) hipFree(B.part);
+        B.part = nullptr;
+        HIP_OK(hipMalloc(&B.part, sizeof(double) * B.cap_blk));
+        B.cap_part = B.cap_blk;
+    }
+    if (B.cap_part2 < B.cap_ax) {
+        if (B.part2) hipFree(B.part2);
+        B.part2 = nullptr;
+        HIP_OK(hipMalloc(&B.part2, sizeof(double) * B.cap_ax));
+        B.cap_part2 = B.cap_ax;
+    }
+    if (!B.ndone) {
+        HIP_OK(hipMalloc(&B.ndone, sizeof(int)));
+        HIP_OK(hipHostMalloc(&B.hndone, sizeof(int), hipHostMallocDefault));
+    }
+    int bo = 0, ao = 0;
+    for (int k = 0; k < ns; k++) {
+        Ctx *c = cs[k];
+        const Packed *pk = c->pk;
+        BScene &S = B.hsc[k];
+        S.P = pk_view(pk);
+        S.b = pk->b;
+        S.x = pk->x;
+        S.r = c->r;
+        S.p = c->pv;
+        S.Ap = c->Ap;
+        S.n_eq = pk->n_eq;
+        S.blk_off = bo;
+        S.nblk = (int)cg_grid(pk->n_eq);
+        S.ax_off = ao;
+        S.nax = pk->n_eq > 0 ? (int)vec_grid(pk->n_eq) : 0;
+        S.max_iter = opts[k].max_iter;
+        S.tol_mode = opts[k].tol_mode;
+        S.tol = opts[k].tol;
+        for (int q = 0; q < S.nblk; q++) B.hbs[bo + q] = k;
+        for (int q = 0; q < S.nax; q++) B.haxs[ao + q] = k;
+        bo += S.nblk;
+        ao += S.nax;
+    }
+    HIP_OK(hipMemcpyAsync(B.sc, B.hsc, sizeof(BScene) * ns, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(B.bs, B.hbs, sizeof(int) * (tot_blk + 1), hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(B.axs, B.haxs, sizeof(int) * (tot_ax + 1), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_fill_zero_int, dim3(1), dim3(64), 0, s, B.ndone, 1);
+    if (tot_ax > 0) hipLaunchKernelGGL(k_b_init, dim3(tot_ax), dim3(CG_BLK), 0, s, B.sc, B.axs, B.part2);
+    hipLaunchKernelGGL(k_b_init_fin, dim3(ns), dim3(CG_BLK), 0, s, B.sc, B.part2, B.ctrl, B.ndone);
+    LAUNCH_CHECK();
+    const size_t lds = sizeof(double) * (size_t)hmax;
+    const spmv_fn_b spmv = opts[0].spmv_variant == 1 ? k_b_spmv<16> : (opts[0].spmv_variant == 2 ? k_b_spmv<4> : k_b_spmv<8>);
+    // the host stops when every scene has stopped: watch ndone == ns
+    hipEvent_t ev[2];
+    HIP_OK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    HIP_OK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    int pending = -1, t = 0, k = 0;
+    while (t < max_iter) {
+        const int t_end = (t + check_every < max_iter) ? t + check_every : max_iter;
+        for (; t < t_end; t++) {
+            prof_begin(PK_SPMV, s);
+            if (tot_blk > 0) hipLaunchKernelGGL(spmv, dim3(tot_blk), dim3(CG_BLK), lds, s, B.sc, B.bs, B.part, B.ctrl);
+            prof_end(PK_SPMV, s);
+            hipLaunchKernelGGL(k_b_spmv_fin, dim3(ns), dim3(CG_BLK), 0, s, B.sc, B.part, B.ctrl, t, B.ndone);
+            if (tot_ax > 0) hipLaunchKernelGGL(k_b_axpy, dim3(tot_ax), dim3(CG_BLK), 0, s, B.sc, B.axs, B.part2, B.ctrl);
+            hipLaunchKernelGGL(k_b_axpy_fin, dim3(ns), dim3(CG_BLK), 0, s, B.sc, B.part2, B.ctrl, t, B.ndone);
+            if (tot_ax > 0) hipLaunchKernelGGL(k_b_pupdate, dim3(tot_ax), dim3(CG_BLK), 0, s, B.sc, B.axs, B.ctrl);
+        }
+        LAUNCH_CHECK();
+        HIP_OK(hipMemcpyAsync(B.hndone, B.ndone, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipEventRecord(ev[k & 1], s));
+        if (pending >= 0) {
+            HIP_OK(hipEventSynchronize(ev[pending & 1]));
+            if (*(volatile int *)B.hndone >= ns) break;
+        }
+        pending = k;
+        k++;
+    }
+    hipEventDestroy(ev[0]);
+    hipEventDestroy(ev[1]);
+    HIP_OK(hipStreamSynchronize(s));
+    HIP_OK(hipMemcpy(B.hctrl, B.ctrl, sizeof(CgCtrl) * ns, hipMemcpyDeviceToHost));
+    for (int q = 0; q < ns; q++) {
+        const CgCtrl &c = B.hctrl[q];
+        res[q].iters = c.done ? c.iters : max_iter;
+        res[q].breakdown = c.breakdown;
+        res[q].res = c.res;
+        res[q].bnorm = c.bnorm;
+        res[q].outer = 1;
+    }
+    return 0;
 }

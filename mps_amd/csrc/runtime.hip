@@ -510,7 +510,10 @@ static int search_only(Ctx *c, const StepConsts &k)
 
 int collide_2d(Ctx *c, const StepConsts &k);   // k_particles.hip
 
-static int run_step(Ctx *c, const SimConfig *cf)
+// The timestep up to the assembled PPE (simulation.c:31-96 before
+// solve_pressure): predictor, collision, sort, search with density and
+// flags, assembly.
+static int step_pre(Ctx *c, const SimConfig *cf, StepConsts *kout)
 {
     StepConsts k = make_consts(cf, c->ps, c->grid, search_radius(cf));
     k.stride = c->stride;
@@ -528,6 +531,12 @@ static int run_step(Ctx *c, const SimConfig *cf)
     if (!rc) rc = launch_assemble(c, k, c->counters);
     if (rc) return rc;
     c->list_valid = true;
+    *kout = k;
+    return 0;
+}
+
+static SolveOpts solve_opts(const SimConfig *cf, const StepConsts &k)
+{
     SolveOpts so{};
     so.solver = cf->solver_type;
     so.max_iter = cf->cg_max_iter;
@@ -539,10 +548,15 @@ static int run_step(Ctx *c, const SimConfig *cf)
     so.dense_ic = (D_ == 2);
     so.spmv_variant = (int)g_opt.spmv;
     so.fused = (int)g_opt.fused;
-    SolveResult sr{};
-    rc = solve_ppe(c, so, &sr);
-    if (rc) return rc;
-    rc = launch_pressure_out(c, k, 0);
+    (void)k;
+    return so;
+}
+
+// The timestep after the solve (simulation.c:96-118): pressure, gradient
+// corrector, boundary, counters and statistics.
+static int step_post(Ctx *c, const StepConsts &k, const SolveResult &sr)
+{
+    int rc = launch_pressure_out(c, k, 0);
     if (!rc) rc = launch_grad_correct(c, k);
     if (!rc) rc = fetch_counters(c);
     if (rc) return rc;
@@ -559,6 +573,39 @@ static int run_step(Ctx *c, const SimConfig *cf)
     c->stats.max_count = c->hcounters->max_count;
     if (g_opt.profile > 0.5) prof_collect();
     return check_overflow(c, k.stride);
+}
+
+static int run_step(Ctx *c, const SimConfig *cf)
+{
+    StepConsts k;
+    int rc = step_pre(c, cf, &k);
+    if (rc) return rc;
+    const SolveOpts so = solve_opts(cf, k);
+    SolveResult sr{};
+    rc = solve_ppe(c, so, &sr);
+    if (rc) return rc;
+    return step_post(c, k, sr);
+}
+
+// Resident device state of ps under configuration cf (uploaded on first
+// use), with a valid neighbour list.
+static int ensure_resident(Ctx *c, ParticleSystem *ps, SimConfig *cf)
+{
+    SimConfig *saved = g_config;
+    g_config = cf;
+    int rc = 0;
+    const double re = search_radius(cf);
+    if (!c->has_state || c->n != ps->num) rc = upload(c, ps, nullptr, grid_for(re, cf->domain_min, cf->domain_max));
+    if (!rc) {
+        c->resident = true;
+        if (!c->list_valid) {
+            StepConsts k = make_consts(cf, ps, c->grid, re);
+            k.stride = c->stride;
+            rc = search_only(c, k);
+        }
+    }
+    g_config = saved;
+    return rc;
 }
 
 // ---------------------------------------------------------------------
@@ -701,6 +748,59 @@ int mps_hip_run_steps(ParticleSystem *ps, int nsteps)
         if (rc) return rc;
     }
     return MPS_OK;
+}
+
+int mps_hip_run_batch(ParticleSystem *const *ps, SimConfig *const *cfgs, int nscenes, int nsteps)
+{
+    if (!ps || !cfgs || nscenes <= 0 || nsteps < 0) return g_last_error = MPS_ERR_ARG;
+    int rc = ensure_device();
+    if (rc) return rc;
+    std::vector<Ctx *> cs(nscenes);
+    for (int q = 0; q < nscenes; q++) {
+        if (!ps[q] || !cfgs[q]) return g_last_error = MPS_ERR_ARG;
+        cs[q] = get_ctx(ps[q]);
+        if (!cs[q]) return g_last_error;
+        rc = ensure_resident(cs[q], ps[q], cfgs[q]);
+        if (rc) return rc;
+    }
+    // the batched solve is the FP64 CG; other solver settings run per scene
+    bool batched = g_opt.precision < 0.5;
+    for (int q = 0; q < nscenes; q++) batched = batched && cfgs[q]->solver_type == 0;
+    std::vector<StepConsts> ks(nscenes);
+    std::vector<SolveOpts> so(nscenes);
+    std::vector<SolveResult> sr(nscenes);
+    hipStream_t bstream = cs[0]->stream;
+    SimConfig *saved = g_config;
+    for (int step = 0; step < nsteps && !rc; step++) {
+        for (int q = 0; q < nscenes && !rc; q++) {
+            g_config = cfgs[q];
+            rc = step_pre(cs[q], cfgs[q], &ks[q]);
+            so[q] = solve_opts(cfgs[q], ks[q]);
+        }
+        if (rc) break;
+        if (batched) {
+            for (int q = 0; q < nscenes && !rc; q++) {
+                rc = cg_alloc_ctrl(cs[q]);
+                if (!rc) rc = pack_ppe(cs[q], 0);
+            }
+            for (int q = 0; q < nscenes && !rc; q++)
+                if (hipStreamSynchronize(cs[q]->stream) != hipSuccess) rc = MPS_ERR_HIP;
+            if (!rc) rc = batch_cg(cs.data(), nscenes, so.data(), sr.data(), bstream);
+            for (int q = 0; q < nscenes && !rc; q++) rc = unpack_solution(cs[q]);
+        } else {
+            for (int q = 0; q < nscenes && !rc; q++) {
+                g_config = cfgs[q];
+                rc = solve_ppe(cs[q], so[q], &sr[q]);
+            }
+        }
+        for (int q = 0; q < nscenes && !rc; q++) {
+            g_config = cfgs[q];
+            rc = step_post(cs[q], ks[q], sr[q]);
+        }
+    }
+    g_config = saved;
+    if (rc) g_last_error = rc;
+    return rc;
 }
 
 int mps_hip_get_stats(ParticleSystem *ps, mps_step_stats *out)

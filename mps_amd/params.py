@@ -41,3 +41,22 @@ def bench_config(lib, w, solver=0, tol=1e-9, max_iter=20000):
     cfg.influence_radius_n = cfg.influence_ratio_n * l0
     cfg.influence_radius_st = cfg.surface_tension_re_ratio * l0
     return cfg
+
+
+ENSEMBLE_SEED = 20260317
+
+
+def ensemble(n=64, seed=ENSEMBLE_SEED, l0=0.01, dt=1e-3):
+    """Independent 3-D dam breaks of uneven size: tank 0.8 (x) x 0.5 (y,
+    vertical) x 0.4 (z) m; water column width ~ U[0.1, 0.4] m, height
+    ~ U[0.2, 0.4] m, the full 0.4 m deep; kinematic viscosity log-uniform
+    in [1e-6, 1e-4] m^2/s; l0 = 0.01 m, dt = 1e-3 s.  Seeded."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        w = float(rng.uniform(0.1, 0.4))
+        h = float(rng.uniform(0.2, 0.4))
+        nu = float(10.0 ** rng.uniform(-6.0, -4.0))
+        out.append(dict(dims=(0.8, 0.5, 0.4, w, h, 0.4), l0=l0, dt=dt, nu=nu))
+    return out

@@ -272,3 +272,56 @@ def test_fused_cg_same_bits(lib3):
     finally:
         lib3.set_option("fused", 0)
     assert out[0] == out[1]
+
+
+def test_batch_equals_single_runs(lib3):
+    """mps_hip_run_batch (particle operators per scene, one batched FP64 CG
+    for all scenes) leaves every scene with the bits of its own single-scene
+    run: scenes of different sizes and viscosities."""
+    import ctypes as C
+    from mps_amd import params
+    scenes_w = [dict(dims=(0.4, 0.3, 0.12, 0.12, 0.2, 0.12), l0=0.01, dt=5e-4, nu=1e-6),
+                dict(dims=(0.6, 0.4, 0.2, 0.2, 0.3, 0.2), l0=0.01, dt=5e-4, nu=3e-5),
+                dict(dims=(0.4, 0.3, 0.12, 0.2, 0.15, 0.12), l0=0.01, dt=5e-4, nu=1e-4)]
+    steps = 3
+    lib3.set_option("tol_mode", 1)
+
+    def make(w):
+        cfg = params.bench_config(lib3, w, 0, 1e-10, 20000)
+        dims = (C.c_double * 6)(*w["dims"])
+        n = lib3.mps_gen_dambreak(dims, w["l0"], 3, 2, None, None)
+        ps = lib3.particle_system_create(n)
+        assert lib3.mps_gen_dambreak(dims, w["l0"], 3, 2, ps, None) == n
+        lib3.set_config(cfg)
+        lib3.particle_system_calc_initial_params(ps)
+        return cfg, ps, n
+
+    def fields(ps, n):
+        arr = np.ctypeslib.as_array(C.cast(ps.contents.particles, C.POINTER(lib3.abi.Particle)), shape=(n,))
+        return {f: np.ascontiguousarray(arr[f]).tobytes() for f in H.FIELDS}
+
+    single = []
+    for w in scenes_w:
+        cfg, ps, n = make(w)
+        lib3.set_config(cfg)
+        say("single scene", n, "nu", w["nu"])
+        lib3.check(lib3.mps_hip_run_steps(ps, steps), "run_steps")
+        lib3.check(lib3.mps_hip_download(ps, None), "download")
+        single.append(fields(ps, n))
+        lib3.mps_hip_release(ps)
+        lib3.particle_system_free(ps)
+    made = [make(w) for w in scenes_w]
+    cfgs = [m[0] for m in made]
+    PSP = C.POINTER(lib3.abi.ParticleSystem)
+    ps_arr = (PSP * len(made))(*[m[1] for m in made])
+    cf_arr = (C.POINTER(lib3.abi.SimConfig) * len(made))(*[C.pointer(c) for c in cfgs])
+    say("batch of", len(made))
+    lib3.check(lib3.mps_hip_run_batch(ps_arr, cf_arr, len(made), steps), "run_batch")
+    for q, (cfg, ps, n) in enumerate(made):
+        lib3.check(lib3.mps_hip_download(ps, None), "download")
+        got = fields(ps, n)
+        for f in H.FIELDS:
+            assert got[f] == single[q][f], (q, f)
+        assert lib3.stats(ps).steps == steps
+        lib3.mps_hip_release(ps)
+        lib3.particle_system_free(ps)
