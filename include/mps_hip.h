@@ -186,6 +186,23 @@ int  mps_hip_run_steps(ParticleSystem *ps, int nsteps);
  * mps_hip_download.  g_config is left as it was. */
 int  mps_hip_run_batch(ParticleSystem *const *ps, SimConfig *const *cfgs, int nscenes, int nsteps);
 
+/* Gradient of L = (1/N_f) sum_fluid |r_i(T) - target_i|^2 after nsteps
+ * steps from the state in ps (no counterpart in the reference, which has
+ * no derivatives).  target: N x DIM positions in particle order.  The
+ * forward keeps a checkpoint (particles and neighbour list, host memory)
+ * every checkpoint_every steps; the reverse sweep recomputes each segment
+ * on the GPU, records its intermediate states, and runs the adjoint of
+ * every operator with the forward's discrete decisions frozen (lists,
+ * flags, clamp, p_min choice, collisions), the pressure adjoint solved by
+ * CG on the same matrix.  Outputs: *loss, *grad_nu (dL/d viscosity),
+ * grad_g[DIM] (dL/d gravity), grad_u0[N x DIM] (dL/d initial velocities;
+ * 0 for non-fluid particles).  ps is left in the final forward state
+ * (device copy released).  Scheme variants without a reverse here
+ * (surface tension, hs_mode, ppe_type 1, cmps_gradient, count-based
+ * surface detection) return MPS_ERR_UNSUPPORTED. */
+int  mps_hip_adjoint(ParticleSystem *ps, int nsteps, int checkpoint_every, const double *target,
+                     double *loss, double *grad_nu, double *grad_g, double *grad_u0);
+
 /* Options (process-wide): keys and values
  *   "tol_mode"       0 absolute sqrt(r.r) < tol (reference), 1 relative
  *                      sqrt(r.r) < tol*sqrt(b.b)

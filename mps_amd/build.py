@@ -18,7 +18,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 INC = os.path.join(ROOT, "include")
-HIP_SRC = ["k_particles.hip", "k_pack.hip", "k_cg.hip", "k_ic.hip", "k_extra.hip", "runtime.hip", "dropins.hip"]
+HIP_SRC = ["k_particles.hip", "k_pack.hip", "k_cg.hip", "k_ic.hip", "k_extra.hip", "k_adjoint.hip", "runtime.hip", "dropins.hip"]
 C_SRC = ["config.c", "particles.c", "io.c"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

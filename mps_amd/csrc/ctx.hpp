@@ -215,6 +215,12 @@ Grid mps_grid_for(double re, const double *dmin, const double *dmax);
 Grid mps_grid_from_cells(const CellList *cl);
 double mps_search_radius(const SimConfig *cf);
 
+// ---- adjoint tape (k_adjoint.hip) -------------------------------------
+// Stages of a step at which a recording adjoint copies the state.
+enum TapeStage { TAPE_START, TAPE_PREDICTED, TAPE_COLLIDED, TAPE_SEARCHED, TAPE_SOLVED };
+void tape_hook(Ctx *c, int stage);
+int mps_ctx_step(Ctx *c, const SimConfig *cf);    // one resident timestep (runtime.hip)
+
 // ---- timing (runtime.hip) ----------------------------------------------
 enum ProfKernel { PK_VISC, PK_COLLIDE, PK_SORT, PK_SEARCH, PK_ASSEMBLE, PK_SPMV, PK_AXPY,
                   PK_PUPDATE, PK_GRAD, PK_IC, PK_PACK, PK_COUNT };

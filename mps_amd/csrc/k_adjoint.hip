@@ -1,0 +1,906 @@
+// k_adjoint.hip -- reverse-mode gradient of the MPS timestep on gfx950.
+//
+// Loss: L = (1/N_f) sum over fluid particles of |r_i(T) - r_i^target|^2
+// after T steps.  Gradients: dL/dnu, dL/dg and dL/du_i(0).
+//
+// The forward is the production step (runtime.hip).  While a segment is
+// recomputed, hooks copy its intermediate states into a tape, in particle
+// (original index) order:
+//   r, u      at the start of the step
+//   r*, u*    after the viscosity/gravity predictor (simulation.c:31-67)
+//   r**, u**  after the collision (operators.c:371-446; 2-D all pairs)
+//   L         the neighbour list searched on r** (CSR over original indices)
+//   x, unk    the PPE solution and unknown flags
+// and the reverse sweep runs the adjoint of each operator with the
+// forward's discrete decisions frozen: the lists, the surface flags and
+// unknown set, the negative-pressure clamp, the neighbour that gives p_min,
+// the set and order of the collisions that fired, and ghosted particles.
+// Only fluid particles move, so only their positions and velocities carry
+// adjoints; the pressure adjoint p-bar covers every particle.
+//
+// The reverse operators are written in pull form wherever the lists allow
+// (j in L(i) <=> i in L(j)): each particle gathers every term that lands on
+// it, so the sums are deterministic.  The collision's partner terms are the
+// exception (scattered with atomics; few particles collide).
+//
+// Per step, in reverse:
+//   corrector   a_i = G sum_j (p_j - p_mu(i)) f(d_ij), f(d) = d w(|d|)/|d|^2,
+//               G = -(D/n0)/rho; u' = u** + dt a; r' = r** + dt^2 a
+//               -> p-bar, r**-bar
+//   clamp       p-bar = 0 where the fluid pressure was clamped
+//   PPE         lambda = A^-1 x-bar (A symmetric; CG on the same matrix),
+//               b-bar = lambda -> n-bar = (relax rho/dt^2)/n0 lambda (fluid
+//               rows), A-bar = -lambda x^T through A_ij = -kappa w(r_ij)
+//               and A_ii = kappa sum_(non-dummy j) w(r_ij) -> r**-bar
+//   density     n_i = sum_j w_n(r_ij) -> r**-bar
+//   collision   the firing sequence of each fluid particle, reversed
+//   predictor   a_i = g + nu*coeff sum_j (u_j - u_i) w(r_ij) over the
+//               previous step's list, u* = u + dt a, r* = r + dt u*
+//               -> r-bar, u-bar, nu-bar, g-bar
+#include <limits.h>
+#include <string.h>
+
+#include <vector>
+
+#include "ctx.hpp"
+
+#define AB 256
+#define COLL_MAX 16      // collisions of one particle in one step that the reverse sweep can hold
+
+static inline unsigned ab(size_t n) { return (unsigned)((n + AB - 1) / AB); }
+
+#define LAUNCH_CHECK()                                                       \
+    do {                                                                     \
+        hipError_t e_ = hipGetLastError();                                   \
+        if (e_ != hipSuccess) {                                              \
+            mps_set_error(MPS_ERR_HIP, __FILE__, __LINE__, hipGetErrorString(e_)); \
+            return MPS_ERR_HIP;                                              \
+        }                                                                    \
+    } while (0)
+
+// w(r) of kernel.c:12-20 and its derivative (0 where r is clamped to r_min
+// or r >= re)
+__device__ __forceinline__ double wdiff(double r, double re, double r_min)
+{
+    if (r >= re || r < r_min) return 0.0;
+    return -re / (r * r);
+}
+
+__device__ __forceinline__ dvec dsub(const dvec &a, const dvec &b)
+{
+    dvec d;
+#pragma unroll
+    for (int k = 0; k < (D_ == 3 ? 4 : 2); k++) d.c[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < D_; k++) d.c[k] = a.c[k] - b.c[k];
+    return d;
+}
+
+__device__ __forceinline__ double ddot(const dvec &a, const dvec &b)
+{
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < D_; k++) s += a.c[k] * b.c[k];
+    return s;
+}
+
+__device__ __forceinline__ void daxpy(dvec &y, double a, const dvec &x)
+{
+#pragma unroll
+    for (int k = 0; k < D_; k++) y.c[k] += a * x.c[k];
+}
+
+__device__ __forceinline__ dvec dzero()
+{
+    dvec d;
+#pragma unroll
+    for (int k = 0; k < (D_ == 3 ? 4 : 2); k++) d.c[k] = 0.0;
+    return d;
+}
+
+// ---------------------------------------------------------------------
+// tape
+// ---------------------------------------------------------------------
+
+struct TapeStep {
+    dvec *r = nullptr, *u = nullptr, *rs = nullptr, *us = nullptr, *rss = nullptr, *uss = nullptr;
+    double *x = nullptr;
+    int *unk = nullptr, *typ = nullptr;
+    int *lst = nullptr, *lidx = nullptr;     // CSR of L over original indices
+    size_t lcap = 0;
+};
+
+static TapeStep *g_rec = nullptr;            // the step being recorded, or null
+static int g_rec_err = 0;
+
+__global__ void k_snap_state(int n, const int *__restrict__ O, const dvec *__restrict__ X, const dvec *__restrict__ V,
+                             const int *__restrict__ T, dvec *__restrict__ r, dvec *__restrict__ u, int *__restrict__ typ)
+{
+    const int s = blockIdx.x * AB + threadIdx.x;
+    if (s >= n) return;
+    const int o = O[s];
+    r[o] = X[s];
+    u[o] = V[s];
+    if (typ) typ[o] = T[s];
+}
+
+__global__ void k_snap_x(int n, const int *__restrict__ O, const double *__restrict__ x, const int *__restrict__ unk,
+                         double *__restrict__ xo, int *__restrict__ unko)
+{
+    const int s = blockIdx.x * AB + threadIdx.x;
+    if (s >= n) return;
+    const int o = O[s];
+    xo[o] = unk[s] ? x[s] : 0.0;
+    unko[o] = unk[s];
+}
+
+static int rec_list(Ctx *c, TapeStep *t)
+{
+    long long total = 0;
+    int rc = launch_list_export(c, &total);
+    if (rc) return rc;
+    if ((size_t)total > t->lcap) {
+        if (t->lidx) hipFree(t->lidx);
+        t->lidx = nullptr;
+        const size_t cap = (size_t)total + (size_t)total / 4 + 64;
+        HIP_OK(hipMalloc(&t->lidx, sizeof(int) * cap));
+        t->lcap = cap;
+    }
+    HIP_OK(hipMemcpyAsync(t->lst, c->off_orig, sizeof(int) * ((size_t)c->n + 1), hipMemcpyDeviceToDevice, c->stream));
+    if (total > 0)
+        HIP_OK(hipMemcpyAsync(t->lidx, c->nl_stage, sizeof(int) * (size_t)total, hipMemcpyDeviceToDevice, c->stream));
+    return 0;
+}
+
+// Called by the step at its stages (runtime.hip) while a tape is recorded.
+void tape_hook(Ctx *c, int stage)
+{
+    TapeStep *t = g_rec;
+    if (!t || g_rec_err) return;
+    const int n = c->n;
+    const unsigned g = ab(n);
+    switch (stage) {
+    case TAPE_START:
+        hipLaunchKernelGGL(k_snap_state, dim3(g), dim3(AB), 0, c->stream, n, c->O[c->ct], c->X[c->cx], c->V[c->cx],
+                           c->T[c->ct], t->r, t->u, t->typ);
+        break;
+    case TAPE_PREDICTED:
+        hipLaunchKernelGGL(k_snap_state, dim3(g), dim3(AB), 0, c->stream, n, c->O[c->ct], c->X[c->cx], c->V[c->cx],
+                           c->T[c->ct], t->rs, t->us, nullptr);
+        break;
+    case TAPE_COLLIDED:
+        hipLaunchKernelGGL(k_snap_state, dim3(g), dim3(AB), 0, c->stream, n, c->O[c->ct], c->X[c->cx], c->V[c->cx],
+                           c->T[c->ct], t->rss, t->uss, nullptr);
+        break;
+    case TAPE_SEARCHED:
+        if (rec_list(c, t)) g_rec_err = MPS_ERR_HIP;
+        break;
+    case TAPE_SOLVED:
+        hipLaunchKernelGGL(k_snap_x, dim3(g), dim3(AB), 0, c->stream, n, c->O[c->ct], c->x, c->unk, t->x, t->unk);
+        break;
+    default:
+        break;
+    }
+    if (hipGetLastError() != hipSuccess) g_rec_err = MPS_ERR_HIP;
+}
+
+static int tape_alloc(TapeStep *t, int n)
+{
+    HIP_OK(hipMalloc(&t->r, sizeof(dvec) * n));
+    HIP_OK(hipMalloc(&t->u, sizeof(dvec) * n));
+    HIP_OK(hipMalloc(&t->rs, sizeof(dvec) * n));
+    HIP_OK(hipMalloc(&t->us, sizeof(dvec) * n));
+    HIP_OK(hipMalloc(&t->rss, sizeof(dvec) * n));
+    HIP_OK(hipMalloc(&t->uss, sizeof(dvec) * n));
+    HIP_OK(hipMalloc(&t->x, sizeof(double) * n));
+    HIP_OK(hipMalloc(&t->unk, sizeof(int) * n));
+    HIP_OK(hipMalloc(&t->typ, sizeof(int) * n));
+    HIP_OK(hipMalloc(&t->lst, sizeof(int) * ((size_t)n + 1)));
+    return 0;
+}
+
+static void tape_free(TapeStep *t)
+{
+    void *b[] = {t->r, t->u, t->rs, t->us, t->rss, t->uss, t->x, t->unk, t->typ, t->lst, t->lidx};
+    for (void *p : b)
+        if (p) hipFree(p);
+    *t = TapeStep();
+}
+
+// ---------------------------------------------------------------------
+// reverse kernels (particle order; one thread per particle)
+// ---------------------------------------------------------------------
+
+// The step's pressures (operators.c:188-199 clamp on fluid), p-bar's
+// neighbour-minimum index mu(i), a-bar and F_i = sum_j f(d_ij).
+__global__ void k_adj_corr_pre(int n, const int *__restrict__ typ, const double *__restrict__ x,
+                               const int *__restrict__ unk, const dvec *__restrict__ rss,
+                               const int *__restrict__ lst, const int *__restrict__ lidx,
+                               const dvec *__restrict__ rb, const dvec *__restrict__ ub, StepConsts k, int clamp,
+                               dvec *__restrict__ abar, dvec *__restrict__ F, int *__restrict__ mu)
+{
+    const int i = blockIdx.x * AB + threadIdx.x;
+    if (i >= n) return;
+    if (typ[i] != FLUID_PARTICLE) {
+        abar[i] = dzero();
+        F[i] = dzero();
+        mu[i] = i;
+        return;
+    }
+    auto pres = [&](int j) {
+        double p = unk[j] ? x[j] : 0.0;
+        if (clamp && typ[j] == FLUID_PARTICLE && p < 0.0) p = 0.0;
+        return p;
+    };
+    double pm = pres(i);
+    int m = i;
+    dvec f = dzero();
+    for (int q = lst[i]; q < lst[i + 1]; q++) {
+        const int j = lidx[q];
+        const double pj = pres(j);
+        if (pj < pm) {
+            pm = pj;
+            m = j;
+        }
+        const dvec d = dsub(rss[j], rss[i]);
+        const double r2 = ddot(d, d);
+        if (r2 < k.grad_min_r2) continue;
+        const double w = wfun(sqrt(r2), k.re_lap, k.r_min);
+        daxpy(f, w / r2, d);
+    }
+    mu[i] = m;
+    F[i] = f;
+    dvec a = dzero();
+    daxpy(a, k.dt, ub[i]);
+    daxpy(a, k.dt * k.dt, rb[i]);
+    abar[i] = a;
+}
+
+// Pull form of the corrector's reverse: p-bar of every particle m and the
+// r**-bar terms of fluid m; x-bar = p-bar through the clamp (0 for
+// non-unknowns).  rbss, ubss start from r-bar', u-bar'.
+__global__ void k_adj_corr(int n, const int *__restrict__ typ, const double *__restrict__ x, const int *__restrict__ unk,
+                           const dvec *__restrict__ rss, const int *__restrict__ lst, const int *__restrict__ lidx,
+                           const dvec *__restrict__ rb, const dvec *__restrict__ ub, const dvec *__restrict__ abar,
+                           const dvec *__restrict__ F, const int *__restrict__ mu, StepConsts k, int clamp,
+                           double *__restrict__ xbar, dvec *__restrict__ rbss, dvec *__restrict__ ubss)
+{
+    const int m = blockIdx.x * AB + threadIdx.x;
+    if (m >= n) return;
+    const double G = k.neg_gc / k.rho;
+    auto pres = [&](int j) {
+        double p = unk[j] ? x[j] : 0.0;
+        if (clamp && typ[j] == FLUID_PARTICLE && p < 0.0) p = 0.0;
+        return p;
+    };
+    const bool mf = typ[m] == FLUID_PARTICLE;
+    const double pm = pres(m);
+    double pb = 0.0;
+    dvec rbm = mf ? rb[m] : dzero();
+    // m's own p_min role
+    if (mf && mu[m] == m) pb -= G * ddot(abar[m], F[m]);
+    for (int q = lst[m]; q < lst[m + 1]; q++) {
+        const int i = lidx[q];
+        const dvec dim = dsub(rss[m], rss[i]);      // d_im = r_m - r_i
+        const double r2 = ddot(dim, dim);
+        const bool pair_ok = !(r2 < k.grad_min_r2);
+        const double rr = sqrt(r2);
+        const double w = wfun(rr, k.re_lap, k.r_min);
+        const double wp = wdiff(rr, k.re_lap, k.r_min);
+        if (typ[i] == FLUID_PARTICLE) {
+            const dvec ai = abar[i];
+            if (pair_ok) {
+                // p_m enters a_i through the pair (i, m)
+                pb += G * (w / r2) * ddot(ai, dim);
+                // r_m enters a_i through d_im
+                if (mf) {
+                    const double qim = pm - pres(mu[i]);
+                    const double gg = w / r2, gp = wp / r2 - 2.0 * w / (r2 * rr);
+                    dvec h = dzero();
+                    daxpy(h, gg, ai);
+                    daxpy(h, (gp / rr) * ddot(dim, ai), dim);
+                    daxpy(rbm, G * qim, h);
+                }
+            }
+            if (mu[i] == m) pb -= G * ddot(ai, F[i]);
+        }
+        if (mf && pair_ok) {
+            // r_m enters a_m through d_mi = r_i - r_m
+            const dvec dmi = dsub(rss[i], rss[m]);
+            const double qmi = pres(i) - pres(mu[m]);
+            const double gg = w / r2, gp = wp / r2 - 2.0 * w / (r2 * rr);
+            const dvec am = abar[m];
+            dvec h = dzero();
+            daxpy(h, gg, am);
+            daxpy(h, (gp / rr) * ddot(dmi, am), dmi);
+            daxpy(rbm, -G * qmi, h);
+        }
+    }
+    double xb = 0.0;
+    if (unk[m]) {
+        const bool clamped = clamp && mf && x[m] < 0.0;
+        xb = clamped ? 0.0 : pb;
+    }
+    xbar[m] = xb;
+    rbss[m] = rbm;
+    ubss[m] = mf ? ub[m] : dzero();
+}
+
+// eq-order CSR of the step's PPE (pressure_solver.c:437-551): row e is the
+// e-th unknown in particle order; the diagonal first, then the unknown
+// neighbours in list order.
+__global__ void k_adj_rowlen(int n, const int *__restrict__ unk, const int *__restrict__ lst,
+                             const int *__restrict__ lidx, int *__restrict__ len)
+{
+    const int i = blockIdx.x * AB + threadIdx.x;
+    if (i >= n) return;
+    int c = 0;
+    if (unk[i]) {
+        c = 1;
+        for (int q = lst[i]; q < lst[i + 1]; q++) c += unk[lidx[q]] ? 1 : 0;
+    }
+    len[i] = c;
+}
+
+__global__ void k_adj_fill(int n, const int *__restrict__ typ, const int *__restrict__ unk,
+                           const int *__restrict__ eq, const int *__restrict__ roff, const dvec *__restrict__ rss,
+                           const int *__restrict__ lst, const int *__restrict__ lidx, const double *__restrict__ xbar,
+                           StepConsts k, int *__restrict__ col, double *__restrict__ val, double *__restrict__ rhs)
+{
+    const int i = blockIdx.x * AB + threadIdx.x;
+    if (i >= n || !unk[i]) return;
+    const int e = eq[i];
+    int at = roff[i];
+    const int dpos = at++;
+    double sumw = 0.0;
+    for (int q = lst[i]; q < lst[i + 1]; q++) {
+        const int j = lidx[q];
+        if (typ[j] == DUMMY_PARTICLE) continue;
+        const dvec d = dsub(rss[j], rss[i]);
+        const double w = wfun(sqrt(ddot(d, d)), k.re_lap, k.r_min);
+        sumw += w;
+        if (unk[j]) {
+            col[at] = eq[j];
+            val[at] = k.neg_coeff * w;
+            at++;
+        }
+    }
+    col[dpos] = e;
+    val[dpos] = k.diag_coeff * sumw;
+    rhs[e] = xbar[i];
+}
+
+__global__ void k_adj_lambda(int n, const int *__restrict__ unk, const int *__restrict__ eq,
+                             const double *__restrict__ sol, double *__restrict__ lam)
+{
+    const int i = blockIdx.x * AB + threadIdx.x;
+    if (i >= n) return;
+    lam[i] = unk[i] ? sol[eq[i]] : 0.0;
+}
+
+// Pull form of the PPE's reverse: A-bar = -lambda x^T through the matrix
+// entries, and n-bar = src/n0 lambda (fluid rows) through the density;
+// adds to r**-bar of fluid m.
+__global__ void k_adj_ppe_pos(int n, const int *__restrict__ typ, const int *__restrict__ unk,
+                              const double *__restrict__ x, const double *__restrict__ lam,
+                              const dvec *__restrict__ rss, const int *__restrict__ lst, const int *__restrict__ lidx,
+                              StepConsts k, dvec *__restrict__ rbss)
+{
+    const int m = blockIdx.x * AB + threadIdx.x;
+    if (m >= n || typ[m] != FLUID_PARTICLE) return;
+    const double kap = k.diag_coeff, kof = -k.neg_coeff;   // A_ii = kap sum w, A_ij = -kof w
+    auto nbar = [&](int i) { return (unk[i] && typ[i] == FLUID_PARTICLE) ? k.src_k / k.n0 * lam[i] : 0.0; };
+    dvec acc = dzero();
+    const double lm = lam[m], xm = x[m], nbm = nbar(m);
+    const bool mu_ = unk[m] != 0;
+    for (int q = lst[m]; q < lst[m + 1]; q++) {
+        const int i = lidx[q];
+        const dvec dim = dsub(rss[m], rss[i]);     // r_m - r_i
+        const double rr = sqrt(ddot(dim, dim));
+        if (!(rr > 0.0)) continue;
+        const double wl = wdiff(rr, k.re_lap, k.r_min), wn = wdiff(rr, k.re_n, k.r_min);
+        // dL/d rho_im for the pair seen from row i (i unknown) and from row m
+        double c = 0.0;
+        if (unk[i]) {
+            if (mu_) c += kof * wl * lam[i] * xm;                      // A_im = -kof w
+            if (typ[m] != DUMMY_PARTICLE) c -= kap * wl * lam[i] * x[i];   // A_ii += kap w
+        }
+        if (mu_) {
+            if (unk[i]) c += kof * wl * lm * x[i];                     // A_mi
+            if (typ[i] != DUMMY_PARTICLE) c -= kap * wl * lm * xm;     // A_mm += kap w
+        }
+        c += (nbar(i) + nbm) * wn;                                     // n_i and n_m
+        // rho_im = |r_m - r_i|: d rho / d r_m = dim / rho
+        daxpy(acc, c / rr, dim);
+    }
+    daxpy(rbss[m], 1.0, acc);
+}
+
+// Collision reverse, own terms: r*-bar = r**-bar; u*-bar = -dt r**-bar
+// (r** = r* + dt (v_after - u*)); the chain adds the rest.
+__global__ void k_adj_coll_init(int n, const int *__restrict__ typ, const dvec *__restrict__ rbss, StepConsts k,
+                                dvec *__restrict__ rbs, dvec *__restrict__ ubs)
+{
+    const int i = blockIdx.x * AB + threadIdx.x;
+    if (i >= n) return;
+    if (typ[i] != FLUID_PARTICLE) {
+        rbs[i] = dzero();
+        ubs[i] = dzero();
+        return;
+    }
+    rbs[i] = rbss[i];
+    dvec u = dzero();
+    daxpy(u, -k.dt, rbss[i]);
+    ubs[i] = u;
+}
+
+__device__ __forceinline__ void atomic_add_vec(dvec *p, const dvec &v)
+{
+#pragma unroll
+    for (int d = 0; d < D_; d++) atomicAdd(&p->c[d], v.c[d]);
+}
+
+// Collision reverse, the firing sequence of fluid i recomputed from r*, u*
+// (candidates: the stale list, every partner closer than the collision
+// distance; 2-D in ascending index as mps_fluid2D/src/operators.c:341-379,
+// 3-D in list order as operators.c:395-432), then run backwards.
+__global__ void k_adj_coll(int n, const int *__restrict__ typ, const dvec *__restrict__ rs, const dvec *__restrict__ us,
+                           const int *__restrict__ lst, const int *__restrict__ lidx, const dvec *__restrict__ rbss,
+                           const dvec *__restrict__ ubss, StepConsts k, dvec *__restrict__ rbs, dvec *__restrict__ ubs,
+                           int *err)
+{
+    const int i = blockIdx.x * AB + threadIdx.x;
+    if (i >= n || typ[i] != FLUID_PARTICLE) return;
+    int js[COLL_MAX];
+    dvec vbefore[COLL_MAX];
+    int K = 0;
+    dvec vi = us[i];
+    const dvec ri = rs[i];
+    if (!k.collide) goto reverse;
+    {
+#if MPS_DIM == 2
+    // ascending index among the partners closer than cd
+    int last = -1;
+    for (;;) {
+        int best = INT_MAX;
+        for (int q = lst[i]; q < lst[i + 1]; q++) {
+            const int j = lidx[q];
+            if (j <= last || j >= best || typ[j] == GHOST_PARTICLE) continue;
+            const dvec d = dsub(rs[j], ri);
+            if (ddot(d, d) >= k.cd2) continue;
+            best = j;
+        }
+        if (best == INT_MAX) break;
+        last = best;
+        const int j = best;
+#else
+    for (int q = lst[i]; q < lst[i + 1]; q++) {
+        const int j = lidx[q];
+        if (typ[j] == GHOST_PARTICLE) continue;
+#endif
+        const dvec d = dsub(rs[j], ri);
+        const double r2 = ddot(d, d);
+        if (r2 >= k.cd2) continue;
+        const double r = sqrt(r2);
+        if (r < k.col_tiny) continue;
+        double vn = 0.0;
+#pragma unroll
+        for (int dd = 0; dd < D_; dd++) vn += (vi.c[dd] - us[j].c[dd]) * (d.c[dd] / r);
+        if (vn <= 0.0) continue;
+        if (K == COLL_MAX) {
+            atomicExch(err, 1);
+            return;
+        }
+        js[K] = j;
+        vbefore[K] = vi;
+        K++;
+        const double imp = k.e_half * vn;
+#pragma unroll
+        for (int dd = 0; dd < D_; dd++) vi.c[dd] -= imp * (d.c[dd] / r);
+    }
+    }
+reverse:
+    // v-bar of the final velocity: u** = v_f, r** = r* + dt (v_f - u*)
+    dvec vb = ubss[i];
+    daxpy(vb, k.dt, rbss[i]);
+    dvec rown = dzero();
+    for (int s = K - 1; s >= 0; s--) {
+        const int j = js[s];
+        const dvec d = dsub(rs[j], ri);
+        const double r = sqrt(ddot(d, d));
+        dvec nh = dzero();
+        daxpy(nh, 1.0 / r, d);
+        const dvec delta = dsub(vbefore[s], us[j]);
+        const double vn = ddot(delta, nh);
+        const double nv = ddot(nh, vb);
+        // v_after = v_before - c vn nh
+        // d/d nh: -c (nv delta + vn vb); through nh = d / r
+        dvec nbar = dzero();
+        daxpy(nbar, -k.e_half * nv, delta);
+        daxpy(nbar, -k.e_half * vn, vb);
+        const double nn = ddot(nh, nbar);
+        dvec dbar = nbar;
+        daxpy(dbar, -nn, nh);
+#pragma unroll
+        for (int dd = 0; dd < D_; dd++) dbar.c[dd] /= r;
+        // d = r*_j - r*_i
+        if (typ[j] == FLUID_PARTICLE) {
+            atomic_add_vec(&rbs[j], dbar);
+            dvec uj = dzero();
+            daxpy(uj, k.e_half * nv, nh);
+            atomic_add_vec(&ubs[j], uj);
+        }
+        daxpy(rown, -1.0, dbar);
+        // propagate to v_before: (I - c nh nh^T) vb
+        daxpy(vb, -k.e_half * nv, nh);
+    }
+    atomic_add_vec(&rbs[i], rown);
+    atomic_add_vec(&ubs[i], vb);
+}
+
+// Predictor reverse, step 1: a-bar of the predictor for every fluid i.
+__global__ void k_adj_pred_pre(int n, const int *__restrict__ typ, const dvec *__restrict__ rbs,
+                               const dvec *__restrict__ ubs, StepConsts k, dvec *__restrict__ apbar,
+                               dvec *__restrict__ ustar_total)
+{
+    const int i = blockIdx.x * AB + threadIdx.x;
+    if (i >= n) return;
+    if (typ[i] != FLUID_PARTICLE) {
+        apbar[i] = dzero();
+        ustar_total[i] = dzero();
+        return;
+    }
+    // r* = r + dt u*: u*-bar += dt r*-bar; u* = u + dt a: a-bar = dt u*-bar
+    dvec ut = ubs[i];
+    daxpy(ut, k.dt, rbs[i]);
+    ustar_total[i] = ut;
+    dvec a = dzero();
+    daxpy(a, k.dt, ut);
+    apbar[i] = a;
+}
+
+// Predictor reverse, step 2 (pull over the stale list): r-bar, u-bar of
+// fluid m, and per-block partials of nu-bar (coeff a-bar . lap) and g-bar.
+__global__ void __launch_bounds__(AB)
+k_adj_pred(int n, const int *__restrict__ typ, const dvec *__restrict__ r, const dvec *__restrict__ u,
+           const int *__restrict__ lst, const int *__restrict__ lidx, const dvec *__restrict__ rbs,
+           const dvec *__restrict__ apbar, const dvec *__restrict__ ustar_total, StepConsts k,
+           dvec *__restrict__ rb_out, dvec *__restrict__ ub_out, double *part)
+{
+    const int m = blockIdx.x * AB + threadIdx.x;
+    double pnu = 0.0, pg[3] = {0.0, 0.0, 0.0};
+    if (m < n) {
+        if (typ[m] != FLUID_PARTICLE) {
+            rb_out[m] = dzero();
+            ub_out[m] = dzero();
+        } else {
+            const dvec am = apbar[m];
+            dvec rbm = rbs[m];
+            dvec ubm = ustar_total[m];
+            dvec lap = dzero();
+            double wsum = 0.0;
+            for (int q = lst[m]; q < lst[m + 1]; q++) {
+                const int i = lidx[q];
+                const dvec dim = dsub(r[m], r[i]);
+                const double rr = sqrt(ddot(dim, dim));
+                const double w = wfun(rr, k.re_lap, k.r_min);
+                const double wp = rr > 0.0 ? wdiff(rr, k.re_lap, k.r_min) : 0.0;
+                const dvec du = dsub(u[i], u[m]);           // u_i - u_m
+                daxpy(lap, w, du);
+                wsum += w;
+                // a_m = ... + K sum_i (u_i - u_m) w(rho_mi)
+                double e = k.visc_coeff * ddot(am, du) * wp;
+                if (typ[i] == FLUID_PARTICLE) {
+                    // a_i = ... + K (u_m - u_i) w(rho_im): u_m's share, r_m through rho_im
+                    const dvec ai = apbar[i];
+                    daxpy(ubm, k.visc_coeff * w, ai);
+                    const dvec dumi = dsub(u[m], u[i]);
+                    e += k.visc_coeff * ddot(ai, dumi) * wp;
+                }
+                if (rr > 0.0) daxpy(rbm, e / rr, dim);
+            }
+            daxpy(ubm, -k.visc_coeff * wsum, am);
+            rb_out[m] = rbm;
+            ub_out[m] = ubm;
+            pnu = k.coeff * ddot(am, lap);
+#pragma unroll
+            for (int d = 0; d < D_; d++) pg[d] = am.c[d];
+        }
+    }
+    pnu = block_sum<double, AB>(pnu);
+    double s0 = block_sum<double, AB>(pg[0]);
+    double s1 = block_sum<double, AB>(pg[1]);
+    double s2 = block_sum<double, AB>(pg[2]);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x * 4 + 0] = pnu;
+        part[blockIdx.x * 4 + 1] = s0;
+        part[blockIdx.x * 4 + 2] = s1;
+        part[blockIdx.x * 4 + 3] = s2;
+    }
+}
+
+// nu-bar, g-bar += the block partials, in block order (one block).
+__global__ void k_adj_param_fin(const double *part, int nblocks, double *acc)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < nblocks; b++)
+        for (int q = 0; q < 4; q++) s[q] += part[b * 4 + q];
+    for (int q = 0; q < 4; q++) acc[q] += s[q];
+}
+
+// ---------------------------------------------------------------------
+// driver
+// ---------------------------------------------------------------------
+
+int mps_ctx_step(Ctx *c, const SimConfig *cf);                     // runtime.hip
+int csr_cg_device(int n, long long nnz, const int *rp, const int *col, const double *val, const double *b,
+                  double *x, const SolveOpts &o, SolveResult *res, hipStream_t s);   // k_ic.hip
+
+struct AdjWork {
+    int n = 0;
+    dvec *rb = nullptr, *ub = nullptr, *rbss = nullptr, *ubss = nullptr, *rbs = nullptr, *ubs = nullptr;
+    dvec *abar = nullptr, *F = nullptr, *apbar = nullptr, *ust = nullptr;
+    int *mu = nullptr, *len = nullptr, *roff = nullptr, *eq = nullptr, *col = nullptr, *err = nullptr, *scan_blk = nullptr;
+    double *xbar = nullptr, *lam = nullptr, *val = nullptr, *rhs = nullptr, *sol = nullptr, *part = nullptr,
+           *params = nullptr;
+    size_t cap_nnz = 0;
+};
+
+static void adj_free(AdjWork &w)
+{
+    void *b[] = {w.rb, w.ub, w.rbss, w.ubss, w.rbs, w.ubs, w.abar, w.F, w.apbar, w.ust, w.mu, w.len, w.roff, w.eq,
+                 w.col, w.err, w.scan_blk, w.xbar, w.lam, w.val, w.rhs, w.sol, w.part, w.params};
+    for (void *p : b)
+        if (p) hipFree(p);
+    w = AdjWork();
+}
+
+static int adj_alloc(AdjWork &w, int n, hipStream_t s)
+{
+    w.n = n;
+    dvec **dv[] = {&w.rb, &w.ub, &w.rbss, &w.ubss, &w.rbs, &w.ubs, &w.abar, &w.F, &w.apbar, &w.ust};
+    for (dvec **p : dv) HIP_OK(hipMalloc((void **)p, sizeof(dvec) * n));
+    HIP_OK(hipMalloc(&w.mu, sizeof(int) * n));
+    HIP_OK(hipMalloc(&w.len, sizeof(int) * n));
+    HIP_OK(hipMalloc(&w.roff, sizeof(int) * ((size_t)n + 1)));
+    HIP_OK(hipMalloc(&w.eq, sizeof(int) * ((size_t)n + 1)));
+    HIP_OK(hipMalloc(&w.err, sizeof(int)));
+    HIP_OK(hipMalloc(&w.scan_blk, sizeof(int) * 8200));
+    HIP_OK(hipMalloc(&w.xbar, sizeof(double) * n));
+    HIP_OK(hipMalloc(&w.lam, sizeof(double) * n));
+    HIP_OK(hipMalloc(&w.rhs, sizeof(double) * n));
+    HIP_OK(hipMalloc(&w.sol, sizeof(double) * n));
+    HIP_OK(hipMalloc(&w.part, sizeof(double) * 4 * ((size_t)ab(n) + 1)));
+    HIP_OK(hipMalloc(&w.params, sizeof(double) * 4));
+    // on the stream the reverse kernels run on (a non-blocking stream does
+    // not wait for null-stream work)
+    HIP_OK(hipMemsetAsync(w.params, 0, sizeof(double) * 4, s));
+    HIP_OK(hipMemsetAsync(w.err, 0, sizeof(int), s));
+    return 0;
+}
+
+// The reverse of one recorded step: (rb, ub) = adjoint of the step's
+// outputs in, adjoint of its inputs out.  ls/lsi: the list the predictor
+// used (the previous step's search).
+static int adj_step(AdjWork &w, const TapeStep &t, const int *ls, const int *lsi, const StepConsts &k, int clamp,
+                    const SolveOpts &so, hipStream_t s)
+{
+    const int n = w.n;
+    const unsigned g = ab(n);
+    hipLaunchKernelGGL(k_adj_corr_pre, dim3(g), dim3(AB), 0, s, n, t.typ, t.x, t.unk, t.rss, t.lst, t.lidx, w.rb, w.ub,
+                       k, clamp, w.abar, w.F, w.mu);
+    hipLaunchKernelGGL(k_adj_corr, dim3(g), dim3(AB), 0, s, n, t.typ, t.x, t.unk, t.rss, t.lst, t.lidx, w.rb, w.ub,
+                       w.abar, w.F, w.mu, k, clamp, w.xbar, w.rbss, w.ubss);
+    // adjoint PPE on the step's matrix
+    hipLaunchKernelGGL(k_adj_rowlen, dim3(g), dim3(AB), 0, s, n, t.unk, t.lst, t.lidx, w.len);
+    LAUNCH_CHECK();
+    int rc = launch_scan_excl(w.len, w.roff, n, w.scan_blk, s);
+    if (!rc) rc = launch_scan_excl(t.unk, w.eq, n, w.scan_blk, s);
+    if (rc) return rc;
+    int tot[2];
+    HIP_OK(hipMemcpyAsync(&tot[0], w.roff + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(&tot[1], w.eq + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    const long long nnz = tot[0];
+    const int neq = tot[1];
+    if ((size_t)nnz > w.cap_nnz) {
+        if (w.col) hipFree(w.col);
+        if (w.val) hipFree(w.val);
+        w.col = nullptr;
+        w.val = nullptr;
+        w.cap_nnz = (size_t)nnz + (size_t)nnz / 4 + 64;
+        HIP_OK(hipMalloc(&w.col, sizeof(int) * w.cap_nnz));
+        HIP_OK(hipMalloc(&w.val, sizeof(double) * w.cap_nnz));
+    }
+    hipLaunchKernelGGL(k_adj_fill, dim3(g), dim3(AB), 0, s, n, t.typ, t.unk, w.eq, w.roff, t.rss, t.lst, t.lidx,
+                       w.xbar, k, w.col, w.val, w.rhs);
+    LAUNCH_CHECK();
+    if (neq > 0) {
+        // row pointers of the eq-order rows: the unknowns' row offsets in order
+        std::vector<int> len_h(n), unk_h(n), roff_h(n + 1);
+        HIP_OK(hipMemcpyAsync(roff_h.data(), w.roff, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(unk_h.data(), t.unk, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        std::vector<int> rp(neq + 1);
+        int e = 0;
+        for (int i = 0; i < n; i++)
+            if (unk_h[i]) rp[e++] = roff_h[i];
+        rp[neq] = (int)nnz;
+        int *rp_d = w.len;    // reuse: n >= neq + 1 unless every particle is an unknown
+        int *tmp = nullptr;
+        if (neq + 1 > n) {
+            HIP_OK(hipMalloc(&tmp, sizeof(int) * (neq + 1)));
+            rp_d = tmp;
+        }
+        HIP_OK(hipMemcpyAsync(rp_d, rp.data(), sizeof(int) * (neq + 1), hipMemcpyHostToDevice, s));
+        SolveResult sr{};
+        rc = csr_cg_device(neq, nnz, rp_d, w.col, w.val, w.rhs, w.sol, so, &sr, s);
+        HIP_OK(hipStreamSynchronize(s));
+        if (tmp) hipFree(tmp);
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_adj_lambda, dim3(g), dim3(AB), 0, s, n, t.unk, w.eq, w.sol, w.lam);
+    hipLaunchKernelGGL(k_adj_ppe_pos, dim3(g), dim3(AB), 0, s, n, t.typ, t.unk, t.x, w.lam, t.rss, t.lst, t.lidx, k,
+                       w.rbss);
+    hipLaunchKernelGGL(k_adj_coll_init, dim3(g), dim3(AB), 0, s, n, t.typ, w.rbss, k, w.rbs, w.ubs);
+    hipLaunchKernelGGL(k_adj_coll, dim3(g), dim3(AB), 0, s, n, t.typ, t.rs, t.us, ls, lsi, w.rbss, w.ubss, k, w.rbs,
+                       w.ubs, w.err);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_adj_pred_pre, dim3(g), dim3(AB), 0, s, n, t.typ, w.rbs, w.ubs, k, w.apbar, w.ust);
+    hipLaunchKernelGGL(k_adj_pred, dim3(g), dim3(AB), 0, s, n, t.typ, t.r, t.u, ls, lsi, w.rbs, w.apbar, w.ust, k, w.rb,
+                       w.ub, w.part);
+    hipLaunchKernelGGL(k_adj_param_fin, dim3(1), dim3(64), 0, s, w.part, (int)g, w.params);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+// Unsupported scheme variants of the adjoint (their operators have no
+// reverse here): returns 0 or a message.
+static const char *adj_unsupported(const SimConfig *cf)
+{
+    if (cf->surface_tension_enabled) return "surface tension";
+    if (cf->hs_mode != 0) return "hs_mode";
+    if (cf->ppe_type != 0) return "ppe_type";
+    if (cf->cmps_gradient != 0) return "cmps_gradient";
+    if (cf->surface_detection_method != 0) return "surface_detection_method";
+    return nullptr;
+}
+
+extern "C" int mps_hip_adjoint(ParticleSystem *ps, int nsteps, int checkpoint_every, const double *target,
+                               double *loss_out, double *grad_nu, double *grad_g, double *grad_u0)
+{
+    if (!ps || nsteps < 1 || !target) return MPS_ERR_ARG;
+    if (!g_config) return MPS_ERR_NO_CONFIG;
+    SimConfig *cf = g_config;
+    if (const char *u = adj_unsupported(cf)) {
+        mps_set_error(MPS_ERR_UNSUPPORTED, __FILE__, __LINE__, u);
+        return MPS_ERR_UNSUPPORTED;
+    }
+    const int n = ps->num;
+    const int K = checkpoint_every > 0 ? checkpoint_every : nsteps;
+    const int nseg = (nsteps + K - 1) / K;
+    Ctx *c = mps_ctx_for(ps, 1);
+    if (!c) return mps_hip_last_error();
+    hipStream_t s = c->stream;
+    const double re = mps_search_radius(cf);
+    const Grid grid = mps_grid_for(re, cf->domain_min, cf->domain_max);
+    NeighborList *nl = neighbor_list_create(n, cf->max_neighbors);
+    if (!nl) return MPS_ERR_ALLOC;
+    int rc = mps_ctx_upload(c, ps, nullptr, grid);
+    StepConsts k = mps_consts(cf, ps, grid, re);
+    k.stride = c->stride;
+    if (!rc) rc = mps_ctx_search(c, k);
+    // forward with checkpoints (particles and the stale list, on the host)
+    std::vector<std::vector<Particle>> ck_p(nseg);
+    std::vector<std::vector<int>> ck_cnt(nseg), ck_nbr(nseg);
+    const size_t nbr_len = (size_t)n * (size_t)cf->max_neighbors;
+    for (int sg = 0; sg < nseg && !rc; sg++) {
+        rc = mps_ctx_download(c, ps, nl);
+        if (rc) break;
+        ck_p[sg].assign(ps->particles, ps->particles + n);
+        ck_cnt[sg].assign(nl->count, nl->count + n);
+        ck_nbr[sg].assign(nl->neighbors, nl->neighbors + nbr_len);
+        const int m = (sg == nseg - 1) ? nsteps - sg * K : K;
+        for (int q = 0; q < m && !rc; q++) rc = mps_ctx_step(c, cf);
+    }
+    if (!rc) rc = mps_ctx_download(c, ps, nullptr);
+    if (rc) {
+        neighbor_list_free(nl);
+        return rc;
+    }
+    const std::vector<Particle> final_p(ps->particles, ps->particles + n);
+    // loss and its gradient in the final positions
+    std::vector<dvec> rb_h(n), zero_h(n);
+    int nf = 0;
+    for (int i = 0; i < n; i++)
+        if (final_p[i].type == FLUID_PARTICLE) nf++;
+    double loss = 0.0;
+    for (int i = 0; i < n; i++) {
+        dvec z;
+        memset(&z, 0, sizeof z);
+        zero_h[i] = z;
+        rb_h[i] = z;
+        if (final_p[i].type != FLUID_PARTICLE) continue;
+        for (int d = 0; d < D_; d++) {
+            const double e = final_p[i].pos[d] - target[(size_t)i * D_ + d];
+            loss += e * e;
+            rb_h[i].c[d] = 2.0 * e / nf;
+        }
+    }
+    loss = nf > 0 ? loss / nf : 0.0;
+    AdjWork w;
+    std::vector<TapeStep> tapes(K);
+    TapeStep ls0;
+    rc = adj_alloc(w, n, s);
+    for (int q = 0; q < K && !rc; q++) rc = tape_alloc(&tapes[q], n);
+    if (!rc) {
+        hipError_t e = hipMalloc(&ls0.lst, sizeof(int) * ((size_t)n + 1));
+        if (e != hipSuccess) rc = MPS_ERR_HIP;
+    }
+    if (!rc) {
+        if (hipMemcpyAsync(w.rb, rb_h.data(), sizeof(dvec) * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipMemcpyAsync(w.ub, zero_h.data(), sizeof(dvec) * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            rc = MPS_ERR_HIP;
+    }
+    SolveOpts so{};
+    so.solver = 0;
+    so.max_iter = cf->cg_max_iter;
+    so.tol = cf->cg_tolerance;
+    so.tol_mode = (int)mps_hip_get_option("tol_mode");
+    so.check_every = (int)mps_hip_get_option("check_every");
+    for (int sg = nseg - 1; sg >= 0 && !rc; sg--) {
+        // recompute the segment from its checkpoint, recording it
+        memcpy(ps->particles, ck_p[sg].data(), sizeof(Particle) * n);
+        memcpy(nl->count, ck_cnt[sg].data(), sizeof(int) * n);
+        memcpy(nl->neighbors, ck_nbr[sg].data(), sizeof(int) * nbr_len);
+        rc = mps_ctx_upload(c, ps, nl, grid);
+        if (!rc) rc = rec_list(c, &ls0);
+        const int m = (sg == nseg - 1) ? nsteps - sg * K : K;
+        for (int q = 0; q < m && !rc; q++) {
+            g_rec = &tapes[q];
+            g_rec_err = 0;
+            rc = mps_ctx_step(c, cf);
+            g_rec = nullptr;
+            if (!rc) rc = g_rec_err;
+        }
+        if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = MPS_ERR_HIP;
+        for (int q = m - 1; q >= 0 && !rc; q--) {
+            const TapeStep &lsrc = q == 0 ? ls0 : tapes[q - 1];
+            rc = adj_step(w, tapes[q], lsrc.lst, lsrc.lidx, k, cf->clamp_negative_pressure, so, s);
+        }
+    }
+    int herr = 0;
+    double params[4] = {0.0, 0.0, 0.0, 0.0};
+    std::vector<dvec> ub_h(n);
+    if (!rc) {
+        if (hipMemcpyAsync(&herr, w.err, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipMemcpyAsync(params, w.params, sizeof params, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipMemcpyAsync(ub_h.data(), w.ub, sizeof(dvec) * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            rc = MPS_ERR_HIP;
+    }
+    if (!rc && herr) {
+        mps_set_error(MPS_ERR_UNSUPPORTED, __FILE__, __LINE__, "more collisions of one particle in one step than the reverse sweep holds");
+        rc = MPS_ERR_UNSUPPORTED;
+    }
+    if (!rc) {
+        if (loss_out) *loss_out = loss;
+        if (grad_nu) *grad_nu = params[0];
+        if (grad_g)
+            for (int d = 0; d < D_; d++) grad_g[d] = params[1 + d];
+        if (grad_u0)
+            for (int i = 0; i < n; i++)
+                for (int d = 0; d < D_; d++) grad_u0[(size_t)i * D_ + d] = ub_h[i].c[d];
+    }
+    // ps holds the final forward state; its device copy is dropped
+    memcpy(ps->particles, final_p.data(), sizeof(Particle) * n);
+    mps_hip_release(ps);
+    for (auto &t : tapes) tape_free(&t);
+    if (ls0.lst) hipFree(ls0.lst);
+    if (ls0.lidx) hipFree(ls0.lidx);
+    adj_free(w);
+    neighbor_list_free(nl);
+    return rc;
+}

@@ -484,7 +484,7 @@ int cg_alloc_ctrl(Ctx *c)
 {
     if (!c->ctrl) {
         HIP_OK(hipMalloc(&c->ctrl, sizeof(CgCtrl)));
-        HIP_OK(hipMemset(c->ctrl, 0, sizeof(CgCtrl)));
+        HIP_OK(hipMemsetAsync(c->ctrl, 0, sizeof(CgCtrl), c->stream));
     }
     if (!c->hctrl) HIP_OK(hipHostMalloc(&c->hctrl, sizeof(CgCtrl), hipHostMallocDefault));
     return 0;

@@ -772,6 +772,32 @@ int iccg_solve(Ctx *c, const SolveOpts &o, SolveResult *res)
     return 0;
 }
 
+// CG (no preconditioner) on a CSR held in device memory, rows with their
+// diagonal included in any column order; used by the adjoint's second
+// solve on the step's matrix.  x receives the solution.
+int csr_cg_device(int n, long long nnz, const int *rp, const int *col, const double *val, const double *b,
+                  double *x, const SolveOpts &o, SolveResult *res, hipStream_t s)
+{
+    static IcData *d = nullptr;
+    int rc = ctrl_ready();
+    if (rc) return rc;
+    if (!d) d = new IcData();
+    rc = ic_reserve(d, n, n, nnz);
+    if (rc) return rc;
+    d->n_eq = n;
+    d->nnz = nnz;
+    HIP_OK(hipMemcpyAsync(d->rp, rp, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToDevice, s));
+    HIP_OK(hipMemcpyAsync(d->col, col, sizeof(int) * (size_t)nnz, hipMemcpyDeviceToDevice, s));
+    HIP_OK(hipMemcpyAsync(d->aval, val, sizeof(double) * (size_t)nnz, hipMemcpyDeviceToDevice, s));
+    HIP_OK(hipMemcpyAsync(d->b, b, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
+    SolveOpts oo = o;
+    oo.solver = 0;
+    rc = pcg_run(d, s, g_dctrl, g_hctrl, oo, 0, res);
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(x, d->x, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
+    return 0;
+}
+
 // Stand-alone CSR solve (rows with sorted columns), CG or ICCG.
 extern "C" int mps_hip_solve_csr(int n, const int *row_ptr, const int *col, const double *val, const double *b,
                                  double *x, int solver_type, int max_iter, double tol, int *iters_out)

@@ -517,17 +517,22 @@ static int step_pre(Ctx *c, const SimConfig *cf, StepConsts *kout)
 {
     StepConsts k = make_consts(cf, c->ps, c->grid, search_radius(cf));
     k.stride = c->stride;
+    tape_hook(c, TAPE_START);
     int rc = launch_visc_predict(c, k);
     if (rc) return rc;
+    tape_hook(c, TAPE_PREDICTED);
 #if MPS_DIM == 2
     rc = collide_2d(c, k);
+    if (!rc) tape_hook(c, TAPE_COLLIDED);
     if (!rc) rc = launch_collide_bin(c, k, 0);
 #else
     rc = launch_collide_bin(c, k, k.collide);
+    if (!rc) tape_hook(c, TAPE_COLLIDED);
 #endif
     if (!rc) rc = launch_sort(c, k, 0);
     if (!rc) rc = reset_counters(c);
     if (!rc) rc = launch_search(c, k, 1, c->counters);
+    if (!rc) tape_hook(c, TAPE_SEARCHED);
     if (!rc) rc = launch_assemble(c, k, c->counters);
     if (rc) return rc;
     c->list_valid = true;
@@ -556,6 +561,7 @@ static SolveOpts solve_opts(const SimConfig *cf, const StepConsts &k)
 // corrector, boundary, counters and statistics.
 static int step_post(Ctx *c, const StepConsts &k, const SolveResult &sr)
 {
+    tape_hook(c, TAPE_SOLVED);
     int rc = launch_pressure_out(c, k, 0);
     if (!rc) rc = launch_grad_correct(c, k);
     if (!rc) rc = fetch_counters(c);
@@ -912,6 +918,7 @@ void simulation_run(ParticleSystem *ps)
 
 // accessors for the other translation units
 Ctx *mps_ctx_for(ParticleSystem *ps, int create) { return create ? get_ctx(ps) : find_ctx(ps); }
+int mps_ctx_step(Ctx *c, const SimConfig *cf) { return run_step(c, cf); }
 int mps_ctx_upload(Ctx *c, ParticleSystem *ps, const NeighborList *nl, const Grid &g) { return upload(c, ps, nl, g); }
 int mps_ctx_download(Ctx *c, ParticleSystem *ps, NeighborList *nl) { return download(c, ps, nl); }
 int mps_ctx_search(Ctx *c, const StepConsts &k) { return search_only(c, k); }

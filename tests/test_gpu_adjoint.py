@@ -1,0 +1,158 @@
+"""Gradients of the 2-D dam break through mps_hip_adjoint against central
+finite differences of the GPU forward (relative step 1e-6).
+
+Loss: mean squared distance of the fluid particles' final positions to a
+target snapshot made by the same solver with nu = 5e-5, displaced by a
+seeded 1 mm random offset -- without it the target lies so close to the
+trajectory (loss 1e-15..1e-11) that the rounding of the positions, not the
+derivative, sets the finite differences.  Parameters:
+viscosity, gravity (both components) and the initial velocities (checked
+along a random direction).  The reference has no derivatives, so finite
+differences of the forward are the check -- of the GPU forward, and for
+the longest case also of the CPU oracle's forward (the reference's
+arithmetic restated).  The CG runs to a relative residual of 1e-13 so that
+the differences resolve the loss.  Steps: 1e-6 relative for gravity and
+the velocities; for the viscosity 1e-6 of the target's 5e-5 (1e-6 of the
+base 1e-6 moves the loss by ~1e-14 relative, the loss's own rounding).
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import scenes
+from mps_amd import abi as A
+from oracle.oracle import Oracle, make_config
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Problem:
+    def __init__(self, lib, name, steps, seed=7, cdr=0.5):
+        self.lib = lib
+        self.steps = steps
+        sc = scenes.CONFIGS[name]
+        self.dim = sc["dim"]
+        self.base = scenes.params_for(name, cg_tolerance=1e-13, cg_max_iter=20000, viscosity=1e-6,
+                                      collision_distance_ratio=cdr)
+        self.pos, vel, self.typ = scenes.dambreak(**sc)
+        rng = np.random.default_rng(seed)
+        fl = self.typ == 0
+        self.u0 = np.zeros_like(vel)
+        self.u0[fl] = rng.normal(0.0, 0.01, size=(fl.sum(), self.dim))
+        lib.set_option("tol_mode", 1)
+        cfg = self.cfg()
+        lib.set_config(cfg)
+        s = A.HostScene(lib.abi, self.pos, self.u0, self.typ)
+        lib.particle_system_calc_initial_params(s.ptr())
+        self.n0, self.lam = s.ps.n0, s.ps.lambda_
+        tpos, ttyp = self.final_positions(nu=5e-5)
+        off = np.random.default_rng(seed + 1).normal(0.0, 1e-3, size=tpos.shape)
+        self.target = (tpos + off * (ttyp == 0)[:, None], ttyp)
+
+    def cfg(self, nu=None, g=None):
+        p = dict(self.base)
+        if nu is not None:
+            p["viscosity"] = nu
+        c = make_config(self.lib.abi, **p)
+        if g is not None:
+            for d in range(self.dim):
+                c.gravity[d] = g[d]
+        return c
+
+    def scene(self, u0=None):
+        s = A.HostScene(self.lib.abi, self.pos, self.u0 if u0 is None else u0, self.typ)
+        s.ps.n0, s.ps.lambda_ = self.n0, self.lam
+        return s
+
+    def final_positions(self, nu=None, g=None, u0=None):
+        cfg = self.cfg(nu, g)
+        self.lib.set_config(cfg)
+        s = self.scene(u0)
+        self.lib.check(self.lib.mps_hip_run_steps(s.ptr(), self.steps), "run_steps")
+        self.lib.check(self.lib.mps_hip_download(s.ptr(), None), "download")
+        self.lib.mps_hip_release(s.ptr())
+        return np.array(s.arr["pos"]), np.array(s.arr["type"])
+
+    def loss(self, **kw):
+        pos, typ = self.final_positions(**kw)
+        fl = typ == 0
+        tgt = self.target[0]
+        return float(np.sum((pos[fl] - tgt[fl]) ** 2) / fl.sum())
+
+    def oracle_loss(self, nu=None, g=None, u0=None):
+        cfg = self.cfg(nu, g)
+        sc = A.HostScene(A.types(self.dim), self.pos, self.u0 if u0 is None else u0, self.typ)
+        sc.ps.n0, sc.ps.lambda_ = self.n0, self.lam
+        sim = Oracle(self.dim).sim(tol_mode=1)
+        sim.begin(sc, cfg)
+        for _ in range(self.steps):
+            sim.step(sc, cfg)
+        pos, typ = np.array(sc.arr["pos"]), np.array(sc.arr["type"])
+        fl = typ == 0
+        return float(np.sum((pos[fl] - self.target[0][fl]) ** 2) / fl.sum())
+
+    def adjoint(self, ckpt):
+        cfg = self.cfg()
+        self.lib.set_config(cfg)
+        s = self.scene()
+        n = s.num
+        tgt = np.ascontiguousarray(self.target[0], dtype=np.float64)
+        loss = np.zeros(1)
+        gnu = np.zeros(1)
+        gg = np.zeros(self.dim)
+        gu = np.zeros((n, self.dim))
+        self.lib.check(self.lib.mps_hip_adjoint(s.ptr(), self.steps, ckpt, _p(tgt), _p(loss), _p(gnu), _p(gg), _p(gu)),
+                       "adjoint")
+        return loss[0], gnu[0], gg, gu
+
+
+def _rel(a, b):
+    return abs(a - b) / max(abs(a), abs(b), 1e-300)
+
+
+def _fd(loss, P, nu, g, hnu, hg, v, hu):
+    out = {"nu": (loss(nu=nu + hnu) - loss(nu=nu - hnu)) / (2 * hnu)}
+    for d in range(2):
+        gp, gm = list(g), list(g)
+        gp[d] += hg
+        gm[d] -= hg
+        out["g%d" % d] = (loss(g=gp) - loss(g=gm)) / (2 * hg)
+    out["u"] = (loss(u0=P.u0 + hu * v) - loss(u0=P.u0 - hu * v)) / (2 * hu)
+    return out
+
+
+@pytest.mark.parametrize("steps,ckpt,cdr", [(2, 1, 0.5), (4, 2, 0.0), (20, 6, 0.5)])
+def test_adjoint_vs_finite_differences(lib2, steps, ckpt, cdr):
+    P = Problem(lib2, "dam2d_l0.01", steps, cdr=cdr)
+    say("adjoint", steps, "steps, checkpoints every", ckpt, "collision ratio", cdr)
+    loss, gnu, gg, gu = P.adjoint(ckpt)
+    assert loss == pytest.approx(P.loss(), rel=1e-12)
+    nu, g = P.base["viscosity"], [0.0, -9.81]
+    rng = np.random.default_rng(11)
+    v = np.zeros_like(P.u0)
+    fl = P.typ == 0
+    v[fl] = rng.normal(size=(fl.sum(), 2))
+    steps_fd = (1e-6 * 5e-5, 1e-6 * 9.81, v, 1e-6 * np.linalg.norm(P.u0) / np.linalg.norm(v))
+    ad = {"nu": gnu, "g0": gg[0], "g1": gg[1], "u": float(np.sum(gu * v))}
+    fds = [("gpu", _fd(P.loss, P, nu, g, *steps_fd))]
+    if steps >= 20:
+        fds.append(("oracle", _fd(P.oracle_loss, P, nu, g, *steps_fd)))
+    say("  loss %.12e" % loss)
+    assert np.all(gu[~fl] == 0.0)
+    for name, fd in fds:
+        for key in ("nu", "g0", "g1", "u"):
+            say("  %-4s adjoint %.10e  fd(%s) %.10e  rel %.2e" % (key, ad[key], name, fd[key], _rel(ad[key], fd[key])))
+    for name, fd in fds:
+        for key in ("nu", "g0", "g1", "u"):
+            assert _rel(ad[key], fd[key]) < 1e-4, (name, key)
