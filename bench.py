@@ -43,6 +43,9 @@ WORKLOADS = {
     "dam3d_l0.01": dict(dims=(1.0, 0.6, 0.5, 0.3, 0.5, 0.5), l0=0.01, dt=5e-4),
     # 64 seeded dam breaks of uneven size and viscosity (mps_amd.params.ensemble)
     "ensemble64": dict(ensemble=64),
+    # gradient of a 200-step 2-D dam break (tank 1.0 x 0.6, column 0.3 x 0.6)
+    "adjoint2d": dict(adjoint=True, dims=(1.0, 0.6, 0.0, 0.3, 0.6, 0.0), l0=0.005, dt=5e-4, steps=200,
+                      checkpoint=20, nu=1e-6, nu_target=5e-5, sigma_u0=0.01, seed=4),
 }
 
 
@@ -197,6 +200,162 @@ def run_ensemble(lib, a):
         lib.particle_system_free(ps)
 
 
+def run_adjoint(a):
+    """Gradient of L = mean |r_i(T) - r_i^target|^2 over the fluid (T = 200
+    steps, checkpoints every 20) with respect to the viscosity, gravity and
+    the initial velocities; the target is the same scene run with
+    nu = 5e-5.  Times mps_hip_adjoint (forward with checkpoints, then the
+    recomputed segments and the reverse sweep) and checks the gradient
+    against central differences (relative step 1e-6; for nu 1e-6 of the
+    target's viscosity): of the free forward, whose decisions flip at this
+    horizon; of the frozen-decision replay, plain and Richardson-
+    extrapolated; and, after one step, of the free forward."""
+    w = WORKLOADS[a.workload]
+    lib = mps_amd.load(2)
+    lib.check(lib.mps_hip_init(0), "mps_hip_init")
+    lib.set_option("tol_mode", 1)
+    lib.set_option("exit_on_error", 0)
+    from mps_amd.params import bench_config
+    steps = w["steps"]
+
+    def cfg_for(nu, g=None):
+        c = bench_config(lib, dict(dims=w["dims"], l0=w["l0"], dt=w["dt"], nu=nu), 0, a.tol)
+        if g is not None:
+            c.gravity[0], c.gravity[1] = g
+        return c
+
+    dims = (C.c_double * 6)(*w["dims"])
+    n = lib.mps_gen_dambreak(dims, w["l0"], 3, 2, None, None)
+    base = lib.particle_system_create(n)
+    lib.mps_gen_dambreak(dims, w["l0"], 3, 2, base, None)
+    arr0 = np.ctypeslib.as_array(C.cast(base.contents.particles, C.POINTER(lib.abi.Particle)), shape=(n,))
+    fl = arr0["type"] == 0
+    u0 = np.zeros((n, 2))
+    u0[fl] = np.random.default_rng(w["seed"]).normal(0.0, w["sigma_u0"], size=(fl.sum(), 2))
+    cfg = cfg_for(w["nu"])
+    lib.set_config(cfg)
+    lib.particle_system_calc_initial_params(base)
+    init = {f: getattr(base.contents, f) for f in ("n0", "lambda_", "C_LL", "C_SL", "n0_count")}
+
+    def scene(u):
+        q = lib.particle_system_create(n)
+        lib.mps_gen_dambreak(dims, w["l0"], 3, 2, q, None)
+        arr = np.ctypeslib.as_array(C.cast(q.contents.particles, C.POINTER(lib.abi.Particle)), shape=(n,))
+        arr["vel"] = u
+        for f, v in init.items():
+            setattr(q.contents, f, v)
+        return q, arr
+
+    def final(nu=None, g=None, u=None):
+        c = cfg_for(w["nu"] if nu is None else nu, g)
+        lib.set_config(c)
+        q, arr = scene(u0 if u is None else u)
+        lib.check(lib.mps_hip_run_steps(q, steps), "forward")
+        lib.check(lib.mps_hip_download(q, None), "download")
+        out = (np.array(arr["pos"]), np.array(arr["type"]))
+        lib.mps_hip_release(q)
+        lib.particle_system_free(q)
+        return out
+
+    target = np.ascontiguousarray(final(nu=w["nu_target"])[0])
+
+    def loss(**kw):
+        pos, typ = final(**kw)
+        f = typ == 0
+        return float(np.sum((pos[f] - target[f]) ** 2) / f.sum())
+
+    lib.set_config(cfg)
+    q, _ = scene(u0)
+    P = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))
+    L, gnu, gg, gu = np.zeros(1), np.zeros(1), np.zeros(2), np.zeros((n, 2))
+    t0 = time.perf_counter()
+    lib.check(lib.mps_hip_adjoint(q, steps, w["checkpoint"], P(target), P(L), P(gnu), P(gg), P(gu)), "adjoint")
+    el = time.perf_counter() - t0
+    lib.particle_system_free(q)
+    t1 = time.perf_counter()
+    loss(); fwd = time.perf_counter() - t1
+    rel = lambda x, y: abs(x - y) / max(abs(x), abs(y), 1e-300)
+    checks = {}
+    hnu = 1e-6 * w["nu_target"]
+    checks["nu"] = rel(gnu[0], (loss(nu=w["nu"] + hnu) - loss(nu=w["nu"] - hnu)) / (2 * hnu))
+    g = [0.0, -9.81]
+    for d in range(2):
+        gp, gm = list(g), list(g)
+        gp[d] += 1e-6 * 9.81
+        gm[d] -= 1e-6 * 9.81
+        checks["g%d" % d] = rel(gg[d], (loss(g=gp) - loss(g=gm)) / (2e-6 * 9.81))
+    v = np.zeros_like(u0)
+    v[fl] = np.random.default_rng(w["seed"] + 1).normal(size=(fl.sum(), 2))
+    hu = 1e-6 * np.linalg.norm(u0) / np.linalg.norm(v)
+    checks["u0_direction"] = rel(float(np.sum(gu * v)), (loss(u=u0 + hu * v) - loss(u=u0 - hu * v)) / (2 * hu))
+    # the same differences of the frozen-decision replay (the function the
+    # adjoint differentiates), at relative steps h = 1e-6 and h/2: the plain
+    # central difference at h, and Richardson's extrapolation
+    # (4 D(h/2) - D(h)) / 3, which cancels the difference's own h^2 term
+    hg = 1e-6 * 9.81
+    sets = [(w["nu"], g, u0)]
+    for f in (1.0, 0.5):
+        sets += [(w["nu"] + f * hnu, g, u0), (w["nu"] - f * hnu, g, u0),
+                 (w["nu"], [f * hg, -9.81], u0), (w["nu"], [-f * hg, -9.81], u0),
+                 (w["nu"], [0.0, -9.81 + f * hg], u0), (w["nu"], [0.0, -9.81 - f * hg], u0),
+                 (w["nu"], g, u0 + f * hu * v), (w["nu"], g, u0 - f * hu * v)]
+    nus = np.array([x[0] for x in sets])
+    gs = np.ascontiguousarray(np.array([x[1] for x in sets], dtype=np.float64))
+    us = np.ascontiguousarray(np.array([x[2] for x in sets], dtype=np.float64))
+    losses = np.zeros(len(sets))
+    lib.set_config(cfg)
+    q, _ = scene(u0)
+    t2 = time.perf_counter()
+    lib.check(lib.mps_hip_replay_losses(q, steps, P(target), len(sets), P(nus), P(gs), P(us), P(losses)), "replay")
+    replay_s = time.perf_counter() - t2
+    lib.particle_system_free(q)
+    adj = {"nu": gnu[0], "g0": gg[0], "g1": gg[1], "u0_direction": float(np.sum(gu * v))}
+    hs = {"nu": hnu, "g0": hg, "g1": hg, "u0_direction": hu}
+    central, richardson = {}, {}
+    for k, key in enumerate(adj):
+        d1 = (losses[1 + 2 * k] - losses[2 + 2 * k]) / (2 * hs[key])
+        d2 = (losses[9 + 2 * k] - losses[10 + 2 * k]) / hs[key]
+        central[key] = rel(adj[key], d1)
+        richardson[key] = rel(adj[key], (4 * d2 - d1) / 3)
+    frozen = {"base_loss_vs_adjoint": rel(losses[0], L[0]), "central_h1e-6": central,
+              "richardson_h1e-6": richardson}
+    # one step: the free forward's decisions do not flip at these steps.  Its
+    # own target (the nu_target run after one step, displaced by a seeded
+    # 1 mm offset) keeps the loss at the size of the differences; the
+    # viscosity step is 1e-4 of the target's, as a 1e-6 step moves the
+    # one-step loss by ~1e-16 relative
+    steps_saved, steps = steps, 1
+    tpos, ttyp = final(nu=w["nu_target"])
+    off = np.random.default_rng(w["seed"] + 2).normal(0.0, 1e-3, size=tpos.shape)
+    target = np.ascontiguousarray(tpos + off * (ttyp == 0)[:, None])
+    one = {}
+    lib.set_config(cfg)
+    q, _ = scene(u0)
+    L1, gnu1, gg1, gu1 = np.zeros(1), np.zeros(1), np.zeros(2), np.zeros((n, 2))
+    lib.check(lib.mps_hip_adjoint(q, 1, 1, P(target), P(L1), P(gnu1), P(gg1), P(gu1)), "adjoint 1 step")
+    lib.particle_system_free(q)
+    h1 = 1e-4 * w["nu_target"]
+    one["nu"] = rel(gnu1[0], (loss(nu=w["nu"] + h1) - loss(nu=w["nu"] - h1)) / (2 * h1))
+    for d in range(2):
+        gp, gm = list(g), list(g)
+        gp[d] += hg
+        gm[d] -= hg
+        one["g%d" % d] = rel(gg1[d], (loss(g=gp) - loss(g=gm)) / (2 * hg))
+    one["u0_direction"] = rel(float(np.sum(gu1 * v)), (loss(u=u0 + hu * v) - loss(u=u0 - hu * v)) / (2 * hu))
+    steps = steps_saved
+    print(json.dumps({
+        "metric": "adjoint gradient of a 200-step 2D dam break (forward + checkpointed reverse)",
+        "value": round(el, 3), "unit": "s per gradient", "higher_is_better": False, "n_gpus": 1,
+        "forward_only_s": round(fwd, 3), "gradient_over_forward": round(el / fwd, 2),
+        "dtype": "f64", "data": "synthetic (reference dam-break lattice generator, seeded initial velocities)",
+        "config": {"workload": a.workload, "particles": n, "fluid": int(fl.sum()), "steps": steps,
+                   "checkpoint_every": w["checkpoint"], "nu": w["nu"], "nu_target": w["nu_target"],
+                   "tolerance": "relative %g" % a.tol},
+        "loss": L[0], "grad_nu": gnu[0], "grad_g": list(gg), "grad_u0_norm": float(np.linalg.norm(gu)),
+        "fd_rel_err_one_step": one, "fd_rel_err_free_forward": checks, "fd_rel_err_frozen_decisions": frozen,
+        "replay_s_for_%d_sets" % len(sets): round(replay_s, 3)}), flush=True)
+
+
 def _fields(lib, ps, n):
     arr = np.ctypeslib.as_array(C.cast(ps.contents.particles, C.POINTER(lib.abi.Particle)), shape=(n,))
     return np.array(arr["pressure"]), np.array(arr["pos"])
@@ -318,6 +477,9 @@ def main():
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
     w = WORKLOADS[a.workload]
+    if w.get("adjoint"):
+        run_adjoint(a)
+        return
     lib = mps_amd.load(3)
     lib.check(lib.mps_hip_init(0), "mps_hip_init")
     if "ensemble" in w:

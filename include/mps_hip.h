@@ -203,6 +203,18 @@ int  mps_hip_run_batch(ParticleSystem *const *ps, SimConfig *const *cfgs, int ns
 int  mps_hip_adjoint(ParticleSystem *ps, int nsteps, int checkpoint_every, const double *target,
                      double *loss, double *grad_nu, double *grad_g, double *grad_u0);
 
+/* The loss of mps_hip_adjoint for nsets parameter sets (viscosity nus[q],
+ * gravity gs[q*DIM..], initial velocities u0s[q*N*DIM..] -- or those of ps
+ * when u0s is NULL), each computed by replaying the nsteps steps recorded
+ * from ps with every discrete decision of that recording frozen (lists,
+ * unknown set, clamp, p_min neighbour, collisions, ghosting): the function
+ * the adjoint differentiates.  Its finite differences check the adjoint at
+ * horizons where those of the free forward cross flipped decisions.
+ * Keeps nsteps recorded steps in device memory; ps is left in the recorded
+ * run's final state (device copy released). */
+int  mps_hip_replay_losses(ParticleSystem *ps, int nsteps, const double *target, int nsets,
+                           const double *nus, const double *gs, const double *u0s, double *losses);
+
 /* Options (process-wide): keys and values
  *   "tol_mode"       0 absolute sqrt(r.r) < tol (reference), 1 relative
  *                      sqrt(r.r) < tol*sqrt(b.b)

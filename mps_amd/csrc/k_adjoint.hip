@@ -66,6 +66,24 @@ __device__ __forceinline__ double wdiff(double r, double re, double r_min)
     return -re / (r * r);
 }
 
+// w at the replayed distance r with its two branches (cut-off at re, clamp
+// at r_min) taken at the recorded distance tr: the function whose
+// derivative wdiff(tr) is, smooth in r along a replay
+__device__ __forceinline__ double wfrozen(double r, double tr, double re, double r_min)
+{
+    if (tr >= re) return 0.0;
+    if (tr < r_min) return re / r_min - 1.0;
+    return re / r - 1.0;
+}
+
+__device__ __forceinline__ double dist(const dvec &a, const dvec &b)
+{
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < D_; k++) s += (a.c[k] - b.c[k]) * (a.c[k] - b.c[k]);
+    return sqrt(s);
+}
+
 __device__ __forceinline__ dvec dsub(const dvec &a, const dvec &b)
 {
     dvec d;
@@ -88,6 +106,13 @@ __device__ __forceinline__ void daxpy(dvec &y, double a, const dvec &x)
 {
 #pragma unroll
     for (int k = 0; k < D_; k++) y.c[k] += a * x.c[k];
+}
+
+static inline dvec dzero_host()
+{
+    dvec d;
+    for (int k = 0; k < (D_ == 3 ? 4 : 2); k++) d.c[k] = 0.0;
+    return d;
 }
 
 __device__ __forceinline__ dvec dzero()
@@ -630,6 +655,208 @@ __global__ void k_adj_param_fin(const double *part, int nblocks, double *acc)
 }
 
 // ---------------------------------------------------------------------
+// Replay: the forward of recorded steps re-run with other parameters
+// (viscosity, gravity, initial velocities) but with every discrete
+// decision of the recording -- lists, unknown set, clamp mask, the
+// neighbour giving p_min, the collisions that fired and their order,
+// the pair cut-offs, ghosting.  This is the function the adjoint
+// differentiates; its finite differences check the adjoint where those of
+// the free forward cross a flipped decision (long horizons).
+// ---------------------------------------------------------------------
+
+// predictor over the stale list, on the replayed state (r, u); non-fluid
+// particles keep the recording's state
+__global__ void k_rep_pred(int n, const int *__restrict__ typ, const dvec *__restrict__ r, const dvec *__restrict__ u,
+                           const int *__restrict__ lst, const int *__restrict__ lidx, StepConsts k, dvec g,
+                           const dvec *__restrict__ tr, const dvec *__restrict__ tu, dvec *__restrict__ rs,
+                           dvec *__restrict__ us)
+{
+    const int i = blockIdx.x * AB + threadIdx.x;
+    if (i >= n) return;
+    if (typ[i] != FLUID_PARTICLE) {
+        rs[i] = tr[i];
+        us[i] = tu[i];
+        return;
+    }
+    dvec lap = dzero();
+    for (int q = lst[i]; q < lst[i + 1]; q++) {
+        const int j = lidx[q];
+        const dvec d = dsub(r[j], r[i]);
+        const double w = wfrozen(sqrt(ddot(d, d)), dist(tr[j], tr[i]), k.re_lap, k.r_min);
+        daxpy(lap, w, dsub(u[j], u[i]));
+    }
+    dvec a = g;
+    daxpy(a, k.visc_coeff, lap);
+    dvec v = u[i];
+    daxpy(v, k.dt, a);
+    dvec x = r[i];
+    daxpy(x, k.dt, v);
+    rs[i] = x;
+    us[i] = v;
+}
+
+// collisions: the recording's sequence (decided on the recorded r*, u*),
+// applied to the replayed r*, u*
+__global__ void k_rep_coll(int n, const int *__restrict__ typ, const dvec *__restrict__ trs,
+                           const dvec *__restrict__ tus, const dvec *__restrict__ rs, const dvec *__restrict__ us,
+                           const int *__restrict__ lst, const int *__restrict__ lidx, StepConsts k,
+                           dvec *__restrict__ rss, dvec *__restrict__ uss)
+{
+    const int i = blockIdx.x * AB + threadIdx.x;
+    if (i >= n) return;
+    if (typ[i] != FLUID_PARTICLE) {
+        rss[i] = rs[i];
+        uss[i] = us[i];
+        return;
+    }
+    dvec vt = tus[i], v = us[i];
+    const dvec ti = trs[i];
+    if (k.collide) {
+#if MPS_DIM == 2
+        int last = -1;
+        for (;;) {
+            int best = INT_MAX;
+            for (int q = lst[i]; q < lst[i + 1]; q++) {
+                const int j = lidx[q];
+                if (j <= last || j >= best || typ[j] == GHOST_PARTICLE) continue;
+                const dvec d = dsub(trs[j], ti);
+                if (ddot(d, d) >= k.cd2) continue;
+                best = j;
+            }
+            if (best == INT_MAX) break;
+            last = best;
+            const int j = best;
+#else
+        for (int q = lst[i]; q < lst[i + 1]; q++) {
+            const int j = lidx[q];
+            if (typ[j] == GHOST_PARTICLE) continue;
+#endif
+            // the decision, on the recording
+            const dvec d = dsub(trs[j], ti);
+            const double r2 = ddot(d, d);
+            if (r2 >= k.cd2) continue;
+            const double r = sqrt(r2);
+            if (r < k.col_tiny) continue;
+            double vn = 0.0;
+#pragma unroll
+            for (int dd = 0; dd < D_; dd++) vn += (vt.c[dd] - tus[j].c[dd]) * (d.c[dd] / r);
+            if (vn <= 0.0) continue;
+#pragma unroll
+            for (int dd = 0; dd < D_; dd++) vt.c[dd] -= k.e_half * vn * (d.c[dd] / r);
+            // the same update on the replayed state
+            const dvec e = dsub(rs[j], rs[i]);
+            const double re_ = sqrt(ddot(e, e));
+            double wn = 0.0;
+#pragma unroll
+            for (int dd = 0; dd < D_; dd++) wn += (v.c[dd] - us[j].c[dd]) * (e.c[dd] / re_);
+#pragma unroll
+            for (int dd = 0; dd < D_; dd++) v.c[dd] -= k.e_half * wn * (e.c[dd] / re_);
+        }
+    }
+    dvec x = rs[i];
+#pragma unroll
+    for (int dd = 0; dd < D_; dd++) x.c[dd] += k.dt * (v.c[dd] - us[i].c[dd]);
+    rss[i] = x;
+    uss[i] = v;
+}
+
+// PPE of the replayed r** with the recording's unknown set: the eq-order
+// rows of k_adj_fill, and b from the replayed density
+__global__ void k_rep_fill(int n, const int *__restrict__ typ, const int *__restrict__ unk, const int *__restrict__ eq,
+                           const int *__restrict__ roff, const dvec *__restrict__ rss, const dvec *__restrict__ trss,
+                           const int *__restrict__ lst, const int *__restrict__ lidx, StepConsts k,
+                           int *__restrict__ col, double *__restrict__ val, double *__restrict__ rhs)
+{
+    const int i = blockIdx.x * AB + threadIdx.x;
+    if (i >= n || !unk[i]) return;
+    const int e = eq[i];
+    int at = roff[i];
+    const int dpos = at++;
+    double sumw = 0.0, nd = 0.0;
+    for (int q = lst[i]; q < lst[i + 1]; q++) {
+        const int j = lidx[q];
+        const dvec d = dsub(rss[j], rss[i]);
+        const double rr = sqrt(ddot(d, d));
+        const double tr = dist(trss[j], trss[i]);
+        nd += wfrozen(rr, tr, k.re_n, k.r_min);
+        if (typ[j] == DUMMY_PARTICLE) continue;
+        const double w = wfrozen(rr, tr, k.re_lap, k.r_min);
+        sumw += w;
+        if (unk[j]) {
+            col[at] = eq[j];
+            val[at] = k.neg_coeff * w;
+            at++;
+        }
+    }
+    col[dpos] = e;
+    val[dpos] = k.diag_coeff * sumw;
+    rhs[e] = typ[i] == FLUID_PARTICLE ? k.src_k * (nd - k.n0) / k.n0 : 0.0;
+}
+
+// pressure gradient with the recording's clamp mask and p_min neighbour
+__global__ void k_rep_corr(int n, const int *__restrict__ typ, const int *__restrict__ unk,
+                           const double *__restrict__ tx, const double *__restrict__ x, const dvec *__restrict__ trss,
+                           const dvec *__restrict__ rss, const dvec *__restrict__ uss, const int *__restrict__ lst,
+                           const int *__restrict__ lidx, StepConsts k, int clamp, dvec *__restrict__ r,
+                           dvec *__restrict__ u)
+{
+    const int i = blockIdx.x * AB + threadIdx.x;
+    if (i >= n) return;
+    if (typ[i] != FLUID_PARTICLE) {
+        r[i] = rss[i];
+        u[i] = uss[i];
+        return;
+    }
+    // recorded pressures decide the clamp and p_min
+    auto tp = [&](int j) {
+        double p = unk[j] ? tx[j] : 0.0;
+        if (clamp && typ[j] == FLUID_PARTICLE && p < 0.0) p = 0.0;
+        return p;
+    };
+    auto pp = [&](int j) {
+        if (!unk[j]) return 0.0;
+        if (clamp && typ[j] == FLUID_PARTICLE && tx[j] < 0.0) return 0.0;
+        return x[j];
+    };
+    double pm = tp(i);
+    int mu = i;
+    for (int q = lst[i]; q < lst[i + 1]; q++) {
+        const int j = lidx[q];
+        if (tp(j) < pm) {
+            pm = tp(j);
+            mu = j;
+        }
+    }
+    const double pmin = pp(mu);
+    dvec grad = dzero();
+    for (int q = lst[i]; q < lst[i + 1]; q++) {
+        const int j = lidx[q];
+        const dvec td = dsub(trss[j], trss[i]);
+        if (ddot(td, td) < k.grad_min_r2) continue;
+        const dvec d = dsub(rss[j], rss[i]);
+        const double r2 = ddot(d, d);
+        const double w = wfrozen(sqrt(r2), sqrt(ddot(td, td)), k.re_lap, k.r_min);
+        daxpy(grad, (pp(j) - pmin) / r2 * w, d);
+    }
+    dvec a = dzero();
+    daxpy(a, k.neg_gc / k.rho, grad);
+    dvec v = uss[i];
+    daxpy(v, k.dt, a);
+    dvec xx = rss[i];
+    daxpy(xx, k.dt * k.dt, a);
+    u[i] = v;
+    r[i] = xx;
+}
+
+__global__ void k_rep_set_u0(int n, const int *__restrict__ typ, const double *__restrict__ u0, dvec *__restrict__ u)
+{
+    const int i = blockIdx.x * AB + threadIdx.x;
+    if (i >= n || typ[i] != FLUID_PARTICLE) return;
+#pragma unroll
+    for (int d = 0; d < D_; d++) u[i].c[d] = u0[(size_t)i * D_ + d];
+}
+
+// ---------------------------------------------------------------------
 // driver
 // ---------------------------------------------------------------------
 
@@ -680,19 +907,15 @@ static int adj_alloc(AdjWork &w, int n, hipStream_t s)
     return 0;
 }
 
-// The reverse of one recorded step: (rb, ub) = adjoint of the step's
-// outputs in, adjoint of its inputs out.  ls/lsi: the list the predictor
-// used (the previous step's search).
-static int adj_step(AdjWork &w, const TapeStep &t, const int *ls, const int *lsi, const StepConsts &k, int clamp,
-                    const SolveOpts &so, hipStream_t s)
+// The step's PPE in eq order (the unknowns of the recording t, rows of
+// k_adj_fill or, with replay, k_rep_fill on positions rss) solved by CG;
+// right-hand side w.xbar (adjoint) or the replayed source; the solution,
+// per eq row, in w.sol.
+static int eq_csr_solve(AdjWork &w, const TapeStep &t, bool replay, const dvec *rss, const StepConsts &k,
+                        const SolveOpts &so, hipStream_t s)
 {
     const int n = w.n;
     const unsigned g = ab(n);
-    hipLaunchKernelGGL(k_adj_corr_pre, dim3(g), dim3(AB), 0, s, n, t.typ, t.x, t.unk, t.rss, t.lst, t.lidx, w.rb, w.ub,
-                       k, clamp, w.abar, w.F, w.mu);
-    hipLaunchKernelGGL(k_adj_corr, dim3(g), dim3(AB), 0, s, n, t.typ, t.x, t.unk, t.rss, t.lst, t.lidx, w.rb, w.ub,
-                       w.abar, w.F, w.mu, k, clamp, w.xbar, w.rbss, w.ubss);
-    // adjoint PPE on the step's matrix
     hipLaunchKernelGGL(k_adj_rowlen, dim3(g), dim3(AB), 0, s, n, t.unk, t.lst, t.lidx, w.len);
     LAUNCH_CHECK();
     int rc = launch_scan_excl(w.len, w.roff, n, w.scan_blk, s);
@@ -713,33 +936,53 @@ static int adj_step(AdjWork &w, const TapeStep &t, const int *ls, const int *lsi
         HIP_OK(hipMalloc(&w.col, sizeof(int) * w.cap_nnz));
         HIP_OK(hipMalloc(&w.val, sizeof(double) * w.cap_nnz));
     }
-    hipLaunchKernelGGL(k_adj_fill, dim3(g), dim3(AB), 0, s, n, t.typ, t.unk, w.eq, w.roff, t.rss, t.lst, t.lidx,
-                       w.xbar, k, w.col, w.val, w.rhs);
+    if (replay)
+        hipLaunchKernelGGL(k_rep_fill, dim3(g), dim3(AB), 0, s, n, t.typ, t.unk, w.eq, w.roff, rss, t.rss, t.lst,
+                           t.lidx, k, w.col, w.val, w.rhs);
+    else
+        hipLaunchKernelGGL(k_adj_fill, dim3(g), dim3(AB), 0, s, n, t.typ, t.unk, w.eq, w.roff, rss, t.lst, t.lidx,
+                           w.xbar, k, w.col, w.val, w.rhs);
     LAUNCH_CHECK();
-    if (neq > 0) {
-        // row pointers of the eq-order rows: the unknowns' row offsets in order
-        std::vector<int> len_h(n), unk_h(n), roff_h(n + 1);
-        HIP_OK(hipMemcpyAsync(roff_h.data(), w.roff, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToHost, s));
-        HIP_OK(hipMemcpyAsync(unk_h.data(), t.unk, sizeof(int) * n, hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
-        std::vector<int> rp(neq + 1);
-        int e = 0;
-        for (int i = 0; i < n; i++)
-            if (unk_h[i]) rp[e++] = roff_h[i];
-        rp[neq] = (int)nnz;
-        int *rp_d = w.len;    // reuse: n >= neq + 1 unless every particle is an unknown
-        int *tmp = nullptr;
-        if (neq + 1 > n) {
-            HIP_OK(hipMalloc(&tmp, sizeof(int) * (neq + 1)));
-            rp_d = tmp;
-        }
-        HIP_OK(hipMemcpyAsync(rp_d, rp.data(), sizeof(int) * (neq + 1), hipMemcpyHostToDevice, s));
-        SolveResult sr{};
-        rc = csr_cg_device(neq, nnz, rp_d, w.col, w.val, w.rhs, w.sol, so, &sr, s);
-        HIP_OK(hipStreamSynchronize(s));
-        if (tmp) hipFree(tmp);
-        if (rc) return rc;
+    if (neq == 0) return 0;
+    // row pointers of the eq rows: the unknowns' offsets, in order
+    std::vector<int> unk_h(n), roff_h(n + 1);
+    HIP_OK(hipMemcpyAsync(roff_h.data(), w.roff, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(unk_h.data(), t.unk, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    std::vector<int> rp(neq + 1);
+    int e = 0;
+    for (int i = 0; i < n; i++)
+        if (unk_h[i]) rp[e++] = roff_h[i];
+    rp[neq] = (int)nnz;
+    int *rp_d = w.len;    // free again; n >= neq + 1 unless every particle is an unknown
+    int *tmp = nullptr;
+    if (neq + 1 > n) {
+        HIP_OK(hipMalloc(&tmp, sizeof(int) * (neq + 1)));
+        rp_d = tmp;
     }
+    HIP_OK(hipMemcpyAsync(rp_d, rp.data(), sizeof(int) * (neq + 1), hipMemcpyHostToDevice, s));
+    SolveResult sr{};
+    rc = csr_cg_device(neq, nnz, rp_d, w.col, w.val, w.rhs, w.sol, so, &sr, s);
+    HIP_OK(hipStreamSynchronize(s));
+    if (tmp) hipFree(tmp);
+    return rc;
+}
+
+// The reverse of one recorded step: (rb, ub) = adjoint of the step's
+// outputs in, adjoint of its inputs out.  ls/lsi: the list the predictor
+// used (the previous step's search).
+static int adj_step(AdjWork &w, const TapeStep &t, const int *ls, const int *lsi, const StepConsts &k, int clamp,
+                    const SolveOpts &so, hipStream_t s)
+{
+    const int n = w.n;
+    const unsigned g = ab(n);
+    hipLaunchKernelGGL(k_adj_corr_pre, dim3(g), dim3(AB), 0, s, n, t.typ, t.x, t.unk, t.rss, t.lst, t.lidx, w.rb, w.ub,
+                       k, clamp, w.abar, w.F, w.mu);
+    hipLaunchKernelGGL(k_adj_corr, dim3(g), dim3(AB), 0, s, n, t.typ, t.x, t.unk, t.rss, t.lst, t.lidx, w.rb, w.ub,
+                       w.abar, w.F, w.mu, k, clamp, w.xbar, w.rbss, w.ubss);
+    // adjoint PPE on the step's matrix
+    int rc = eq_csr_solve(w, t, false, t.rss, k, so, s);
+    if (rc) return rc;
     hipLaunchKernelGGL(k_adj_lambda, dim3(g), dim3(AB), 0, s, n, t.unk, w.eq, w.sol, w.lam);
     hipLaunchKernelGGL(k_adj_ppe_pos, dim3(g), dim3(AB), 0, s, n, t.typ, t.unk, t.x, w.lam, t.rss, t.lst, t.lidx, k,
                        w.rbss);
@@ -902,5 +1145,115 @@ extern "C" int mps_hip_adjoint(ParticleSystem *ps, int nsteps, int checkpoint_ev
     if (ls0.lidx) hipFree(ls0.lidx);
     adj_free(w);
     neighbor_list_free(nl);
+    return rc;
+}
+
+// Loss of the frozen-decision replay (see above) for several parameter
+// sets: the steps are recorded once from ps under g_config, then each set
+// (nus[q], gs[q*DIM..], u0s[q*N*DIM..] or ps's velocities when u0s is null)
+// replays them.  Checks the adjoint by finite differences that cannot
+// cross a decision.  ps is left in the recorded run's final state.
+extern "C" int mps_hip_replay_losses(ParticleSystem *ps, int nsteps, const double *target, int nsets,
+                                     const double *nus, const double *gs, const double *u0s, double *losses)
+{
+    if (!ps || nsteps < 1 || !target || nsets < 1 || !nus || !gs || !losses) return MPS_ERR_ARG;
+    if (!g_config) return MPS_ERR_NO_CONFIG;
+    SimConfig *cf = g_config;
+    if (const char *u = adj_unsupported(cf)) {
+        mps_set_error(MPS_ERR_UNSUPPORTED, __FILE__, __LINE__, u);
+        return MPS_ERR_UNSUPPORTED;
+    }
+    const int n = ps->num;
+    Ctx *c = mps_ctx_for(ps, 1);
+    if (!c) return mps_hip_last_error();
+    hipStream_t s = c->stream;
+    const double re = mps_search_radius(cf);
+    const Grid grid = mps_grid_for(re, cf->domain_min, cf->domain_max);
+    int rc = mps_ctx_upload(c, ps, nullptr, grid);
+    StepConsts k = mps_consts(cf, ps, grid, re);
+    k.stride = c->stride;
+    if (!rc) rc = mps_ctx_search(c, k);
+    std::vector<TapeStep> tapes(nsteps);
+    TapeStep ls0;
+    for (int t = 0; t < nsteps && !rc; t++) rc = tape_alloc(&tapes[t], n);
+    if (!rc && hipMalloc(&ls0.lst, sizeof(int) * ((size_t)n + 1)) != hipSuccess) rc = MPS_ERR_HIP;
+    if (!rc) rc = rec_list(c, &ls0);
+    for (int t = 0; t < nsteps && !rc; t++) {
+        g_rec = &tapes[t];
+        g_rec_err = 0;
+        rc = mps_ctx_step(c, cf);
+        g_rec = nullptr;
+        if (!rc) rc = g_rec_err;
+    }
+    if (!rc) rc = mps_ctx_download(c, ps, nullptr);
+    AdjWork w;
+    dvec *r = nullptr, *u = nullptr, *rs = nullptr, *us = nullptr, *rss = nullptr, *uss = nullptr;
+    double *xr = nullptr, *u0d = nullptr;
+    if (!rc) rc = adj_alloc(w, n, s);
+    if (!rc) {
+        dvec **bufs[] = {&r, &u, &rs, &us, &rss, &uss};
+        for (dvec **b : bufs)
+            if (hipMalloc((void **)b, sizeof(dvec) * n) != hipSuccess) rc = MPS_ERR_HIP;
+        if (hipMalloc(&xr, sizeof(double) * n) != hipSuccess) rc = MPS_ERR_HIP;
+        if (u0s && hipMalloc(&u0d, sizeof(double) * (size_t)n * D_) != hipSuccess) rc = MPS_ERR_HIP;
+    }
+    SolveOpts so{};
+    so.solver = 0;
+    so.max_iter = cf->cg_max_iter;
+    so.tol = cf->cg_tolerance;
+    so.tol_mode = (int)mps_hip_get_option("tol_mode");
+    so.check_every = (int)mps_hip_get_option("check_every");
+    const unsigned g = ab(n);
+    std::vector<dvec> rh(n);
+    for (int q = 0; q < nsets && !rc; q++) {
+        StepConsts kq = k;
+        kq.visc_coeff = nus[q] * k.coeff;
+        dvec gq = dzero_host();
+        for (int d = 0; d < D_; d++) gq.c[d] = gs[(size_t)q * D_ + d];
+        HIP_OK(hipMemcpyAsync(r, tapes[0].r, sizeof(dvec) * n, hipMemcpyDeviceToDevice, s));
+        HIP_OK(hipMemcpyAsync(u, tapes[0].u, sizeof(dvec) * n, hipMemcpyDeviceToDevice, s));
+        if (u0s) {
+            HIP_OK(hipMemcpyAsync(u0d, u0s + (size_t)q * n * D_, sizeof(double) * (size_t)n * D_,
+                                  hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_rep_set_u0, dim3(g), dim3(AB), 0, s, n, tapes[0].typ, u0d, u);
+        }
+        for (int t = 0; t < nsteps && !rc; t++) {
+            const TapeStep &T = tapes[t];
+            const TapeStep &L = t == 0 ? ls0 : tapes[t - 1];
+            hipLaunchKernelGGL(k_rep_pred, dim3(g), dim3(AB), 0, s, n, T.typ, r, u, L.lst, L.lidx, kq, gq, T.r, T.u, rs,
+                               us);
+            hipLaunchKernelGGL(k_rep_coll, dim3(g), dim3(AB), 0, s, n, T.typ, T.rs, T.us, rs, us, L.lst, L.lidx, kq, rss,
+                               uss);
+            LAUNCH_CHECK();
+            rc = eq_csr_solve(w, T, true, rss, kq, so, s);
+            if (rc) break;
+            hipLaunchKernelGGL(k_adj_lambda, dim3(g), dim3(AB), 0, s, n, T.unk, w.eq, w.sol, xr);
+            hipLaunchKernelGGL(k_rep_corr, dim3(g), dim3(AB), 0, s, n, T.typ, T.unk, T.x, xr, T.rss, rss, uss, T.lst,
+                               T.lidx, kq, cf->clamp_negative_pressure, r, u);
+            LAUNCH_CHECK();
+        }
+        if (rc) break;
+        HIP_OK(hipMemcpyAsync(rh.data(), r, sizeof(dvec) * n, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        double l = 0.0;
+        int nf = 0;
+        for (int i = 0; i < n; i++) {
+            if (ps->particles[i].type != FLUID_PARTICLE) continue;
+            nf++;
+            for (int d = 0; d < D_; d++) {
+                const double e = rh[i].c[d] - target[(size_t)i * D_ + d];
+                l += e * e;
+            }
+        }
+        losses[q] = nf > 0 ? l / nf : 0.0;
+    }
+    mps_hip_release(ps);
+    for (auto &t : tapes) tape_free(&t);
+    if (ls0.lst) hipFree(ls0.lst);
+    if (ls0.lidx) hipFree(ls0.lidx);
+    void *bufs[] = {r, u, rs, us, rss, uss, xr, u0d};
+    for (void *b : bufs)
+        if (b) hipFree(b);
+    adj_free(w);
     return rc;
 }

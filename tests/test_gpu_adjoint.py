@@ -13,7 +13,10 @@ the longest case also of the CPU oracle's forward (the reference's
 arithmetic restated).  The CG runs to a relative residual of 1e-13 so that
 the differences resolve the loss.  Steps: 1e-6 relative for gravity and
 the velocities; for the viscosity 1e-6 of the target's 5e-5 (1e-6 of the
-base 1e-6 moves the loss by ~1e-14 relative, the loss's own rounding).
+base 1e-6 moves the loss by ~1e-14 relative, the loss's own rounding);
+after a single step, where that step moves the loss by ~1e-16 relative,
+1e-4 of it (5e-9, which moves positions by ~1e-12 m, far too little to
+change a decision).
 """
 import ctypes as C
 import sys
@@ -132,8 +135,9 @@ def _fd(loss, P, nu, g, hnu, hg, v, hu):
     return out
 
 
-@pytest.mark.parametrize("steps,ckpt,cdr", [(2, 1, 0.5), (4, 2, 0.0), (20, 6, 0.5)])
-def test_adjoint_vs_finite_differences(lib2, steps, ckpt, cdr):
+@pytest.mark.parametrize("steps,ckpt,cdr,rnu", [(1, 1, 0.5, 1e-4), (2, 1, 0.5, 1e-6), (4, 2, 0.0, 1e-6),
+                                               (20, 6, 0.5, 1e-6)])
+def test_adjoint_vs_finite_differences(lib2, steps, ckpt, cdr, rnu):
     P = Problem(lib2, "dam2d_l0.01", steps, cdr=cdr)
     say("adjoint", steps, "steps, checkpoints every", ckpt, "collision ratio", cdr)
     loss, gnu, gg, gu = P.adjoint(ckpt)
@@ -143,7 +147,7 @@ def test_adjoint_vs_finite_differences(lib2, steps, ckpt, cdr):
     v = np.zeros_like(P.u0)
     fl = P.typ == 0
     v[fl] = rng.normal(size=(fl.sum(), 2))
-    steps_fd = (1e-6 * 5e-5, 1e-6 * 9.81, v, 1e-6 * np.linalg.norm(P.u0) / np.linalg.norm(v))
+    steps_fd = (rnu * 5e-5, 1e-6 * 9.81, v, 1e-6 * np.linalg.norm(P.u0) / np.linalg.norm(v))
     ad = {"nu": gnu, "g0": gg[0], "g1": gg[1], "u": float(np.sum(gu * v))}
     fds = [("gpu", _fd(P.loss, P, nu, g, *steps_fd))]
     if steps >= 20:
@@ -156,3 +160,55 @@ def test_adjoint_vs_finite_differences(lib2, steps, ckpt, cdr):
     for name, fd in fds:
         for key in ("nu", "g0", "g1", "u"):
             assert _rel(ad[key], fd[key]) < 1e-4, (name, key)
+
+
+def test_adjoint_200_steps_vs_frozen_replay(lib2):
+    """200 steps, checkpoints every 20.  Over this horizon a perturbation of
+    1e-6 flips discrete decisions of the free forward (collisions, clamps,
+    flags, list membership), so the differences are taken of the replay
+    (mps_hip_replay_losses): the forward re-run with every decision of the
+    recorded trajectory, the function the adjoint differentiates.  The
+    replay's loss at the recorded parameters must equal the adjoint's.
+    Central differences at relative step h = 1e-6 carry an h^2 error of a
+    few 1e-4 at this horizon; Richardson's extrapolation from h and h/2
+    cancels it."""
+    steps = 200
+    P = Problem(lib2, "dam2d_l0.01", steps)
+    say("adjoint", steps, "steps, checkpoints every 20")
+    loss, gnu, gg, gu = P.adjoint(20)
+    nu, g = P.base["viscosity"], [0.0, -9.81]
+    rng = np.random.default_rng(11)
+    v = np.zeros_like(P.u0)
+    fl = P.typ == 0
+    v[fl] = rng.normal(size=(fl.sum(), 2))
+    hnu, hg = 1e-6 * 5e-5, 1e-6 * 9.81
+    hu = 1e-6 * np.linalg.norm(P.u0) / np.linalg.norm(v)
+    sets = [(nu, g, P.u0)]
+    for f in (1.0, 0.5):
+        sets += [(nu + f * hnu, g, P.u0), (nu - f * hnu, g, P.u0),
+                 (nu, [f * hg, -9.81], P.u0), (nu, [-f * hg, -9.81], P.u0),
+                 (nu, [0.0, -9.81 + f * hg], P.u0), (nu, [0.0, -9.81 - f * hg], P.u0),
+                 (nu, g, P.u0 + f * hu * v), (nu, g, P.u0 - f * hu * v)]
+    nus = np.array([x[0] for x in sets])
+    gs = np.ascontiguousarray(np.array([x[1] for x in sets], dtype=np.float64))
+    us = np.ascontiguousarray(np.array([x[2] for x in sets], dtype=np.float64))
+    losses = np.zeros(len(sets))
+    P.lib.set_config(P.cfg())
+    s = P.scene()
+    tgt = np.ascontiguousarray(P.target[0], dtype=np.float64)
+    P.lib.check(P.lib.mps_hip_replay_losses(s.ptr(), steps, _p(tgt), len(sets), _p(nus), _p(gs), _p(us),
+                                            _p(losses)), "replay")
+    say("  loss %.12e  replay %.12e" % (loss, losses[0]))
+    assert losses[0] == pytest.approx(loss, rel=1e-12)
+    ad = {"nu": gnu, "g0": gg[0], "g1": gg[1], "u": float(np.sum(gu * v))}
+    hs = {"nu": hnu, "g0": hg, "g1": hg, "u": hu}
+    errs = {}
+    for k, key in enumerate(ad):
+        d1 = (losses[1 + 2 * k] - losses[2 + 2 * k]) / (2 * hs[key])
+        d2 = (losses[9 + 2 * k] - losses[10 + 2 * k]) / hs[key]
+        rich = (4 * d2 - d1) / 3
+        errs[key] = _rel(ad[key], rich)
+        say("  %-3s adjoint %.10e  central %.10e (rel %.2e)  richardson %.10e (rel %.2e)"
+            % (key, ad[key], d1, _rel(ad[key], d1), rich, errs[key]))
+    for key, e in errs.items():
+        assert e < 1e-4, key
