@@ -215,6 +215,22 @@ int  mps_hip_adjoint(ParticleSystem *ps, int nsteps, int checkpoint_every, const
 int  mps_hip_replay_losses(ParticleSystem *ps, int nsteps, const double *target, int nsets,
                            const double *nus, const double *gs, const double *u0s, double *losses);
 
+/* Backward of one timestep (no counterpart in the reference): runs
+ * simulation_step's step from ps under g_config -- predictor on the stale
+ * list nl as simulation_step (3DP/include/simulation.h:17) receives it, or
+ * on a search of ps's positions when nl is NULL -- then its reverse, with
+ * the decisions of that step frozen, for the adjoints of the step's output
+ * positions and velocities rbar_out, ubar_out (N x DIM each).  Outputs
+ * (each may be NULL): rbar_in, ubar_in (N x DIM, adjoints of the input
+ * positions and velocities; 0 for non-fluid particles, whose state the
+ * step holds fixed), *grad_nu and grad_g[DIM] (this step's share of the
+ * viscosity and gravity gradients).  ps and nl are left as simulation_step
+ * leaves them (device copy released).  Chained backwards over steps, with
+ * each step's input state, it gives mps_hip_adjoint's gradient. */
+int  mps_hip_step_backward(ParticleSystem *ps, NeighborList *nl, const double *rbar_out,
+                           const double *ubar_out, double *rbar_in, double *ubar_in, double *grad_nu,
+                           double *grad_g);
+
 /* Options (process-wide): keys and values
  *   "tol_mode"       0 absolute sqrt(r.r) < tol (reference), 1 relative
  *                      sqrt(r.r) < tol*sqrt(b.b)

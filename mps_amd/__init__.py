@@ -98,6 +98,7 @@ class Lib:
             "mps_hip_run_batch": (C.c_int, [C.POINTER(PS), C.POINTER(C.POINTER(a.SimConfig)), C.c_int, C.c_int]),
             "mps_hip_adjoint": (C.c_int, [PS, C.c_int, C.c_int, D, D, D, D, D]),
             "mps_hip_replay_losses": (C.c_int, [PS, C.c_int, D, C.c_int, D, D, D, D]),
+            "mps_hip_step_backward": (C.c_int, [PS, NL, D, D, D, D, D, D]),
             "mps_hip_set_option": (C.c_int, [C.c_char_p, C.c_double]),
             "mps_hip_get_option": (C.c_double, [C.c_char_p]),
             "mps_hip_get_stats": (C.c_int, [PS, C.POINTER(StepStats)]),

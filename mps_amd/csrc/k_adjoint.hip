@@ -1257,3 +1257,96 @@ extern "C" int mps_hip_replay_losses(ParticleSystem *ps, int nsteps, const doubl
     adj_free(w);
     return rc;
 }
+
+// One step forward from ps (the stale list nl as simulation_step receives
+// it, or a search of ps's positions when nl is null), recorded, then its
+// reverse for the output adjoints (rbar_out, ubar_out): a vector-Jacobian
+// product of the step.  ps and nl advance as simulation_step leaves them.
+extern "C" int mps_hip_step_backward(ParticleSystem *ps, NeighborList *nl, const double *rbar_out,
+                                     const double *ubar_out, double *rbar_in, double *ubar_in, double *grad_nu,
+                                     double *grad_g)
+{
+    if (!ps || !rbar_out || !ubar_out) return MPS_ERR_ARG;
+    if (!g_config) return MPS_ERR_NO_CONFIG;
+    SimConfig *cf = g_config;
+    if (const char *u = adj_unsupported(cf)) {
+        mps_set_error(MPS_ERR_UNSUPPORTED, __FILE__, __LINE__, u);
+        return MPS_ERR_UNSUPPORTED;
+    }
+    const int n = ps->num;
+    Ctx *c = mps_ctx_for(ps, 1);
+    if (!c) return mps_hip_last_error();
+    hipStream_t s = c->stream;
+    const double re = mps_search_radius(cf);
+    const Grid grid = mps_grid_for(re, cf->domain_min, cf->domain_max);
+    int rc = mps_ctx_upload(c, ps, nl, grid);
+    StepConsts k = mps_consts(cf, ps, grid, re);
+    k.stride = c->stride;
+    if (!rc && !nl) rc = mps_ctx_search(c, k);
+    AdjWork w;
+    TapeStep t, ls0;
+    if (!rc) rc = adj_alloc(w, n, s);
+    if (!rc) rc = tape_alloc(&t, n);
+    if (!rc && hipMalloc(&ls0.lst, sizeof(int) * ((size_t)n + 1)) != hipSuccess) rc = MPS_ERR_HIP;
+    if (!rc) rc = rec_list(c, &ls0);
+    if (!rc) {
+        g_rec = &t;
+        g_rec_err = 0;
+        rc = mps_ctx_step(c, cf);
+        g_rec = nullptr;
+        if (!rc) rc = g_rec_err;
+    }
+    std::vector<dvec> rb_h(n), ub_h(n);
+    for (int i = 0; i < n; i++) {
+        rb_h[i] = dzero_host();
+        ub_h[i] = dzero_host();
+        for (int d = 0; d < D_; d++) {
+            rb_h[i].c[d] = rbar_out[(size_t)i * D_ + d];
+            ub_h[i].c[d] = ubar_out[(size_t)i * D_ + d];
+        }
+    }
+    if (!rc) {
+        if (hipMemcpyAsync(w.rb, rb_h.data(), sizeof(dvec) * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipMemcpyAsync(w.ub, ub_h.data(), sizeof(dvec) * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            rc = MPS_ERR_HIP;
+    }
+    SolveOpts so{};
+    so.solver = 0;
+    so.max_iter = cf->cg_max_iter;
+    so.tol = cf->cg_tolerance;
+    so.tol_mode = (int)mps_hip_get_option("tol_mode");
+    so.check_every = (int)mps_hip_get_option("check_every");
+    if (!rc) rc = adj_step(w, t, ls0.lst, ls0.lidx, k, cf->clamp_negative_pressure, so, s);
+    int herr = 0;
+    double params[4] = {0.0, 0.0, 0.0, 0.0};
+    if (!rc) {
+        if (hipMemcpyAsync(&herr, w.err, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipMemcpyAsync(params, w.params, sizeof params, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipMemcpyAsync(rb_h.data(), w.rb, sizeof(dvec) * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipMemcpyAsync(ub_h.data(), w.ub, sizeof(dvec) * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            rc = MPS_ERR_HIP;
+    }
+    if (!rc && herr) {
+        mps_set_error(MPS_ERR_UNSUPPORTED, __FILE__, __LINE__, "more collisions of one particle in one step than the reverse sweep holds");
+        rc = MPS_ERR_UNSUPPORTED;
+    }
+    if (!rc) rc = mps_ctx_download(c, ps, nl);
+    if (!rc) {
+        if (grad_nu) *grad_nu = params[0];
+        if (grad_g)
+            for (int d = 0; d < D_; d++) grad_g[d] = params[1 + d];
+        for (int i = 0; i < n; i++)
+            for (int d = 0; d < D_; d++) {
+                if (rbar_in) rbar_in[(size_t)i * D_ + d] = rb_h[i].c[d];
+                if (ubar_in) ubar_in[(size_t)i * D_ + d] = ub_h[i].c[d];
+            }
+    }
+    mps_hip_release(ps);
+    tape_free(&t);
+    if (ls0.lst) hipFree(ls0.lst);
+    if (ls0.lidx) hipFree(ls0.lidx);
+    adj_free(w);
+    return rc;
+}
