@@ -453,7 +453,9 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=("fp64", "mixed", "mixed32"),
                     help="fp64; mixed = FP32 matrix, FP64 vectors; mixed32 = FP32 matrix and vectors")
     ap.add_argument("--tol", type=float, default=1e-9)
-    ap.add_argument("--spmv", type=int, default=0, help="SpMV kernel variant of the FP64 CG (0-4)")
+    ap.add_argument("--spmv", type=int, default=0, help="SpMV kernel variant of the FP64 CG (0-2)")
+    ap.add_argument("--fused", type=int, default=0,
+                    help="FP64 CG iteration: 0 five kernels, 1 two (each reduction repeated by the next kernel's blocks)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--sync", action="store_true",
@@ -493,6 +495,7 @@ def main():
     lib.set_option("precision", {"fp64": 0, "mixed": 1, "mixed32": 2}[a.precision])
     lib.set_option("exit_on_error", 0)
     lib.set_option("spmv", a.spmv)
+    lib.set_option("fused", a.fused)
     ps, n = build_scene(lib, w)
     print("bench: %s, %d particles, solver %s, %s" % (a.workload, n, a.solver, a.precision),
           file=sys.stderr, flush=True)
