@@ -275,6 +275,14 @@ def run_adjoint(a):
     t1 = time.perf_counter()
     loss(); fwd = time.perf_counter() - t1
     rel = lambda x, y: abs(x - y) / max(abs(x), abs(y), 1e-300)
+    if a.no_check:
+        print(json.dumps({
+            "metric": "adjoint gradient of a 200-step 2D dam break (forward + checkpointed reverse)",
+            "value": round(el, 3), "unit": "s per gradient", "higher_is_better": False, "n_gpus": 1,
+            "forward_only_s": round(fwd, 3), "gradient_over_forward": round(el / fwd, 2), "dtype": "f64",
+            "config": {"workload": a.workload, "particles": n, "steps": steps, "checkpoint_every": w["checkpoint"]},
+            "loss": L[0], "grad_nu": gnu[0], "grad_g": list(gg)}), flush=True)
+        return
     checks = {}
     hnu = 1e-6 * w["nu_target"]
     checks["nu"] = rel(gnu[0], (loss(nu=w["nu"] + hnu) - loss(nu=w["nu"] - hnu)) / (2 * hnu))
@@ -458,6 +466,8 @@ def main():
                     help="FP64 CG iteration: 0 five kernels, 1 two (each reduction repeated by the next kernel's blocks)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-check", action="store_true",
+                    help="adjoint workload: time the gradient only, without the finite-difference checks")
     ap.add_argument("--sync", action="store_true",
                     help="time the reference-style per-step drop-in calls (simulation_step with the "
                          "state and list copied in and out each step), PCIe included; not the headline")
