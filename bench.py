@@ -462,6 +462,10 @@ def main():
                     help="fp64; mixed = FP32 matrix, FP64 vectors; mixed32 = FP32 matrix and vectors")
     ap.add_argument("--tol", type=float, default=1e-9)
     ap.add_argument("--spmv", type=int, default=0, help="SpMV kernel variant of the FP64 CG (0-2)")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="1: pick the SpMV variant and polling interval with mps_hip_autotune first (FP64 CG)")
+    ap.add_argument("--graph", type=int, default=0,
+                    help="1: replay the particle phases of each step from captured HIP graphs")
     ap.add_argument("--fused", type=int, default=0,
                     help="FP64 CG iteration: 0 five kernels, 1 two (each reduction repeated by the next kernel's blocks)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -497,6 +501,8 @@ def main():
     if "ensemble" in w:
         lib.set_option("tol_mode", 1)
         lib.set_option("exit_on_error", 0)
+        lib.set_option("spmv", a.spmv)
+        lib.set_option("graph", a.graph)
         run_ensemble(lib, a)
         return
     cfg = make_config(lib, w, 1 if a.solver == "iccg" else 0, a.tol)
@@ -506,6 +512,7 @@ def main():
     lib.set_option("exit_on_error", 0)
     lib.set_option("spmv", a.spmv)
     lib.set_option("fused", a.fused)
+    lib.set_option("graph", a.graph)
     ps, n = build_scene(lib, w)
     print("bench: %s, %d particles, solver %s, %s" % (a.workload, n, a.solver, a.precision),
           file=sys.stderr, flush=True)
@@ -516,6 +523,14 @@ def main():
     if a.sync:
         sync_rate(lib, ps, n, cfg, a)
         return
+    tune = None
+    if a.autotune and a.precision == "fp64" and a.solver == "cg":
+        res = mps_amd.TuneResult()
+        t_tune = time.perf_counter()
+        lib.check(lib.mps_hip_autotune(ps, 1, C.byref(res)), "autotune")
+        tune = {"spmv": res.spmv, "check_every": res.check_every,
+                "spmv_ms": [round(v, 5) for v in res.spmv_ms], "solve_ms": [round(v, 2) for v in res.solve_ms],
+                "seconds": round(time.perf_counter() - t_tune, 2)}
     lib.check(lib.mps_hip_run_steps(ps, a.warmup), "warm-up steps")
     # the timed steps time only the SpMV launches (two HIP events each)
     lib.set_option("profile", 0 if a.no_profile else 2)
@@ -574,7 +589,10 @@ def main():
         "config": {"workload": a.workload, "particles": n, "fluid": None, "l0": w["l0"],
                    "dt": w["dt"], "solver": a.solver, "precision": a.precision,
                    "tolerance": "relative %g" % a.tol, "mean_iters_per_step": round(iters, 1),
-                   "n_eq": st.n_eq, "nnz": st.nnz, "pairs": st.n_pairs, "parallelism": "1 GPU"},
+                   "n_eq": st.n_eq, "nnz": st.nnz, "pairs": st.n_pairs, "parallelism": "1 GPU",
+                   "spmv_variant": int(lib.L.mps_hip_get_option(b"spmv")),
+                   "check_every": int(lib.L.mps_hip_get_option(b"check_every")), "graph": a.graph},
+        "autotune": tune,
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels_ms_one_step": kernels,

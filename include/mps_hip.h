@@ -249,6 +249,11 @@ int  mps_hip_step_backward(ParticleSystem *ps, NeighborList *nl, const double *r
  *                      1: two (the previous iteration's reduction done
  *                      redundantly by every block of the next kernel),
  *                      same results bit for bit, measured 2 % slower
+ *   "graph"          1: the particle phases of each step (predictor to
+ *                      assembly) are captured once as a HIP graph and
+ *                      replayed, one graph per buffer parity; same results
+ *                      bit for bit; not used while kernels are timed or a
+ *                      tape records (0, default: plain launches)
  *   "profile"        1 times the main kernels with HIP events, 2 only the
  *                      CG SpMV
  * Returns 0, or MPS_ERR_ARG for an unknown key. */
@@ -269,6 +274,23 @@ typedef struct {
     int    max_count;       /* largest neighbour count of the last search */
 } mps_step_stats;
 int  mps_hip_get_stats(ParticleSystem *ps, mps_step_stats *out);
+
+/* Autotuner (no counterpart in the reference): times the choices of the
+ * FP64 CG that leave every result bit unchanged -- the SpMV's entries per
+ * load batch (option "spmv") and the host's polling interval (option
+ * "check_every") -- on a private device copy of ps's host particles under
+ * g_config (one step's PPE, solved once per candidate).  ps and its device
+ * state are not touched.  With apply != 0 the fastest choices become the
+ * options.  Returns 0 or an error code. */
+typedef struct {
+    int    spmv;            /* fastest "spmv" variant                      */
+    int    check_every;     /* fastest "check_every"                       */
+    double spmv_ms[3];      /* mean SpMV launch, variants 0 (8 entries per
+                               batch), 1 (16), 2 (4)                       */
+    double solve_ms[4];     /* one solve polling every 8, 16, 32, 64 iters */
+    int    iters;           /* CG iterations of that solve                 */
+} mps_tune_result;
+int  mps_hip_autotune(ParticleSystem *ps, int apply, mps_tune_result *out);
 
 /* Exports of the last assembled PPE in the reference's layout
  * (mps_fluid3D_parallel/src/pressure_solver.c:411-551): rows in eq order

@@ -26,6 +26,11 @@ class StepStats(C.Structure):
                 ("last_bnorm", C.c_double), ("ic_levels", C.c_int), ("max_count", C.c_int)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("spmv", C.c_int), ("check_every", C.c_int), ("spmv_ms", C.c_double * 3),
+                ("solve_ms", C.c_double * 4), ("iters", C.c_int)]
+
+
 _libs = {}
 
 
@@ -102,6 +107,7 @@ class Lib:
             "mps_hip_set_option": (C.c_int, [C.c_char_p, C.c_double]),
             "mps_hip_get_option": (C.c_double, [C.c_char_p]),
             "mps_hip_get_stats": (C.c_int, [PS, C.POINTER(StepStats)]),
+            "mps_hip_autotune": (C.c_int, [PS, C.c_int, C.POINTER(TuneResult)]),
             "mps_hip_export_ppe": (C.c_int, [PS, I, I, I, D, D, D]),
             "mps_hip_export_solution": (C.c_int, [PS, D]),
             "mps_hip_export_ic": (C.c_int, [PS, I, I, D, D]),

@@ -157,6 +157,16 @@ struct Ctx {
     IcData *ic = nullptr;
     MixedBufs *mixed = nullptr;
     Packed *pk = nullptr;
+    // The particle phases of a step (predictor to assembly) captured as a
+    // HIP graph, one per buffer parity at entry (every step flips the
+    // X/V, T/O/... and P indices); valid while the buffers and the step
+    // constants stay those of the capture (runtime.hip).
+    struct StepGraph {
+        hipGraphExec_t exec = nullptr;
+        int in[4] = {0, 0, 0, 0};     // cx, ct, cp, n at entry
+        int out[3] = {0, 0, 0};       // cx, ct, cp after
+        StepConsts k{};
+    } sg[2];
 };
 
 // ---- launchers (k_particles.hip) -------------------------------------
@@ -204,6 +214,7 @@ int pack_ppe(Ctx *c, int with_f32);        // k_pack.hip
 int unpack_solution(Ctx *c);               // packed x -> c->x (slot order)
 void packed_free(Packed *p);
 int solve_ppe(Ctx *c, const SolveOpts &o, SolveResult *out);
+int cg_time_spmv(Ctx *c, int variant, int reps, double *ms);   // k_cg.hip
 
 // ---- runtime accessors (runtime.hip) -----------------------------------
 Ctx *mps_ctx_for(ParticleSystem *ps, int create);
@@ -217,6 +228,7 @@ double mps_search_radius(const SimConfig *cf);
 
 // ---- adjoint tape (k_adjoint.hip) -------------------------------------
 // Stages of a step at which a recording adjoint copies the state.
+bool tape_recording();   // k_adjoint.hip: a tape is armed
 enum TapeStage { TAPE_START, TAPE_PREDICTED, TAPE_COLLIDED, TAPE_SEARCHED, TAPE_SOLVED };
 void tape_hook(Ctx *c, int stage);
 int mps_ctx_step(Ctx *c, const SimConfig *cf);    // one resident timestep (runtime.hip)

@@ -177,6 +177,8 @@ static int rec_list(Ctx *c, TapeStep *t)
     return 0;
 }
 
+bool tape_recording() { return g_rec != nullptr; }
+
 // Called by the step at its stages (runtime.hip) while a tape is recorded.
 void tape_hook(Ctx *c, int stage)
 {

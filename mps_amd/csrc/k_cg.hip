@@ -526,6 +526,39 @@ static int run_chunks(hipStream_t s, int max_iter, int check_every, const int *d
     return rc;
 }
 
+// Mean launch time (ms) of FP64 SpMV variant `variant` on c's packed
+// system, over `reps` launches after two warm-up launches.  The control
+// block is re-armed first (the kernel skips its work once a solve has
+// stopped); the next solve initialises it again.  Ap and the partials are
+// scratch between solves.
+int cg_time_spmv(Ctx *c, int variant, int reps, double *ms)
+{
+    const Packed *pk = c->pk;
+    if (!pk || pk->n_eq == 0 || !c->ctrl || reps < 1) return MPS_ERR_ARG;
+    hipStream_t s = c->stream;
+    const PkView P = pk_view(pk);
+    const int n = pk->n_eq;
+    const unsigned g = cg_grid(n);
+    const size_t lds = sizeof(double) * (size_t)pk->hmax;
+    const spmv_fn f = pick_spmv(variant);
+    HIP_OK(hipMemsetAsync(&c->ctrl->done, 0, sizeof(int), s));
+    for (int q = 0; q < 2; q++) hipLaunchKernelGGL(f, dim3(g), dim3(CG_BLK), lds, s, n, P, c->pv, c->Ap, c->partA, c->ctrl);
+    LAUNCH_CHECK();
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    HIP_OK(hipEventRecord(e0, s));
+    for (int q = 0; q < reps; q++) hipLaunchKernelGGL(f, dim3(g), dim3(CG_BLK), lds, s, n, P, c->pv, c->Ap, c->partA, c->ctrl);
+    HIP_OK(hipEventRecord(e1, s));
+    HIP_OK(hipEventSynchronize(e1));
+    float t = 0.0f;
+    HIP_OK(hipEventElapsedTime(&t, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    *ms = (double)t / reps;
+    return 0;
+}
+
 // FP64 CG on the packed system; the solution is left in the packed x.
 static int cg_fp64(Ctx *c, const SolveOpts &o, SolveResult *res)
 {

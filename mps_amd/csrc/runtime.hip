@@ -16,6 +16,8 @@
 #include <string>
 #include <vector>
 
+#include <chrono>
+
 #include "ctx.hpp"
 
 // ---------------------------------------------------------------------
@@ -33,7 +35,7 @@ void mps_set_error(int code, const char *file, int line, const char *msg)
 
 struct Options {
     double tol_mode = 0, precision = 0, inner_tol = 1e-3, exit_on_error = 1, check_every = 16;
-    double spmv = 0, profile = 0, verbose = 0, fused = 0;
+    double spmv = 0, profile = 0, verbose = 0, fused = 0, graph = 0;
 };
 static Options g_opt;
 
@@ -160,8 +162,19 @@ static int dalloc(T **p, size_t n)
 void ic_free(IcData *d);   // k_ic.hip
 void mixed_free(MixedBufs *m);   // k_cg.hip
 
+// Drops the captured step graphs (their kernel arguments hold the
+// context's buffer addresses).
+static void graph_reset(Ctx *c)
+{
+    for (auto &g : c->sg) {
+        if (g.exec) hipGraphExecDestroy(g.exec);
+        g = Ctx::StepGraph();
+    }
+}
+
 void ctx_free(Ctx *c)
 {
+    graph_reset(c);
     for (int q = 0; q < 2; q++) {
         DFREE(c->X[q]); DFREE(c->V[q]); DFREE(c->ACC[q]); DFREE(c->T[q]); DFREE(c->O[q]);
         DFREE(c->SURF[q]); DFREE(c->NCOUNT[q]); DFREE(c->P[q]); DFREE(c->ND[q]);
@@ -221,6 +234,7 @@ int ctx_alloc(Ctx *c, int n, int stride, int grid_cells)
         HIP_OK(hipMemsetAsync(c->ncnt, 0, sizeof(int) * N, c->stream));
     }
     if (grid_cells + 4 > c->grid_cap) {
+        graph_reset(c);
         DFREE(c->ccount); DFREE(c->cstart); DFREE(c->cfill);
         ALLOC(c->ccount, (size_t)grid_cells + 4);
         ALLOC(c->cstart, (size_t)grid_cells + 4);
@@ -459,12 +473,20 @@ static int download(Ctx *c, ParticleSystem *ps, NeighborList *nl)
 // the timestep
 // ---------------------------------------------------------------------
 
-static int reset_counters(Ctx *c)
+__global__ void k_reset_counters(DevCounters *c)
 {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
     DevCounters z{};
     z.overflow_orig = INT_MAX;
-    *c->hcounters = z;
-    HIP_OK(hipMemcpyAsync(c->counters, c->hcounters, sizeof(DevCounters), hipMemcpyHostToDevice, c->stream));
+    *c = z;
+}
+
+// A kernel rather than a copy from host memory, so that a captured step
+// replays it with the same effect.
+static int reset_counters(Ctx *c)
+{
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, c->stream, c->counters);
+    HIP_OK(hipGetLastError());
     return 0;
 }
 
@@ -513,10 +535,8 @@ int collide_2d(Ctx *c, const StepConsts &k);   // k_particles.hip
 // The timestep up to the assembled PPE (simulation.c:31-96 before
 // solve_pressure): predictor, collision, sort, search with density and
 // flags, assembly.
-static int step_pre(Ctx *c, const SimConfig *cf, StepConsts *kout)
+static int step_pre_launch(Ctx *c, const StepConsts &k)
 {
-    StepConsts k = make_consts(cf, c->ps, c->grid, search_radius(cf));
-    k.stride = c->stride;
     tape_hook(c, TAPE_START);
     int rc = launch_visc_predict(c, k);
     if (rc) return rc;
@@ -534,6 +554,63 @@ static int step_pre(Ctx *c, const SimConfig *cf, StepConsts *kout)
     if (!rc) rc = launch_search(c, k, 1, c->counters);
     if (!rc) tape_hook(c, TAPE_SEARCHED);
     if (!rc) rc = launch_assemble(c, k, c->counters);
+    return rc;
+}
+
+// The particle phases through a captured graph: replays the capture made
+// from the same entry state (buffer indices, particle count, constants),
+// or captures one; then moves the buffer indices as the launches did.
+static int step_pre_graph(Ctx *c, const StepConsts &k)
+{
+    const int in[4] = {c->cx, c->ct, c->cp, c->n};
+    Ctx::StepGraph *hit = nullptr;
+    for (auto &g : c->sg)
+        if (g.exec && memcmp(g.in, in, sizeof in) == 0 && memcmp(&g.k, &k, sizeof k) == 0) hit = &g;
+    if (!hit) {
+        Ctx::StepGraph &g = c->sg[c->cx & 1];
+        if (g.exec) hipGraphExecDestroy(g.exec);
+        g = Ctx::StepGraph();
+        HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = step_pre_launch(c, k);
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &gr);
+        const int out[3] = {c->cx, c->ct, c->cp};
+        c->cx = in[0];
+        c->ct = in[1];
+        c->cp = in[2];
+        if (!rc && e != hipSuccess) {
+            mps_set_error(MPS_ERR_HIP, __FILE__, __LINE__, hipGetErrorString(e));
+            rc = MPS_ERR_HIP;
+        }
+        if (!rc && hipGraphInstantiate(&g.exec, gr, nullptr, nullptr, 0) != hipSuccess) {
+            g.exec = nullptr;
+            mps_set_error(MPS_ERR_HIP, __FILE__, __LINE__, "hipGraphInstantiate failed");
+            rc = MPS_ERR_HIP;
+        }
+        if (gr) hipGraphDestroy(gr);
+        if (rc) return rc;
+        memcpy(g.in, in, sizeof in);
+        memcpy(g.out, out, sizeof out);
+        g.k = k;
+        hit = &g;
+    }
+    HIP_OK(hipGraphLaunch(hit->exec, c->stream));
+    c->cx = hit->out[0];
+    c->ct = hit->out[1];
+    c->cp = hit->out[2];
+    return 0;
+}
+
+// The step up to the solve (simulation.c:31-95): predictor, collision,
+// sort, search with density and flags, assembly.  With the option "graph"
+// the launches are replayed from a capture, except while a tape records
+// the step or kernels are timed.
+static int step_pre(Ctx *c, const SimConfig *cf, StepConsts *kout)
+{
+    StepConsts k = make_consts(cf, c->ps, c->grid, search_radius(cf));
+    k.stride = c->stride;
+    const bool graph = g_opt.graph > 0.5 && !tape_recording() && g_opt.profile < 0.5;
+    const int rc = graph ? step_pre_graph(c, k) : step_pre_launch(c, k);
     if (rc) return rc;
     c->list_valid = true;
     *kout = k;
@@ -671,6 +748,7 @@ int mps_hip_set_option(const char *key, double value)
     else if (k == "check_every") g_opt.check_every = value;
     else if (k == "spmv") g_opt.spmv = value;
     else if (k == "fused") g_opt.fused = value;
+    else if (k == "graph") g_opt.graph = value;
     else if (k == "profile") g_opt.profile = value;
     else if (k == "verbose") g_opt.verbose = value;
     else return MPS_ERR_ARG;
@@ -688,6 +766,7 @@ double mps_hip_get_option(const char *key)
     if (k == "check_every") return g_opt.check_every;
     if (k == "spmv") return g_opt.spmv;
     if (k == "fused") return g_opt.fused;
+    if (k == "graph") return g_opt.graph;
     if (k == "profile") return g_opt.profile;
     if (k == "verbose") return g_opt.verbose;
     return NAN;
@@ -807,6 +886,67 @@ int mps_hip_run_batch(ParticleSystem *const *ps, SimConfig *const *cfgs, int nsc
     g_config = saved;
     if (rc) g_last_error = rc;
     return rc;
+}
+
+// Times the bit-preserving choices of the FP64 solve on a private device
+// copy of ps (its host particles, under g_config): one step up to the
+// assembly, then each SpMV variant on the packed system, then whole solves
+// of that system with each host polling interval.  Neither choice changes
+// a result bit: the variants sum every row in the same order, and the
+// polling interval only decides how far past convergence the host has
+// queued iterations that exit at once.
+int mps_hip_autotune(ParticleSystem *ps, int apply, mps_tune_result *out)
+{
+    if (!ps || !out) return g_last_error = MPS_ERR_ARG;
+    if (!g_config) return g_last_error = MPS_ERR_NO_CONFIG;
+    int rc = ensure_device();
+    if (rc) return rc;
+    memset(out, 0, sizeof *out);
+    ParticleSystem shadow = *ps;     // same host particles, a context of its own
+    Ctx *c = get_ctx(&shadow);
+    if (!c) return g_last_error;
+    const Options saved = g_opt;
+    g_opt.profile = 0;
+    g_opt.graph = 0;
+    g_opt.precision = 0;
+    g_opt.fused = 0;
+    rc = ensure_resident(c, &shadow, g_config);
+    StepConsts k;
+    if (!rc) rc = step_pre(c, g_config, &k);
+    SolveOpts so{};
+    SolveResult sr{};
+    if (!rc) {
+        so = solve_opts(g_config, k);
+        so.solver = 0;
+        rc = solve_ppe(c, so, &sr);
+    }
+    int best_v = 0;
+    for (int v = 0; v < 3 && !rc; v++) {
+        rc = cg_time_spmv(c, v, 20, &out->spmv_ms[v]);
+        if (!rc && (v == 0 || out->spmv_ms[v] < out->spmv_ms[best_v])) best_v = v;
+    }
+    const int ce[4] = {8, 16, 32, 64};
+    int best_c = 0;
+    for (int q = 0; q < 4 && !rc; q++) {
+        so.spmv_variant = best_v;
+        so.check_every = ce[q];
+        if (hipStreamSynchronize(c->stream) != hipSuccess) rc = MPS_ERR_HIP;
+        const auto t0 = std::chrono::steady_clock::now();
+        if (!rc) rc = solve_ppe(c, so, &sr);
+        out->solve_ms[q] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (!rc && (q == 0 || out->solve_ms[q] < out->solve_ms[best_c])) best_c = q;
+    }
+    g_opt = saved;
+    mps_hip_release(&shadow);
+    if (rc) return g_last_error = rc;
+    out->spmv = best_v;
+    out->check_every = ce[best_c];
+    out->iters = sr.iters;
+    if (apply) {
+        g_opt.spmv = best_v;
+        g_opt.check_every = ce[best_c];
+    }
+    return MPS_OK;
 }
 
 int mps_hip_get_stats(ParticleSystem *ps, mps_step_stats *out)

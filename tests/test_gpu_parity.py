@@ -274,6 +274,71 @@ def test_fused_cg_same_bits(lib3):
     assert out[0] == out[1]
 
 
+@pytest.mark.parametrize("name,libname", [("dam3d_mid", "lib3"), ("dam2d_l0.01", "lib2")])
+def test_graph_same_bits(request, name, libname):
+    """The particle phases replayed from captured HIP graphs (option
+    "graph") leave every field with the bits of plain launches, over steps
+    of both buffer parities."""
+    lib = request.getfixturevalue(libname)
+    dim, cfg, pos, vel, typ = H.build(name, cg_tolerance=1e-10)
+    lib.set_option("tol_mode", 1)
+    out = []
+    try:
+        for gmode in (0, 1):
+            lib.set_option("graph", gmode)
+            say("graph", gmode, name)
+            g = H.GpuRun(lib, cfg, pos, vel, typ)
+            lib.mps_hip_set_resident(g.scene.ptr(), 1)
+            assert lib.mps_hip_run_steps(g.scene.ptr(), 5) == 0
+            assert lib.mps_hip_download(g.scene.ptr(), None) == 0
+            st = lib.stats(g.scene)
+            out.append(([np.ascontiguousarray(g.scene.arr[f]).tobytes() for f in H.FIELDS], st.total_iters))
+            g.close()
+    finally:
+        lib.set_option("graph", 0)
+    assert out[0][1] == out[1][1]
+    for f, a, b in zip(H.FIELDS, out[0][0], out[1][0]):
+        assert a == b, f
+
+
+def test_autotune_keeps_bits(lib3):
+    """mps_hip_autotune times the SpMV variants and polling intervals on a
+    private copy of the scene, leaves the caller's particles as they were,
+    applies the fastest choices, and the steps that follow give the bits of
+    the default options."""
+    import mps_amd
+    dim, cfg, pos, vel, typ = H.build("dam3d_mid", cg_tolerance=1e-10)
+    lib3.set_option("tol_mode", 1)
+    out = []
+    try:
+        for tune in (False, True):
+            g = H.GpuRun(lib3, cfg, pos, vel, typ)
+            if tune:
+                names = g.scene.arr.dtype.names
+                before = [np.ascontiguousarray(g.scene.arr[f]).tobytes() for f in names]
+                res = mps_amd.TuneResult()
+                assert lib3.mps_hip_autotune(g.scene.ptr(), 1, C.byref(res)) == 0
+                say("autotune: spmv %d (%s ms), check_every %d (%s ms), %d iterations"
+                    % (res.spmv, ["%.4f" % v for v in res.spmv_ms], res.check_every,
+                       ["%.2f" % v for v in res.solve_ms], res.iters))
+                assert [np.ascontiguousarray(g.scene.arr[f]).tobytes() for f in names] == before
+                assert res.spmv in (0, 1, 2) and res.check_every in (8, 16, 32, 64)
+                assert res.spmv_ms[res.spmv] == min(res.spmv_ms)
+                assert res.solve_ms[(8, 16, 32, 64).index(res.check_every)] == min(res.solve_ms)
+                assert min(res.spmv_ms) > 0 and min(res.solve_ms) > 0 and res.iters > 0
+                assert lib3.L.mps_hip_get_option(b"spmv") == res.spmv
+                assert lib3.L.mps_hip_get_option(b"check_every") == res.check_every
+            for _ in range(2):
+                assert g.step() == 0
+            out.append([np.ascontiguousarray(g.scene.arr[f]).tobytes() for f in H.FIELDS])
+            g.close()
+    finally:
+        lib3.set_option("spmv", 0)
+        lib3.set_option("check_every", 16)
+    for f, a, b in zip(H.FIELDS, out[0], out[1]):
+        assert a == b, f
+
+
 def test_batch_equals_single_runs(lib3):
     """mps_hip_run_batch (particle operators per scene, one batched FP64 CG
     for all scenes) leaves every scene with the bits of its own single-scene
