@@ -64,6 +64,18 @@ def build_scene(lib, w):
     return ps, n
 
 
+PRECISIONS = {"fp64": 0, "mixed": 1, "mixed32": 2, "mixed32q": 3}
+
+
+def coded_bytes(st):
+    """Bytes of one application of the distance-coded FP32 operator
+    (k_mx_spmv_q): one 4-byte word (halo index and distance code) per
+    off-diagonal, and per row its length, FP64 Dirichlet diagonal, FP32 p
+    read and FP32 Ap written.  Its own minimum, not the CSR figure."""
+    z, ne = st.nnz, st.n_eq
+    return 4 * (z - ne) + ne * (4 + 8 + 4 + 4)
+
+
 def spmv_bytes(st, s_val=8, s_vec=8):
     """Algorithmic bytes of one SpMV y = A x over the reference's CSR:
     Z (value + 4-byte column) + 4 (n_eq + 1) row pointers + x read once and
@@ -102,6 +114,72 @@ def cpu_baseline(w, tol, seconds_hint=30.0):
     return dict(value=len(typ) * steps / el, unit="particle-steps/s", cores=threads, kind="port",
                 sample="%d step(s) from rest of the %d-particle workload, CG relative 1e-9, "
                        "mean %.0f iterations/step, %.1f s" % (steps, len(typ), iters / steps, el))
+
+
+def gpu_bnorms(lib, w, tol, k):
+    """sqrt(b.b) of the PPE at steps 1..k from rest, from the GPU; its
+    right-hand side is bit-exact with the reference's."""
+    cfg = make_config(lib, w, 0, tol)
+    lib.set_config(cfg)
+    ps, n = build_scene(lib, w)
+    lib.particle_system_calc_initial_params(ps)
+    out, iters = [], []
+    for _ in range(k):
+        lib.check(lib.mps_hip_run_steps(ps, 1), "step")
+        st = lib.stats(ps)
+        out.append(st.last_bnorm)
+        iters.append(st.last_iters)
+    lib.mps_hip_release(ps)
+    lib.particle_system_free(ps)
+    return out, iters
+
+
+def cpu_baseline_reference(lib, w, tol, seconds=20.0, max_steps=8):
+    """The reference itself, compiled from its own sources with its OpenMP
+    flags (oracle/_ref/libref3dp.so, built by oracle/Makefile), on this
+    host: simulation_step of the same workload from rest, for as many steps
+    as fit about `seconds` (at least one).  The reference's CG stops on an
+    absolute residual (pressure_solver.c:90-131); step s gets tol * |b_s|,
+    with |b_s| recorded by the GPU run of the same steps, so both stop on
+    the same relative criterion.  Initial parameters come from the oracle
+    (bit-identical to the reference's, whose O(N^2) pass takes hours at
+    this size) and the initial search is not timed, as in simulation_run.
+    None when oracle/_ref is absent."""
+    from oracle.oracle import Oracle, Reference
+    if not Reference.available("3dp"):
+        return None
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import scenes  # the reference generator's lattice rules
+    bn, gi = gpu_bnorms(lib, w, tol, max_steps)
+    ref = Reference("3dp")
+    cfg_o = make_config(lib, w, 0, tol)
+    cfg_r = ref.config_from(cfg_o)
+    d = w["dims"]
+    pos, vel, typ = scenes.dambreak(3, d[:3], d[3:], w["l0"], 3, 2)
+    so = A.HostScene(A.types(3), pos, vel, typ)
+    Oracle(3).init_params(so, cfg_o)
+    sr = A.HostScene(ref.abi, pos, vel, typ)
+    for f in ("n0", "lambda_", "C_LL", "C_SL", "n0_count"):
+        setattr(sr.ps, f, getattr(so.ps, f))
+    ref.set_config(cfg_r)
+    nl, cl = ref.make_lists(sr.num)
+    ref.L.neighbor_search_cell_linked_list(nl, sr.ptr(), cl, ref.search_radius())
+    steps, el = 0, 0.0
+    for s in range(1, max_steps + 1):
+        cfg_r.cg_tolerance = tol * bn[s - 1]
+        t0 = time.perf_counter()
+        ref.L.simulation_step(sr.ptr(), nl, cl, s)
+        el += time.perf_counter() - t0
+        steps += 1
+        if el > seconds:
+            break
+    ref.free_lists(nl, cl)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=len(typ) * steps / el, unit="particle-steps/s", cores=threads, kind="reference",
+                sample="%d step(s) from rest of the %d-particle workload through the reference's own "
+                       "simulation_step (OpenMP, %d threads), CG to tol*|b| per step (the GPU's CG took "
+                       "%.0f iterations/step on those steps), %.1f s"
+                % (steps, len(typ), threads, sum(gi[:steps]) / steps, el))
 
 
 def profiled_traffic(precision, workload, nnz):
@@ -180,7 +258,9 @@ def run_ensemble(lib, a):
     cpu = None
     if not a.no_cpu_baseline:
         try:
-            cpu = cpu_baseline(ws[0], a.tol)
+            cpu = cpu_baseline_reference(lib, ws[0], a.tol) if a.cpu_kind == "reference" else None
+            if cpu is None:
+                cpu = cpu_baseline(ws[0], a.tol)
             cpu["sample"] = "scene 0 of the ensemble: " + cpu["sample"]
         except Exception as e:  # baseline is reported, never required
             cpu = dict(error=str(e))
@@ -373,7 +453,7 @@ def compare_forms(lib, ps, n, w, a):
     """The same scene, from the same initial parameters, resident for
     --steps steps in FP64 and in the mixed form: particle-steps/s of both
     and the relative L2 differences of pressure and positions at the end."""
-    mixed = {"fp64": 1, "mixed": 1, "mixed32": 2}[a.precision]
+    mixed = PRECISIONS[a.precision] or 1
     init = {f: getattr(ps.contents, f) for f in ("n0", "lambda_", "C_LL", "C_SL", "n0_count")}
     out = {}
     for name, prec in (("fp64", 0), (a.precision if a.precision != "fp64" else "mixed", mixed)):
@@ -458,7 +538,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="dam3d_l0.005", choices=sorted(WORKLOADS))
     ap.add_argument("--solver", default="cg", choices=("cg", "iccg"))
-    ap.add_argument("--precision", default="fp64", choices=("fp64", "mixed", "mixed32"),
+    ap.add_argument("--precision", default="fp64", choices=tuple(PRECISIONS),
                     help="fp64; mixed = FP32 matrix, FP64 vectors; mixed32 = FP32 matrix and vectors")
     ap.add_argument("--tol", type=float, default=1e-9)
     ap.add_argument("--spmv", type=int, default=0, help="SpMV kernel variant of the FP64 CG (0-2)")
@@ -469,6 +549,9 @@ def main():
     ap.add_argument("--fused", type=int, default=0,
                     help="FP64 CG iteration: 0 five kernels, 1 two (each reduction repeated by the next kernel's blocks)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-kind", default="reference", choices=("reference", "port"),
+                    help="CPU baseline: the reference built from its sources (oracle/_ref; the oracle "
+                         "port where that is absent) or the oracle port")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-check", action="store_true",
                     help="adjoint workload: time the gradient only, without the finite-difference checks")
@@ -508,7 +591,7 @@ def main():
     cfg = make_config(lib, w, 1 if a.solver == "iccg" else 0, a.tol)
     lib.set_config(cfg)
     lib.set_option("tol_mode", 1)
-    lib.set_option("precision", {"fp64": 0, "mixed": 1, "mixed32": 2}[a.precision])
+    lib.set_option("precision", PRECISIONS[a.precision])
     lib.set_option("exit_on_error", 0)
     lib.set_option("spmv", a.spmv)
     lib.set_option("fused", a.fused)
@@ -553,12 +636,12 @@ def main():
     roof = None
     if cnt.value > 0:
         byt = {"fp64": spmv_bytes(st, 8, 8), "mixed": spmv_bytes(st, 4, 8),
-               "mixed32": spmv_bytes(st, 4, 4)}[a.precision]
+               "mixed32": spmv_bytes(st, 4, 4), "mixed32q": coded_bytes(st)}[a.precision]
         ach = byt / (spmv_ms * 1e-3) / 1e9
         traffic, tsrc = profiled_traffic(a.precision, a.workload, st.nnz)
         roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(ach / HBM_PEAK_GBS, 4), traffic=traffic, traffic_source=tsrc,
-                    kernel="k_spmv_pk" if a.precision == "fp64" else "k_mx_spmv_pk", bytes_per_launch=int(byt),
+                    kernel={"fp64": "k_spmv_pk", "mixed32q": "k_mx_spmv_q"}.get(a.precision, "k_mx_spmv_pk"), bytes_per_launch=int(byt),
                     mean_launch_ms=round(spmv_ms, 5), launches=spmv_launches)
     # per-kernel breakdown: one more step, every kernel class timed
     kernels = {}
@@ -575,7 +658,9 @@ def main():
     cpu = None
     if not a.no_cpu_baseline and rank == 0:
         try:
-            cpu = cpu_baseline(w, a.tol)
+            cpu = cpu_baseline_reference(lib, w, a.tol) if a.cpu_kind == "reference" else None
+            if cpu is None:
+                cpu = cpu_baseline(w, a.tol)
         except Exception as e:  # baseline is reported, never required
             cpu = dict(error=str(e))
     value = n * a.steps / el

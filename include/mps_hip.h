@@ -239,7 +239,10 @@ int  mps_hip_step_backward(ParticleSystem *ps, NeighborList *nl, const double *r
  *                      solution, and make reliable updates (true residual
  *                      b - A x recomputed with the FP64 matrix), testing
  *                      convergence on that FP64 residual: 1 FP64 CG
- *                      vectors, 2 FP32 CG vectors and FP32 SpMV sums
+ *                      vectors, 2 FP32 CG vectors and FP32 SpMV sums,
+ *                      3 as 2 with each off-diagonal applied from a 20-bit
+ *                      code of its pair distance packed with a 12-bit halo
+ *                      index (4 bytes per nonzero instead of 6)
  *   "inner_tol"      residual reduction between reliable updates (1e-3)
  *   "exit_on_error"  1 (reference behaviour) or 0
  *   "check_every"    CG iterations between host convergence polls (16)

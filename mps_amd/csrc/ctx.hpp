@@ -107,6 +107,8 @@ struct Packed {
     int *c32 = nullptr;
     float *fval = nullptr;           // mixed precision: FP32 values
     double *ddir = nullptr;          // mixed precision: Dirichlet part of each row
+    unsigned *cq = nullptr;          // precision 3: (distance code << 16) | halo index, per entry
+    size_t cap_cq = 0;
     int *dmax = nullptr;             // device: largest halo
     int *htot = nullptr;             // pinned: scan totals
 };
@@ -196,12 +198,16 @@ struct SolveOpts {
     int max_iter;
     double tol;
     int tol_mode;      // 0 absolute, 1 relative
-    int precision;     // 0 FP64; 1 FP32 matrix, FP64 vectors; 2 FP32 matrix and vectors
+    int precision;     // 0 FP64; 1 FP32 matrix, FP64 vectors; 2 FP32 matrix and vectors;
+                       // 3 FP32 vectors, off-diagonals from 16-bit distance codes
     double inner_tol;
     int check_every;
     int dense_ic;      // 2-D dense-formula IC(0)
     int spmv_variant;  // entries per load batch in the SpMV
     int fused;         // FP64 CG in two kernels per iteration
+    // the PPE's off-diagonal A_ij = neg_coeff w(r_ij) (w of re_lap, r
+    // clamped at r_min), for the distance-coded operator (precision 3)
+    double neg_coeff, re_lap, r_min;
 };
 struct SolveResult {
     int iters, outer, breakdown;

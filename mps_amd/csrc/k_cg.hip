@@ -718,6 +718,94 @@ k_mx_spmv_pk(int n, PkView P, const V *__restrict__ p, V *__restrict__ Ap, doubl
     if (threadIdx.x == 0) part[blockIdx.x] = v;
 }
 
+// ---- precision 3: off-diagonals from 16-bit distance codes ------------
+// Every off-diagonal is A_ij = neg_coeff w(r_ij), w(r) = re / max(r, r_min)
+// - 1 for r < re (pressure_solver.c:490-531, kernel.c:12-20).  In halo-mode
+// blocks one 32-bit word per entry carries the entry's halo index (12 bits:
+// a staged halo holds at most PK_HALO_CAP = 4096 rows) and a 20-bit code of
+// its distance, r = r_min + code * dr with dr = (re - r_min) / (2^20 - 1),
+// recovered from the FP64 value when the system is packed
+// (r = re / (A_ij / neg_coeff + 1)).  The SpMV applies
+// fl32(neg_coeff) (re / r' - 1) in the row-sum-preserving form.  Rounding r
+// to half a code step (1.5e-6 l0 at re = 3.1 l0) moves w by about 2e-6
+// relative at r = l0 and 3e-5 at r = 0.98 re, by under 1e-6 absolute
+// nearer re, and the operator error keeps zero row sums.  The SpMV streams
+// 4 B per nonzero instead of 6 (FP32 value and 16-bit index).  Blocks
+// stored with 32-bit columns apply their FP32 values.
+static_assert(PK_HALO_CAP <= 4096, "halo indices of the coded operator have 12 bits");
+#define Q_CODE_MAX 1048575.0
+struct QConsts {
+    float neg_coeff, re, r_lo, dr;
+};
+
+__global__ void __launch_bounds__(CG_BLK)
+k_mx_q16_build(int n_eq, PkView P, double neg_coeff, double re, double r_min, unsigned *__restrict__ cq)
+{
+    const int k = blockIdx.x * CG_BLK + threadIdx.x;
+    if (k >= n_eq) return;
+    const int4 mt = P.smeta[k >> 6];
+    if (!mt.z) return;
+    const int lane = k & 63;
+    const int m = P.len[k];
+    const size_t vb = (size_t)mt.x * 64u + lane;
+    const unsigned *wrow = P.cw16 + (size_t)mt.y * 64u + lane;
+    const double dr = (re - r_min) / Q_CODE_MAX;
+    for (int q = 0; q < m; q++) {
+        const unsigned wd = wrow[(size_t)(q >> 1) * 64u];
+        const unsigned idx = (q & 1) ? (wd >> 16) : (wd & 0xffffu);
+        const double w = P.val[vb + (size_t)q * 64u] / neg_coeff;
+        double r = re / (w + 1.0);
+        if (r < r_min) r = r_min;
+        double code = rint((r - r_min) / dr);
+        code = code < 0.0 ? 0.0 : (code > Q_CODE_MAX ? Q_CODE_MAX : code);
+        cq[vb + (size_t)q * 64u] = ((unsigned)code << 12) | (idx & 0xfffu);
+    }
+}
+
+template <int U>
+__global__ void __launch_bounds__(CG_BLK)
+k_mx_spmv_q(int n, PkView P, const unsigned *__restrict__ cq, QConsts qk, const float *__restrict__ p,
+            float *__restrict__ Ap, double *part, const MixedCtrl *c)
+{
+    extern __shared__ unsigned char smem_raw[];
+    float *hx = reinterpret_cast<float *>(smem_raw);
+    if (c->done) return;
+    stage_halo(P, XPlain<float>{p}, hx);
+    const int i = blockIdx.x * CG_BLK + threadIdx.x;
+    double v = 0.0;
+    if (i < n) {
+        const float pi = p[i];
+        float s = (float)P.ddir[i] * pi;
+        const int4 mt = P.smeta[i >> 6];
+        const int lane = i & 63;
+        const int m = P.len[i];
+        if (mt.z) {
+            const unsigned *qrow = cq + (size_t)mt.x * 64u + lane;
+            for (int q0 = 0; q0 < m; q0 += U) {
+                unsigned w[U];
+                float h[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) w[u] = (q0 + u < m) ? qrow[(size_t)(q0 + u) * 64u] : 0u;
+#pragma unroll
+                for (int u = 0; u < U; u++) h[u] = hx[w[u] & 0xfffu];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const float r = qk.r_lo + (float)(w[u] >> 12) * qk.dr;
+                    const float a = qk.neg_coeff * (qk.re * __builtin_amdgcn_rcpf(r) - 1.0f);
+                    if (q0 + u < m) s += a * (h[u] - pi);
+                }
+            }
+        } else {
+            s = row_sum_global<U, true, float>(m, P.fval + (size_t)mt.x * 64u + lane,
+                                               P.c32 + (size_t)mt.y * 64u + lane, XPlain<float>{p}, pi, i, s);
+        }
+        Ap[i] = s;
+        v = (double)pi * (double)s;
+    }
+    v = block_sum<double, CG_BLK>(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
 // alpha, after folding in the true residual norm if the previous iteration
 // made a reliable update
 __global__ void __launch_bounds__(CG_BLK)
@@ -844,9 +932,37 @@ static int cg_mixed_run(Ctx *c, const SolveOpts &o, SolveResult *res, V *r, V *p
     typedef void (*mx_fn)(int, PkView, const V *, V *, double *, const MixedCtrl *);
     const mx_fn spmv = o.spmv_variant == 1 ? k_mx_spmv_pk<16, V> : k_mx_spmv_pk<8, V>;
     const size_t lds = sizeof(V) * (size_t)pk->hmax;
+    const bool coded = sizeof(V) == sizeof(float) && o.precision == 3;
+    QConsts qk{};
+    if (coded) {
+        Packed *pw = c->pk;
+        const size_t need = pw->tot_val * 64;
+        if (pw->cap_cq < need) {
+            if (pw->cq) hipFree(pw->cq);
+            pw->cq = nullptr;
+            pw->cap_cq = 0;
+            HIP_OK(hipMalloc(&pw->cq, sizeof(unsigned) * (need + need / 8 + 64)));
+            pw->cap_cq = need + need / 8 + 64;
+        }
+        hipLaunchKernelGGL(k_mx_q16_build, dim3(g), dim3(CG_BLK), 0, s, n, P, o.neg_coeff, o.re_lap, o.r_min, pw->cq);
+        LAUNCH_CHECK();
+        qk.neg_coeff = (float)o.neg_coeff;
+        qk.re = (float)o.re_lap;
+        qk.r_lo = (float)o.r_min;
+        qk.dr = (float)((o.re_lap - o.r_min) / Q_CODE_MAX);
+    }
+    const unsigned *cq = c->pk->cq;
     int rc = run_chunks(s, o.max_iter, o.check_every, &mb->d->done, &mb->h->done, [&](int t) {
         prof_begin(PK_SPMV, s);
-        hipLaunchKernelGGL(spmv, dim3(g), dim3(CG_BLK), lds, s, n, P, p, Ap, c->partA, mb->d);
+        if constexpr (sizeof(V) == sizeof(float)) {
+            if (coded)
+                hipLaunchKernelGGL(o.spmv_variant == 1 ? k_mx_spmv_q<16> : k_mx_spmv_q<8>, dim3(g), dim3(CG_BLK), lds, s,
+                                   n, P, cq, qk, p, Ap, c->partA, mb->d);
+            else
+                hipLaunchKernelGGL(spmv, dim3(g), dim3(CG_BLK), lds, s, n, P, p, Ap, c->partA, mb->d);
+        } else {
+            hipLaunchKernelGGL(spmv, dim3(g), dim3(CG_BLK), lds, s, n, P, p, Ap, c->partA, mb->d);
+        }
         prof_end(PK_SPMV, s);
         hipLaunchKernelGGL(k_mx_fin1, dim3(1), dim3(CG_BLK), 0, s, c->partA, g, mb->part_ru, mb->d, t);
         prof_begin(PK_AXPY, s);
@@ -874,7 +990,7 @@ static int cg_mixed(Ctx *c, const SolveOpts &o, SolveResult *res)
     if (!c->mixed) c->mixed = new MixedBufs();
     MixedBufs *mb = c->mixed;
     const size_t NS = ((size_t)(c->cap > 0 ? c->cap : 1) + 63) / 64 * 64;
-    if (o.precision == 2 && !c->fr) {
+    if (o.precision >= 2 && !c->fr) {
         HIP_OK(hipMalloc(&c->fr, sizeof(float) * NS));
         HIP_OK(hipMalloc(&c->fp, sizeof(float) * NS));
         HIP_OK(hipMalloc(&c->fAp, sizeof(float) * NS));
@@ -884,7 +1000,7 @@ static int cg_mixed(Ctx *c, const SolveOpts &o, SolveResult *res)
         HIP_OK(hipHostMalloc(&mb->h, sizeof(MixedCtrl), hipHostMallocDefault));
         HIP_OK(hipMalloc(&mb->part_ru, sizeof(double) * 65536));
     }
-    if (o.precision == 2) return cg_mixed_run<float>(c, o, res, c->fr, c->fp, c->fAp);
+    if (o.precision >= 2) return cg_mixed_run<float>(c, o, res, c->fr, c->fp, c->fAp);
     return cg_mixed_run<double>(c, o, res, c->r, c->pv, c->Ap);
 }
 
@@ -910,7 +1026,7 @@ int solve_ppe(Ctx *c, const SolveOpts &o, SolveResult *res)
         return MPS_ERR_ARG;
     }
     if (o.solver == 1) return iccg_solve(c, o, res);
-    const int mixed = (o.precision == 1 || o.precision == 2);
+    const int mixed = (o.precision >= 1 && o.precision <= 3);
     rc = pack_ppe(c, mixed);
     if (rc) return rc;
     if (c->pk->n_eq == 0) {

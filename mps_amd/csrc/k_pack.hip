@@ -293,7 +293,7 @@ void packed_free(Packed *p)
     if (!p) return;
     void *bufs[] = {p->row_of_slot, p->slot_of_row, p->len, p->diag, p->b, p->x, p->sw, p->w16, p->w32,
                     p->bh, p->bmode, p->voff, p->c16off, p->c32off, p->hoff, p->smeta, p->bmeta, p->halo,
-                    p->val, p->cw16, p->c32, p->fval, p->ddir, p->dmax};
+                    p->val, p->cw16, p->c32, p->fval, p->ddir, p->dmax, p->cq};
     for (void *b : bufs)
         if (b) hipFree(b);
     if (p->htot) hipHostFree(p->htot);

@@ -630,7 +630,9 @@ static SolveOpts solve_opts(const SimConfig *cf, const StepConsts &k)
     so.dense_ic = (D_ == 2);
     so.spmv_variant = (int)g_opt.spmv;
     so.fused = (int)g_opt.fused;
-    (void)k;
+    so.neg_coeff = k.neg_coeff;
+    so.re_lap = k.re_lap;
+    so.r_min = k.r_min;
     return so;
 }
 

@@ -1,6 +1,6 @@
 """Mixed precision PPE solves (option "precision" 1: FP32 matrix with FP64
-CG vectors; 2: FP32 matrix and vectors) against the FP64 GPU solve of the
-same scene.
+CG vectors; 2: FP32 matrix and vectors; 3: FP32 vectors, off-diagonals
+from 16-bit distance codes) against the FP64 GPU solve of the same scene.
 
 Nothing in the reference pins the mixed form; its contract is the FP64
 solution: relative L2 pressure error <= 1e-6 at every step, positions within
@@ -22,7 +22,7 @@ def say(*a):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("precision", [1, 2])
+@pytest.mark.parametrize("precision", [1, 2, 3])
 @pytest.mark.parametrize("name,steps", [("dam3d_small", 6), ("dam3d_mid", 3)])
 def test_mixed_matches_fp64(lib3, name, steps, precision):
     dim, cfg, pos, vel, typ = H.build(name, cg_tolerance=1e-9)
