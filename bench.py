@@ -449,14 +449,16 @@ def _fields(lib, ps, n):
     return np.array(arr["pressure"]), np.array(arr["pos"])
 
 
-def compare_forms(lib, ps, n, w, a):
+def compare_forms(lib, ps, n, w, a, precision=None, emit=True):
     """The same scene, from the same initial parameters, resident for
     --steps steps in FP64 and in the mixed form: particle-steps/s of both
-    and the relative L2 differences of pressure and positions at the end."""
-    mixed = PRECISIONS[a.precision] or 1
+    and the relative L2 differences of pressure and positions at the end.
+    Returns the result; prints it as one JSON line with emit."""
+    precision = precision or a.precision
+    mixed = PRECISIONS[precision] or 1
     init = {f: getattr(ps.contents, f) for f in ("n0", "lambda_", "C_LL", "C_SL", "n0_count")}
     out = {}
-    for name, prec in (("fp64", 0), (a.precision if a.precision != "fp64" else "mixed", mixed)):
+    for name, prec in (("fp64", 0), (precision if precision != "fp64" else "mixed", mixed)):
         q, _ = build_scene(lib, w)
         for f, v in init.items():
             setattr(q.contents, f, v)
@@ -474,13 +476,17 @@ def compare_forms(lib, ps, n, w, a):
     names = list(out)
     (p0, x0), (p1, x1) = out[names[0]]["fields"], out[names[1]]["fields"]
     rel = lambda u, v: float(np.linalg.norm(u - v) / max(np.linalg.norm(v), 1e-300))
-    print(json.dumps({"metric": "mixed vs FP64, same scene", "workload": a.workload, "particles": n,
-                      "steps": a.steps, "forms": {k: {"particle_steps_per_s": round(v["rate"], 1),
-                                                      "ms_per_step": round(v["ms"], 3),
-                                                      "mean_iters_per_step": round(v["iters"], 1)}
-                                                  for k, v in out.items()},
-                      "speedup": round(out[names[1]]["rate"] / out[names[0]]["rate"], 3),
-                      "rel_l2_pressure": rel(p1, p0), "rel_l2_position": rel(x1, x0)}), flush=True)
+    res = {"metric": "mixed vs FP64, same scene", "workload": a.workload, "particles": n,
+           "steps": a.steps, "forms": {k: {"particle_steps_per_s": round(v["rate"], 1),
+                                           "ms_per_step": round(v["ms"], 3),
+                                           "mean_iters_per_step": round(v["iters"], 1)}
+                                       for k, v in out.items()},
+           "speedup": round(out[names[1]]["rate"] / out[names[0]]["rate"], 3),
+           "rel_l2_pressure": rel(p1, p0), "rel_l2_position": rel(x1, x0)}
+    lib.set_option("precision", 0)
+    if emit:
+        print(json.dumps(res), flush=True)
+    return res
 
 
 def sync_rate(lib, ps, n, cfg, a):
@@ -549,6 +555,8 @@ def main():
     ap.add_argument("--fused", type=int, default=0,
                     help="FP64 CG iteration: 0 five kernels, 1 two (each reduction repeated by the next kernel's blocks)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mixed-check", default="mixed32q", choices=("none",) + tuple(p for p in PRECISIONS if p != "fp64"),
+                    help="FP64 runs: also run this mixed form on the same scene and steps and report it beside")
     ap.add_argument("--cpu-kind", default="reference", choices=("reference", "port"),
                     help="CPU baseline: the reference built from its sources (oracle/_ref; the oracle "
                          "port where that is absent) or the oracle port")
@@ -655,6 +663,17 @@ def main():
         if lib.mps_hip_profile_get(name.encode(), C.byref(ms), C.byref(cnt)) == 0 and cnt.value:
             kernels[name] = dict(total_ms=round(ms.value, 3), launches=cnt.value)
     lib.mps_hip_release(ps)
+    mixed = None
+    if a.precision == "fp64" and a.solver == "cg" and a.mixed_check != "none":
+        # the metric's other half: the mixed form on the same scene and steps
+        r = compare_forms(lib, ps, n, w, a, precision=a.mixed_check, emit=False)
+        fm = r["forms"]
+        mixed = {"precision": a.mixed_check, "steps": a.steps,
+                 "fp64_particle_steps_per_s": fm["fp64"]["particle_steps_per_s"],
+                 "mixed_particle_steps_per_s": fm[a.mixed_check]["particle_steps_per_s"],
+                 "speedup": r["speedup"], "rel_l2_pressure": r["rel_l2_pressure"],
+                 "rel_l2_position": r["rel_l2_position"],
+                 "mean_iters_per_step": {k: v["mean_iters_per_step"] for k, v in fm.items()}}
     cpu = None
     if not a.no_cpu_baseline and rank == 0:
         try:
@@ -678,6 +697,7 @@ def main():
                    "spmv_variant": int(lib.L.mps_hip_get_option(b"spmv")),
                    "check_every": int(lib.L.mps_hip_get_option(b"check_every")), "graph": a.graph},
         "autotune": tune,
+        "mixed": mixed,
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels_ms_one_step": kernels,
