@@ -116,41 +116,47 @@ def cpu_baseline(w, tol, seconds_hint=30.0):
                        "mean %.0f iterations/step, %.1f s" % (steps, len(typ), iters / steps, el))
 
 
-def gpu_bnorms(lib, w, tol, k):
-    """sqrt(b.b) of the PPE at steps 1..k from rest, from the GPU; its
-    right-hand side is bit-exact with the reference's."""
-    cfg = make_config(lib, w, 0, tol)
-    lib.set_config(cfg)
-    ps, n = build_scene(lib, w)
-    lib.particle_system_calc_initial_params(ps)
-    out, iters = [], []
-    for _ in range(k):
-        lib.check(lib.mps_hip_run_steps(ps, 1), "step")
-        st = lib.stats(ps)
-        out.append(st.last_bnorm)
-        iters.append(st.last_iters)
-    lib.mps_hip_release(ps)
-    lib.particle_system_free(ps)
-    return out, iters
+class GpuBnorms:
+    """sqrt(b.b) of the PPE step by step from rest, from a GPU run of the
+    same scene; its right-hand side is bit-exact with the reference's."""
+
+    def __init__(self, lib, w, tol):
+        self.lib = lib
+        self.cfg = make_config(lib, w, 0, tol)
+        lib.set_config(self.cfg)
+        self.ps, _ = build_scene(lib, w)
+        lib.particle_system_calc_initial_params(self.ps)
+        self.iters = []
+
+    def next(self):
+        self.lib.set_config(self.cfg)
+        self.lib.check(self.lib.mps_hip_run_steps(self.ps, 1), "step")
+        st = self.lib.stats(self.ps)
+        self.iters.append(st.last_iters)
+        return st.last_bnorm
+
+    def close(self):
+        self.lib.mps_hip_release(self.ps)
+        self.lib.particle_system_free(self.ps)
 
 
-def cpu_baseline_reference(lib, w, tol, seconds=20.0, max_steps=8):
+def cpu_baseline_reference(lib, w, tol, seconds=20.0, max_steps=300):
     """The reference itself, compiled from its own sources with its OpenMP
     flags (oracle/_ref/libref3dp.so, built by oracle/Makefile), on this
     host: simulation_step of the same workload from rest, for as many steps
     as fit about `seconds` (at least one).  The reference's CG stops on an
     absolute residual (pressure_solver.c:90-131); step s gets tol * |b_s|,
-    with |b_s| recorded by the GPU run of the same steps, so both stop on
-    the same relative criterion.  Initial parameters come from the oracle
-    (bit-identical to the reference's, whose O(N^2) pass takes hours at
-    this size) and the initial search is not timed, as in simulation_run.
-    None when oracle/_ref is absent."""
+    with |b_s| from a GPU run of the same steps kept in lockstep, so both
+    stop on the same relative criterion.  Initial parameters come from the
+    oracle (bit-identical to the reference's, whose O(N^2) pass takes hours
+    at this size) and the initial search is not timed, as in
+    simulation_run.  None when oracle/_ref is absent."""
     from oracle.oracle import Oracle, Reference
     if not Reference.available("3dp"):
         return None
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import scenes  # the reference generator's lattice rules
-    bn, gi = gpu_bnorms(lib, w, tol, max_steps)
+    gb = GpuBnorms(lib, w, tol)
     ref = Reference("3dp")
     cfg_o = make_config(lib, w, 0, tol)
     cfg_r = ref.config_from(cfg_o)
@@ -166,7 +172,7 @@ def cpu_baseline_reference(lib, w, tol, seconds=20.0, max_steps=8):
     ref.L.neighbor_search_cell_linked_list(nl, sr.ptr(), cl, ref.search_radius())
     steps, el = 0, 0.0
     for s in range(1, max_steps + 1):
-        cfg_r.cg_tolerance = tol * bn[s - 1]
+        cfg_r.cg_tolerance = tol * gb.next()
         t0 = time.perf_counter()
         ref.L.simulation_step(sr.ptr(), nl, cl, s)
         el += time.perf_counter() - t0
@@ -174,12 +180,14 @@ def cpu_baseline_reference(lib, w, tol, seconds=20.0, max_steps=8):
         if el > seconds:
             break
     ref.free_lists(nl, cl)
+    gi = gb.iters
+    gb.close()
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return dict(value=len(typ) * steps / el, unit="particle-steps/s", cores=threads, kind="reference",
                 sample="%d step(s) from rest of the %d-particle workload through the reference's own "
                        "simulation_step (OpenMP, %d threads), CG to tol*|b| per step (the GPU's CG took "
                        "%.0f iterations/step on those steps), %.1f s"
-                % (steps, len(typ), threads, sum(gi[:steps]) / steps, el))
+                % (steps, len(typ), threads, sum(gi) / steps, el))
 
 
 def profiled_traffic(precision, workload, nnz):
@@ -226,6 +234,15 @@ def run_ensemble(lib, a):
     total = sum(m[2] for m in scenes)
     print("bench: %d scenes, %d particles (%d-%d per scene)" % (len(scenes), total, min(m[2] for m in scenes),
                                                                   max(m[2] for m in scenes)), file=sys.stderr, flush=True)
+    tune = None
+    if a.autotune and a.solver == "cg":
+        # tiles of the batched CG from the largest scene
+        big = max(scenes, key=lambda m: m[2])
+        lib.set_config(big[0])
+        res = mps_amd.TuneResult()
+        lib.check(lib.mps_hip_autotune(big[1], 1, C.byref(res)), "autotune")
+        tune = {"scene_particles": big[2], "spmv": res.spmv, "check_every": res.check_every,
+                "spmv_ms": [round(v, 5) for v in res.spmv_ms], "solve_ms": [round(v, 2) for v in res.solve_ms]}
     PSP = C.POINTER(lib.abi.ParticleSystem)
     ps_arr = (PSP * len(scenes))(*[m[1] for m in scenes])
     cf_arr = (C.POINTER(lib.abi.SimConfig) * len(scenes))(*[C.pointer(m[0]) for m in scenes])
@@ -273,8 +290,10 @@ def run_ensemble(lib, a):
         "config": {"workload": a.workload, "scenes": len(scenes), "particles": total, "l0": ws[0]["l0"],
                    "dt": ws[0]["dt"], "seed": params.ENSEMBLE_SEED,
                    "tolerance": "relative %g" % a.tol, "mean_iters_per_step": round(sum(iters) / len(iters), 1),
-                   "max_iters_per_step": round(max(iters), 1), "parallelism": "1 GPU, batched CG"},
-        "sequential": seq, "cpu_baseline": cpu}), flush=True)
+                   "max_iters_per_step": round(max(iters), 1), "parallelism": "1 GPU, batched CG",
+                   "graph": a.graph, "spmv_variant": int(lib.L.mps_hip_get_option(b"spmv")),
+                   "check_every": int(lib.L.mps_hip_get_option(b"check_every"))},
+        "autotune": tune, "sequential": seq, "cpu_baseline": cpu}), flush=True)
     for cfg, ps, n in scenes:
         lib.mps_hip_release(ps)
         lib.particle_system_free(ps)
@@ -550,8 +569,9 @@ def main():
     ap.add_argument("--spmv", type=int, default=0, help="SpMV kernel variant of the FP64 CG (0-2)")
     ap.add_argument("--autotune", type=int, default=1,
                     help="1: pick the SpMV variant and polling interval with mps_hip_autotune first (FP64 CG)")
-    ap.add_argument("--graph", type=int, default=0,
-                    help="1: replay the particle phases of each step from captured HIP graphs")
+    ap.add_argument("--graph", type=int, default=None,
+                    help="1: replay the particle phases of each step from captured HIP graphs "
+                         "(default: 1 for the ensemble workloads, 0 otherwise)")
     ap.add_argument("--fused", type=int, default=0,
                     help="FP64 CG iteration: 0 five kernels, 1 two (each reduction repeated by the next kernel's blocks)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -590,12 +610,16 @@ def main():
     lib = mps_amd.load(3)
     lib.check(lib.mps_hip_init(0), "mps_hip_init")
     if "ensemble" in w:
+        if a.graph is None:
+            a.graph = 1
         lib.set_option("tol_mode", 1)
         lib.set_option("exit_on_error", 0)
         lib.set_option("spmv", a.spmv)
         lib.set_option("graph", a.graph)
         run_ensemble(lib, a)
         return
+    if a.graph is None:
+        a.graph = 0
     cfg = make_config(lib, w, 1 if a.solver == "iccg" else 0, a.tol)
     lib.set_config(cfg)
     lib.set_option("tol_mode", 1)
