@@ -572,8 +572,8 @@ def main():
     ap.add_argument("--graph", type=int, default=None,
                     help="1: replay the particle phases of each step from captured HIP graphs "
                          "(default: 1 for the ensemble workloads, 0 otherwise)")
-    ap.add_argument("--fused", type=int, default=0,
-                    help="FP64 CG iteration: 0 five kernels, 1 two (each reduction repeated by the next kernel's blocks)")
+    ap.add_argument("--fused", type=int, default=2,
+                    help="FP64 CG iteration (same bits in every form): 2 three kernels (default), 0 five, 1 two")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mixed-check", default="mixed32q", choices=("none",) + tuple(p for p in PRECISIONS if p != "fp64"),
                     help="FP64 runs: also run this mixed form on the same scene and steps and report it beside")

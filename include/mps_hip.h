@@ -248,10 +248,13 @@ int  mps_hip_step_backward(ParticleSystem *ps, NeighborList *nl, const double *r
  *   "check_every"    CG iterations between host convergence polls (16)
  *   "spmv"           entries per load batch in the CG's SpMV:
  *                      0 eight, 1 sixteen, 2 four
- *   "fused"          0 (default): FP64 CG in five kernels per iteration;
- *                      1: two (the previous iteration's reduction done
- *                      redundantly by every block of the next kernel),
- *                      same results bit for bit, measured 2 % slower
+ *   "fused"          FP64 CG iteration, same results bit for bit in all
+ *                      forms: 2 (default) three kernels, each finaliser's
+ *                      reduction repeated by the blocks of the next vector
+ *                      kernel (1.3 % faster than 0); 0 five kernels (SpMV,
+ *                      one-block finaliser, axpy, finaliser, p update);
+ *                      1 two (the previous iteration closed inside the
+ *                      SpMV, p formed where it is read; 2 % slower than 0)
  *   "graph"          1: the particle phases of each step (predictor to
  *                      assembly) are captured once as a HIP graph and
  *                      replayed, one graph per buffer parity; same results

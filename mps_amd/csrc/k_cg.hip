@@ -480,6 +480,56 @@ k_cg_iter_b(int n, int t, const double *__restrict__ p, const double *__restrict
     if (threadIdx.x == 0) partB[blockIdx.x] = v;
 }
 
+// ---- three-kernel form (option fused = 2) ----------------------------
+// The SpMV of the five-kernel form; then k_cg_iter_b (every block sums the
+// SpMV partials as k_spmv_fin does, then x, r and r.r); then k_cg3_pupdate:
+// every block sums the r.r partials as k_axpy_fin does, takes its stop
+// tests (block 0 records them), and forms p = r + beta p.  Only the
+// 512-block vector kernels repeat a reduction, over 3,366 and 512 partials;
+// the arithmetic is the five-kernel form's, so the bits are too.
+__global__ void __launch_bounds__(CG_BLK)
+k_cg3_pupdate(int n, int t, int max_iter, const double *__restrict__ r, double *__restrict__ p, const double *partB,
+              unsigned nb, CgCtrl *ctrl)
+{
+    __shared__ double sh_beta;
+    __shared__ int sh_stop;
+    if (ctrl->done) return;
+    const double rr_new = fin_sum(partB, nb);
+    if (threadIdx.x == 0) {
+        const double rr = ctrl->rr[t & 1];
+        int stop = 0, iters = 0;
+        double bt = 0.0;
+        if (rr < 1.0e-60) {
+            stop = 1;
+            iters = t;
+        } else {
+            bt = rr_new / rr;
+            if (t + 1 >= max_iter) {
+                stop = 1;
+                iters = max_iter;
+            } else if (sqrt(rr_new) < ctrl->stop) {
+                stop = 1;
+                iters = t + 1;
+            }
+        }
+        if (blockIdx.x == 0) {
+            ctrl->rr[(t + 1) & 1] = rr_new;
+            ctrl->res = sqrt(rr_new);
+            ctrl->beta = bt;
+            if (stop) {
+                ctrl->iters = iters;
+                ctrl->done = 1;
+            }
+        }
+        sh_beta = bt;
+        sh_stop = stop;
+    }
+    __syncthreads();
+    if (sh_stop) return;
+    const double beta = sh_beta;
+    for (int i = blockIdx.x * CG_BLK + threadIdx.x; i < n; i += gridDim.x * CG_BLK) p[i] = r[i] + beta * p[i];
+}
+
 int cg_alloc_ctrl(Ctx *c)
 {
     if (!c->ctrl) {
@@ -574,7 +624,23 @@ static int cg_fp64(Ctx *c, const SolveOpts &o, SolveResult *res)
     LAUNCH_CHECK();
     const size_t lds = sizeof(double) * (size_t)pk->hmax;
     int rc = 0;
-    if (o.fused) {
+    if (o.fused == 2) {
+        const spmv_fn spmv = pick_spmv(o.spmv_variant);
+        rc = run_chunks(s, o.max_iter, o.check_every, &c->ctrl->done, &c->hctrl->done, [&](int t) {
+            prof_begin(PK_SPMV, s);
+            hipLaunchKernelGGL(spmv, dim3(g), dim3(CG_BLK), lds, s, n, P, p, Ap, c->partA, c->ctrl);
+            prof_end(PK_SPMV, s);
+            prof_begin(PK_AXPY, s);
+            hipLaunchKernelGGL(k_cg_iter_b, dim3(gv), dim3(CG_BLK), 0, s, n, t, p, Ap, x, r, c->partA, g, c->partB,
+                               c->ctrl);
+            prof_end(PK_AXPY, s);
+            prof_begin(PK_PUPDATE, s);
+            hipLaunchKernelGGL(k_cg3_pupdate, dim3(gv), dim3(CG_BLK), 0, s, n, t, o.max_iter, r, p, c->partB, gv,
+                               c->ctrl);
+            prof_end(PK_PUPDATE, s);
+        });
+        if (rc) return rc;
+    } else if (o.fused) {
         // p ping-pongs: iteration t reads pb[t & 1] (p_{t-1}; b at t = 0)
         // and writes p_t to pb[(t + 1) & 1]
         if (!c->pv2) HIP_OK(hipMalloc(&c->pv2, sizeof(double) * (((size_t)c->cap + 63) / 64 * 64)));

@@ -35,7 +35,7 @@ void mps_set_error(int code, const char *file, int line, const char *msg)
 
 struct Options {
     double tol_mode = 0, precision = 0, inner_tol = 1e-3, exit_on_error = 1, check_every = 16;
-    double spmv = 0, profile = 0, verbose = 0, fused = 0, graph = 0;
+    double spmv = 0, profile = 0, verbose = 0, fused = 2, graph = 0;
 };
 static Options g_opt;
 

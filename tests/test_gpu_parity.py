@@ -253,13 +253,15 @@ def test_spmv_variants_same_bits(lib3):
 
 
 def test_fused_cg_same_bits(lib3):
-    """The two-kernel CG iteration (previous iteration closed inside the
-    SpMV, p formed where it is read) gives the five-kernel form's bits."""
+    """The fused CG iterations give the five-kernel form's bits: two kernels
+    (previous iteration closed inside the SpMV, p formed where it is read)
+    and three (each finaliser's reduction repeated by the blocks of the next
+    vector kernel)."""
     dim, cfg, pos, vel, typ = H.build("dam3d_mid", cg_tolerance=1e-10)
     lib3.set_option("tol_mode", 1)
     out = []
     try:
-        for f in (0, 1):
+        for f in (0, 1, 2):
             lib3.set_option("fused", f)
             say("fused", f)
             g = H.GpuRun(lib3, cfg, pos, vel, typ)
@@ -272,6 +274,7 @@ def test_fused_cg_same_bits(lib3):
     finally:
         lib3.set_option("fused", 0)
     assert out[0] == out[1]
+    assert out[0] == out[2]
 
 
 @pytest.mark.parametrize("name,libname", [("dam3d_mid", "lib3"), ("dam2d_l0.01", "lib2")])
