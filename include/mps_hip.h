@@ -100,6 +100,11 @@ void collision(ParticleSystem *ps);
 #endif
 void calc_surface_tension(ParticleSystem *ps, NeighborList *nl);
 void calc_surface_tension_SL(ParticleSystem *ps, NeighborList *nl);
+#ifdef MPS_FRICTION_FIELDS
+/* mps_fluid3D/include/operators.h:42 -- hybrid wall friction, in the 3-D
+ * serial program's build (libmps_hip3ds.so) */
+void apply_friction(ParticleSystem *ps, NeighborList *nl);
+#endif
 
 /* --- boundary (mps_fluid3D_parallel/include/boundary.h:8,12-14) -------- */
 void apply_wall_boundary(ParticleSystem *ps);

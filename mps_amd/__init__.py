@@ -15,8 +15,10 @@ from . import abi
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def lib_path(dim=3):
-    return os.path.join(HERE, "libmps_hip%dd.so" % dim)
+def lib_path(dim=3, serial=False):
+    """libmps_hip3d.so / libmps_hip2d.so, or libmps_hip3ds.so (the 3-D serial
+    program's SimConfig layout, with its hybrid wall friction)."""
+    return os.path.join(HERE, "libmps_hip%dd%s.so" % (dim, "s" if serial else ""))
 
 
 class StepStats(C.Structure):
@@ -37,12 +39,12 @@ _libs = {}
 class Lib:
     """Prototyped handle on libmps_hip{dim}d.so."""
 
-    def __init__(self, dim=3):
-        path = lib_path(dim)
+    def __init__(self, dim=3, serial=False):
+        path = lib_path(dim, serial)
         if not os.path.exists(path):
             raise RuntimeError("%s is not built (python -m mps_amd.build)" % path)
         self.dim = dim
-        self.abi = abi.types(dim)
+        self.abi = abi.types(dim, serial)
         self.L = C.CDLL(path)
         a = self.abi
         L = self.L
@@ -117,6 +119,8 @@ class Lib:
             "mps_hip_fp64_probe": (C.c_int, [C.c_int, D, D, D, D]),
         }
         self.missing = []
+        if serial:
+            sig["apply_friction"] = (None, [PS, NL])
         for name, (res, args) in sig.items():
             try:
                 f = getattr(L, name)
@@ -178,7 +182,8 @@ class Lib:
         return x[:n_eq]
 
 
-def load(dim=3):
-    if dim not in _libs:
-        _libs[dim] = Lib(dim)
-    return _libs[dim]
+def load(dim=3, serial=False):
+    key = (dim, serial)
+    if key not in _libs:
+        _libs[key] = Lib(dim, serial)
+    return _libs[key]

@@ -1,11 +1,14 @@
-"""Build recipe of libmps_hip{2,3}d.so and the mps_sim{2,3}d drivers.
+"""Build recipe of libmps_hip{3,2}d.so, libmps_hip3ds.so and the mps_sim
+drivers.  libmps_hip3ds.so is the 3-D library in the layout of the 3-D
+serial program (mps_fluid3D: SimConfig with the hybrid-friction fields,
+MPS_FRICTION_FIELDS), with its wall friction.
 
 HIP sources are compiled by hipcc for gfx950 only, FP64 contraction off (the
 reference's x86 build has no FMA and parity is bit-exact), packed-FP32 ops
 disabled; host C sources by gcc with OpenMP.  Everything is written in-tree
 under mps_amd/ so the built libraries travel with the repository snapshot.
 
-    python -m mps_amd.build            # both dimensions
+    python -m mps_amd.build            # 3d, 2d and 3ds
     python -m mps_amd.build --dim 3
 """
 import argparse
@@ -37,14 +40,17 @@ def _run(cmd):
     return p.stdout + p.stderr
 
 
-def lib_path(dim):
-    return os.path.join(HERE, "libmps_hip%dd.so" % dim)
+def lib_path(dim, serial=False):
+    return os.path.join(HERE, "libmps_hip%dd%s.so" % (dim, "s" if serial else ""))
 
 
-def build_dim(dim, jobs=8, verbose=False):
-    bdir = os.path.join(HERE, "_build", "%dd" % dim)
+def build_dim(dim, jobs=8, verbose=False, serial=False):
+    tag = "%dd%s" % (dim, "s" if serial else "")
+    bdir = os.path.join(HERE, "_build", tag)
     os.makedirs(bdir, exist_ok=True)
     defs = ["-DMPS_DIM=%d" % dim, "-I" + INC, "-I" + os.path.join(HERE, "csrc")]
+    if serial:
+        defs.append("-DMPS_FRICTION_FIELDS")
     cmds = []
     objs = []
     for s in HIP_SRC:
@@ -65,22 +71,24 @@ def build_dim(dim, jobs=8, verbose=False):
                 print(out)
     gomp = subprocess.run(["gcc", "-print-file-name=libgomp.so"], capture_output=True,
                           text=True).stdout.strip()
-    tmp = lib_path(dim) + ".tmp"
+    tmp = lib_path(dim, serial) + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs +
          [gomp, "-Wl,-Bsymbolic"])
-    os.replace(tmp, lib_path(dim))
+    os.replace(tmp, lib_path(dim, serial))
     # stand-alone driver, same CLI as the reference's mps_sim
-    exe = os.path.join(HERE, "mps_sim%dd" % dim)
-    _run(["gcc", "-O2", "-Wall", "-DMPS_DIM=%d" % dim, "-I" + INC,
+    exe = os.path.join(HERE, "mps_sim%s" % tag)
+    _run(["gcc", "-O2", "-Wall"] + [x for x in defs if x.startswith("-D")] + ["-I" + INC,
           os.path.join(HERE, "host", "main.c"), "-o", exe, "-L" + HERE,
-          "-l:libmps_hip%dd.so" % dim, "-Wl,-rpath,$ORIGIN"])
-    return lib_path(dim)
+          "-l:libmps_hip%s.so" % tag, "-Wl,-rpath,$ORIGIN"])
+    return lib_path(dim, serial)
 
 
-def build(dims=(3, 2), jobs=8, verbose=False):
+def build(dims=(3, 2), jobs=8, verbose=False, serial3=True):
     out = []
     for d in dims:
         out.append(build_dim(d, jobs=jobs, verbose=verbose))
+    if serial3 and 3 in dims:
+        out.append(build_dim(3, jobs=jobs, verbose=verbose, serial=True))
     return out
 
 

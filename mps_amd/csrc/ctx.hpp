@@ -62,6 +62,10 @@ struct StepConsts {
     int stride;                      // max_neighbors
     int surf_method, ppe_type, hs_mode, cmps, clamp, collide;
     int st_enabled, st_sl_on;
+    // hybrid wall friction of the 3-D serial program (built with
+    // MPS_FRICTION_FIELDS; mps_fluid3D/src/operators.c:337-463)
+    int fric;
+    double fr_theta_s, fr_dth, fr_vc, fr_dyn;   // radians, radians, m/s, alpha*nu*2*DIM/(n0*lambda)
 };
 
 // Counters written by the search and assembly kernels.
@@ -180,6 +184,7 @@ int  launch_scan_excl(const int *in, int *out, int n, int *blk, hipStream_t s);
 int  launch_unpack(Ctx *c, const void *stage_dev, int with_list);
 int  launch_pack(Ctx *c, void *stage_dev);
 int  launch_visc_predict(Ctx *c, const StepConsts &k);
+int  launch_friction(Ctx *c, const StepConsts &k);   // the friction alone, on the current velocities
 int  launch_collide_bin(Ctx *c, const StepConsts &k, int collide);
 int  launch_bin(Ctx *c, const StepConsts &k);
 int  launch_sort(Ctx *c, const StepConsts &k, int full);

@@ -240,6 +240,19 @@ void calc_viscosity_term(ParticleSystem *ps, NeighborList *nl)
     end_call(cl, ps);
 }
 
+#ifdef MPS_FRICTION_FIELDS
+// apply_friction (mps_fluid3D/src/operators.c:337-463): the hybrid wall
+// friction on the current velocities, over nl
+void apply_friction(ParticleSystem *ps, NeighborList *nl)
+{
+    if (!g_config || !g_config->friction_enabled) return;
+    Call cl;
+    if (!begin_call(cl, ps, nl, true)) return;
+    launch_friction(cl.c, cl.k);
+    end_call(cl, ps);
+}
+#endif
+
 void calc_pressure_gradient(ParticleSystem *ps, NeighborList *nl)
 {
     Call cl;

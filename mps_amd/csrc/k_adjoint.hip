@@ -1009,6 +1009,9 @@ static const char *adj_unsupported(const SimConfig *cf)
     if (cf->ppe_type != 0) return "ppe_type";
     if (cf->cmps_gradient != 0) return "cmps_gradient";
     if (cf->surface_detection_method != 0) return "surface_detection_method";
+#ifdef MPS_FRICTION_FIELDS
+    if (cf->friction_enabled) return "hybrid wall friction";
+#endif
     return nullptr;
 }
 

@@ -293,6 +293,141 @@ int launch_list_import(Ctx *c, const int *counts_dev, const int *offsets_dev)
 //    simulation.c:31-67, operators.c:50-85
 // ---------------------------------------------------------------------
 
+// Hybrid wall friction of the 3-D serial program (mps_fluid3D/src/
+// operators.c:337-463) on the predicted velocity v of fluid slot i, over
+// the stale list, in the reference's arithmetic and order.  A particle is
+// on the contact line when its fluid-only density is below
+// surface_threshold n0 and a wall particle lies within re_st; the nearest
+// one (first strict minimum in list order) gives the wall normal from its
+// own wall neighbours, the particle's fluid neighbours the free-surface
+// normal, their angle the contact angle.  Slow particles (|v| <= v_c) are
+// held when that angle is within delta_theta of the wetting angle; faster
+// ones lose slip velocity along the wall, at most all of it.  Only v
+// changes and only positions and types are read, so particles are
+// independent, as in the reference's serial loop.
+__device__ void wall_friction(int i, const dvec &xi, const dvec *__restrict__ X, const int *__restrict__ T,
+                              const int *__restrict__ nbr, const int *__restrict__ ncnt, const StepConsts &k,
+                              double v[D_])
+{
+    const int m = ncnt[i];
+    double nf = 0.0, r2s = 1.0e30, dr[D_];
+    int js = -1;
+    for (int q = 0; q < m; q++) {
+        const int j = nbr[sell_off(i, q, k.stride)];
+        const double r2 = dist2(xi, X[j], dr);
+        const int tj = T[j];
+        if (tj == FLUID_PARTICLE) {
+            if (r2 < k.re_n2) nf += wfun(sqrt(r2), k.re_n, k.r_min);
+        } else if (tj == WALL_PARTICLE && r2 < k.re_st * k.re_st && r2 < r2s) {
+            r2s = r2;
+            js = j;
+        }
+    }
+    if (js < 0 || nf >= k.thr_n0) return;
+    double nw[D_], nlv[D_], es[D_];
+#pragma unroll
+    for (int d = 0; d < D_; d++) nw[d] = nlv[d] = 0.0;
+    const dvec xw = X[js];
+    const int mw = ncnt[js];
+    for (int q = 0; q < mw; q++) {
+        const int mm = nbr[sell_off(js, q, k.stride)];
+        if (T[mm] != WALL_PARTICLE) continue;
+        const double r = sqrt(dist2(X[mm], xw, dr));
+        if (r < k.st_tiny) continue;
+        const double w = wfun(r, k.re_st, k.r_min);
+#pragma unroll
+        for (int d = 0; d < D_; d++) nw[d] += w * dr[d] / r;
+    }
+    double m2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D_; d++) m2 += nw[d] * nw[d];
+    const double magw = sqrt(m2);
+    if (magw < 1.0e-9) return;
+#pragma unroll
+    for (int d = 0; d < D_; d++) nw[d] /= magw;
+    int nfl = 0;
+    for (int q = 0; q < m; q++) {
+        const int j = nbr[sell_off(i, q, k.stride)];
+        if (T[j] != FLUID_PARTICLE) continue;
+        const double r = sqrt(dist2(xi, X[j], dr));
+        if (r < k.st_tiny) continue;
+        const double w = wfun(r, k.re_st, k.r_min);
+#pragma unroll
+        for (int d = 0; d < D_; d++) nlv[d] -= w * dr[d] / r;
+        nfl++;
+    }
+    if (nfl == 0) return;
+    m2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D_; d++) m2 += nlv[d] * nlv[d];
+    const double magl = sqrt(m2);
+    if (magl < 1.0e-9) return;
+#pragma unroll
+    for (int d = 0; d < D_; d++) nlv[d] /= magl;
+    double cs = 0.0;
+#pragma unroll
+    for (int d = 0; d < D_; d++) cs -= nw[d] * nlv[d];
+    const double theta = acos(fmax(-1.0, fmin(1.0, cs)));
+    double dot = 0.0;
+#pragma unroll
+    for (int d = 0; d < D_; d++) dot += nlv[d] * nw[d];
+    m2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D_; d++) {
+        es[d] = nlv[d] - dot * nw[d];
+        m2 += es[d] * es[d];
+    }
+    const double mage = sqrt(m2);
+    if (mage < 1.0e-9) return;
+#pragma unroll
+    for (int d = 0; d < D_; d++) es[d] /= mage;
+    double v2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D_; d++) v2 += v[d] * v[d];
+    if (sqrt(v2) <= k.fr_vc) {
+        if (fabs(theta - k.fr_theta_s) <= k.fr_dth)
+#pragma unroll
+            for (int d = 0; d < D_; d++) v[d] = 0.0;
+        return;
+    }
+    double sw = 0.0;
+    for (int q = 0; q < m; q++) {
+        const int j = nbr[sell_off(i, q, k.stride)];
+        if (T[j] != WALL_PARTICLE) continue;
+        const double r = sqrt(dist2(xi, X[j], dr));
+        if (r < k.st_tiny) continue;
+        sw += wfun(r, k.re_st, k.r_min);
+    }
+    double vn = 0.0;
+#pragma unroll
+    for (int d = 0; d < D_; d++) vn += v[d] * es[d];
+    double dvn = -k.fr_dyn * k.dt * sw * vn;
+    if (dvn > vn) dvn = vn;
+    if (dvn < -vn) dvn = -vn;
+#pragma unroll
+    for (int d = 0; d < D_; d++) v[d] += dvn * es[d];
+}
+
+// the friction alone on the current velocities (apply_friction drop-in)
+__global__ void __launch_bounds__(BLK)
+k_friction(int n, const dvec *__restrict__ X, dvec *__restrict__ V, const int *__restrict__ T,
+           const int *__restrict__ nbr, const int *__restrict__ ncnt, StepConsts k)
+{
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i >= n || T[i] != FLUID_PARTICLE) return;
+    dvec v = V[i];
+    wall_friction(i, X[i], X, T, nbr, ncnt, k, v.c);
+    V[i] = v;
+}
+
+int launch_friction(Ctx *c, const StepConsts &k)
+{
+    hipLaunchKernelGGL(k_friction, dim3(nblk(c->n)), dim3(BLK), 0, c->stream, c->n, c->X[c->cx], c->V[c->cx],
+                       c->T[c->ct], c->nbr, c->ncnt, k);
+    LAUNCH_CHECK();
+    return 0;
+}
+
 __global__ void __launch_bounds__(BLK)
 k_visc_predict(int n, const dvec *__restrict__ X, const dvec *__restrict__ V, const int *__restrict__ T,
                const int *__restrict__ nbr, const int *__restrict__ ncnt, dvec *__restrict__ Xo,
@@ -342,10 +477,10 @@ k_visc_predict(int n, const dvec *__restrict__ X, const dvec *__restrict__ V, co
     }
     dvec xo = xi, vo = vi;
 #pragma unroll
-    for (int d = 0; d < D_; d++) {
-        vo.c[d] = vi.c[d] + k.dt * acc[d];
-        xo.c[d] = xi.c[d] + k.dt * vo.c[d];
-    }
+    for (int d = 0; d < D_; d++) vo.c[d] = vi.c[d] + k.dt * acc[d];
+    if (k.fric) wall_friction(i, xi, X, T, nbr, ncnt, k, vo.c);
+#pragma unroll
+    for (int d = 0; d < D_; d++) xo.c[d] = xi.c[d] + k.dt * vo.c[d];
     Xo[i] = xo;
     Vo[i] = vo;
 }

@@ -375,6 +375,14 @@ static StepConsts make_consts(const SimConfig *cf, const ParticleSystem *ps, con
 #endif
     k.st_sl_on = ps->C_SL != 0.0;
     k.st_tiny = 1.0e-9 * cf->particle_distance;
+#ifdef MPS_FRICTION_FIELDS
+    // mps_fluid3D/src/operators.c:339-349, same products
+    k.fric = cf->friction_enabled;
+    k.fr_theta_s = cf->wetting_angle_SL * M_PI / 180.0;
+    k.fr_dth = cf->friction_delta_theta * M_PI / 180.0;
+    k.fr_vc = cf->friction_velocity_threshold;
+    k.fr_dyn = cf->dynamic_friction_alpha * cf->viscosity * 2.0 * D_ / (ps->n0 * ps->lambda);
+#endif
     return k;
 }
 

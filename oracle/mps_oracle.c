@@ -354,9 +354,137 @@ static void surface_tension(ParticleSystem *ps, const SimConfig *c, const orc_nl
     }
 }
 
+#ifdef MPS_FRICTION_FIELDS
+/* Hybrid wall friction of the 3-D serial program (mps_fluid3D/src/
+ * operators.c:337-463), applied to u* before r* = r + dt u*
+ * (mps_fluid3D/src/simulation.c:50-53), over the stale list.  A fluid
+ * particle is on the contact line when its fluid-only number density is
+ * below surface_threshold n0 and a wall particle lies within re_st; the
+ * nearest such wall particle (first strict minimum in list order) gives
+ * the wall normal from its own wall neighbours, the particle's fluid
+ * neighbours give the free-surface normal, and their angle the contact
+ * angle.  Slow particles (|u*| <= v_c) are held when that angle is within
+ * delta_theta of the wetting angle; faster ones lose slip velocity along
+ * the wall, at most all of it.  Each particle changes only its own
+ * velocity and reads positions and types, so the order over i is free. */
+static void friction(ParticleSystem *ps, const SimConfig *c, const orc_nlist *nl)
+{
+    const double re_st = c->influence_radius_st, re_n = c->influence_radius_n, l0 = c->particle_distance;
+    const double theta_s = c->wetting_angle_SL * M_PI / 180.0;
+    const double dth = c->friction_delta_theta * M_PI / 180.0;
+    const double v_c = c->friction_velocity_threshold;
+    const double dyn = c->dynamic_friction_alpha * c->viscosity * 2.0 * D / (ps->n0 * ps->lambda);
+    for (int i = 0; i < ps->num; i++) {
+        Particle *pi = &ps->particles[i];
+        if (pi->type != FLUID_PARTICLE) continue;
+        double nf = 0.0, r2s = 1.0e30, dr[D];
+        int js = -1;
+        for (long k = nl->start[i]; k < nl->start[i + 1]; k++) {
+            const int j = nl->idx[k];
+            const Particle *pj = &ps->particles[j];
+            const double r2 = dist2(pi, pj, dr);
+            if (pj->type == FLUID_PARTICLE) {
+                if (r2 < re_n * re_n) nf += orc_weight(sqrt(r2), re_n, l0);
+            } else if (pj->type == WALL_PARTICLE && r2 < re_st * re_st && r2 < r2s) {
+                r2s = r2;
+                js = j;
+            }
+        }
+        if (js < 0 || nf >= c->surface_threshold * ps->n0) continue;
+        /* wall normal: sum over the wall neighbours m of j* of
+         * w(|r_j* - r_m|) (r_j* - r_m) / |r_j* - r_m|, normalised */
+        const Particle *pw = &ps->particles[js];
+        double nw[D], nlv[D], es[D];
+        for (int d = 0; d < D; d++) nw[d] = nlv[d] = 0.0;
+        for (long k = nl->start[js]; k < nl->start[js + 1]; k++) {
+            const Particle *pm = &ps->particles[nl->idx[k]];
+            if (pm->type != WALL_PARTICLE) continue;
+            const double r = sqrt(dist2(pm, pw, dr));
+            if (r < 1.0e-9 * l0) continue;
+            const double w = orc_weight(r, re_st, l0);
+            for (int d = 0; d < D; d++) nw[d] += w * dr[d] / r;
+        }
+        double m2 = 0.0;
+        for (int d = 0; d < D; d++) m2 += nw[d] * nw[d];
+        const double mw = sqrt(m2);
+        if (mw < 1.0e-9) continue;
+        for (int d = 0; d < D; d++) nw[d] /= mw;
+        /* outward free-surface normal: minus the same sum over the fluid
+         * neighbours of i */
+        int nfl = 0;
+        for (long k = nl->start[i]; k < nl->start[i + 1]; k++) {
+            const Particle *pj = &ps->particles[nl->idx[k]];
+            if (pj->type != FLUID_PARTICLE) continue;
+            const double r = sqrt(dist2(pi, pj, dr));
+            if (r < 1.0e-9 * l0) continue;
+            const double w = orc_weight(r, re_st, l0);
+            for (int d = 0; d < D; d++) nlv[d] -= w * dr[d] / r;
+            nfl++;
+        }
+        if (nfl == 0) continue;
+        m2 = 0.0;
+        for (int d = 0; d < D; d++) m2 += nlv[d] * nlv[d];
+        const double ml = sqrt(m2);
+        if (ml < 1.0e-9) continue;
+        for (int d = 0; d < D; d++) nlv[d] /= ml;
+        /* contact angle, cos = -n_w . n_lv */
+        double cs = 0.0;
+        for (int d = 0; d < D; d++) cs -= nw[d] * nlv[d];
+        const double theta = acos(fmax(-1.0, fmin(1.0, cs)));
+        /* slip direction: n_lv projected on the wall plane, normalised */
+        double dot = 0.0;
+        for (int d = 0; d < D; d++) dot += nlv[d] * nw[d];
+        m2 = 0.0;
+        for (int d = 0; d < D; d++) {
+            es[d] = nlv[d] - dot * nw[d];
+            m2 += es[d] * es[d];
+        }
+        const double me = sqrt(m2);
+        if (me < 1.0e-9) continue;
+        for (int d = 0; d < D; d++) es[d] /= me;
+        double v2 = 0.0;
+        for (int d = 0; d < D; d++) v2 += pi->vel[d] * pi->vel[d];
+        if (sqrt(v2) <= v_c) {
+            if (fabs(theta - theta_s) <= dth)
+                for (int d = 0; d < D; d++) pi->vel[d] = 0.0;
+        } else {
+            double sw = 0.0;
+            for (long k = nl->start[i]; k < nl->start[i + 1]; k++) {
+                const Particle *pj = &ps->particles[nl->idx[k]];
+                if (pj->type != WALL_PARTICLE) continue;
+                const double r = sqrt(dist2(pi, pj, dr));
+                if (r < 1.0e-9 * l0) continue;
+                sw += orc_weight(r, re_st, l0);
+            }
+            double vn = 0.0;
+            for (int d = 0; d < D; d++) vn += pi->vel[d] * es[d];
+            double dvn = -dyn * c->dt * sw * vn;
+            if (dvn > vn) dvn = vn;
+            if (dvn < -vn) dvn = -vn;
+            for (int d = 0; d < D; d++) pi->vel[d] += dvn * es[d];
+        }
+    }
+}
+
+/* apply_friction as a call of its own (mps_fluid3D/include/operators.h:42):
+ * the friction on ps's current velocities over the list of ps's current
+ * positions.  Returns 0, or -1 when the list cannot be built. */
+int orc_apply_friction(ParticleSystem *ps, const SimConfig *c)
+{
+    orc_nlist nl = {0};
+    double re = fmax(c->influence_radius_lap, c->influence_radius_n);
+    if (c->surface_tension_enabled) re = fmax(re, c->influence_radius_st);
+    if (orc_search(ps, c, re, &nl)) return -1;
+    if (c->friction_enabled) friction(ps, c, &nl);
+    orc_nlist_free(&nl);
+    return 0;
+}
+#endif
+
 /* acc = gravity (fluid) or 0; viscosity over the stale list
  * (operators.c:50-85); surface tension if enabled; u* = u + dt acc;
- * r* = r + dt u*  (simulation.c:31-67) */
+ * hybrid friction in the 3-D serial layout; r* = r + dt u*
+ * (simulation.c:31-67) */
 void orc_visc_predict(ParticleSystem *ps, const SimConfig *c, const orc_nlist *stale)
 {
     const int n = ps->num;
@@ -389,6 +517,9 @@ void orc_visc_predict(ParticleSystem *ps, const SimConfig *c, const orc_nlist *s
         if (p->type != FLUID_PARTICLE) continue;
         for (int d = 0; d < D; d++) p->vel[d] += dt * p->acc[d];
     }
+#ifdef MPS_FRICTION_FIELDS
+    if (c->friction_enabled) friction(ps, c, stale);
+#endif
     for (int i = 0; i < n; i++) {
         Particle *p = &ps->particles[i];
         if (p->type != FLUID_PARTICLE) continue;

@@ -52,12 +52,13 @@ def make_config(abi, gravity=None, domain_min=None, domain_max=None, **kw):
 
 
 class Oracle:
-    """The C restatement (liboracle{dim}d.so)."""
+    """The C restatement (liboracle{dim}d.so; liboracle3ds.so in the 3-D
+    serial program's layout, with its hybrid wall friction)."""
 
-    def __init__(self, dim=3):
+    def __init__(self, dim=3, friction=False):
         self.dim = dim
-        self.abi = _abi.types(dim)
-        path = os.path.join(HERE, "liboracle%dd.so" % dim)
+        self.abi = _abi.types(dim, friction)
+        path = os.path.join(HERE, "liboracle%dd%s.so" % (dim, "s" if friction else ""))
         if not os.path.exists(path):
             raise RuntimeError("oracle library missing: %s (run make -C oracle)" % path)
         L = C.CDLL(path)
@@ -81,12 +82,20 @@ class Oracle:
                                     C.POINTER(C.c_double), C.c_int, C.c_int, C.c_int,
                                     C.c_double, vp, C.POINTER(C.c_double)]
         L.orc_solve_pressure.argtypes = [vp, PS, CF]
+        if friction:
+            L.orc_apply_friction.argtypes = [PS, CF]
 
     def sim(self, tol_mode=0, dot_order=0):
         return OracleSim(self, tol_mode, dot_order)
 
     def init_params(self, scene, cfg):
         self.L.orc_init_params(scene.ptr(), C.byref(cfg))
+
+    def apply_friction(self, scene, cfg):
+        """The hybrid wall friction alone (serial layout), on the scene's
+        velocities over the list of its positions."""
+        if self.L.orc_apply_friction(scene.ptr(), C.byref(cfg)) != 0:
+            raise RuntimeError("orc_apply_friction failed")
 
     def solve_csr(self, row_ptr, col, val, b, solver=0, dense_ic=None, max_iter=10000,
                   tol=1e-10, tol_mode=0, dot_order=0):

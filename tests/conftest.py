@@ -35,6 +35,18 @@ def lib3():
 
 
 @pytest.fixture(scope="session")
+def lib3s():
+    """The 3-D library in the serial program's layout (with friction)."""
+    import mps_amd
+    lib = mps_amd.load(3, serial=True)
+    rc = lib.mps_hip_init(0)
+    if rc != 0:
+        raise RuntimeError("GPU library did not initialise: %s" % lib.mps_hip_error_string(rc).decode())
+    lib.set_option("exit_on_error", 0)
+    return lib
+
+
+@pytest.fixture(scope="session")
 def lib2():
     import mps_amd
     lib = mps_amd.load(2)

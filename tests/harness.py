@@ -5,6 +5,8 @@ the same SimConfig field values, the same n0/lambda) and the same solver
 stopping rule, so their outputs can be compared field by field.
 """
 import ctypes as C
+import json
+import os
 
 import numpy as np
 
@@ -13,13 +15,33 @@ from mps_amd import abi as A
 from oracle.oracle import Oracle, make_config
 
 FIELDS = ("pos", "vel", "acc", "pressure", "n", "type", "on_surface")
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-def build(name, **over):
+def friction_fixture(abi):
+    """The wall-friction fixture (golden/friction3ds.npz, written by
+    golden/make_friction_fixture.py from the reference's apply_friction):
+    the scene in `abi`'s layout as it was before the friction, its config
+    (3-D serial layout), the expected fluid velocities after, and the fluid
+    mask."""
+    g = np.load(os.path.join(GOLDEN, "friction3ds.npz"))
+    params = json.loads(str(g["params"]))
+    dim, cfg, pos, vel, typ = build(str(g["scene"]), friction=True, **params)
+    typ = g["type"].astype(np.int32)
+    fl = typ == 0
+    pos[fl] = g["fluid_pos"]
+    vel[fl] = g["fluid_vel_in"]
+    sc = A.HostScene(abi, pos, vel, typ)
+    sc.ps.n0 = float(g["n0"])
+    sc.ps.lambda_ = float(g["lam"])
+    return sc, cfg, np.array(g["fluid_vel_out"]), fl
+
+
+def build(name, friction=False, **over):
     sc = scenes.CONFIGS[name]
     dim = sc["dim"]
     p = scenes.params_for(name, **over)
-    cfg = make_config(A.types(dim), **p)
+    cfg = make_config(A.types(dim, friction), **p)
     pos, vel, typ = scenes.dambreak(**sc)
     return dim, cfg, pos, vel, typ
 
@@ -64,8 +86,8 @@ class GpuRun:
 
 
 class OracleRun:
-    def __init__(self, cfg, pos, vel, typ, dim, tol_mode=0, dot_order=0, n0=None, lam=None):
-        self.orc = Oracle(dim)
+    def __init__(self, cfg, pos, vel, typ, dim, tol_mode=0, dot_order=0, n0=None, lam=None, friction=False):
+        self.orc = Oracle(dim, friction)
         self.cfg = cfg
         self.scene = A.HostScene(A.types(dim), pos, vel, typ)
         if n0 is None:

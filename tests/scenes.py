@@ -34,6 +34,9 @@ CONFIGS = {
                         wl=3, dl=2, dt=5e-4),
     "dam3d_mid": dict(dim=3, domain=(0.6, 0.4, 0.2), water=(0.2, 0.3, 0.2), l0=0.01,
                       wl=3, dl=2, dt=5e-4),
+    # a small tank for the wall-friction fixture (7,300 particles, 320 fluid)
+    "fric3d_tiny": dict(dim=3, domain=(0.2, 0.15, 0.05), water=(0.08, 0.1, 0.05), l0=0.01,
+                        wl=3, dl=2, dt=5e-4),
 }
 
 

@@ -16,25 +16,29 @@ from oracle.oracle import Reference, make_config
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
+def declared_functions(serial=False):
+    """Function names include/mps_hip.h declares; those inside an
+    #ifdef MPS_FRICTION_FIELDS block only for the 3-D serial layout."""
     names = set()
     for h in ("mps_hip.h",):
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        if not serial:
+            text = re.sub(r"#ifdef MPS_FRICTION_FIELDS.*?#endif", "", text, flags=re.S)
         for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(\w+)\s*\(", text, flags=re.M):
             if m.group(1) not in ("if", "defined"):
                 names.add(m.group(1))
     return names
 
 
-@pytest.fixture(scope="module", params=[3, 2])
+@pytest.fixture(scope="module", params=[(3, False), (2, False), (3, True)], ids=["3d", "2d", "3ds"])
 def lib(request):
     import mps_amd
-    return mps_amd.load(request.param)
+    return mps_amd.load(*request.param)
 
 
 def test_every_declared_symbol_is_exported(lib):
-    names = declared_functions()
+    names = declared_functions(serial=lib.abi.SimConfig is not None and hasattr(lib.abi.SimConfig, "friction_enabled"))
     assert len(names) > 50
     missing = [n for n in sorted(names) if not hasattr(lib.L, n)]
     assert not missing, missing

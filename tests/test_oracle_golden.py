@@ -64,3 +64,16 @@ def test_oracle_reproduces_reference(name):
         st, ix = sim.neighbor_lists()
         assert list_digest(st, ix) == meta["digests"]["step%d_lists" % step], (name, step)
         assert np.array_equal(s.arr["pressure"], gold["step%d_pressure" % step])
+
+
+def test_friction_fixture():
+    """The oracle's hybrid wall friction (serial layout) reproduces the
+    reference's own apply_friction on the stored state, bit for bit
+    (golden/friction3ds.npz; 282 of its 320 fluid particles change)."""
+    import harness as H
+    sc, cfg, want, fl = H.friction_fixture(A.types(3, True))
+    before = np.array(sc.arr["vel"])[fl]
+    Oracle(3, True).apply_friction(sc, cfg)
+    got = np.array(sc.arr["vel"])[fl]
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+    assert np.any(got != before)

@@ -30,12 +30,12 @@ def _serial_openmp():
         pass
 
 
-def run_pair(scene, variant, steps, **over):
+def run_pair(scene, variant, steps, friction=False, final=None, **over):
     _serial_openmp()
     sc = scenes.CONFIGS[scene]
     dim = sc["dim"]
     p = scenes.params_for(scene, **over)
-    cfg_o = make_config(A.types(dim), **p)
+    cfg_o = make_config(A.types(dim, friction), **p)
     ref = Reference(variant)
     cfg_r = ref.config_from(cfg_o)
     ref.set_config(cfg_r)
@@ -43,7 +43,7 @@ def run_pair(scene, variant, steps, **over):
     sr = A.HostScene(A.types(dim), pos, vel, typ)
     so = A.HostScene(A.types(dim), pos, vel, typ)
     ref.L.particle_system_calc_initial_params(sr.ptr())
-    orc = Oracle(dim)
+    orc = Oracle(dim, friction)
     orc.init_params(so, cfg_o)
     for f in ("n0", "lambda_", "C_LL", "C_SL", "n0_count"):
         assert getattr(sr.ps, f) == getattr(so.ps, f), f
@@ -61,6 +61,8 @@ def run_pair(scene, variant, steps, **over):
     rl = A.padded_lists(nl.contents, sr.num)
     assert all(np.array_equal(rl[i], ix[st[i]:st[i + 1]]) for i in range(sr.num))
     ref.free_lists(nl, cl)
+    if final is not None:
+        final.update({f: np.array(so.arr[f]) for f in ("pos", "vel")})
     return sim.ppe(so.num)
 
 
@@ -89,6 +91,28 @@ def test_3d_parallel_variants(over):
 
 def test_3d_serial_variant():
     run_pair("shipped3d", "3d", 4, cg_tolerance=1e-6, cmps_gradient=1)
+
+
+FRICTION = dict(friction_enabled=1, friction_delta_theta=30.0, dynamic_friction_alpha=600.0,
+                wetting_angle_SL=60.0)
+
+
+@pytest.mark.parametrize("v_c", [0.01, 0.002])
+def test_3d_serial_friction(v_c):
+    """The hybrid wall friction (mps_fluid3D/src/operators.c:337-463) in the
+    oracle's serial layout against the reference's own 3-D serial build,
+    bit for bit over 12 steps of the shipped dam break, with the default
+    switch speed v_c = 0.01 m/s and with 0.002 m/s, which move particles
+    between the static and the dynamic branch.  The friction must change
+    the result."""
+    on, off = {}, {}
+    run_pair("shipped3d", "3d", 12, friction=True, final=on, cg_tolerance=1e-6,
+             friction_velocity_threshold=v_c, **FRICTION)
+    run_pair("shipped3d", "3d", 12, friction=True, final=off, cg_tolerance=1e-6,
+             friction_velocity_threshold=v_c, **dict(FRICTION, friction_enabled=0))
+    changed = np.any(on["vel"] != off["vel"], axis=1)
+    print("friction changed the velocity of %d particles" % changed.sum())
+    assert changed.sum() > 0
 
 
 @pytest.mark.parametrize("over", [dict(), dict(solver_type=1, hs_mode=1, influence_ratio_lap=4.0),
