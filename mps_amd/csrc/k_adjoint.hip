@@ -962,10 +962,11 @@ static int eq_csr_solve(AdjWork &w, const TapeStep &t, bool replay, const dvec *
         HIP_OK(hipMalloc(&tmp, sizeof(int) * (neq + 1)));
         rp_d = tmp;
     }
-    HIP_OK(hipMemcpyAsync(rp_d, rp.data(), sizeof(int) * (neq + 1), hipMemcpyHostToDevice, s));
     SolveResult sr{};
-    rc = csr_cg_device(neq, nnz, rp_d, w.col, w.val, w.rhs, w.sol, so, &sr, s);
-    HIP_OK(hipStreamSynchronize(s));
+    if (hipMemcpyAsync(rp_d, rp.data(), sizeof(int) * (neq + 1), hipMemcpyHostToDevice, s) != hipSuccess)
+        rc = MPS_ERR_HIP;
+    if (!rc) rc = csr_cg_device(neq, nnz, rp_d, w.col, w.val, w.rhs, w.sol, so, &sr, s);
+    if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = MPS_ERR_HIP;
     if (tmp) hipFree(tmp);
     return rc;
 }
@@ -1215,13 +1216,15 @@ extern "C" int mps_hip_replay_losses(ParticleSystem *ps, int nsteps, const doubl
         kq.visc_coeff = nus[q] * k.coeff;
         dvec gq = dzero_host();
         for (int d = 0; d < D_; d++) gq.c[d] = gs[(size_t)q * D_ + d];
-        HIP_OK(hipMemcpyAsync(r, tapes[0].r, sizeof(dvec) * n, hipMemcpyDeviceToDevice, s));
-        HIP_OK(hipMemcpyAsync(u, tapes[0].u, sizeof(dvec) * n, hipMemcpyDeviceToDevice, s));
-        if (u0s) {
-            HIP_OK(hipMemcpyAsync(u0d, u0s + (size_t)q * n * D_, sizeof(double) * (size_t)n * D_,
-                                  hipMemcpyHostToDevice, s));
-            hipLaunchKernelGGL(k_rep_set_u0, dim3(g), dim3(AB), 0, s, n, tapes[0].typ, u0d, u);
+        // errors end the loop and fall through to the frees below
+        if (hipMemcpyAsync(r, tapes[0].r, sizeof(dvec) * n, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+            hipMemcpyAsync(u, tapes[0].u, sizeof(dvec) * n, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+            (u0s && hipMemcpyAsync(u0d, u0s + (size_t)q * n * D_, sizeof(double) * (size_t)n * D_,
+                                   hipMemcpyHostToDevice, s) != hipSuccess)) {
+            rc = MPS_ERR_HIP;
+            break;
         }
+        if (u0s) hipLaunchKernelGGL(k_rep_set_u0, dim3(g), dim3(AB), 0, s, n, tapes[0].typ, u0d, u);
         for (int t = 0; t < nsteps && !rc; t++) {
             const TapeStep &T = tapes[t];
             const TapeStep &L = t == 0 ? ls0 : tapes[t - 1];
@@ -1229,17 +1232,23 @@ extern "C" int mps_hip_replay_losses(ParticleSystem *ps, int nsteps, const doubl
                                us);
             hipLaunchKernelGGL(k_rep_coll, dim3(g), dim3(AB), 0, s, n, T.typ, T.rs, T.us, rs, us, L.lst, L.lidx, kq, rss,
                                uss);
-            LAUNCH_CHECK();
+            if (hipGetLastError() != hipSuccess) {
+                rc = MPS_ERR_HIP;
+                break;
+            }
             rc = eq_csr_solve(w, T, true, rss, kq, so, s);
             if (rc) break;
             hipLaunchKernelGGL(k_adj_lambda, dim3(g), dim3(AB), 0, s, n, T.unk, w.eq, w.sol, xr);
             hipLaunchKernelGGL(k_rep_corr, dim3(g), dim3(AB), 0, s, n, T.typ, T.unk, T.x, xr, T.rss, rss, uss, T.lst,
                                T.lidx, kq, cf->clamp_negative_pressure, r, u);
-            LAUNCH_CHECK();
+            if (hipGetLastError() != hipSuccess) rc = MPS_ERR_HIP;
         }
         if (rc) break;
-        HIP_OK(hipMemcpyAsync(rh.data(), r, sizeof(dvec) * n, hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
+        if (hipMemcpyAsync(rh.data(), r, sizeof(dvec) * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+            rc = MPS_ERR_HIP;
+            break;
+        }
         double l = 0.0;
         int nf = 0;
         for (int i = 0; i < n; i++) {
