@@ -76,6 +76,19 @@ def coded_bytes(st):
     return 4 * (z - ne) + ne * (4 + 8 + 4 + 4)
 
 
+def step_bytes(st, n, iters, s=8):
+    """Algorithmic bytes of one whole timestep, B_particles + I * B_cg:
+    six neighbour passes of 4 B per directed pair (search, density,
+    viscosity, collision, assembly, p_min + gradient as two), the CSR and b
+    written once, sort scatter and state traffic of about (22 s + 40) B per
+    particle; per CG iteration the CSR once plus nine vector sweeps of s
+    bytes per row."""
+    e, z, ne = st.n_pairs, st.nnz, st.n_eq
+    b_part = 28 * e + z * (s + 4) + 4 * (ne + 1) + 8 * ne + (22 * s + 40) * n
+    b_cg = z * (s + 4) + 4 * (ne + 1) + 9 * s * ne
+    return b_part + iters * b_cg
+
+
 def spmv_bytes(st, s_val=8, s_vec=8):
     """Algorithmic bytes of one SpMV y = A x over the reference's CSR:
     Z (value + 4-byte column) + 4 (n_eq + 1) row pointers + x read once and
@@ -687,6 +700,13 @@ def main():
         if lib.mps_hip_profile_get(name.encode(), C.byref(ms), C.byref(cnt)) == 0 and cnt.value:
             kernels[name] = dict(total_ms=round(ms.value, 3), launches=cnt.value)
     lib.mps_hip_release(ps)
+    step_roof = None
+    if a.precision == "fp64" and a.solver == "cg":
+        sb = step_bytes(st, n, iters)
+        sach = sb / (el / a.steps) / 1e9
+        step_roof = dict(bound="hbm", bytes_per_step=int(sb), iters_per_step=round(iters, 1),
+                         achieved=round(sach, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(sach / HBM_PEAK_GBS, 4),
+                         particle_steps_per_s_at_peak=round(n / (sb / (HBM_PEAK_GBS * 1e9)), 1))
     mixed = None
     if a.precision == "fp64" and a.solver == "cg" and a.mixed_check != "none":
         # the metric's other half: the mixed form on the same scene and steps
@@ -723,6 +743,7 @@ def main():
         "autotune": tune,
         "mixed": mixed,
         "roofline": roof,
+        "step_roofline": step_roof,
         "cpu_baseline": cpu,
         "kernels_ms_one_step": kernels,
     }
