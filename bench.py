@@ -298,12 +298,13 @@ def run_ensemble(lib, a):
         "metric": "particle-steps/s, ensemble of independent 3D dam breaks",
         "value": round(total * a.steps / el, 1), "unit": "particle-steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(1e3 * el / a.steps, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if a.precision == "fp64" else "f32+f64",
         "data": "synthetic (reference dam-break lattice generator, seeded column sizes and viscosities)",
         "config": {"workload": a.workload, "scenes": len(scenes), "particles": total, "l0": ws[0]["l0"],
                    "dt": ws[0]["dt"], "seed": params.ENSEMBLE_SEED,
                    "tolerance": "relative %g" % a.tol, "mean_iters_per_step": round(sum(iters) / len(iters), 1),
                    "max_iters_per_step": round(max(iters), 1), "parallelism": "1 GPU, batched CG",
+                   "precision": a.precision,
                    "graph": a.graph, "spmv_variant": int(lib.L.mps_hip_get_option(b"spmv")),
                    "check_every": int(lib.L.mps_hip_get_option(b"check_every"))},
         "autotune": tune, "sequential": seq, "cpu_baseline": cpu}), flush=True)
@@ -629,6 +630,7 @@ def main():
         lib.set_option("exit_on_error", 0)
         lib.set_option("spmv", a.spmv)
         lib.set_option("graph", a.graph)
+        lib.set_option("precision", PRECISIONS[a.precision])
         run_ensemble(lib, a)
         return
     if a.graph is None:

@@ -221,6 +221,7 @@ struct SolveResult {
 int cg_alloc_ctrl(Ctx *c);
 // FP64 CG over the packed systems of several contexts at once (k_cg.hip)
 int batch_cg(Ctx *const *cs, int ns, const SolveOpts *opts, SolveResult *res, hipStream_t s);
+int batch_cg_mixed(Ctx *const *cs, int ns, const SolveOpts *opts, SolveResult *res, hipStream_t s);
 int pack_ppe(Ctx *c, int with_f32);        // k_pack.hip
 int unpack_solution(Ctx *c);               // packed x -> c->x (slot order)
 void packed_free(Packed *p);

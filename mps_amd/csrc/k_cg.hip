@@ -804,11 +804,10 @@ struct QConsts {
     float neg_coeff, re, r_lo, dr;
 };
 
-__global__ void __launch_bounds__(CG_BLK)
-k_mx_q16_build(int n_eq, PkView P, double neg_coeff, double re, double r_min, unsigned *__restrict__ cq)
+// the coded words of row k (halo-mode slices only)
+__device__ __forceinline__ void q16_build_row(int k, const PkView &P, double neg_coeff, double re, double r_min,
+                                              unsigned *__restrict__ cq)
 {
-    const int k = blockIdx.x * CG_BLK + threadIdx.x;
-    if (k >= n_eq) return;
     const int4 mt = P.smeta[k >> 6];
     if (!mt.z) return;
     const int lane = k & 63;
@@ -828,20 +827,19 @@ k_mx_q16_build(int n_eq, PkView P, double neg_coeff, double re, double r_min, un
     }
 }
 
-template <int U>
 __global__ void __launch_bounds__(CG_BLK)
-k_mx_spmv_q(int n, PkView P, const unsigned *__restrict__ cq, QConsts qk, const float *__restrict__ p,
-            float *__restrict__ Ap, double *part, const MixedCtrl *c)
+k_mx_q16_build(int n_eq, PkView P, double neg_coeff, double re, double r_min, unsigned *__restrict__ cq)
 {
-    extern __shared__ unsigned char smem_raw[];
-    float *hx = reinterpret_cast<float *>(smem_raw);
-    if (c->done) return;
-    stage_halo(P, XPlain<float>{p}, hx);
-    const int i = blockIdx.x * CG_BLK + threadIdx.x;
-    double v = 0.0;
-    if (i < n) {
-        const float pi = p[i];
-        float s = (float)P.ddir[i] * pi;
+    const int k = blockIdx.x * CG_BLK + threadIdx.x;
+    if (k < n_eq) q16_build_row(k, P, neg_coeff, re, r_min, cq);
+}
+
+// (A' p)_i of the coded operator, the halo of p staged in hx
+template <int U>
+__device__ __forceinline__ float coded_row(const PkView &P, const unsigned *__restrict__ cq, const QConsts &qk,
+                                           const float *__restrict__ p, const float *hx, int i, float pi)
+{
+    float s = (float)P.ddir[i] * pi;
         const int4 mt = P.smeta[i >> 6];
         const int lane = i & 63;
         const int m = P.len[i];
@@ -865,6 +863,23 @@ k_mx_spmv_q(int n, PkView P, const unsigned *__restrict__ cq, QConsts qk, const 
             s = row_sum_global<U, true, float>(m, P.fval + (size_t)mt.x * 64u + lane,
                                                P.c32 + (size_t)mt.y * 64u + lane, XPlain<float>{p}, pi, i, s);
         }
+    return s;
+}
+
+template <int U>
+__global__ void __launch_bounds__(CG_BLK)
+k_mx_spmv_q(int n, PkView P, const unsigned *__restrict__ cq, QConsts qk, const float *__restrict__ p,
+            float *__restrict__ Ap, double *part, const MixedCtrl *c)
+{
+    extern __shared__ unsigned char smem_raw[];
+    float *hx = reinterpret_cast<float *>(smem_raw);
+    if (c->done) return;
+    stage_halo(P, XPlain<float>{p}, hx);
+    const int i = blockIdx.x * CG_BLK + threadIdx.x;
+    double v = 0.0;
+    if (i < n) {
+        const float pi = p[i];
+        const float s = coded_row<U>(P, cq, qk, p, hx, i, pi);
         Ap[i] = s;
         v = (double)pi * (double)s;
     }
@@ -1001,16 +1016,7 @@ static int cg_mixed_run(Ctx *c, const SolveOpts &o, SolveResult *res, V *r, V *p
     const bool coded = sizeof(V) == sizeof(float) && o.precision == 3;
     QConsts qk{};
     if (coded) {
-        Packed *pw = c->pk;
-        const size_t need = pw->tot_val * 64;
-        if (pw->cap_cq < need) {
-            if (pw->cq) hipFree(pw->cq);
-            pw->cq = nullptr;
-            pw->cap_cq = 0;
-            HIP_OK(hipMalloc(&pw->cq, sizeof(unsigned) * (need + need / 8 + 64)));
-            pw->cap_cq = need + need / 8 + 64;
-        }
-        hipLaunchKernelGGL(k_mx_q16_build, dim3(g), dim3(CG_BLK), 0, s, n, P, o.neg_coeff, o.re_lap, o.r_min, pw->cq);
+        hipLaunchKernelGGL(k_mx_q16_build, dim3(g), dim3(CG_BLK), 0, s, n, P, o.neg_coeff, o.re_lap, o.r_min, c->pk->cq);
         LAUNCH_CHECK();
         qk.neg_coeff = (float)o.neg_coeff;
         qk.re = (float)o.re_lap;
@@ -1051,12 +1057,13 @@ static int cg_mixed_run(Ctx *c, const SolveOpts &o, SolveResult *res, V *r, V *p
     return 0;
 }
 
-static int cg_mixed(Ctx *c, const SolveOpts &o, SolveResult *res)
+// the FP32 vectors (precisions 2 and 3) and the control block of c's mixed solve
+static int mixed_reserve(Ctx *c, int precision)
 {
     if (!c->mixed) c->mixed = new MixedBufs();
     MixedBufs *mb = c->mixed;
     const size_t NS = ((size_t)(c->cap > 0 ? c->cap : 1) + 63) / 64 * 64;
-    if (o.precision >= 2 && !c->fr) {
+    if (precision >= 2 && !c->fr) {
         HIP_OK(hipMalloc(&c->fr, sizeof(float) * NS));
         HIP_OK(hipMalloc(&c->fp, sizeof(float) * NS));
         HIP_OK(hipMalloc(&c->fAp, sizeof(float) * NS));
@@ -1066,6 +1073,29 @@ static int cg_mixed(Ctx *c, const SolveOpts &o, SolveResult *res)
         HIP_OK(hipHostMalloc(&mb->h, sizeof(MixedCtrl), hipHostMallocDefault));
         HIP_OK(hipMalloc(&mb->part_ru, sizeof(double) * 65536));
     }
+    return 0;
+}
+
+// room for the coded words of c's packed system (one per stored entry)
+static int coded_reserve(Ctx *c)
+{
+    Packed *pw = c->pk;
+    const size_t need = pw->tot_val * 64;
+    if (pw->cap_cq < need) {
+        if (pw->cq) hipFree(pw->cq);
+        pw->cq = nullptr;
+        pw->cap_cq = 0;
+        HIP_OK(hipMalloc(&pw->cq, sizeof(unsigned) * (need + need / 8 + 64)));
+        pw->cap_cq = need + need / 8 + 64;
+    }
+    return 0;
+}
+
+static int cg_mixed(Ctx *c, const SolveOpts &o, SolveResult *res)
+{
+    int rc = mixed_reserve(c, o.precision);
+    if (!rc && o.precision == 3) rc = coded_reserve(c);
+    if (rc) return rc;
     if (o.precision >= 2) return cg_mixed_run<float>(c, o, res, c->fr, c->fp, c->fAp);
     return cg_mixed_run<double>(c, o, res, c->r, c->pv, c->Ap);
 }
@@ -1171,7 +1201,7 @@ k_b_init_fin(const BScene *__restrict__ sc, const double *part2, CgCtrl *ctrl, i
         c->iters = 0;
         c->breakdown = 0;
         c->res = sqrt(bb);
-        if (S.max_iter <= 0 || sqrt(bb) < c->stop) {
+        if (S.n_eq == 0 || S.max_iter <= 0 || sqrt(bb) < c->stop) {
             c->done = 1;
             atomicAdd(ndone, 1);
         }
@@ -1419,4 +1449,403 @@ int batch_cg(Ctx *const *cs, int ns, const SolveOpts *opts, SolveResult *res, hi
         res[q].outer = 1;
     }
     return 0;
+}
+
+// ---------------------------------------------------------------------
+// Batched mixed CG (precisions 1-3): the mixed iteration above over many
+// scenes, with the block tables of the batched FP64 CG.  Every scene keeps
+// the block partition of its own single-scene mixed solve (SpMV and
+// p-update: one block per 256 rows; init and axpy: vec_grid(n_eq) blocks,
+// grid-stride), the finalisers sum the same partials in the same order,
+// and the reliable updates fire on the same iterations, so each scene ends
+// with the bits of its single-scene solve.
+// ---------------------------------------------------------------------
+
+template <typename V>
+struct BMScene {
+    PkView P;
+    const double *b;
+    double *x;
+    V *r, *p, *Ap;
+    unsigned *cq;
+    QConsts qk;
+    int n_eq;
+    int blk_off, nblk;       // SpMV / p-update blocks
+    int ax_off, nax;         // init / axpy blocks
+    int max_iter, tol_mode;
+    double tol, delta, neg_coeff, re, r_min;
+};
+
+__global__ void __launch_bounds__(CG_BLK)
+k_bm_q16_build(const BMScene<float> *__restrict__ sc, const int *__restrict__ bs)
+{
+    const BMScene<float> &S = sc[bs[blockIdx.x]];
+    const int k = (blockIdx.x - S.blk_off) * CG_BLK + threadIdx.x;
+    if (k < S.n_eq) q16_build_row(k, S.P, S.neg_coeff, S.re, S.r_min, S.cq);
+}
+
+template <typename V>
+__global__ void __launch_bounds__(CG_BLK)
+k_bm_init(const BMScene<V> *__restrict__ sc, const int *__restrict__ axs, double *part2, double *part3)
+{
+    const BMScene<V> &S = sc[axs[blockIdx.x]];
+    const int j = blockIdx.x - S.ax_off;
+    double v = 0.0, w = 0.0;
+    for (int i = j * CG_BLK + threadIdx.x; i < S.n_eq; i += S.nax * CG_BLK) {
+        const double bi = S.b[i];
+        S.x[i] = 0.0;
+        const V f = (V)bi;
+        S.r[i] = f;
+        S.p[i] = f;
+        v += (double)f * (double)f;
+        w += bi * bi;
+    }
+    v = block_sum<double, CG_BLK>(v);
+    w = block_sum<double, CG_BLK>(w);
+    if (threadIdx.x == 0) {
+        part2[blockIdx.x] = v;
+        part3[blockIdx.x] = w;
+    }
+}
+
+template <typename V>
+__global__ void __launch_bounds__(CG_BLK)
+k_bm_init_fin(const BMScene<V> *__restrict__ sc, const double *part2, const double *part3, MixedCtrl *ctrl,
+              int *ndone)
+{
+    const BMScene<V> &S = sc[blockIdx.x];
+    MixedCtrl *c = ctrl + blockIdx.x;
+    const double rr = fin_sum(part2 + S.ax_off, S.nax);
+    const double bb = fin_sum(part3 + S.ax_off, S.nax);
+    if (threadIdx.x == 0) {
+        c->rr = rr;
+        c->rr_ru = rr;
+        c->bnorm = sqrt(bb);
+        c->stop = S.tol_mode ? S.tol * sqrt(bb) : S.tol;
+        c->delta2 = S.delta * S.delta;
+        c->done = 0;
+        c->iters = 0;
+        c->breakdown = 0;
+        c->ru = 0;
+        c->n_ru = 0;
+        c->res = sqrt(bb);
+        // an empty system is solved (the single-scene solve skips it)
+        if (S.n_eq == 0 || S.max_iter <= 0 || sqrt(bb) < c->stop) {
+            c->done = 1;
+            atomicAdd(ndone, 1);
+        }
+    }
+}
+
+template <int U, typename V, bool CODED>
+__global__ void __launch_bounds__(CG_BLK)
+k_bm_spmv(const BMScene<V> *__restrict__ sc, const int *__restrict__ bs, double *part, const MixedCtrl *ctrl)
+{
+    extern __shared__ unsigned char smem_raw[];
+    V *hx = reinterpret_cast<V *>(smem_raw);
+    const int si = bs[blockIdx.x];
+    if (ctrl[si].done) return;
+    const BMScene<V> &S = sc[si];
+    const int lb = blockIdx.x - S.blk_off;
+    const V *p = S.p;
+    stage_halo_of(S.P, lb, XPlain<V>{p}, hx);
+    const int i = lb * CG_BLK + threadIdx.x;
+    double v = 0.0;
+    if (i < S.n_eq) {
+        const V pi = p[i];
+        V s;
+        if constexpr (CODED)
+            s = coded_row<U>(S.P, S.cq, S.qk, p, hx, i, pi);
+        else
+            s = packed_row<U, true, V>(S.P, S.P.fval, i, XPlain<V>{p}, hx, pi, (V)S.P.ddir[i] * pi);
+        S.Ap[i] = s;
+        v = (double)pi * (double)s;
+    }
+    v = block_sum<double, CG_BLK>(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+template <typename V>
+__global__ void __launch_bounds__(CG_BLK)
+k_bm_fin1(const BMScene<V> *__restrict__ sc, const double *part, const double *part_ru, MixedCtrl *ctrl, int t,
+          int *ndone)
+{
+    MixedCtrl *c = ctrl + blockIdx.x;
+    if (c->done) return;
+    const BMScene<V> &S = sc[blockIdx.x];
+    const double pap = fin_sum(part + S.blk_off, S.nblk);
+    const int ru = c->ru;
+    double rr_true = 0.0;
+    if (ru) rr_true = fin_sum(part_ru + S.blk_off, S.nblk);
+    if (threadIdx.x == 0) {
+        if (ru) {
+            c->ru = 0;
+            c->n_ru++;
+            c->rr = rr_true;
+            c->rr_ru = rr_true;
+            c->res = sqrt(rr_true);
+            if (sqrt(rr_true) < c->stop) {
+                c->done = 1;
+                c->iters = t;
+                atomicAdd(ndone, 1);
+                return;
+            }
+        }
+        c->pap = pap;
+        if (fabs(pap) < 1.0e-30) {
+            c->done = 1;
+            c->iters = t;
+            c->breakdown = 1;
+            atomicAdd(ndone, 1);
+        } else {
+            c->alpha = c->rr / pap;
+        }
+    }
+}
+
+template <typename V>
+__global__ void __launch_bounds__(CG_BLK)
+k_bm_axpy(const BMScene<V> *__restrict__ sc, const int *__restrict__ axs, double *part2, const MixedCtrl *ctrl)
+{
+    const int si = axs[blockIdx.x];
+    if (ctrl[si].done) return;
+    const BMScene<V> &S = sc[si];
+    const int j = blockIdx.x - S.ax_off;
+    const double alpha = ctrl[si].alpha;
+    const V av = (V)alpha;
+    double v = 0.0;
+    for (int i = j * CG_BLK + threadIdx.x; i < S.n_eq; i += S.nax * CG_BLK) {
+        S.x[i] += alpha * (double)S.p[i];
+        const V ri = S.r[i] - av * S.Ap[i];
+        S.r[i] = ri;
+        v += (double)ri * (double)ri;
+    }
+    v = block_sum<double, CG_BLK>(v);
+    if (threadIdx.x == 0) part2[blockIdx.x] = v;
+}
+
+template <typename V>
+__global__ void __launch_bounds__(CG_BLK)
+k_bm_fin2(const BMScene<V> *__restrict__ sc, const double *part2, MixedCtrl *ctrl, int t, int *ndone)
+{
+    MixedCtrl *c = ctrl + blockIdx.x;
+    if (c->done) return;
+    const BMScene<V> &S = sc[blockIdx.x];
+    const double rr_new = fin_sum(part2 + S.ax_off, S.nax);
+    if (threadIdx.x == 0) {
+        const double rr = c->rr;
+        if (rr < 1.0e-60 || t + 1 >= S.max_iter) {
+            c->done = 1;
+            c->iters = t + 1;
+            c->res = sqrt(rr_new);
+            atomicAdd(ndone, 1);
+            return;
+        }
+        c->beta = rr_new / rr;
+        c->rr = rr_new;
+        c->ru = (rr_new < c->delta2 * c->rr_ru) || (sqrt(rr_new) < c->stop);
+    }
+}
+
+template <int U, typename V>
+__global__ void __launch_bounds__(CG_BLK)
+k_bm_pupdate(const BMScene<V> *__restrict__ sc, const int *__restrict__ bs, double *part_ru, const MixedCtrl *ctrl)
+{
+    const int si = bs[blockIdx.x];
+    if (ctrl[si].done) return;
+    const BMScene<V> &S = sc[si];
+    const V beta = (V)ctrl[si].beta;
+    const int ru = ctrl[si].ru;
+    const int i = (blockIdx.x - S.blk_off) * CG_BLK + threadIdx.x;
+    double v = 0.0;
+    if (i < S.n_eq) {
+        V ri = S.r[i];
+        if (ru) {
+            const double xi = S.x[i];
+            const double s = packed_row_global<U>(S.P, S.P.val, i, S.x, S.P.diag[i] * xi);
+            ri = (V)(S.b[i] - s);
+            v = (double)ri * (double)ri;
+            S.r[i] = ri;
+        }
+        S.p[i] = ri + beta * S.p[i];
+    }
+    if (ru) {
+        v = block_sum<double, CG_BLK>(v);
+        if (threadIdx.x == 0) part_ru[blockIdx.x] = v;
+    }
+}
+
+struct BatchMixedBufs {
+    int cap_sc = 0, cap_ctrl = 0, cap_part3 = 0, cap_ru = 0;
+    unsigned char *sc = nullptr, *hsc = nullptr;
+    MixedCtrl *ctrl = nullptr, *hctrl = nullptr;
+    double *part3 = nullptr, *part_ru = nullptr;
+};
+static BatchMixedBufs g_bmixed;
+
+static int reserve_dev(double **d, int *cap, int need)
+{
+    if (*cap >= need) return 0;
+    if (*d) hipFree(*d);
+    *d = nullptr;
+    const int n = need + need / 4 + 16;
+    HIP_OK(hipMalloc((void **)d, sizeof(double) * n));
+    *cap = n;
+    return 0;
+}
+
+template <typename V>
+static int batch_mixed_run(Ctx *const *cs, int ns, const SolveOpts *opts, SolveResult *res, hipStream_t s)
+{
+    BatchBufs &B = g_batch;
+    BatchMixedBufs &M = g_bmixed;
+    const bool coded = sizeof(V) == sizeof(float) && opts[0].precision == 3;
+    int tot_blk = 0, tot_ax = 0, hmax = 0, max_iter = 0, check_every = 16;
+    for (int k = 0; k < ns; k++) {
+        const int n = cs[k]->pk->n_eq;
+        tot_blk += (int)cg_grid(n);
+        tot_ax += n > 0 ? (int)vec_grid(n) : 0;
+        if (cs[k]->pk->hmax > hmax) hmax = cs[k]->pk->hmax;
+        if (opts[k].max_iter > max_iter) max_iter = opts[k].max_iter;
+        if (opts[k].check_every > 0) check_every = opts[k].check_every;
+    }
+    int rc = reserve_pair(&M.sc, &M.hsc, &M.cap_sc, (int)(sizeof(BMScene<V>) * ns));
+    if (!rc) rc = reserve_pair(&B.bs, &B.hbs, &B.cap_blk, tot_blk + 1);
+    if (!rc) rc = reserve_pair(&B.axs, &B.haxs, &B.cap_ax, tot_ax + 1);
+    if (!rc) rc = reserve_pair(&M.ctrl, &M.hctrl, &M.cap_ctrl, ns);
+    if (!rc) rc = reserve_dev(&B.part, &B.cap_part, tot_blk + 1);
+    if (!rc) rc = reserve_dev(&M.part_ru, &M.cap_ru, tot_blk + 1);
+    if (!rc) rc = reserve_dev(&B.part2, &B.cap_part2, tot_ax + 1);
+    if (!rc) rc = reserve_dev(&M.part3, &M.cap_part3, tot_ax + 1);
+    if (rc) return rc;
+    if (!B.ndone) {
+        HIP_OK(hipMalloc(&B.ndone, sizeof(int)));
+        HIP_OK(hipHostMalloc(&B.hndone, sizeof(int), hipHostMallocDefault));
+    }
+    BMScene<V> *hsc = reinterpret_cast<BMScene<V> *>(M.hsc);
+    const BMScene<V> *dsc = reinterpret_cast<const BMScene<V> *>(M.sc);
+    int bo = 0, ao = 0;
+    for (int k = 0; k < ns; k++) {
+        Ctx *c = cs[k];
+        const Packed *pk = c->pk;
+        const SolveOpts &o = opts[k];
+        BMScene<V> &S = hsc[k];
+        S.P = pk_view(pk);
+        S.b = pk->b;
+        S.x = pk->x;
+        if constexpr (sizeof(V) == sizeof(float)) {
+            S.r = c->fr;
+            S.p = c->fp;
+            S.Ap = c->fAp;
+        } else {
+            S.r = c->r;
+            S.p = c->pv;
+            S.Ap = c->Ap;
+        }
+        S.cq = pk->cq;
+        S.qk.neg_coeff = (float)o.neg_coeff;
+        S.qk.re = (float)o.re_lap;
+        S.qk.r_lo = (float)o.r_min;
+        S.qk.dr = (float)((o.re_lap - o.r_min) / Q_CODE_MAX);
+        S.n_eq = pk->n_eq;
+        S.blk_off = bo;
+        S.nblk = (int)cg_grid(pk->n_eq);
+        S.ax_off = ao;
+        S.nax = pk->n_eq > 0 ? (int)vec_grid(pk->n_eq) : 0;
+        S.max_iter = o.max_iter;
+        S.tol_mode = o.tol_mode;
+        S.tol = o.tol;
+        S.delta = o.inner_tol > 0 ? o.inner_tol : 1e-3;
+        S.neg_coeff = o.neg_coeff;
+        S.re = o.re_lap;
+        S.r_min = o.r_min;
+        for (int q = 0; q < S.nblk; q++) B.hbs[bo + q] = k;
+        for (int q = 0; q < S.nax; q++) B.haxs[ao + q] = k;
+        bo += S.nblk;
+        ao += S.nax;
+    }
+    HIP_OK(hipMemcpyAsync(M.sc, M.hsc, sizeof(BMScene<V>) * ns, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(B.bs, B.hbs, sizeof(int) * (tot_blk + 1), hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(B.axs, B.haxs, sizeof(int) * (tot_ax + 1), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_fill_zero_int, dim3(1), dim3(64), 0, s, B.ndone, 1);
+    if (coded && tot_blk > 0)
+        hipLaunchKernelGGL(k_bm_q16_build, dim3(tot_blk), dim3(CG_BLK), 0, s,
+                           reinterpret_cast<const BMScene<float> *>(dsc), B.bs);
+    if (tot_ax > 0) hipLaunchKernelGGL(k_bm_init<V>, dim3(tot_ax), dim3(CG_BLK), 0, s, dsc, B.axs, B.part2, M.part3);
+    hipLaunchKernelGGL(k_bm_init_fin<V>, dim3(ns), dim3(CG_BLK), 0, s, dsc, B.part2, M.part3, M.ctrl, B.ndone);
+    LAUNCH_CHECK();
+    typedef void (*bm_fn)(const BMScene<V> *, const int *, double *, const MixedCtrl *);
+    const bool u16 = opts[0].spmv_variant == 1;
+    bm_fn spmv = u16 ? k_bm_spmv<16, V, false> : k_bm_spmv<8, V, false>;
+    if constexpr (sizeof(V) == sizeof(float))
+        if (coded) spmv = u16 ? k_bm_spmv<16, V, true> : k_bm_spmv<8, V, true>;
+    const size_t lds = sizeof(V) * (size_t)hmax;
+    // the host stops when every scene has stopped: watch ndone == ns
+    hipEvent_t ev[2];
+    HIP_OK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    HIP_OK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    int pending = -1, t = 0, k = 0;
+    hipError_t e = hipSuccess;
+    while (t < max_iter && e == hipSuccess) {
+        const int t_end = (t + check_every < max_iter) ? t + check_every : max_iter;
+        for (; t < t_end; t++) {
+            prof_begin(PK_SPMV, s);
+            if (tot_blk > 0) hipLaunchKernelGGL(spmv, dim3(tot_blk), dim3(CG_BLK), lds, s, dsc, B.bs, B.part, M.ctrl);
+            prof_end(PK_SPMV, s);
+            hipLaunchKernelGGL(k_bm_fin1<V>, dim3(ns), dim3(CG_BLK), 0, s, dsc, B.part, M.part_ru, M.ctrl, t, B.ndone);
+            prof_begin(PK_AXPY, s);
+            if (tot_ax > 0) hipLaunchKernelGGL(k_bm_axpy<V>, dim3(tot_ax), dim3(CG_BLK), 0, s, dsc, B.axs, B.part2, M.ctrl);
+            prof_end(PK_AXPY, s);
+            hipLaunchKernelGGL(k_bm_fin2<V>, dim3(ns), dim3(CG_BLK), 0, s, dsc, B.part2, M.ctrl, t, B.ndone);
+            prof_begin(PK_PUPDATE, s);
+            if (tot_blk > 0)
+                hipLaunchKernelGGL((k_bm_pupdate<8, V>), dim3(tot_blk), dim3(CG_BLK), 0, s, dsc, B.bs, M.part_ru, M.ctrl);
+            prof_end(PK_PUPDATE, s);
+        }
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(B.hndone, B.ndone, sizeof(int), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipEventRecord(ev[k & 1], s);
+        if (e == hipSuccess && pending >= 0) {
+            e = hipEventSynchronize(ev[pending & 1]);
+            if (e == hipSuccess && *(volatile int *)B.hndone >= ns) break;
+        }
+        pending = k;
+        k++;
+    }
+    hipEventDestroy(ev[0]);
+    hipEventDestroy(ev[1]);
+    if (e != hipSuccess) {
+        mps_set_error(MPS_ERR_HIP, __FILE__, __LINE__, hipGetErrorString(e));
+        return MPS_ERR_HIP;
+    }
+    HIP_OK(hipStreamSynchronize(s));
+    HIP_OK(hipMemcpy(M.hctrl, M.ctrl, sizeof(MixedCtrl) * ns, hipMemcpyDeviceToHost));
+    for (int q = 0; q < ns; q++) {
+        const MixedCtrl &c = M.hctrl[q];
+        res[q].iters = c.done ? c.iters : opts[q].max_iter;
+        res[q].breakdown = c.breakdown;
+        res[q].res = c.res;
+        res[q].bnorm = c.bnorm;
+        res[q].outer = c.n_ru;
+    }
+    return 0;
+}
+
+// Mixed-precision counterpart of batch_cg (opts[*].precision 1-3, all
+// alike): each scene's system packed by pack_ppe(c, 1); solutions stay in
+// each scene's packed x.
+int batch_cg_mixed(Ctx *const *cs, int ns, const SolveOpts *opts, SolveResult *res, hipStream_t s)
+{
+    const int prec = opts[0].precision;
+    for (int k = 0; k < ns; k++) {
+        if (opts[k].precision != prec || prec < 1 || prec > 3) {
+            mps_set_error(MPS_ERR_ARG, __FILE__, __LINE__, "batched mixed solve: one precision (1-3) for all scenes");
+            return MPS_ERR_ARG;
+        }
+        int rc = mixed_reserve(cs[k], prec);
+        if (!rc && prec == 3) rc = coded_reserve(cs[k]);
+        if (rc) return rc;
+    }
+    if (prec >= 2) return batch_mixed_run<float>(cs, ns, opts, res, s);
+    return batch_mixed_run<double>(cs, ns, opts, res, s);
 }

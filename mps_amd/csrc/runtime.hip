@@ -858,8 +858,8 @@ int mps_hip_run_batch(ParticleSystem *const *ps, SimConfig *const *cfgs, int nsc
         rc = ensure_resident(cs[q], ps[q], cfgs[q]);
         if (rc) return rc;
     }
-    // the batched solve is the FP64 CG; other solver settings run per scene
-    bool batched = g_opt.precision < 0.5;
+    // CG (FP64 or mixed) is solved batched; ICCG runs per scene
+    bool batched = true;
     for (int q = 0; q < nscenes; q++) batched = batched && cfgs[q]->solver_type == 0;
     std::vector<StepConsts> ks(nscenes);
     std::vector<SolveOpts> so(nscenes);
@@ -874,13 +874,16 @@ int mps_hip_run_batch(ParticleSystem *const *ps, SimConfig *const *cfgs, int nsc
         }
         if (rc) break;
         if (batched) {
+            const int mixed = so[0].precision >= 1 && so[0].precision <= 3;
             for (int q = 0; q < nscenes && !rc; q++) {
                 rc = cg_alloc_ctrl(cs[q]);
-                if (!rc) rc = pack_ppe(cs[q], 0);
+                if (!rc) rc = pack_ppe(cs[q], mixed);
             }
             for (int q = 0; q < nscenes && !rc; q++)
                 if (hipStreamSynchronize(cs[q]->stream) != hipSuccess) rc = MPS_ERR_HIP;
-            if (!rc) rc = batch_cg(cs.data(), nscenes, so.data(), sr.data(), bstream);
+            if (!rc)
+                rc = mixed ? batch_cg_mixed(cs.data(), nscenes, so.data(), sr.data(), bstream)
+                           : batch_cg(cs.data(), nscenes, so.data(), sr.data(), bstream);
             for (int q = 0; q < nscenes && !rc; q++) rc = unpack_solution(cs[q]);
         } else {
             for (int q = 0; q < nscenes && !rc; q++) {

@@ -342,10 +342,20 @@ def test_autotune_keeps_bits(lib3):
         assert a == b, f
 
 
-def test_batch_equals_single_runs(lib3):
-    """mps_hip_run_batch (particle operators per scene, one batched FP64 CG
-    for all scenes) leaves every scene with the bits of its own single-scene
+@pytest.mark.parametrize("precision", [0, 1, 2, 3])
+def test_batch_equals_single_runs(lib3, precision):
+    """mps_hip_run_batch (particle operators per scene, one batched CG for
+    all scenes: FP64, or the mixed forms of precisions 1-3) leaves every
+    scene with the bits and CG iteration counts of its own single-scene
     run: scenes of different sizes and viscosities."""
+    lib3.set_option("precision", precision)
+    try:
+        _batch_equals_single(lib3)
+    finally:
+        lib3.set_option("precision", 0)
+
+
+def _batch_equals_single(lib3):
     import ctypes as C
     from mps_amd import params
     scenes_w = [dict(dims=(0.4, 0.3, 0.12, 0.12, 0.2, 0.12), l0=0.01, dt=5e-4, nu=1e-6),
@@ -375,7 +385,7 @@ def test_batch_equals_single_runs(lib3):
         say("single scene", n, "nu", w["nu"])
         lib3.check(lib3.mps_hip_run_steps(ps, steps), "run_steps")
         lib3.check(lib3.mps_hip_download(ps, None), "download")
-        single.append(fields(ps, n))
+        single.append((fields(ps, n), lib3.stats(ps).total_iters))
         lib3.mps_hip_release(ps)
         lib3.particle_system_free(ps)
     made = [make(w) for w in scenes_w]
@@ -389,7 +399,8 @@ def test_batch_equals_single_runs(lib3):
         lib3.check(lib3.mps_hip_download(ps, None), "download")
         got = fields(ps, n)
         for f in H.FIELDS:
-            assert got[f] == single[q][f], (q, f)
+            assert got[f] == single[q][0][f], (q, f)
         assert lib3.stats(ps).steps == steps
+        assert lib3.stats(ps).total_iters == single[q][1]
         lib3.mps_hip_release(ps)
         lib3.particle_system_free(ps)
