@@ -184,10 +184,10 @@ int  mps_hip_run_steps(ParticleSystem *ps, int nsteps);
  * runs one scene per process): nsteps timesteps of every ps[i] under its
  * own configuration cfgs[i] (viscosity, geometry, ... may differ; same
  * dimensionality), state resident on the GPU.  The particle operators run
- * per scene; the FP64 CG runs batched -- one launch per CG phase for all
- * scenes, per-scene reductions and stop tests -- and gives every scene
- * the bits of its own single-scene solve.  Scenes that use ICCG or a mixed
- * precision are solved one by one.  ps[i] stays resident: read it back with
+ * per scene; the CG (FP64, or the mixed form of option "precision") runs
+ * batched -- one launch per CG phase for all scenes, per-scene reductions
+ * and stop tests -- and gives every scene the bits of its own single-scene
+ * solve.  Scenes that use ICCG are solved one by one.  ps[i] stays resident: read it back with
  * mps_hip_download.  g_config is left as it was. */
 int  mps_hip_run_batch(ParticleSystem *const *ps, SimConfig *const *cfgs, int nscenes, int nsteps);
 
