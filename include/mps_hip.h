@@ -251,8 +251,10 @@ int  mps_hip_step_backward(ParticleSystem *ps, NeighborList *nl, const double *r
  *   "inner_tol"      residual reduction between reliable updates (1e-3)
  *   "exit_on_error"  1 (reference behaviour) or 0
  *   "check_every"    CG iterations between host convergence polls (16)
- *   "spmv"           entries per load batch in the CG's SpMV:
- *                      0 eight, 1 sixteen, 2 four
+ *   "spmv"           SpMV kernel of the CG (same bits in all): 0 eight
+ *                      entries per load batch, 1 sixteen, matrix loaded
+ *                      non-temporally in both; 2 sixteen, matrix loaded
+ *                      through the caches
  *   "fused"          FP64 CG iteration, same results bit for bit in all
  *                      forms: 2 (default) three kernels, each finaliser's
  *                      reduction repeated by the blocks of the next vector
