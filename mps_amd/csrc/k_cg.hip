@@ -978,33 +978,63 @@ k_mx_fin2(const double *part, unsigned np, MixedCtrl *c, int t, int max_iter)
     }
 }
 
-// p = r + beta p; on a reliable update first r = b - A x with the FP64
-// matrix for the own row (partial |r|^2 for the next fin1)
+// p = r + beta p over PU_R consecutive 256-row chunks per block (c0 is
+// the first; nch of them exist), so that each thread has PU_R rows in
+// flight.  On a reliable update first r = b - A x with the FP64 matrix for
+// each row, and part_ru[chunk] = the chunk's |r|^2, summed with the
+// thread-to-row map of one 256-thread block per chunk -- the partition of
+// the SpMV's blocks, which the next fin1 sums.
+#define PU_R 4
 template <int U, typename V>
-__global__ void __launch_bounds__(CG_BLK)
-k_mx_pupdate_pk(int n, PkView P, const double *__restrict__ b, const double *__restrict__ x, V *__restrict__ r,
-                V *__restrict__ p, double *part_ru, const MixedCtrl *c)
+__device__ __forceinline__ void pupdate_chunks(const PkView &P, const double *__restrict__ b,
+                                               const double *__restrict__ x, V *__restrict__ r, V *__restrict__ p,
+                                               int n, int c0, int nch, V beta, int ru, double *part_ru)
 {
-    if (c->done) return;
-    const V beta = (V)c->beta;
-    const int ru = c->ru;
-    const int i = blockIdx.x * CG_BLK + threadIdx.x;
-    double v = 0.0;
-    if (i < n) {
-        V ri = r[i];
-        if (ru) {
+    if (!ru) {
+        V rv[PU_R], pv[PU_R];
+#pragma unroll
+        for (int c = 0; c < PU_R; c++) {
+            const int i = (c0 + c) * CG_BLK + threadIdx.x;
+            if (c < nch && i < n) {
+                rv[c] = r[i];
+                pv[c] = p[i];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < PU_R; c++) {
+            const int i = (c0 + c) * CG_BLK + threadIdx.x;
+            if (c < nch && i < n) p[i] = rv[c] + beta * pv[c];
+        }
+        return;
+    }
+    for (int c = 0; c < nch; c++) {
+        const int i = (c0 + c) * CG_BLK + threadIdx.x;
+        double v = 0.0;
+        if (i < n) {
             const double xi = x[i];
             const double s = packed_row_global<U>(P, P.val, i, x, P.diag[i] * xi);
-            ri = (V)(b[i] - s);
+            const V ri = (V)(b[i] - s);
             v = (double)ri * (double)ri;
             r[i] = ri;
+            p[i] = ri + beta * p[i];
         }
-        p[i] = ri + beta * p[i];
-    }
-    if (ru) {
         v = block_sum<double, CG_BLK>(v);
-        if (threadIdx.x == 0) part_ru[blockIdx.x] = v;
+        if (threadIdx.x == 0) part_ru[c0 + c] = v;
     }
+}
+
+static inline unsigned pu_grid(unsigned nblk) { return (nblk + PU_R - 1) / PU_R; }
+
+// the p update of the mixed iteration over nblk = cg_grid(n) chunks
+template <int U, typename V>
+__global__ void __launch_bounds__(CG_BLK)
+k_mx_pupdate_pk(int n, unsigned nblk, PkView P, const double *__restrict__ b, const double *__restrict__ x,
+                V *__restrict__ r, V *__restrict__ p, double *part_ru, const MixedCtrl *c)
+{
+    if (c->done) return;
+    const int c0 = blockIdx.x * PU_R;
+    const int nch = min(PU_R, (int)nblk - c0);
+    pupdate_chunks<U, V>(P, b, x, r, p, n, c0, nch, (V)c->beta, c->ru, part_ru);
 }
 
 struct MixedBufs {
@@ -1062,7 +1092,7 @@ static int cg_mixed_run(Ctx *c, const SolveOpts &o, SolveResult *res, V *r, V *p
         prof_end(PK_AXPY, s);
         hipLaunchKernelGGL(k_mx_fin2, dim3(1), dim3(CG_BLK), 0, s, c->partB, gv, mb->d, t, o.max_iter);
         prof_begin(PK_PUPDATE, s);
-        hipLaunchKernelGGL((k_mx_pupdate_pk<8, V>), dim3(g), dim3(CG_BLK), 0, s, n, P, pk->b, x, r, p,
+        hipLaunchKernelGGL((k_mx_pupdate_pk<8, V>), dim3(pu_grid(g)), dim3(CG_BLK), 0, s, n, g, P, pk->b, x, r, p,
                            mb->part_ru, mb->d);
         prof_end(PK_PUPDATE, s);
     });
@@ -1491,7 +1521,8 @@ struct BMScene {
     unsigned *cq;
     QConsts qk;
     int n_eq;
-    int blk_off, nblk;       // SpMV / p-update blocks
+    int blk_off, nblk;       // SpMV blocks (256-row chunks)
+    int pb_off, npb;         // p-update blocks (PU_R chunks each)
     int ax_off, nax;         // init / axpy blocks
     int max_iter, tol_mode;
     double tol, delta, neg_coeff, re, r_min;
@@ -1670,35 +1701,21 @@ k_bm_fin2(const BMScene<V> *__restrict__ sc, const double *part2, MixedCtrl *ctr
 
 template <int U, typename V>
 __global__ void __launch_bounds__(CG_BLK)
-k_bm_pupdate(const BMScene<V> *__restrict__ sc, const int *__restrict__ bs, double *part_ru, const MixedCtrl *ctrl)
+k_bm_pupdate(const BMScene<V> *__restrict__ sc, const int *__restrict__ pbs, double *part_ru, const MixedCtrl *ctrl)
 {
-    const int si = bs[blockIdx.x];
+    const int si = pbs[blockIdx.x];
     if (ctrl[si].done) return;
     const BMScene<V> &S = sc[si];
-    const V beta = (V)ctrl[si].beta;
-    const int ru = ctrl[si].ru;
-    const int i = (blockIdx.x - S.blk_off) * CG_BLK + threadIdx.x;
-    double v = 0.0;
-    if (i < S.n_eq) {
-        V ri = S.r[i];
-        if (ru) {
-            const double xi = S.x[i];
-            const double s = packed_row_global<U>(S.P, S.P.val, i, S.x, S.P.diag[i] * xi);
-            ri = (V)(S.b[i] - s);
-            v = (double)ri * (double)ri;
-            S.r[i] = ri;
-        }
-        S.p[i] = ri + beta * S.p[i];
-    }
-    if (ru) {
-        v = block_sum<double, CG_BLK>(v);
-        if (threadIdx.x == 0) part_ru[blockIdx.x] = v;
-    }
+    const int c0 = (blockIdx.x - S.pb_off) * PU_R;
+    const int nch = min(PU_R, S.nblk - c0);
+    pupdate_chunks<U, V>(S.P, S.b, S.x, S.r, S.p, S.n_eq, c0, nch, (V)ctrl[si].beta, ctrl[si].ru,
+                         part_ru + S.blk_off);
 }
 
 struct BatchMixedBufs {
-    int cap_sc = 0, cap_ctrl = 0, cap_part3 = 0, cap_ru = 0;
+    int cap_sc = 0, cap_ctrl = 0, cap_part3 = 0, cap_ru = 0, cap_pb = 0;
     unsigned char *sc = nullptr, *hsc = nullptr;
+    int *pbs = nullptr, *hpbs = nullptr;
     MixedCtrl *ctrl = nullptr, *hctrl = nullptr;
     double *part3 = nullptr, *part_ru = nullptr;
 };
@@ -1721,16 +1738,18 @@ static int batch_mixed_run(Ctx *const *cs, int ns, const SolveOpts *opts, SolveR
     BatchBufs &B = g_batch;
     BatchMixedBufs &M = g_bmixed;
     const bool coded = sizeof(V) == sizeof(float) && opts[0].precision == 3;
-    int tot_blk = 0, tot_ax = 0, hmax = 0, max_iter = 0, check_every = 16;
+    int tot_blk = 0, tot_pb = 0, tot_ax = 0, hmax = 0, max_iter = 0, check_every = 16;
     for (int k = 0; k < ns; k++) {
         const int n = cs[k]->pk->n_eq;
         tot_blk += (int)cg_grid(n);
+        tot_pb += (int)pu_grid(cg_grid(n));
         tot_ax += n > 0 ? (int)vec_grid(n) : 0;
         if (cs[k]->pk->hmax > hmax) hmax = cs[k]->pk->hmax;
         if (opts[k].max_iter > max_iter) max_iter = opts[k].max_iter;
         if (opts[k].check_every > 0) check_every = opts[k].check_every;
     }
     int rc = reserve_pair(&M.sc, &M.hsc, &M.cap_sc, (int)(sizeof(BMScene<V>) * ns));
+    if (!rc) rc = reserve_pair(&M.pbs, &M.hpbs, &M.cap_pb, tot_pb + 1);
     if (!rc) rc = reserve_pair(&B.bs, &B.hbs, &B.cap_blk, tot_blk + 1);
     if (!rc) rc = reserve_pair(&B.axs, &B.haxs, &B.cap_ax, tot_ax + 1);
     if (!rc) rc = reserve_pair(&M.ctrl, &M.hctrl, &M.cap_ctrl, ns);
@@ -1745,7 +1764,7 @@ static int batch_mixed_run(Ctx *const *cs, int ns, const SolveOpts *opts, SolveR
     }
     BMScene<V> *hsc = reinterpret_cast<BMScene<V> *>(M.hsc);
     const BMScene<V> *dsc = reinterpret_cast<const BMScene<V> *>(M.sc);
-    int bo = 0, ao = 0;
+    int bo = 0, po = 0, ao = 0;
     for (int k = 0; k < ns; k++) {
         Ctx *c = cs[k];
         const Packed *pk = c->pk;
@@ -1771,6 +1790,8 @@ static int batch_mixed_run(Ctx *const *cs, int ns, const SolveOpts *opts, SolveR
         S.n_eq = pk->n_eq;
         S.blk_off = bo;
         S.nblk = (int)cg_grid(pk->n_eq);
+        S.pb_off = po;
+        S.npb = (int)pu_grid(cg_grid(pk->n_eq));
         S.ax_off = ao;
         S.nax = pk->n_eq > 0 ? (int)vec_grid(pk->n_eq) : 0;
         S.max_iter = o.max_iter;
@@ -1781,11 +1802,14 @@ static int batch_mixed_run(Ctx *const *cs, int ns, const SolveOpts *opts, SolveR
         S.re = o.re_lap;
         S.r_min = o.r_min;
         for (int q = 0; q < S.nblk; q++) B.hbs[bo + q] = k;
+        for (int q = 0; q < S.npb; q++) M.hpbs[po + q] = k;
         for (int q = 0; q < S.nax; q++) B.haxs[ao + q] = k;
         bo += S.nblk;
+        po += S.npb;
         ao += S.nax;
     }
     HIP_OK(hipMemcpyAsync(M.sc, M.hsc, sizeof(BMScene<V>) * ns, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(M.pbs, M.hpbs, sizeof(int) * (tot_pb + 1), hipMemcpyHostToDevice, s));
     HIP_OK(hipMemcpyAsync(B.bs, B.hbs, sizeof(int) * (tot_blk + 1), hipMemcpyHostToDevice, s));
     HIP_OK(hipMemcpyAsync(B.axs, B.haxs, sizeof(int) * (tot_ax + 1), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_fill_zero_int, dim3(1), dim3(64), 0, s, B.ndone, 1);
@@ -1820,8 +1844,8 @@ static int batch_mixed_run(Ctx *const *cs, int ns, const SolveOpts *opts, SolveR
             prof_end(PK_AXPY, s);
             hipLaunchKernelGGL(k_bm_fin2<V>, dim3(ns), dim3(CG_BLK), 0, s, dsc, B.part2, M.ctrl, t, B.ndone);
             prof_begin(PK_PUPDATE, s);
-            if (tot_blk > 0)
-                hipLaunchKernelGGL((k_bm_pupdate<8, V>), dim3(tot_blk), dim3(CG_BLK), 0, s, dsc, B.bs, M.part_ru, M.ctrl);
+            if (tot_pb > 0)
+                hipLaunchKernelGGL((k_bm_pupdate<8, V>), dim3(tot_pb), dim3(CG_BLK), 0, s, dsc, M.pbs, M.part_ru, M.ctrl);
             prof_end(PK_PUPDATE, s);
         }
         e = hipGetLastError();
