@@ -223,6 +223,10 @@ int cg_alloc_ctrl(Ctx *c);
 int batch_cg(Ctx *const *cs, int ns, const SolveOpts *opts, SolveResult *res, hipStream_t s);
 int batch_cg_mixed(Ctx *const *cs, int ns, const SolveOpts *opts, SolveResult *res, hipStream_t s);
 int pack_ppe(Ctx *c, int with_f32);        // k_pack.hip
+int pack_stage_a(Ctx *c);                   // pack_ppe = a, wait, b, wait, c
+int pack_stage_b(Ctx *c);
+int pack_stage_c(Ctx *c, int with_f32);
+int pack_wait(Ctx *c);
 int unpack_solution(Ctx *c);               // packed x -> c->x (slot order)
 void packed_free(Packed *p);
 int solve_ppe(Ctx *c, const SolveOpts &o, SolveResult *out);

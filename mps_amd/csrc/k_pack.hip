@@ -340,7 +340,11 @@ static int pk_reserve_rows(Packed *p, int n)
     return 0;
 }
 
-int pack_ppe(Ctx *c, int with_f32)
+// pack_ppe in three stages, each ending with copies of sizes to pinned
+// host memory that the next stage reads once the stream has reached them
+// (pack_wait).  Callers packing many systems run a stage for all of them
+// before waiting, so the systems' kernels overlap.
+int pack_stage_a(Ctx *c)
 {
     if (!c->pk) c->pk = new Packed();
     Packed *p = c->pk;
@@ -352,7 +356,21 @@ int pack_ppe(Ctx *c, int with_f32)
     rc = launch_scan_excl(c->unk, p->row_of_slot, n, c->scan_blk, s);
     if (rc) return rc;
     HIP_OK(hipMemcpyAsync(&p->htot[0], p->row_of_slot + n, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
+    return 0;
+}
+
+int pack_wait(Ctx *c)
+{
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int pack_stage_b(Ctx *c)
+{
+    Packed *p = c->pk;
+    const int n = c->n;
+    hipStream_t s = c->stream;
+    int rc = 0;
     const int n_eq = p->htot[0];
     p->n_eq = n_eq;
     p->n_slices = (n_eq + 63) / 64;
@@ -380,7 +398,16 @@ int pack_ppe(Ctx *c, int with_f32)
     HIP_OK(hipMemcpyAsync(&p->htot[3], p->c32off + ns, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_OK(hipMemcpyAsync(&p->htot[4], p->hoff + nb, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_OK(hipMemcpyAsync(&p->htot[5], p->dmax, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
+    return 0;
+}
+
+int pack_stage_c(Ctx *c, int with_f32)
+{
+    Packed *p = c->pk;
+    hipStream_t s = c->stream;
+    const int n_eq = p->n_eq;
+    if (n_eq == 0) return 0;
+    const int ns = p->n_slices, nb = p->n_blocks;
     p->tot_val = (size_t)p->htot[1];
     p->tot_w16 = (size_t)p->htot[2];
     p->tot_c32 = (size_t)p->htot[3];
@@ -403,6 +430,16 @@ int pack_ppe(Ctx *c, int with_f32)
     }
     prof_end(PK_PACK, s);
     return 0;
+}
+
+int pack_ppe(Ctx *c, int with_f32)
+{
+    int rc = pack_stage_a(c);
+    if (!rc) rc = pack_wait(c);
+    if (!rc) rc = pack_stage_b(c);
+    if (!rc) rc = pack_wait(c);
+    if (!rc) rc = pack_stage_c(c, with_f32);
+    return rc;
 }
 
 int unpack_solution(Ctx *c)

@@ -646,13 +646,22 @@ static SolveOpts solve_opts(const SimConfig *cf, const StepConsts &k)
 
 // The timestep after the solve (simulation.c:96-118): pressure, gradient
 // corrector, boundary, counters and statistics.
-static int step_post(Ctx *c, const StepConsts &k, const SolveResult &sr)
+// In two halves: the launches with an asynchronous copy of the counters,
+// then the wait and the bookkeeping, so that an ensemble can launch every
+// scene's half before it waits.
+static int step_post_launch(Ctx *c, const StepConsts &k)
 {
     tape_hook(c, TAPE_SOLVED);
     int rc = launch_pressure_out(c, k, 0);
     if (!rc) rc = launch_grad_correct(c, k);
-    if (!rc) rc = fetch_counters(c);
     if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(c->hcounters, c->counters, sizeof(DevCounters), hipMemcpyDeviceToHost, c->stream));
+    return 0;
+}
+
+static int step_post_finish(Ctx *c, const StepConsts &k, const SolveResult &sr)
+{
+    HIP_OK(hipStreamSynchronize(c->stream));
     c->ppe_valid = true;
     c->stats.steps++;
     c->stats.last_iters = sr.iters;
@@ -666,6 +675,12 @@ static int step_post(Ctx *c, const StepConsts &k, const SolveResult &sr)
     c->stats.max_count = c->hcounters->max_count;
     if (g_opt.profile > 0.5) prof_collect();
     return check_overflow(c, k.stride);
+}
+
+static int step_post(Ctx *c, const StepConsts &k, const SolveResult &sr)
+{
+    const int rc = step_post_launch(c, k);
+    return rc ? rc : step_post_finish(c, k, sr);
 }
 
 static int run_step(Ctx *c, const SimConfig *cf)
@@ -875,12 +890,21 @@ int mps_hip_run_batch(ParticleSystem *const *ps, SimConfig *const *cfgs, int nsc
         if (rc) break;
         if (batched) {
             const int mixed = so[0].precision >= 1 && so[0].precision <= 3;
+            // each packing stage for every scene before its waits, so the
+            // scenes' packing kernels overlap
             for (int q = 0; q < nscenes && !rc; q++) {
                 rc = cg_alloc_ctrl(cs[q]);
-                if (!rc) rc = pack_ppe(cs[q], mixed);
+                if (!rc) rc = pack_stage_a(cs[q]);
             }
-            for (int q = 0; q < nscenes && !rc; q++)
-                if (hipStreamSynchronize(cs[q]->stream) != hipSuccess) rc = MPS_ERR_HIP;
+            for (int q = 0; q < nscenes && !rc; q++) {
+                rc = pack_wait(cs[q]);
+                if (!rc) rc = pack_stage_b(cs[q]);
+            }
+            for (int q = 0; q < nscenes && !rc; q++) {
+                rc = pack_wait(cs[q]);
+                if (!rc) rc = pack_stage_c(cs[q], mixed);
+            }
+            for (int q = 0; q < nscenes && !rc; q++) rc = pack_wait(cs[q]);
             if (!rc)
                 rc = mixed ? batch_cg_mixed(cs.data(), nscenes, so.data(), sr.data(), bstream)
                            : batch_cg(cs.data(), nscenes, so.data(), sr.data(), bstream);
@@ -893,7 +917,11 @@ int mps_hip_run_batch(ParticleSystem *const *ps, SimConfig *const *cfgs, int nsc
         }
         for (int q = 0; q < nscenes && !rc; q++) {
             g_config = cfgs[q];
-            rc = step_post(cs[q], ks[q], sr[q]);
+            rc = step_post_launch(cs[q], ks[q]);
+        }
+        for (int q = 0; q < nscenes && !rc; q++) {
+            g_config = cfgs[q];
+            rc = step_post_finish(cs[q], ks[q], sr[q]);
         }
     }
     g_config = saved;
