@@ -979,12 +979,15 @@ k_mx_fin2(const double *part, unsigned np, MixedCtrl *c, int t, int max_iter)
 }
 
 // p = r + beta p over PU_R consecutive 256-row chunks per block (c0 is
-// the first; nch of them exist), so that each thread has PU_R rows in
-// flight.  On a reliable update first r = b - A x with the FP64 matrix for
-// each row, and part_ru[chunk] = the chunk's |r|^2, summed with the
-// thread-to-row map of one 256-thread block per chunk -- the partition of
-// the SpMV's blocks, which the next fin1 sums.
-#define PU_R 4
+// the first; nch of them exist).  On a reliable update first r = b - A x
+// with the FP64 matrix for each row, and part_ru[chunk] = the chunk's
+// |r|^2, summed with the thread-to-row map of one 256-thread block per
+// chunk -- the partition of the SpMV's blocks, which the next fin1 sums.
+// One chunk per block measured fastest (1, 2, 4 on one box: 63.7, 64.2,
+// 64.2 ms per mixed step at 1.08M particles; 173.6, 176.1, 179.3 ms per
+// step of the 64-scene ensemble): the reliable-update SpMV wants rows
+// spread over threads more than the plain update wants rows per thread.
+#define PU_R 1
 template <int U, typename V>
 __device__ __forceinline__ void pupdate_chunks(const PkView &P, const double *__restrict__ b,
                                                const double *__restrict__ x, V *__restrict__ r, V *__restrict__ p,
@@ -1007,6 +1010,10 @@ __device__ __forceinline__ void pupdate_chunks(const PkView &P, const double *__
         }
         return;
     }
+    // each chunk's sum as block_sum forms it: wave sums, then the waves'
+    // sums in wave order; one barrier for all chunks
+    __shared__ double wpart[PU_R][CG_BLK / WAVE];
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
     for (int c = 0; c < nch; c++) {
         const int i = (c0 + c) * CG_BLK + threadIdx.x;
         double v = 0.0;
@@ -1018,8 +1025,15 @@ __device__ __forceinline__ void pupdate_chunks(const PkView &P, const double *__
             r[i] = ri;
             p[i] = ri + beta * p[i];
         }
-        v = block_sum<double, CG_BLK>(v);
-        if (threadIdx.x == 0) part_ru[c0 + c] = v;
+        v = wave_sum(v);
+        if (lane == 0) wpart[c][wv] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < nch) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < CG_BLK / WAVE; w++) s += wpart[threadIdx.x][w];
+        part_ru[c0 + threadIdx.x] = s;
     }
 }
 
