@@ -85,33 +85,34 @@ struct DevCounters {
 // longest row) and grouped 256 at a time into blocks, one per SpMV
 // workgroup.  A block's halo is the sorted set of rows its entries
 // reference (halo[bmeta[b].x ...], bmeta[b].y of them); in halo mode
-// (smeta[s].z = 1) an entry's column is a 16-bit index into that halo, two
-// entries per 32-bit word (word (smeta[s].y + q/2) * 64 + lane of cw16),
-// else a 32-bit row number (c32[(smeta[s].y + q) * 64 + lane]).  Entries of
+// (smeta[s].z = 1) an entry's column is a 12-bit index into that halo;
+// entries 8g..8g+7 of a row fill the 96 bits (entry u at bit 12u, word
+// 0 lowest) of words smeta[s].y + 3g, +1, +2 (word w at w * 64 + lane of
+// chw); else a 32-bit row number (c32[(smeta[s].y + q) * 64 + lane]).  Entries of
 // a row keep the assembly's order, so every row sums as before.
 #define PK_ROWS 256                  // rows per block = SpMV workgroup size
-#define PK_HALO_CAP 4096             // largest halo staged in LDS
+#define PK_HALO_CAP 4096             // largest halo staged in LDS (12-bit indices)
 struct Packed {
     int n_eq = 0, n_slices = 0, n_blocks = 0, hmax = 0;
     int cap_rows = 0, cap_slots = 0;
-    size_t cap_val = 0, cap_w16 = 0, cap_c32 = 0, cap_fval = 0, cap_halo = 0;
-    size_t tot_val = 0, tot_w16 = 0, tot_c32 = 0, tot_halo = 0;
+    size_t cap_val = 0, cap_hw = 0, cap_c32 = 0, cap_fval = 0, cap_halo = 0;
+    size_t tot_val = 0, tot_hw = 0, tot_c32 = 0, tot_halo = 0;
     int *row_of_slot = nullptr;      // exclusive scan of unk (n + 1)
     int *slot_of_row = nullptr;
     int *len = nullptr;              // off-diagonal entries of the row
     double *diag = nullptr, *b = nullptr, *x = nullptr;
-    int *sw = nullptr, *w16 = nullptr, *w32 = nullptr;       // per slice
+    int *sw = nullptr, *whw = nullptr, *w32 = nullptr;       // per slice
     int *bh = nullptr, *bmode = nullptr;                     // per block
-    int *voff = nullptr, *c16off = nullptr, *c32off = nullptr, *hoff = nullptr;   // scans
+    int *voff = nullptr, *chwoff = nullptr, *c32off = nullptr, *hoff = nullptr;   // scans
     int4 *smeta = nullptr;           // per slice {value offset, column offset, halo mode, 0}
     int2 *bmeta = nullptr;           // per block {halo offset, halo length}
     int *halo = nullptr;
     double *val = nullptr;
-    unsigned *cw16 = nullptr;
+    unsigned *chw = nullptr;
     int *c32 = nullptr;
     float *fval = nullptr;           // mixed precision: FP32 values
     double *ddir = nullptr;          // mixed precision: Dirichlet part of each row
-    unsigned *cq = nullptr;          // precision 3: (distance code << 16) | halo index, per entry
+    unsigned *cq = nullptr;          // precision 3: (distance code << 12) | halo index, per entry
     size_t cap_cq = 0;
     int *dmax = nullptr;             // device: largest halo
     int *htot = nullptr;             // pinned: scan totals
@@ -204,7 +205,7 @@ struct SolveOpts {
     double tol;
     int tol_mode;      // 0 absolute, 1 relative
     int precision;     // 0 FP64; 1 FP32 matrix, FP64 vectors; 2 FP32 matrix and vectors;
-                       // 3 FP32 vectors, off-diagonals from 16-bit distance codes
+                       // 3 FP32 vectors, off-diagonals from 20-bit distance codes
     double inner_tol;
     int check_every;
     int dense_ic;      // 2-D dense-formula IC(0)

@@ -7,7 +7,7 @@
 // order of the dot products differs from the reference's serial loop.
 //
 // The system is the packed copy of the PPE (k_pack.hip): one row per
-// unknown, 64-row slices, 16-bit column offsets where they fit.
+// unknown, 64-row slices, 12-bit halo indices where they fit.
 // Per iteration:
 //   spmv       Ap = A p (one thread per row, entries U at a time), partial p.Ap
 //   spmv_fin   one block: p.Ap, alpha (or breakdown)
@@ -81,7 +81,7 @@ struct PkView {
     const int *halo;
     const double *diag, *val, *ddir;
     const float *fval;
-    const unsigned *cw16;
+    const unsigned *chw;
     const int *c32;
 };
 
@@ -96,7 +96,7 @@ static PkView pk_view(const Packed *p)
     v.val = p->val;
     v.ddir = p->ddir;
     v.fval = p->fval;
-    v.cw16 = p->cw16;
+    v.chw = p->chw;
     v.c32 = p->c32;
     return v;
 }
@@ -157,24 +157,56 @@ __device__ __forceinline__ ACC row_sum_global(int m, const VT *__restrict__ vrow
     return s;
 }
 
-// The same sum with 16-bit halo indices (two per word): x is read from
-// `hx`, either the block's halo staged in LDS or, with INDIRECT, global x
-// at halo[hbase + index].  U is even.
+// 12-bit halo indices: entries 8g..8g+7 of a row fill the 96 bits of three
+// consecutive words (entry u at bit 12u, word 0 lowest).  Word j of a group
+// holds bits of entries from IDX_FIRST[j] on, so it is read only when that
+// entry exists.
+#define IDX_GROUP_WORDS 3
+static_assert(PK_HALO_CAP <= 4096, "halo indices have 12 bits");
+__device__ __forceinline__ int idx_first(int j) { return j == 0 ? 0 : (j == 1 ? 2 : 5); }
+
+// entry u (0-7) of a group whose words are g[0..2]
+__device__ __forceinline__ unsigned idx_of(const unsigned *g, int u)
+{
+    const int bit = 12 * u, wd = bit >> 5, sh = bit & 31;
+    unsigned v = g[wd] >> sh;
+    if (sh > 20) v |= g[wd + 1] << (32 - sh);
+    return v & 0xfffu;
+}
+
+// entry q of a row whose first index word is wrow (stride 64 words)
+__device__ __forceinline__ unsigned idx_at(const unsigned *__restrict__ wrow, int q)
+{
+    const int bit = 12 * (q & 7), wd = bit >> 5, sh = bit & 31;
+    const unsigned *w = wrow + (size_t)((q >> 3) * IDX_GROUP_WORDS + wd) * 64u;
+    unsigned v = w[0] >> sh;
+    if (sh > 20) v |= w[64] << (32 - sh);
+    return v & 0xfffu;
+}
+
+// The same sum with 12-bit halo indices: x is read from `hx`, either the
+// block's halo staged in LDS or, with INDIRECT, global x at
+// halo[hbase + index].  U is a multiple of 8 (whole index groups).
 template <int U, bool DIFF, bool INDIRECT, typename ACC, bool NT = false, typename VT, typename XT>
 __device__ __forceinline__ ACC row_sum_halo(int m, const VT *__restrict__ vrow, const unsigned *__restrict__ wrow,
                                             const XT *hx, const int *__restrict__ hidx, XT xi, ACC s)
 {
+    static_assert(U % 8 == 0, "whole groups of eight indices");
+    constexpr int NW = U / 8 * IDX_GROUP_WORDS;
     for (int q0 = 0; q0 < m; q0 += U) {
-        unsigned w[U / 2];
+        unsigned w[NW];
         VT a[U];
         XT xv[U];
 #pragma unroll
-        for (int j = 0; j < U / 2; j++) w[j] = (q0 + 2 * j < m) ? ldm<NT>(wrow + (size_t)(q0 / 2 + j) * 64u) : 0u;
+        for (int j = 0; j < NW; j++) {
+            const int first = (j / IDX_GROUP_WORDS) * 8 + idx_first(j % IDX_GROUP_WORDS);
+            w[j] = (q0 + first < m) ? ldm<NT>(wrow + (size_t)(q0 / 8 * IDX_GROUP_WORDS + j) * 64u) : 0u;
+        }
 #pragma unroll
         for (int u = 0; u < U; u++) a[u] = (q0 + u < m) ? ldm<NT>(vrow + (size_t)(q0 + u) * 64u) : VT(0);
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const unsigned li = (q0 + u < m) ? ((u & 1) ? (w[u >> 1] >> 16) : (w[u >> 1] & 0xffffu)) : 0u;
+            const unsigned li = (q0 + u < m) ? idx_of(w + (u / 8) * IDX_GROUP_WORDS, u % 8) : 0u;
             xv[u] = INDIRECT ? hx[hidx[li]] : hx[li];
         }
 #pragma unroll
@@ -195,7 +227,7 @@ __device__ __forceinline__ ACC packed_row(const PkView &P, const VT *vals, int i
     const int m = P.len[i];
     const VT *vrow = vals + (size_t)mt.x * 64u + lane;
     if (mt.z)
-        return row_sum_halo<U, DIFF, false, ACC, NT>(m, vrow, P.cw16 + (size_t)mt.y * 64u + lane, hx, nullptr, xi, s);
+        return row_sum_halo<U, DIFF, false, ACC, NT>(m, vrow, P.chw + (size_t)mt.y * 64u + lane, hx, nullptr, xi, s);
     return row_sum_global<U, DIFF, ACC, NT>(m, vrow, P.c32 + (size_t)mt.y * 64u + lane, x, xi, i, s);
 }
 
@@ -210,7 +242,7 @@ __device__ __forceinline__ double packed_row_global(const PkView &P, const VT *v
     const VT *vrow = vals + (size_t)mt.x * 64u + lane;
     if (mt.z) {
         const int *hidx = P.halo + P.bmeta[i / PK_ROWS].x;
-        return row_sum_halo<U, false, true, double>(m, vrow, P.cw16 + (size_t)mt.y * 64u + lane, x, hidx, 0.0, s);
+        return row_sum_halo<U, false, true, double>(m, vrow, P.chw + (size_t)mt.y * 64u + lane, x, hidx, 0.0, s);
     }
     return row_sum_global<U, false, double>(m, vrow, P.c32 + (size_t)mt.y * 64u + lane, XPlain<double>{x}, 0.0,
                                             i, s);
@@ -801,7 +833,7 @@ k_mx_spmv_pk(int n, PkView P, const V *__restrict__ p, V *__restrict__ Ap, doubl
     if (threadIdx.x == 0) part[blockIdx.x] = v;
 }
 
-// ---- precision 3: off-diagonals from 16-bit distance codes ------------
+// ---- precision 3: off-diagonals from 20-bit distance codes ------------
 // Every off-diagonal is A_ij = neg_coeff w(r_ij), w(r) = re / max(r, r_min)
 // - 1 for r < re (pressure_solver.c:490-531, kernel.c:12-20).  In halo-mode
 // blocks one 32-bit word per entry carries the entry's halo index (12 bits:
@@ -813,9 +845,8 @@ k_mx_spmv_pk(int n, PkView P, const V *__restrict__ p, V *__restrict__ Ap, doubl
 // to half a code step (1.5e-6 l0 at re = 3.1 l0) moves w by about 2e-6
 // relative at r = l0 and 3e-5 at r = 0.98 re, by under 1e-6 absolute
 // nearer re, and the operator error keeps zero row sums.  The SpMV streams
-// 4 B per nonzero instead of 6 (FP32 value and 16-bit index).  Blocks
+// 4 B per nonzero instead of 5.5 (FP32 value and 12-bit index).  Blocks
 // stored with 32-bit columns apply their FP32 values.
-static_assert(PK_HALO_CAP <= 4096, "halo indices of the coded operator have 12 bits");
 #define Q_CODE_MAX 1048575.0
 struct QConsts {
     float neg_coeff, re, r_lo, dr;
@@ -830,11 +861,10 @@ __device__ __forceinline__ void q16_build_row(int k, const PkView &P, double neg
     const int lane = k & 63;
     const int m = P.len[k];
     const size_t vb = (size_t)mt.x * 64u + lane;
-    const unsigned *wrow = P.cw16 + (size_t)mt.y * 64u + lane;
+    const unsigned *wrow = P.chw + (size_t)mt.y * 64u + lane;
     const double dr = (re - r_min) / Q_CODE_MAX;
     for (int q = 0; q < m; q++) {
-        const unsigned wd = wrow[(size_t)(q >> 1) * 64u];
-        const unsigned idx = (q & 1) ? (wd >> 16) : (wd & 0xffffu);
+        const unsigned idx = idx_at(wrow, q);
         const double w = P.val[vb + (size_t)q * 64u] / neg_coeff;
         double r = re / (w + 1.0);
         if (r < r_min) r = r_min;

@@ -13,14 +13,15 @@
 //     reference -- its halo, about 1,900 rows for a 256-row block of the
 //     1.08M-particle dam break, each used ~15 times -- are listed once; the
 //     SpMV stages the vector at the halo in LDS and gathers from there, and
-//     an entry's column is a 16-bit index into the halo, two per 32-bit
-//     word.  A block whose halo is larger than PK_HALO_CAP, or whose
+//     an entry's column is a 12-bit index into the halo, eight per three
+//     32-bit words.  A block whose halo is larger than PK_HALO_CAP, or whose
 //     columns span more than 2^17 rows, keeps 32-bit row numbers and
 //     gathers from global memory.
 // Entries keep their order inside a row, so each row's sum is unchanged.
 // The halo of a block is found with an LDS bitmap over its column span:
 // set a bit per referenced row, then an entry's halo index is the number
-// of set bits below its own.
+// of set bits below its own.  Halo indices are 12-bit (a staged halo holds
+// at most PK_HALO_CAP = 4096 rows), eight to three 32-bit words.
 #include <limits.h>
 
 #include "ctx.hpp"
@@ -134,7 +135,7 @@ __device__ __forceinline__ void block_bitmap(const BlockRows &br, int stride, co
 __global__ void __launch_bounds__(PKB)
 k_pk_pass1(int n_eq, int n_slices, int stride, const int *__restrict__ slot_of_row,
            const int *__restrict__ row_of_slot, const int *__restrict__ Alen, const int *__restrict__ Acol,
-           int *__restrict__ sw, int *__restrict__ w16, int *__restrict__ w32, int *__restrict__ bh,
+           int *__restrict__ sw, int *__restrict__ whw, int *__restrict__ w32, int *__restrict__ bh,
            int *__restrict__ bmode, int *dmax)
 {
     __shared__ unsigned bits[BM_WORDS];
@@ -156,7 +157,7 @@ k_pk_pass1(int n_eq, int n_slices, int stride, const int *__restrict__ slot_of_r
     const int sl = blockIdx.x * (PKB / WAVE) + threadIdx.x / WAVE;
     if (sl < n_slices && (threadIdx.x & (WAVE - 1)) == 0) {
         sw[sl] = wmax;
-        w16[sl] = mode ? (wmax + 1) / 2 : 0;
+        whw[sl] = mode ? (wmax + 7) / 8 * 3 : 0;     // three words per 8 entries
         w32[sl] = mode ? 0 : wmax;
     }
     if (threadIdx.x == 0) {
@@ -173,8 +174,8 @@ k_pk_pass2(int n_eq, int n_slices, int stride, const int *__restrict__ slot_of_r
            const int *__restrict__ row_of_slot, const int *__restrict__ Alen, const double *__restrict__ Aval,
            const int *__restrict__ Acol, const double *__restrict__ diag, const double *__restrict__ b,
            const int *__restrict__ bmode, const int *__restrict__ bh, const int *__restrict__ hoff,
-           const int *__restrict__ voff, const int *__restrict__ c16off, const int *__restrict__ c32off,
-           int *__restrict__ halo, double *__restrict__ val, unsigned short *__restrict__ c16,
+           const int *__restrict__ voff, const int *__restrict__ chwoff, const int *__restrict__ c32off,
+           int *__restrict__ halo, double *__restrict__ val, unsigned *__restrict__ chw,
            int *__restrict__ c32, int *__restrict__ len, double *__restrict__ pdiag, double *__restrict__ pb,
            int4 *__restrict__ smeta, int2 *__restrict__ bmeta)
 {
@@ -216,12 +217,13 @@ k_pk_pass2(int n_eq, int n_slices, int stride, const int *__restrict__ slot_of_r
     const int lane = threadIdx.x & (WAVE - 1);
     const int sl = blockIdx.x * (PKB / WAVE) + threadIdx.x / WAVE;
     if (sl < n_slices && lane == 0)
-        smeta[sl] = make_int4(voff[sl], mode ? c16off[sl] : c32off[sl], mode, 0);
+        smeta[sl] = make_int4(voff[sl], mode ? chwoff[sl] : c32off[sl], mode, 0);
     if (threadIdx.x == 0) bmeta[blockIdx.x] = make_int2(mode ? hoff[blockIdx.x] : 0, bh[blockIdx.x]);
     if (br.slot < 0) return;
     const int vo = voff[sl];
-    const int co = mode ? c16off[sl] : c32off[sl];
+    const int co = mode ? chwoff[sl] : c32off[sl];
     const int m = br.m;
+    unsigned g0 = 0u, g1 = 0u, g2 = 0u;      // the row's current group of eight 12-bit indices
     for (int q = 0; q < m; q++) {
         const size_t o = sell_off(br.slot, q, stride);
         const int cr = row_of_slot[Acol[o]];
@@ -229,13 +231,29 @@ k_pk_pass2(int n_eq, int n_slices, int stride, const int *__restrict__ slot_of_r
         if (mode) {
             const int c = cr - br.bmn;
             const unsigned below = bits[c >> 5] & ((1u << (c & 31)) - 1u);
-            const int li = pre[c >> 5] + __popc(below);
-            c16[((size_t)(co + q / 2) * 64u + lane) * 2u + (q & 1)] = (unsigned short)li;
+            const unsigned li = (unsigned)(pre[c >> 5] + __popc(below));   // < PK_HALO_CAP
+            const int bit = 12 * (q & 7), wd = bit >> 5, sh = bit & 31;
+            const unsigned lo = li << sh, hi = sh > 20 ? li >> (32 - sh) : 0u;
+            if (wd == 0) {
+                g0 |= lo;
+                g1 |= hi;
+            } else if (wd == 1) {
+                g1 |= lo;
+                g2 |= hi;
+            } else {
+                g2 |= lo;
+            }
+            if ((q & 7) == 7 || q == m - 1) {
+                const size_t w = (size_t)(co + (q >> 3) * 3) * 64u + lane;
+                chw[w] = g0;
+                chw[w + 64u] = g1;
+                chw[w + 128u] = g2;
+                g0 = g1 = g2 = 0u;
+            }
         } else {
             c32[(size_t)(co + q) * 64u + lane] = cr;
         }
     }
-    if (mode && (m & 1)) c16[((size_t)(co + m / 2) * 64u + lane) * 2u + 1u] = 0;   // unused half
     len[br.k] = m;
     pdiag[br.k] = diag[br.slot];
     pb[br.k] = b[br.slot];
@@ -291,9 +309,9 @@ static int grow(T **p, size_t *cap, size_t need)
 void packed_free(Packed *p)
 {
     if (!p) return;
-    void *bufs[] = {p->row_of_slot, p->slot_of_row, p->len, p->diag, p->b, p->x, p->sw, p->w16, p->w32,
-                    p->bh, p->bmode, p->voff, p->c16off, p->c32off, p->hoff, p->smeta, p->bmeta, p->halo,
-                    p->val, p->cw16, p->c32, p->fval, p->ddir, p->dmax, p->cq};
+    void *bufs[] = {p->row_of_slot, p->slot_of_row, p->len, p->diag, p->b, p->x, p->sw, p->whw, p->w32,
+                    p->bh, p->bmode, p->voff, p->chwoff, p->c32off, p->hoff, p->smeta, p->bmeta, p->halo,
+                    p->val, p->chw, p->c32, p->fval, p->ddir, p->dmax, p->cq};
     for (void *b : bufs)
         if (b) hipFree(b);
     if (p->htot) hipHostFree(p->htot);
@@ -311,8 +329,8 @@ static int pk_reserve_rows(Packed *p, int n)
     }
     const int rows = (n + PKB - 1) / PKB * PKB;
     if (p->cap_rows < rows) {
-        void *old[] = {p->slot_of_row, p->len, p->diag, p->b, p->x, p->ddir, p->sw, p->w16, p->w32, p->bh,
-                       p->bmode, p->voff, p->c16off, p->c32off, p->hoff, p->smeta, p->bmeta};
+        void *old[] = {p->slot_of_row, p->len, p->diag, p->b, p->x, p->ddir, p->sw, p->whw, p->w32, p->bh,
+                       p->bmode, p->voff, p->chwoff, p->c32off, p->hoff, p->smeta, p->bmeta};
         for (void *b : old)
             if (b) hipFree(b);
         const size_t ns = (size_t)rows / 64 + 1, nb = (size_t)rows / PKB + 1;
@@ -323,10 +341,10 @@ static int pk_reserve_rows(Packed *p, int n)
         HIP_OK(hipMalloc(&p->x, sizeof(double) * rows));
         HIP_OK(hipMalloc(&p->ddir, sizeof(double) * rows));
         HIP_OK(hipMalloc(&p->sw, sizeof(int) * ns));
-        HIP_OK(hipMalloc(&p->w16, sizeof(int) * ns));
+        HIP_OK(hipMalloc(&p->whw, sizeof(int) * ns));
         HIP_OK(hipMalloc(&p->w32, sizeof(int) * ns));
         HIP_OK(hipMalloc(&p->voff, sizeof(int) * ns));
-        HIP_OK(hipMalloc(&p->c16off, sizeof(int) * ns));
+        HIP_OK(hipMalloc(&p->chwoff, sizeof(int) * ns));
         HIP_OK(hipMalloc(&p->c32off, sizeof(int) * ns));
         HIP_OK(hipMalloc(&p->smeta, sizeof(int4) * ns));
         HIP_OK(hipMalloc(&p->bh, sizeof(int) * nb));
@@ -386,15 +404,15 @@ int pack_stage_b(Ctx *c)
     rc = launch_fill_int(p->dmax, 0, 1, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_pk_pass1, dim3(nb), dim3(PKB), 0, s, n_eq, ns, c->stride, p->slot_of_row,
-                       p->row_of_slot, c->Alen, c->Acol, p->sw, p->w16, p->w32, p->bh, p->bmode, p->dmax);
+                       p->row_of_slot, c->Alen, c->Acol, p->sw, p->whw, p->w32, p->bh, p->bmode, p->dmax);
     LAUNCH_CHECK();
     rc = launch_scan_excl(p->sw, p->voff, ns, c->scan_blk, s);
-    if (!rc) rc = launch_scan_excl(p->w16, p->c16off, ns, c->scan_blk, s);
+    if (!rc) rc = launch_scan_excl(p->whw, p->chwoff, ns, c->scan_blk, s);
     if (!rc) rc = launch_scan_excl(p->w32, p->c32off, ns, c->scan_blk, s);
     if (!rc) rc = launch_scan_excl(p->bh, p->hoff, nb, c->scan_blk, s);
     if (rc) return rc;
     HIP_OK(hipMemcpyAsync(&p->htot[1], p->voff + ns, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_OK(hipMemcpyAsync(&p->htot[2], p->c16off + ns, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(&p->htot[2], p->chwoff + ns, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_OK(hipMemcpyAsync(&p->htot[3], p->c32off + ns, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_OK(hipMemcpyAsync(&p->htot[4], p->hoff + nb, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_OK(hipMemcpyAsync(&p->htot[5], p->dmax, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -409,17 +427,17 @@ int pack_stage_c(Ctx *c, int with_f32)
     if (n_eq == 0) return 0;
     const int ns = p->n_slices, nb = p->n_blocks;
     p->tot_val = (size_t)p->htot[1];
-    p->tot_w16 = (size_t)p->htot[2];
+    p->tot_hw = (size_t)p->htot[2];
     p->tot_c32 = (size_t)p->htot[3];
     p->tot_halo = (size_t)p->htot[4];
     p->hmax = p->htot[5];
     GROW(p->val, p->cap_val, p->tot_val * 64);
-    GROW(p->cw16, p->cap_w16, p->tot_w16 * 64);
+    GROW(p->chw, p->cap_hw, p->tot_hw * 64);
     GROW(p->c32, p->cap_c32, p->tot_c32 * 64);
     GROW(p->halo, p->cap_halo, p->tot_halo);
     hipLaunchKernelGGL(k_pk_pass2, dim3(nb), dim3(PKB), 0, s, n_eq, ns, c->stride, p->slot_of_row, p->row_of_slot,
-                       c->Alen, c->Aval, c->Acol, c->diag, c->b, p->bmode, p->bh, p->hoff, p->voff, p->c16off,
-                       p->c32off, p->halo, p->val, (unsigned short *)p->cw16, p->c32, p->len, p->diag, p->b,
+                       c->Alen, c->Aval, c->Acol, c->diag, c->b, p->bmode, p->bh, p->hoff, p->voff, p->chwoff,
+                       p->c32off, p->halo, p->val, p->chw, p->c32, p->len, p->diag, p->b,
                        p->smeta, p->bmeta);
     LAUNCH_CHECK();
     if (with_f32) {
