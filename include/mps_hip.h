@@ -163,7 +163,11 @@ int  mps_gen_dambreak(const double dims[6], double l0, int wall_layers,
 
 int  mps_hip_init(int device);          /* optional; first call does it   */
 void mps_hip_finalize(void);            /* frees every device context     */
+/* The error of the last failed call.  Each void entry point of Part 1
+ * clears it when it starts, so after such a call it reports that call. */
 int  mps_hip_last_error(void);
+/* Text of an error code: the detailed message (with the particle index of
+ * a neighbour-list overflow) when the last error had this code. */
 const char *mps_hip_error_string(int code);
 /* gfx arch name of the active device, e.g. "gfx950" ("" if none). */
 const char *mps_hip_device_arch(void);

@@ -33,6 +33,7 @@
     } while (0)
 
 void mps_set_error(int code, const char *file, int line, const char *msg);
+void mps_clear_error(void);
 
 // Step constants computed on the host in the reference's association order
 // (host and device FP64 arithmetic are both IEEE, so precomputed products

@@ -128,6 +128,7 @@ struct Call {
 
 static bool begin_call(Call &cl, ParticleSystem *ps, const NeighborList *nl, bool need_list)
 {
+    mps_clear_error();
     if (!ps || !g_config) return false;
     cl.c = mps_ctx_for(ps, 1);
     if (!cl.c) return false;
@@ -156,6 +157,7 @@ extern "C" {
 // neighbor_search_cell_linked_list (neighbor_search.c:146-240)
 void neighbor_search_cell_linked_list(NeighborList *nl, ParticleSystem *ps, CellList *cl, double re)
 {
+    mps_clear_error();
     if (!ps || !nl || !g_config) return;
     Ctx *c = mps_ctx_for(ps, 1);
     if (!c) return;
@@ -197,6 +199,7 @@ void neighbor_search_cell_linked_list(NeighborList *nl, ParticleSystem *ps, Cell
 // neighbor_search_brute_force (neighbor_search.c:44-81)
 void neighbor_search_brute_force(NeighborList *nl, ParticleSystem *ps, double re)
 {
+    mps_clear_error();
     if (!ps || !nl || !g_config) return;
     Ctx *c = mps_ctx_for(ps, 1);
     if (!c) return;
@@ -218,6 +221,9 @@ void neighbor_search_brute_force(NeighborList *nl, ParticleSystem *ps, double re
         fprintf(stderr, "Error: neighbor count exceeded max_neighbors (%d) for particle %d. "
                         "Increase max_neighbors in params.txt.\n", c->stride, h.overflow_orig);
         if (mps_hip_get_option("exit_on_error") > 0.5) exit(EXIT_FAILURE);
+        char msg[128];
+        snprintf(msg, sizeof msg, "neighbour list overflow at particle %d", h.overflow_orig);
+        mps_set_error(MPS_ERR_NEIGHBOR_OVERFLOW, __FILE__, __LINE__, msg);
         return;
     }
     c->list_valid = true;
@@ -366,6 +372,7 @@ void remove_out_of_bounds(ParticleSystem *ps, double xmin, double xmax, double y
 // scatter -- from the caller's list, densities and flags.
 void solve_pressure(ParticleSystem *ps, NeighborList *nl)
 {
+    mps_clear_error();
     Call cl;
     if (!begin_call(cl, ps, nl, true)) return;
     Ctx *c = cl.c;
@@ -404,6 +411,7 @@ void solve_pressure(ParticleSystem *ps, NeighborList *nl)
 // particle_system_calc_initial_params (particle.c:69-221) on the GPU
 void particle_system_calc_initial_params(ParticleSystem *ps)
 {
+    mps_clear_error();
     if (!ps || !g_config) return;
     const SimConfig *cf = g_config;
     const double re_n = cf->influence_radius_n, re_lap = cf->influence_radius_lap;
@@ -462,6 +470,9 @@ void particle_system_calc_initial_params(ParticleSystem *ps)
         h.overflow_orig != INT_MAX) {
         fprintf(stderr, "Error: more than max_neighbors particles within the influence radius of "
                         "particle %d\n", h.overflow_orig);
+        char msg[128];
+        snprintf(msg, sizeof msg, "neighbour list overflow at particle %d", h.overflow_orig);
+        mps_set_error(MPS_ERR_NEIGHBOR_OVERFLOW, __FILE__, __LINE__, msg);
         return;
     }
     c->has_state = false;   // the pass used the state buffers as scratch

@@ -27,11 +27,18 @@
 static int g_last_error = MPS_OK;
 static char g_last_msg[512] = "";
 
+static int g_msg_code = MPS_OK;   // the code g_last_msg was written for
+
 void mps_set_error(int code, const char *file, int line, const char *msg)
 {
     g_last_error = code;
+    g_msg_code = code;
     snprintf(g_last_msg, sizeof g_last_msg, "%s:%d: %s", file, line, msg ? msg : "");
 }
+
+// Each void entry point of the reference's API starts with this, so that
+// mps_hip_last_error() afterwards reports that call alone.
+void mps_clear_error(void) { g_last_error = MPS_OK; }
 
 struct Options {
     double tol_mode = 0, precision = 0, inner_tol = 1e-3, exit_on_error = 1, check_every = 16;
@@ -742,15 +749,17 @@ int mps_hip_last_error(void) { return g_last_error; }
 
 const char *mps_hip_error_string(int code)
 {
+    // the detailed message of the last error, when it was written for this code
+    if (code != MPS_OK && code == g_msg_code && g_last_msg[0]) return g_last_msg;
     switch (code) {
     case MPS_OK: return "ok";
-    case MPS_ERR_HIP: return g_last_msg[0] ? g_last_msg : "HIP runtime error";
+    case MPS_ERR_HIP: return "HIP runtime error";
     case MPS_ERR_NO_DEVICE: return "no gfx950 device";
     case MPS_ERR_ALLOC: return "device allocation failed";
     case MPS_ERR_NEIGHBOR_OVERFLOW: return "neighbour list overflow";
     case MPS_ERR_ARG: return "invalid argument";
     case MPS_ERR_NO_CONFIG: return "g_config is NULL";
-    case MPS_ERR_UNSUPPORTED: return g_last_msg[0] ? g_last_msg : "unsupported";
+    case MPS_ERR_UNSUPPORTED: return "unsupported";
     case MPS_ERR_SOLVER: return "solver failure";
     default: return "unknown error";
     }
@@ -1027,6 +1036,7 @@ void mps_hip_profile_reset(void)
 void simulation_step(ParticleSystem *ps, NeighborList *nl, CellList *cl, int step)
 {
     (void)step;
+    mps_clear_error();
     if (!g_config) { g_last_error = MPS_ERR_NO_CONFIG; return; }
     Ctx *c = get_ctx(ps);
     if (!c) return;
@@ -1047,6 +1057,7 @@ void simulation_step(ParticleSystem *ps, NeighborList *nl, CellList *cl, int ste
 // output step.
 void simulation_run(ParticleSystem *ps)
 {
+    mps_clear_error();
     if (!g_config) { g_last_error = MPS_ERR_NO_CONFIG; return; }
     const double re = search_radius(g_config);
     const double dt = g_config->dt;
