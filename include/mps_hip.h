@@ -259,13 +259,14 @@ int  mps_hip_step_backward(ParticleSystem *ps, NeighborList *nl, const double *r
  *                      entries per load batch, 1 sixteen, matrix loaded
  *                      non-temporally in both; 2 sixteen, matrix loaded
  *                      through the caches
- *   "fused"          FP64 CG iteration, same results bit for bit in all
+ *   "fused"          CG iteration, same results bit for bit in all
  *                      forms: 2 (default) three kernels, each finaliser's
  *                      reduction repeated by the blocks of the next vector
- *                      kernel (1.3 % faster than 0); 0 five kernels (SpMV,
- *                      one-block finaliser, axpy, finaliser, p update);
- *                      1 two (the previous iteration closed inside the
- *                      SpMV, p formed where it is read; 2 % slower than 0)
+ *                      kernel (FP64 1.3 %, mixed 0.8 % faster than 0);
+ *                      0 five kernels (SpMV, one-block finaliser, axpy,
+ *                      finaliser, p update); 1 (FP64 only; the mixed forms
+ *                      run five) two kernels, the previous iteration closed
+ *                      inside the SpMV, p formed where it is read (slower)
  *   "graph"          1: the particle phases of each step (predictor to
  *                      assembly) are captured once as a HIP graph and
  *                      replayed, one graph per buffer parity; same results

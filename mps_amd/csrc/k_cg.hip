@@ -765,6 +765,13 @@ static int cg_fp64(Ctx *c, const SolveOpts &o, SolveResult *res)
 struct MixedCtrl {
     double rr, rr_ru, pap, alpha, beta, bnorm, stop, res, delta2;
     int done, iters, breakdown, ru, n_ru, pad;
+    // three-kernel form: r.r, its value at the last reliable update and the
+    // reliable-update flag entering iteration t sit in slot t & 1 (written
+    // by the previous p update, read by every block of the next axpy);
+    // m_rr, m_rr_ru are the axpy's values after folding in the true
+    // residual, read by every block of the p update
+    double f_rr[2], f_rr_ru[2], m_rr, m_rr_ru;
+    int f_ru[2];
 };
 
 template <typename V>
@@ -808,6 +815,9 @@ k_mx_init_fin(const double *part, const double *bpart, unsigned np, MixedCtrl *c
         c->ru = 0;
         c->n_ru = 0;
         c->res = sqrt(bb);
+        c->f_rr[0] = rr;
+        c->f_rr_ru[0] = rr;
+        c->f_ru[0] = 0;
         if (max_iter <= 0 || sqrt(bb) < c->stop) c->done = 1;
     }
 }
@@ -1081,6 +1091,116 @@ k_mx_pupdate_pk(int n, unsigned nblk, PkView P, const double *__restrict__ b, co
     pupdate_chunks<U, V>(P, b, x, r, p, n, c0, nch, (V)c->beta, c->ru, part_ru);
 }
 
+// ---- the mixed iteration in three kernels (option fused = 2) ----------
+// SpMV, then k_mx3_axpy (every block repeats k_mx_fin1's sums and
+// decisions; block 0 records them), then k_mx3_pupdate (every block
+// repeats k_mx_fin2's; block 0 records).  The arithmetic is the five-kernel
+// form's, so the iterates have its bits.  A block that finds the solve
+// stopped, or reads the stop flag block 0 of the same kernel has just set,
+// does no update -- the same outcome either way.
+template <typename V>
+__global__ void __launch_bounds__(CG_BLK)
+k_mx3_axpy(int n, int t, const V *__restrict__ p, const V *__restrict__ Ap, double *__restrict__ x,
+           V *__restrict__ r, const double *part, const double *part_ru, unsigned np, double *part2, MixedCtrl *c)
+{
+    __shared__ double sh_alpha;
+    __shared__ int sh_go;
+    if (c->done) return;
+    const double pap = fin_sum(part, np);
+    const int ru = c->f_ru[t & 1];
+    double rr_true = 0.0;
+    if (ru) rr_true = fin_sum(part_ru, np);
+    if (threadIdx.x == 0) {
+        double rr = c->f_rr[t & 1], rr_ru = c->f_rr_ru[t & 1];
+        int go = 1, conv = 0, brk = 0;
+        if (ru) {
+            rr = rr_true;
+            rr_ru = rr_true;
+            conv = sqrt(rr_true) < c->stop;
+        }
+        if (!conv) brk = fabs(pap) < 1.0e-30;
+        go = !conv && !brk;
+        const double alpha = go ? rr / pap : 0.0;
+        if (blockIdx.x == 0) {
+            if (ru) {
+                c->n_ru++;
+                c->rr = rr_true;
+                c->rr_ru = rr_true;
+                c->res = sqrt(rr_true);
+            }
+            if (!conv) c->pap = pap;
+            if (go) {
+                c->alpha = alpha;
+                c->m_rr = rr;
+                c->m_rr_ru = rr_ru;
+            } else {
+                c->iters = t;
+                c->breakdown = brk;
+                c->done = 1;
+            }
+        }
+        sh_alpha = alpha;
+        sh_go = go;
+    }
+    __syncthreads();
+    if (!sh_go) return;
+    const double alpha = sh_alpha;
+    const V av = (V)alpha;
+    double v = 0.0;
+    for (int i = blockIdx.x * CG_BLK + threadIdx.x; i < n; i += gridDim.x * CG_BLK) {
+        x[i] += alpha * (double)p[i];
+        const V ri = r[i] - av * Ap[i];
+        r[i] = ri;
+        v += (double)ri * (double)ri;
+    }
+    v = block_sum<double, CG_BLK>(v);
+    if (threadIdx.x == 0) part2[blockIdx.x] = v;
+}
+
+template <int U, typename V>
+__global__ void __launch_bounds__(CG_BLK)
+k_mx3_pupdate(int n, unsigned nblk, int t, int max_iter, PkView P, const double *__restrict__ b,
+              const double *__restrict__ x, V *__restrict__ r, V *__restrict__ p, const double *part2,
+              unsigned np2, double *part_ru, MixedCtrl *c)
+{
+    __shared__ double sh_beta;
+    __shared__ int sh_go, sh_ru;
+    if (c->done) return;
+    const double rr_new = fin_sum(part2, np2);
+    if (threadIdx.x == 0) {
+        const double rr = c->m_rr, rr_ru = c->m_rr_ru;
+        const int go = !(rr < 1.0e-60 || t + 1 >= max_iter);
+        double beta = 0.0;
+        int ru = 0;
+        if (go) {
+            beta = rr_new / rr;
+            ru = (rr_new < c->delta2 * rr_ru) || (sqrt(rr_new) < c->stop);
+        }
+        if (blockIdx.x == 0) {
+            if (go) {
+                c->beta = beta;
+                c->rr = rr_new;
+                c->ru = ru;
+                c->f_rr[(t + 1) & 1] = rr_new;
+                c->f_rr_ru[(t + 1) & 1] = rr_ru;
+                c->f_ru[(t + 1) & 1] = ru;
+            } else {
+                c->iters = t + 1;
+                c->res = sqrt(rr_new);
+                c->done = 1;
+            }
+        }
+        sh_beta = beta;
+        sh_go = go;
+        sh_ru = ru;
+    }
+    __syncthreads();
+    if (!sh_go) return;
+    const int c0 = blockIdx.x * PU_R;
+    const int nch = min(PU_R, (int)nblk - c0);
+    pupdate_chunks<U, V>(P, b, x, r, p, n, c0, nch, (V)sh_beta, sh_ru, part_ru);
+}
+
 struct MixedBufs {
     MixedCtrl *d = nullptr, *h = nullptr;
     double *part_ru = nullptr;
@@ -1119,6 +1239,7 @@ static int cg_mixed_run(Ctx *c, const SolveOpts &o, SolveResult *res, V *r, V *p
     typedef void (*q_fn)(int, PkView, const unsigned *, QConsts, const float *, float *, double *, const MixedCtrl *);
     const q_fn spmv_q = o.spmv_variant == 1 ? k_mx_spmv_q<16, true>
                                             : (o.spmv_variant == 2 ? k_mx_spmv_q<16, false> : k_mx_spmv_q<8, true>);
+    const bool three = o.fused == 2;
     int rc = run_chunks(s, o.max_iter, o.check_every, &mb->d->done, &mb->h->done, [&](int t) {
         prof_begin(PK_SPMV, s);
         if constexpr (sizeof(V) == sizeof(float)) {
@@ -1130,6 +1251,17 @@ static int cg_mixed_run(Ctx *c, const SolveOpts &o, SolveResult *res, V *r, V *p
             hipLaunchKernelGGL(spmv, dim3(g), dim3(CG_BLK), lds, s, n, P, p, Ap, c->partA, mb->d);
         }
         prof_end(PK_SPMV, s);
+        if (three) {
+            prof_begin(PK_AXPY, s);
+            hipLaunchKernelGGL(k_mx3_axpy<V>, dim3(gv), dim3(CG_BLK), 0, s, n, t, p, Ap, x, r, c->partA, mb->part_ru,
+                               g, c->partB, mb->d);
+            prof_end(PK_AXPY, s);
+            prof_begin(PK_PUPDATE, s);
+            hipLaunchKernelGGL((k_mx3_pupdate<8, V>), dim3(pu_grid(g)), dim3(CG_BLK), 0, s, n, g, t, o.max_iter, P,
+                               pk->b, x, r, p, c->partB, gv, mb->part_ru, mb->d);
+            prof_end(PK_PUPDATE, s);
+            return;
+        }
         hipLaunchKernelGGL(k_mx_fin1, dim3(1), dim3(CG_BLK), 0, s, c->partA, g, mb->part_ru, mb->d, t);
         prof_begin(PK_AXPY, s);
         hipLaunchKernelGGL(k_mx_axpy<V>, dim3(gv), dim3(CG_BLK), 0, s, n, p, Ap, x, r, c->partB, mb->d);

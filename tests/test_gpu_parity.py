@@ -252,29 +252,33 @@ def test_spmv_variants_same_bits(lib3):
     assert all(o == out[0] for o in out)
 
 
-def test_fused_cg_same_bits(lib3):
-    """The fused CG iterations give the five-kernel form's bits: two kernels
-    (previous iteration closed inside the SpMV, p formed where it is read)
-    and three (each finaliser's reduction repeated by the blocks of the next
-    vector kernel)."""
+@pytest.mark.parametrize("precision", [0, 1, 2, 3])
+def test_fused_cg_same_bits(lib3, precision):
+    """The fused CG iterations give the five-kernel form's bits.  FP64: two
+    kernels (previous iteration closed inside the SpMV, p formed where it
+    is read) and three (each finaliser's reduction repeated by the blocks
+    of the next vector kernel).  Mixed forms: three kernels, the
+    reliable-update fold and the stop tests repeated the same way."""
     dim, cfg, pos, vel, typ = H.build("dam3d_mid", cg_tolerance=1e-10)
     lib3.set_option("tol_mode", 1)
+    lib3.set_option("precision", precision)
+    forms = (0, 1, 2) if precision == 0 else (0, 2)
     out = []
     try:
-        for f in (0, 1, 2):
+        for f in forms:
             lib3.set_option("fused", f)
-            say("fused", f)
+            say("precision", precision, "fused", f)
             g = H.GpuRun(lib3, cfg, pos, vel, typ)
             for _ in range(2):
                 assert g.step() == 0
             st = lib3.stats(g.scene)
             out.append((np.ascontiguousarray(g.scene.arr["pressure"]).tobytes(), st.last_iters,
-                        st.last_residual))
+                        st.last_residual, st.last_outer, st.total_iters))
             g.close()
     finally:
-        lib3.set_option("fused", 0)
-    assert out[0] == out[1]
-    assert out[0] == out[2]
+        lib3.set_option("fused", 2)
+        lib3.set_option("precision", 0)
+    assert all(o == out[0] for o in out)
 
 
 @pytest.mark.parametrize("name,libname", [("dam3d_mid", "lib3"), ("dam2d_l0.01", "lib2")])
