@@ -504,6 +504,8 @@ def compare_forms(lib, ps, n, w, a, precision=None, emit=True):
         st = lib.stats(q)
         out[name] = dict(rate=n * (a.steps - 1) / el, ms=1e3 * el / (a.steps - 1),
                          iters=st.total_iters / max(st.steps, 1), fields=_fields(lib, q, n))
+        print("bench: %s form, %d steps, %.3f ms/step" % (name, a.steps, out[name]["ms"]), file=sys.stderr,
+              flush=True)
         lib.mps_hip_release(q)
         lib.particle_system_free(q)
     names = list(out)
