@@ -224,6 +224,7 @@ int cg_alloc_ctrl(Ctx *c);
 // FP64 CG over the packed systems of several contexts at once (k_cg.hip)
 int batch_cg(Ctx *const *cs, int ns, const SolveOpts *opts, SolveResult *res, hipStream_t s);
 int batch_cg_mixed(Ctx *const *cs, int ns, const SolveOpts *opts, SolveResult *res, hipStream_t s);
+void cg_batch_free();
 int pack_ppe(Ctx *c, int with_f32);        // k_pack.hip
 int pack_stage_a(Ctx *c);                   // pack_ppe = a, wait, b, wait, c
 int pack_stage_b(Ctx *c);

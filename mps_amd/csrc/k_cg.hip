@@ -2071,3 +2071,25 @@ int batch_cg_mixed(Ctx *const *cs, int ns, const SolveOpts *opts, SolveResult *r
     if (prec >= 2) return batch_mixed_run<float>(cs, ns, opts, res, s);
     return batch_mixed_run<double>(cs, ns, opts, res, s);
 }
+
+// Frees the batched solvers' block tables, partials and control blocks
+// (mps_hip_finalize); the next batch allocates them again.
+void cg_batch_free()
+{
+    BatchBufs &B = g_batch;
+    void *dev[] = {B.sc, B.bs, B.axs, B.part, B.part2, B.ctrl, B.ndone};
+    void *host[] = {B.hsc, B.hbs, B.haxs, B.hctrl, B.hndone};
+    for (void *p : dev)
+        if (p) hipFree(p);
+    for (void *p : host)
+        if (p) hipHostFree(p);
+    B = BatchBufs();
+    BatchMixedBufs &M = g_bmixed;
+    void *mdev[] = {M.sc, M.pbs, M.ctrl, M.part3, M.part_ru};
+    void *mhost[] = {M.hsc, M.hpbs, M.hctrl};
+    for (void *p : mdev)
+        if (p) hipFree(p);
+    for (void *p : mhost)
+        if (p) hipHostFree(p);
+    M = BatchMixedBufs();
+}

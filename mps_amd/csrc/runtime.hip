@@ -743,6 +743,7 @@ void mps_hip_finalize(void)
         delete kv.second;
     }
     g_ctx.clear();
+    cg_batch_free();
 }
 
 int mps_hip_last_error(void) { return g_last_error; }

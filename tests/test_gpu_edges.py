@@ -225,3 +225,44 @@ def test_batch_with_empty_scene(lib3, precision):
         assert single[0][1] == 0
     finally:
         lib3.set_option("precision", 0)
+
+
+def test_finalize_then_batch_again(lib3):
+    """mps_hip_finalize frees every context and the batched solvers'
+    buffers; a batch run afterwards allocates them again and gives the
+    bits of the same batch run before."""
+    lib3.set_option("tol_mode", 1)
+    lib3.set_option("precision", 3)
+    try:
+        dim, cfg_a, pa, va, ta = H.build("dam3d_small", cg_tolerance=1e-10)
+        dim, cfg_b, pb, vb, tb = H.build("dam3d_small", cg_tolerance=1e-10, viscosity=1e-4)
+        o = H.OracleRun(cfg_a, pa, va, ta, dim)
+        n0, lam = o.scene.ps.n0, o.scene.ps.lambda_
+
+        def run():
+            scs = []
+            for cfg, pos, vel, typ in ((cfg_a, pa, va, ta), (cfg_b, pb, vb, tb)):
+                lib3.set_config(cfg)
+                sc = A.HostScene(lib3.abi, pos, vel, typ)
+                sc.ps.n0 = n0
+                sc.ps.lambda_ = lam
+                scs.append(sc)
+            PSP = C.POINTER(lib3.abi.ParticleSystem)
+            ps_arr = (PSP * 2)(*[C.pointer(s.ps) for s in scs])
+            cf_arr = (C.POINTER(lib3.abi.SimConfig) * 2)(C.pointer(cfg_a), C.pointer(cfg_b))
+            lib3.check(lib3.mps_hip_run_batch(ps_arr, cf_arr, 2, 2), "run_batch")
+            out = []
+            for sc in scs:
+                lib3.check(lib3.mps_hip_download(sc.ptr(), None), "download")
+                out.append(_fields(sc))
+                lib3.mps_hip_release(sc.ptr())
+            return out
+
+        say("batch before finalize")
+        first = run()
+        lib3.mps_hip_finalize()
+        say("batch after finalize")
+        second = run()
+        assert first == second
+    finally:
+        lib3.set_option("precision", 0)
