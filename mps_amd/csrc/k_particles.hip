@@ -428,6 +428,12 @@ int launch_friction(Ctx *c, const StepConsts &k)
     return 0;
 }
 
+
+// Neighbour loops load a batch of list entries, then the neighbours' data,
+// before summing in list order: the gathers of a batch overlap.
+#define ASM_U 8    // neighbours per load batch of the assembly
+#define NB_U 4     // of the viscosity and pressure-gradient sums
+
 __global__ void __launch_bounds__(BLK)
 k_visc_predict(int n, const dvec *__restrict__ X, const dvec *__restrict__ V, const int *__restrict__ T,
                const int *__restrict__ nbr, const int *__restrict__ ncnt, dvec *__restrict__ Xo,
@@ -445,13 +451,26 @@ k_visc_predict(int n, const dvec *__restrict__ X, const dvec *__restrict__ V, co
 #pragma unroll
     for (int d = 0; d < D_; d++) lap[d] = 0.0;
     const int m = ncnt[i];
-    for (int q = 0; q < m; q++) {
-        const int j = nbr[sell_off(i, q, k.stride)];
-        const dvec xj = X[j], vj = V[j];
-        double dr[D_];
-        const double w = wfun(sqrt(dist2(xi, xj, dr)), k.re_lap, k.r_min);
+    // NB_U neighbours at a time: indices, then positions and velocities,
+    // loaded together; summed in list order
+    for (int q0 = 0; q0 < m; q0 += NB_U) {
+        int jj[NB_U];
+        dvec xx[NB_U], vv[NB_U];
 #pragma unroll
-        for (int d = 0; d < D_; d++) lap[d] += (vj.c[d] - vi.c[d]) * w;
+        for (int u = 0; u < NB_U; u++) jj[u] = (q0 + u < m) ? nbr[sell_off(i, q0 + u, k.stride)] : i;
+#pragma unroll
+        for (int u = 0; u < NB_U; u++) {
+            xx[u] = X[jj[u]];
+            vv[u] = V[jj[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < NB_U; u++) {
+            if (q0 + u >= m) break;
+            double dr[D_];
+            const double w = wfun(sqrt(dist2(xi, xx[u], dr)), k.re_lap, k.r_min);
+#pragma unroll
+            for (int d = 0; d < D_; d++) lap[d] += (vv[u].c[d] - vi.c[d]) * w;
+        }
     }
     double acc[D_];
 #pragma unroll
@@ -921,7 +940,34 @@ k_assemble(int n, const dvec *__restrict__ X, const dvec *__restrict__ V, const 
             double hs = 0.0, div = 0.0;
             int m = 0;
             const int cnt = ncnt[i];
-            for (int q = 0; q < cnt; q++) {
+            if (!need_v) {
+                // the common case, eight neighbours at a time: their
+                // indices, then their flags and positions, are loaded
+                // together (the gathers overlap); the entries are formed
+                // and stored in list order as below
+                for (int q0 = 0; q0 < cnt; q0 += ASM_U) {
+                    int jj[ASM_U], ok[ASM_U];
+                    dvec xx[ASM_U];
+#pragma unroll
+                    for (int u = 0; u < ASM_U; u++) jj[u] = (q0 + u < cnt) ? nbr[sell_off(i, q0 + u, k.stride)] : i;
+#pragma unroll
+                    for (int u = 0; u < ASM_U; u++) ok[u] = q0 + u < cnt && unk[jj[u]] && T[jj[u]] != DUMMY_PARTICLE;
+#pragma unroll
+                    for (int u = 0; u < ASM_U; u++)
+                        if (ok[u]) xx[u] = X[jj[u]];
+#pragma unroll
+                    for (int u = 0; u < ASM_U; u++) {
+                        if (!ok[u]) continue;
+                        double dr[D_];
+                        const double r2 = dist2(xi, xx[u], dr);
+                        const double w = wfun(sqrt(r2), k.re_lap, k.r_min);
+                        Aval[sell_off(i, m, k.stride)] = k.neg_coeff * w;
+                        Acol[sell_off(i, m, k.stride)] = jj[u];
+                        m++;
+                    }
+                }
+            }
+            for (int q = 0; need_v && q < cnt; q++) {
                 const int j = nbr[sell_off(i, q, k.stride)];
                 const int uj = unk[j];
                 if (!uj && !need_v) continue;
@@ -1049,27 +1095,46 @@ k_grad_correct(int n, GradIO io, const int *__restrict__ nbr, const int *__restr
             pimin = io.pmin[i];
         } else if (k.cmps == 0) {
             pimin = pi;
-            for (int q = 0; q < cnt; q++) {
-                const double pj = io.P[nbr[sell_off(i, q, k.stride)]];
-                if (pj < pimin) pimin = pj;
+            for (int q0 = 0; q0 < cnt; q0 += 2 * NB_U) {
+                double pp[2 * NB_U];
+#pragma unroll
+                for (int u = 0; u < 2 * NB_U; u++)
+                    pp[u] = (q0 + u < cnt) ? io.P[nbr[sell_off(i, q0 + u, k.stride)]] : pimin;
+#pragma unroll
+                for (int u = 0; u < 2 * NB_U; u++)
+                    if (pp[u] < pimin) pimin = pp[u];
             }
         }
         double grad[D_];
 #pragma unroll
         for (int d = 0; d < D_; d++) grad[d] = 0.0;
-        for (int q = 0; q < cnt; q++) {
-            const int j = nbr[sell_off(i, q, k.stride)];
-            double dr[D_];
-            const double r2 = dist2(xi, io.X[j], dr);
-            if (r2 < k.grad_min_r2) continue;
-            const double w = wfun(sqrt(r2), k.re_lap, k.r_min);
-            const double pj = io.P[j];
-            double dp;
-            if (k.cmps == 1) dp = pi + pj - pimin - io.pmin[j];
-            else if (k.cmps == 2) dp = pi + pj;
-            else dp = pj - pimin;
+        // NB_U neighbours at a time, summed in list order
+        for (int q0 = 0; q0 < cnt; q0 += NB_U) {
+            int jj[NB_U];
+            dvec xx[NB_U];
+            double pp[NB_U];
 #pragma unroll
-            for (int d = 0; d < D_; d++) grad[d] += dp / r2 * dr[d] * w;
+            for (int u = 0; u < NB_U; u++) jj[u] = (q0 + u < cnt) ? nbr[sell_off(i, q0 + u, k.stride)] : i;
+#pragma unroll
+            for (int u = 0; u < NB_U; u++) {
+                xx[u] = io.X[jj[u]];
+                pp[u] = io.P[jj[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < NB_U; u++) {
+                if (q0 + u >= cnt) break;
+                double dr[D_];
+                const double r2 = dist2(xi, xx[u], dr);
+                if (r2 < k.grad_min_r2) continue;
+                const double w = wfun(sqrt(r2), k.re_lap, k.r_min);
+                const double pj = pp[u];
+                double dp;
+                if (k.cmps == 1) dp = pi + pj - pimin - io.pmin[jj[u]];
+                else if (k.cmps == 2) dp = pi + pj;
+                else dp = pj - pimin;
+#pragma unroll
+                for (int d = 0; d < D_; d++) grad[d] += dp / r2 * dr[d] * w;
+            }
         }
 #pragma unroll
         for (int d = 0; d < D_; d++) {
