@@ -537,22 +537,35 @@ k_collide_bin(int n, const dvec *__restrict__ X, const dvec *__restrict__ V, con
 #pragma unroll
         for (int d = 0; d < D_; d++) w[d] = vi.c[d];
         const int m = ncnt[i];
-        for (int q = 0; q < m; q++) {
-            const int j = nbr[sell_off(i, q, k.stride)];
-            if (T[j] == GHOST_PARTICLE) continue;
-            double dr[D_];
-            const double r2 = dist2(xi, X[j], dr);
-            if (r2 >= k.cd2) continue;
-            const double r = sqrt(r2);
-            if (r < k.col_tiny) continue;
-            const dvec vj = V[j];
-            double vn = 0.0;
+        for (int q0 = 0; q0 < m; q0 += NB_U * 2) {
+            int jj[NB_U * 2], tt[NB_U * 2];
+            dvec xx[NB_U * 2];
 #pragma unroll
-            for (int d = 0; d < D_; d++) vn += (w[d] - vj.c[d]) * (dr[d] / r);
-            if (vn <= 0.0) continue;
-            const double imp = k.e_half * vn;
+            for (int u = 0; u < NB_U * 2; u++) jj[u] = (q0 + u < m) ? nbr[sell_off(i, q0 + u, k.stride)] : i;
 #pragma unroll
-            for (int d = 0; d < D_; d++) w[d] -= imp * (dr[d] / r);
+            for (int u = 0; u < NB_U * 2; u++) {
+                tt[u] = T[jj[u]];
+                xx[u] = X[jj[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < NB_U * 2; u++) {
+                if (q0 + u >= m) break;
+                const int j = jj[u];
+                if (tt[u] == GHOST_PARTICLE) continue;
+                double dr[D_];
+                const double r2 = dist2(xi, xx[u], dr);
+                if (r2 >= k.cd2) continue;
+                const double r = sqrt(r2);
+                if (r < k.col_tiny) continue;
+                const dvec vj = V[j];
+                double vn = 0.0;
+#pragma unroll
+                for (int d = 0; d < D_; d++) vn += (w[d] - vj.c[d]) * (dr[d] / r);
+                if (vn <= 0.0) continue;
+                const double imp = k.e_half * vn;
+#pragma unroll
+                for (int d = 0; d < D_; d++) w[d] -= imp * (dr[d] / r);
+            }
         }
 #pragma unroll
         for (int d = 0; d < D_; d++) {
@@ -850,23 +863,34 @@ k_search(int n, const dvec *__restrict__ X, const int *__restrict__ T, const int
                     for (int cxx = lo[0]; cxx <= hi[0]; cxx++) {
                         const int cid = cell_key(k.grid, cxx, cy, cz);
                         const int j1 = cstart[cid + 1];
-                        for (int j = cstart[cid]; j < j1; j++) {
-                            if (j == i) continue;
-                            double dr[D_];
-                            const double r2 = dist2(xi, X[j], dr);
-                            if (!(r2 < k.re2)) continue;
-                            if (cnt < k.stride) out.nbr[sell_off(i, cnt, k.stride)] = j;
-                            cnt++;
-                            if (with_density) {
-                                const int tj = T[j];
-                                const bool need_n = r2 <= k.re_n2_hi;
-                                const bool need_l = tj != DUMMY_PARTICLE;
-                                if (need_n || need_l) {
-                                    const double r = sqrt(r2);
-                                    if (need_n) nsum += wfun(r, k.re_n, k.r_min);
-                                    if (need_l) sumw += wfun(r, k.re_lap, k.r_min);
+                        // the cell's candidates NB_U at a time: positions
+                        // loaded together, tested and listed in order
+                        for (int j0 = cstart[cid]; j0 < j1; j0 += NB_U) {
+                            dvec xx[NB_U];
+#pragma unroll
+                            for (int u = 0; u < NB_U; u++)
+                                if (j0 + u < j1) xx[u] = X[j0 + u];
+#pragma unroll
+                            for (int u = 0; u < NB_U; u++) {
+                                const int j = j0 + u;
+                                if (j >= j1) break;
+                                if (j == i) continue;
+                                double dr[D_];
+                                const double r2 = dist2(xi, xx[u], dr);
+                                if (!(r2 < k.re2)) continue;
+                                if (cnt < k.stride) out.nbr[sell_off(i, cnt, k.stride)] = j;
+                                cnt++;
+                                if (with_density) {
+                                    const int tj = T[j];
+                                    const bool need_n = r2 <= k.re_n2_hi;
+                                    const bool need_l = tj != DUMMY_PARTICLE;
+                                    if (need_n || need_l) {
+                                        const double r = sqrt(r2);
+                                        if (need_n) nsum += wfun(r, k.re_n, k.r_min);
+                                        if (need_l) sumw += wfun(r, k.re_lap, k.r_min);
+                                    }
+                                    cnt_n += (r2 < k.re_n2);
                                 }
-                                cnt_n += (r2 < k.re_n2);
                             }
                         }
                     }
