@@ -28,6 +28,7 @@
 
 #define PKB 256
 #define BM_WORDS 4096     // bitmap of 2^17 rows
+#define PK_U 8            // row entries per load batch (gathers overlap)
 static_assert(PKB == PK_ROWS, "pack block = SpMV block");
 
 static inline unsigned pk_blocks(size_t n) { return (unsigned)((n + PKB - 1) / PKB); }
@@ -93,10 +94,17 @@ __device__ __forceinline__ BlockRows block_rows(int n_eq, int stride, const int 
     if (br.k < n_eq) {
         br.slot = slot_of_row[br.k];
         br.m = Alen[br.slot];
-        for (int q = 0; q < br.m; q++) {
-            const int cr = row_of_slot[Acol[sell_off(br.slot, q, stride)]];
-            mn = cr < mn ? cr : mn;
-            mx = cr > mx ? cr : mx;
+        for (int q0 = 0; q0 < br.m; q0 += PK_U) {
+            int cs[PK_U];
+#pragma unroll
+            for (int u = 0; u < PK_U; u++) cs[u] = (q0 + u < br.m) ? Acol[sell_off(br.slot, q0 + u, stride)] : -1;
+#pragma unroll
+            for (int u = 0; u < PK_U; u++) {
+                if (cs[u] < 0) continue;
+                const int cr = row_of_slot[cs[u]];
+                mn = cr < mn ? cr : mn;
+                mx = cr > mx ? cr : mx;
+            }
         }
     }
     mn = wave_min_i(mn);
@@ -124,9 +132,16 @@ __device__ __forceinline__ void block_bitmap(const BlockRows &br, int stride, co
 {
     for (int w = threadIdx.x; w < BM_WORDS; w += PKB) bits[w] = 0u;
     __syncthreads();
-    for (int q = 0; q < br.m; q++) {
-        const int c = row_of_slot[Acol[sell_off(br.slot, q, stride)]] - br.bmn;
-        atomicOr(&bits[c >> 5], 1u << (c & 31));
+    for (int q0 = 0; q0 < br.m; q0 += PK_U) {
+        int cs[PK_U];
+#pragma unroll
+        for (int u = 0; u < PK_U; u++) cs[u] = (q0 + u < br.m) ? Acol[sell_off(br.slot, q0 + u, stride)] : -1;
+#pragma unroll
+        for (int u = 0; u < PK_U; u++) {
+            if (cs[u] < 0) continue;
+            const int c = row_of_slot[cs[u]] - br.bmn;
+            atomicOr(&bits[c >> 5], 1u << (c & 31));
+        }
     }
     __syncthreads();
 }
@@ -224,34 +239,49 @@ k_pk_pass2(int n_eq, int n_slices, int stride, const int *__restrict__ slot_of_r
     const int co = mode ? chwoff[sl] : c32off[sl];
     const int m = br.m;
     unsigned g0 = 0u, g1 = 0u, g2 = 0u;      // the row's current group of eight 12-bit indices
-    for (int q = 0; q < m; q++) {
-        const size_t o = sell_off(br.slot, q, stride);
-        const int cr = row_of_slot[Acol[o]];
-        val[(size_t)(vo + q) * 64u + lane] = Aval[o];
-        if (mode) {
-            const int c = cr - br.bmn;
-            const unsigned below = bits[c >> 5] & ((1u << (c & 31)) - 1u);
-            const unsigned li = (unsigned)(pre[c >> 5] + __popc(below));   // < PK_HALO_CAP
-            const int bit = 12 * (q & 7), wd = bit >> 5, sh = bit & 31;
-            const unsigned lo = li << sh, hi = sh > 20 ? li >> (32 - sh) : 0u;
-            if (wd == 0) {
-                g0 |= lo;
-                g1 |= hi;
-            } else if (wd == 1) {
-                g1 |= lo;
-                g2 |= hi;
+    for (int q0 = 0; q0 < m; q0 += PK_U) {
+        int crs[PK_U];
+        double avs[PK_U];
+#pragma unroll
+        for (int u = 0; u < PK_U; u++) {
+            const size_t o = sell_off(br.slot, q0 + u, stride);
+            crs[u] = (q0 + u < m) ? Acol[o] : 0;
+            avs[u] = (q0 + u < m) ? Aval[o] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < PK_U; u++)
+            if (q0 + u < m) crs[u] = row_of_slot[crs[u]];
+#pragma unroll
+        for (int u = 0; u < PK_U; u++) {
+            const int q = q0 + u;
+            if (q >= m) break;
+            const int cr = crs[u];
+            val[(size_t)(vo + q) * 64u + lane] = avs[u];
+            if (mode) {
+                const int c = cr - br.bmn;
+                const unsigned below = bits[c >> 5] & ((1u << (c & 31)) - 1u);
+                const unsigned li = (unsigned)(pre[c >> 5] + __popc(below));   // < PK_HALO_CAP
+                const int bit = 12 * (q & 7), wd = bit >> 5, sh = bit & 31;
+                const unsigned lo = li << sh, hi = sh > 20 ? li >> (32 - sh) : 0u;
+                if (wd == 0) {
+                    g0 |= lo;
+                    g1 |= hi;
+                } else if (wd == 1) {
+                    g1 |= lo;
+                    g2 |= hi;
+                } else {
+                    g2 |= lo;
+                }
+                if ((q & 7) == 7 || q == m - 1) {
+                    const size_t w = (size_t)(co + (q >> 3) * 3) * 64u + lane;
+                    chw[w] = g0;
+                    chw[w + 64u] = g1;
+                    chw[w + 128u] = g2;
+                    g0 = g1 = g2 = 0u;
+                }
             } else {
-                g2 |= lo;
+                c32[(size_t)(co + q) * 64u + lane] = cr;
             }
-            if ((q & 7) == 7 || q == m - 1) {
-                const size_t w = (size_t)(co + (q >> 3) * 3) * 64u + lane;
-                chw[w] = g0;
-                chw[w + 64u] = g1;
-                chw[w + 128u] = g2;
-                g0 = g1 = g2 = 0u;
-            }
-        } else {
-            c32[(size_t)(co + q) * 64u + lane] = cr;
         }
     }
     len[br.k] = m;
