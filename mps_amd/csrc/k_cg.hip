@@ -174,16 +174,6 @@ __device__ __forceinline__ unsigned idx_of(const unsigned *g, int u)
     return v & 0xfffu;
 }
 
-// entry q of a row whose first index word is wrow (stride 64 words)
-__device__ __forceinline__ unsigned idx_at(const unsigned *__restrict__ wrow, int q)
-{
-    const int bit = 12 * (q & 7), wd = bit >> 5, sh = bit & 31;
-    const unsigned *w = wrow + (size_t)((q >> 3) * IDX_GROUP_WORDS + wd) * 64u;
-    unsigned v = w[0] >> sh;
-    if (sh > 20) v |= w[64] << (32 - sh);
-    return v & 0xfffu;
-}
-
 // The same sum with 12-bit halo indices: x is read from `hx`, either the
 // block's halo staged in LDS or, with INDIRECT, global x at
 // halo[hbase + index].  U is a multiple of 8 (whole index groups).
@@ -873,14 +863,26 @@ __device__ __forceinline__ void q16_build_row(int k, const PkView &P, double neg
     const size_t vb = (size_t)mt.x * 64u + lane;
     const unsigned *wrow = P.chw + (size_t)mt.y * 64u + lane;
     const double dr = (re - r_min) / Q_CODE_MAX;
-    for (int q = 0; q < m; q++) {
-        const unsigned idx = idx_at(wrow, q);
-        const double w = P.val[vb + (size_t)q * 64u] / neg_coeff;
-        double r = re / (w + 1.0);
-        if (r < r_min) r = r_min;
-        double code = rint((r - r_min) / dr);
-        code = code < 0.0 ? 0.0 : (code > Q_CODE_MAX ? Q_CODE_MAX : code);
-        cq[vb + (size_t)q * 64u] = ((unsigned)code << 12) | (idx & 0xfffu);
+    // a group of eight entries at a time: values and index words first
+    for (int q0 = 0; q0 < m; q0 += 8) {
+        double a[8];
+        unsigned g[IDX_GROUP_WORDS];
+#pragma unroll
+        for (int u = 0; u < 8; u++) a[u] = (q0 + u < m) ? P.val[vb + (size_t)(q0 + u) * 64u] : 0.0;
+#pragma unroll
+        for (int j = 0; j < IDX_GROUP_WORDS; j++)
+            g[j] = (q0 + idx_first(j) < m) ? wrow[(size_t)(q0 / 8 * IDX_GROUP_WORDS + j) * 64u] : 0u;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            if (q0 + u >= m) break;
+            const unsigned idx = idx_of(g, u);
+            const double w = a[u] / neg_coeff;
+            double r = re / (w + 1.0);
+            if (r < r_min) r = r_min;
+            double code = rint((r - r_min) / dr);
+            code = code < 0.0 ? 0.0 : (code > Q_CODE_MAX ? Q_CODE_MAX : code);
+            cq[vb + (size_t)(q0 + u) * 64u] = ((unsigned)code << 12) | (idx & 0xfffu);
+        }
     }
 }
 

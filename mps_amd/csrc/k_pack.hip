@@ -300,10 +300,16 @@ __global__ void k_pk_f32(int n_eq, const int *__restrict__ len, const int4 *__re
     const int m = len[k];
     const size_t base = (size_t)smeta[k >> 6].x * 64u + (size_t)(k & 63);
     double d = pdiag[k];
-    for (int q = 0; q < m; q++) {
-        const double a = val[base + (size_t)q * 64u];
-        fval[base + (size_t)q * 64u] = (float)a;
-        d += a;
+    for (int q0 = 0; q0 < m; q0 += PK_U) {
+        double a[PK_U];
+#pragma unroll
+        for (int u = 0; u < PK_U; u++) a[u] = (q0 + u < m) ? val[base + (size_t)(q0 + u) * 64u] : 0.0;
+#pragma unroll
+        for (int u = 0; u < PK_U; u++) {
+            if (q0 + u >= m) break;
+            fval[base + (size_t)(q0 + u) * 64u] = (float)a[u];
+            d += a[u];
+        }
     }
     ddir[k] = d;
 }
