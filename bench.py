@@ -93,7 +93,8 @@ def spmv_bytes(st, s_val=8, s_vec=8):
     """Algorithmic bytes of one SpMV y = A x over the reference's CSR:
     Z (value + 4-byte column) + 4 (n_eq + 1) row pointers + x read once and
     y written once (Z = nonzeros with the diagonal).  The packed matrix the
-    kernel reads stores 16-bit columns, so it moves fewer bytes than this."""
+    kernel reads stores 12-bit halo indices, so it moves fewer bytes than
+    this."""
     z, ne = st.nnz, st.n_eq
     return z * (s_val + 4) + 4 * (ne + 1) + 2 * s_vec * ne
 
@@ -255,7 +256,8 @@ def run_ensemble(lib, a):
         res = mps_amd.TuneResult()
         lib.check(lib.mps_hip_autotune(big[1], 1, C.byref(res)), "autotune")
         tune = {"scene_particles": big[2], "spmv": res.spmv, "check_every": res.check_every,
-                "spmv_ms": [round(v, 5) for v in res.spmv_ms], "solve_ms": [round(v, 2) for v in res.solve_ms]}
+                "spmv_ms": [round(v, 5) for v in res.spmv_ms], "solve_ms": [round(v, 2) for v in res.solve_ms],
+                "pblk": res.pblk, "pblk_ms": [round(v, 3) for v in res.pblk_ms]}
     PSP = C.POINTER(lib.abi.ParticleSystem)
     ps_arr = (PSP * len(scenes))(*[m[1] for m in scenes])
     cf_arr = (C.POINTER(lib.abi.SimConfig) * len(scenes))(*[C.pointer(m[0]) for m in scenes])
@@ -662,6 +664,7 @@ def main():
         lib.check(lib.mps_hip_autotune(ps, 1, C.byref(res)), "autotune")
         tune = {"spmv": res.spmv, "check_every": res.check_every,
                 "spmv_ms": [round(v, 5) for v in res.spmv_ms], "solve_ms": [round(v, 2) for v in res.solve_ms],
+                "pblk": res.pblk, "pblk_ms": [round(v, 3) for v in res.pblk_ms],
                 "seconds": round(time.perf_counter() - t_tune, 2)}
     lib.check(lib.mps_hip_run_steps(ps, a.warmup), "warm-up steps")
     # the timed steps time only the SpMV launches (two HIP events each)

@@ -274,6 +274,9 @@ int  mps_hip_step_backward(ParticleSystem *ps, NeighborList *nl, const double *r
  *                      tape records (0, default: plain launches)
  *   "profile"        1 times the main kernels with HIP events, 2 only the
  *                      CG SpMV
+ *   "pblk"           block size of the per-particle kernels (collision,
+ *                      predictor, search, assembly, gradient): 128, 256
+ *                      (default) or 512; same results bit for bit
  * Returns 0, or MPS_ERR_ARG for an unknown key. */
 int    mps_hip_set_option(const char *key, double value);
 double mps_hip_get_option(const char *key);
@@ -293,20 +296,25 @@ typedef struct {
 } mps_step_stats;
 int  mps_hip_get_stats(ParticleSystem *ps, mps_step_stats *out);
 
-/* Autotuner (no counterpart in the reference): times the choices of the
- * FP64 CG that leave every result bit unchanged -- the SpMV's entries per
- * load batch (option "spmv") and the host's polling interval (option
- * "check_every") -- on a private device copy of ps's host particles under
- * g_config (one step's PPE, solved once per candidate).  ps and its device
- * state are not touched.  With apply != 0 the fastest choices become the
- * options.  Returns 0 or an error code. */
+/* Autotuner (no counterpart in the reference): times the choices that
+ * leave every result bit unchanged -- the CG SpMV kernel (option "spmv"),
+ * the host's polling interval (option "check_every") and the block size of
+ * the per-particle kernels (option "pblk") -- on a private device copy of
+ * ps's host particles under g_config (one step's PPE, solved once per
+ * candidate; the particle phases three times per block size).  ps and its
+ * device state are not touched.  With apply != 0 the fastest choices
+ * become the options.  Returns 0 or an error code. */
 typedef struct {
     int    spmv;            /* fastest "spmv" variant                      */
     int    check_every;     /* fastest "check_every"                       */
     double spmv_ms[3];      /* mean SpMV launch, variants 0 (8 entries per
-                               batch), 1 (16), 2 (4)                       */
+                               batch, non-temporal), 1 (16, non-temporal),
+                               2 (16, through the caches)                  */
     double solve_ms[4];     /* one solve polling every 8, 16, 32, 64 iters */
     int    iters;           /* CG iterations of that solve                 */
+    int    pblk;            /* fastest "pblk"                              */
+    double pblk_ms[3];      /* particle phases of a step at block sizes
+                               128, 256, 512                               */
 } mps_tune_result;
 int  mps_hip_autotune(ParticleSystem *ps, int apply, mps_tune_result *out);
 

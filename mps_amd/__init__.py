@@ -30,7 +30,8 @@ class StepStats(C.Structure):
 
 class TuneResult(C.Structure):
     _fields_ = [("spmv", C.c_int), ("check_every", C.c_int), ("spmv_ms", C.c_double * 3),
-                ("solve_ms", C.c_double * 4), ("iters", C.c_int)]
+                ("solve_ms", C.c_double * 4), ("iters", C.c_int), ("pblk", C.c_int),
+                ("pblk_ms", C.c_double * 3)]
 
 
 _libs = {}

@@ -34,6 +34,7 @@
 
 void mps_set_error(int code, const char *file, int line, const char *msg);
 void mps_clear_error(void);
+int particle_block();                       // block size of the per-particle kernels (runtime.hip)
 
 // Step constants computed on the host in the reference's association order
 // (host and device FP64 arithmetic are both IEEE, so precomputed products

@@ -23,6 +23,21 @@
 
 static inline unsigned nblk(size_t n, unsigned b = BLK) { return (unsigned)((n + b - 1) / b); }
 
+// The per-particle kernels (collision, predictor, search, assembly,
+// gradient) at the block size particle_block() names (128, 256 or 512;
+// option "pblk"): one thread per particle and integer block counters, so
+// every size gives the same bits.
+#define PARTICLE_LAUNCH(kern, c, ...)                                                                   \
+    do {                                                                                              \
+        const int pb_ = particle_block();                                                             \
+        if (pb_ == 128)                                                                               \
+            hipLaunchKernelGGL(kern<128>, dim3(nblk((c)->n, 128)), dim3(128), 0, (c)->stream, __VA_ARGS__); \
+        else if (pb_ == 512)                                                                          \
+            hipLaunchKernelGGL(kern<512>, dim3(nblk((c)->n, 512)), dim3(512), 0, (c)->stream, __VA_ARGS__); \
+        else                                                                                          \
+            hipLaunchKernelGGL(kern<256>, dim3(nblk((c)->n, 256)), dim3(256), 0, (c)->stream, __VA_ARGS__); \
+    } while (0)
+
 #define LAUNCH_CHECK()                                                       \
     do {                                                                     \
         hipError_t e_ = hipGetLastError();                                   \
@@ -434,12 +449,13 @@ int launch_friction(Ctx *c, const StepConsts &k)
 #define ASM_U 8    // neighbours per load batch of the assembly
 #define NB_U 4     // of the viscosity and pressure-gradient sums
 
-__global__ void __launch_bounds__(BLK)
+template <int B>
+__global__ void __launch_bounds__(B)
 k_visc_predict(int n, const dvec *__restrict__ X, const dvec *__restrict__ V, const int *__restrict__ T,
                const int *__restrict__ nbr, const int *__restrict__ ncnt, dvec *__restrict__ Xo,
                dvec *__restrict__ Vo, StepConsts k)
 {
-    const int i = blockIdx.x * BLK + threadIdx.x;
+    const int i = blockIdx.x * B + threadIdx.x;
     if (i >= n) return;
     const dvec xi = X[i], vi = V[i];
     if (T[i] != FLUID_PARTICLE) {
@@ -507,7 +523,7 @@ k_visc_predict(int n, const dvec *__restrict__ X, const dvec *__restrict__ V, co
 int launch_visc_predict(Ctx *c, const StepConsts &k)
 {
     prof_begin(PK_VISC, c->stream);
-    hipLaunchKernelGGL(k_visc_predict, dim3(nblk(c->n)), dim3(BLK), 0, c->stream, c->n,
+    PARTICLE_LAUNCH(k_visc_predict, c, c->n,
                        c->X[c->cx], c->V[c->cx], c->T[c->ct], c->nbr, c->ncnt,
                        c->X[1 - c->cx], c->V[1 - c->cx], k);
     prof_end(PK_VISC, c->stream);
@@ -521,13 +537,14 @@ int launch_visc_predict(Ctx *c, const StepConsts &k)
 //    every particle and the per-cell histogram of the counting sort
 // ---------------------------------------------------------------------
 
-__global__ void __launch_bounds__(BLK)
+template <int B>
+__global__ void __launch_bounds__(B)
 k_collide_bin(int n, const dvec *__restrict__ X, const dvec *__restrict__ V, const int *__restrict__ T,
               const int *__restrict__ nbr, const int *__restrict__ ncnt, dvec *__restrict__ Xo,
               dvec *__restrict__ Vo, int *__restrict__ cell, int *__restrict__ ccount, StepConsts k,
               int collide)
 {
-    const int i = blockIdx.x * BLK + threadIdx.x;
+    const int i = blockIdx.x * B + threadIdx.x;
     if (i >= n) return;
     const int t = T[i];
     const dvec xi = X[i], vi = V[i];
@@ -588,7 +605,7 @@ int launch_collide_bin(Ctx *c, const StepConsts &k, int collide)
     int rc = launch_fill_int(c->ccount, 0, (size_t)k.grid.total + 1, c->stream);
     if (rc) return rc;
     prof_begin(PK_COLLIDE, c->stream);
-    hipLaunchKernelGGL(k_collide_bin, dim3(nblk(c->n)), dim3(BLK), 0, c->stream, c->n,
+    PARTICLE_LAUNCH(k_collide_bin, c, c->n,
                        c->X[c->cx], c->V[c->cx], c->T[c->ct], c->nbr, c->ncnt,
                        c->X[1 - c->cx], c->V[1 - c->cx], c->cell, c->ccount, k, collide);
     prof_end(PK_COLLIDE, c->stream);
@@ -840,11 +857,12 @@ struct SearchOut {
     DevCounters *ctr;
 };
 
-__global__ void __launch_bounds__(BLK)
+template <int B>
+__global__ void __launch_bounds__(B)
 k_search(int n, const dvec *__restrict__ X, const int *__restrict__ T, const int *__restrict__ O,
          const int *__restrict__ cstart, SearchOut out, StepConsts k, int with_density)
 {
-    const int i = blockIdx.x * BLK + threadIdx.x;
+    const int i = blockIdx.x * B + threadIdx.x;
     int cnt = 0, cnt_n = 0, u = 0;
     if (i < n) {
         const int t = T[i];
@@ -917,8 +935,8 @@ k_search(int n, const dvec *__restrict__ X, const int *__restrict__ T, const int
         }
     }
     // counters: unknowns, directed pairs, largest list
-    const long long pairs = block_sum<long long, BLK>((long long)cnt);
-    const long long nu = block_sum<long long, BLK>((long long)u);
+    const long long pairs = block_sum<long long, B>((long long)cnt);
+    const long long nu = block_sum<long long, B>((long long)u);
     if (threadIdx.x == 0) {
         atomicAdd(&out.ctr->n_pairs, (unsigned long long)pairs);
         atomicAdd(&out.ctr->n_eq, (int)nu);
@@ -932,7 +950,7 @@ int launch_search(Ctx *c, const StepConsts &k, int with_density, DevCounters *ct
     o.nbr = c->nbr; o.ncnt = c->ncnt; o.surf = c->SURF[c->ct]; o.unk = c->unk;
     o.ncount = c->NCOUNT[c->ct]; o.nd = c->ND[c->ct]; o.diag = c->diag; o.b = c->b; o.ctr = ctr;
     prof_begin(PK_SEARCH, c->stream);
-    hipLaunchKernelGGL(k_search, dim3(nblk(c->n)), dim3(BLK), 0, c->stream, c->n, c->X[c->cx],
+    PARTICLE_LAUNCH(k_search, c, c->n, c->X[c->cx],
                        c->T[c->ct], c->O[c->ct], c->cstart, o, k, with_density);
     prof_end(PK_SEARCH, c->stream);
     LAUNCH_CHECK();
@@ -944,13 +962,14 @@ int launch_search(Ctx *c, const StepConsts &k, int with_density, DevCounters *ct
 //    need velocities (HS :342-362, weakly compressible :381-409)
 // ---------------------------------------------------------------------
 
-__global__ void __launch_bounds__(BLK)
+template <int B>
+__global__ void __launch_bounds__(B)
 k_assemble(int n, const dvec *__restrict__ X, const dvec *__restrict__ V, const int *__restrict__ T,
            const double *__restrict__ ND, const int *__restrict__ unk, const int *__restrict__ nbr,
            const int *__restrict__ ncnt, double *__restrict__ Aval, int *__restrict__ Acol,
            int *__restrict__ Alen, double *__restrict__ b, DevCounters *ctr, StepConsts k)
 {
-    const int i = blockIdx.x * BLK + threadIdx.x;
+    const int i = blockIdx.x * B + threadIdx.x;
     long long row_nnz = 0;
     if (i < n) {
         if (!unk[i]) {
@@ -1032,14 +1051,14 @@ k_assemble(int n, const dvec *__restrict__ X, const dvec *__restrict__ V, const 
             }
         }
     }
-    const long long tot = block_sum<long long, BLK>(row_nnz);
+    const long long tot = block_sum<long long, B>(row_nnz);
     if (threadIdx.x == 0) atomicAdd(&ctr->nnz, (unsigned long long)tot);
 }
 
 int launch_assemble(Ctx *c, const StepConsts &k, DevCounters *ctr)
 {
     prof_begin(PK_ASSEMBLE, c->stream);
-    hipLaunchKernelGGL(k_assemble, dim3(nblk(c->n)), dim3(BLK), 0, c->stream, c->n, c->X[c->cx],
+    PARTICLE_LAUNCH(k_assemble, c, c->n, c->X[c->cx],
                        c->V[c->cx], c->T[c->ct], c->ND[c->ct], c->unk, c->nbr, c->ncnt, c->Aval,
                        c->Acol, c->Alen, c->b, ctr, k);
     prof_end(PK_ASSEMBLE, c->stream);
@@ -1100,10 +1119,11 @@ struct GradIO {
     int *T;
 };
 
-__global__ void __launch_bounds__(BLK)
+template <int B>
+__global__ void __launch_bounds__(B)
 k_grad_correct(int n, GradIO io, const int *__restrict__ nbr, const int *__restrict__ ncnt, StepConsts k)
 {
-    const int i = blockIdx.x * BLK + threadIdx.x;
+    const int i = blockIdx.x * B + threadIdx.x;
     if (i >= n) return;
     int t = io.T[i];
     const dvec xi = io.X[i], vi = io.V[i];
@@ -1201,8 +1221,7 @@ int launch_grad_correct(Ctx *c, const StepConsts &k)
     io.Xo = c->X[1 - c->cx]; io.Vo = c->V[1 - c->cx]; io.A = c->ACC[c->ct];
     io.Po = c->P[1 - c->cp]; io.ND = c->ND[c->ct]; io.T = c->T[c->ct];
     prof_begin(PK_GRAD, c->stream);
-    hipLaunchKernelGGL(k_grad_correct, dim3(nblk(c->n)), dim3(BLK), 0, c->stream, c->n, io, c->nbr,
-                       c->ncnt, k);
+    PARTICLE_LAUNCH(k_grad_correct, c, c->n, io, c->nbr, c->ncnt, k);
     prof_end(PK_GRAD, c->stream);
     LAUNCH_CHECK();
     c->cx ^= 1;

@@ -42,9 +42,11 @@ void mps_clear_error(void) { g_last_error = MPS_OK; }
 
 struct Options {
     double tol_mode = 0, precision = 0, inner_tol = 1e-3, exit_on_error = 1, check_every = 16;
-    double spmv = 0, profile = 0, verbose = 0, fused = 2, graph = 0;
+    double spmv = 0, profile = 0, verbose = 0, fused = 2, graph = 0, pblk = 256;
 };
 static Options g_opt;
+
+int particle_block() { return (int)g_opt.pblk; }
 
 // Kernel timing: HIP events recorded on the launching stream around each
 // launch of a kernel class, drawn from a pool so recording costs no
@@ -786,7 +788,12 @@ int mps_hip_set_option(const char *key, double value)
     else if (k == "graph") g_opt.graph = value;
     else if (k == "profile") g_opt.profile = value;
     else if (k == "verbose") g_opt.verbose = value;
-    else return MPS_ERR_ARG;
+    else if (k == "pblk") {
+        if (value != 128 && value != 256 && value != 512) return MPS_ERR_ARG;
+        if (value != g_opt.pblk)
+            for (auto &kv : g_ctx) graph_reset(kv.second);   // captures hold the launch sizes
+        g_opt.pblk = value;
+    } else return MPS_ERR_ARG;
     return MPS_OK;
 }
 
@@ -804,6 +811,7 @@ double mps_hip_get_option(const char *key)
     if (k == "graph") return g_opt.graph;
     if (k == "profile") return g_opt.profile;
     if (k == "verbose") return g_opt.verbose;
+    if (k == "pblk") return g_opt.pblk;
     return NAN;
 }
 
@@ -987,15 +995,37 @@ int mps_hip_autotune(ParticleSystem *ps, int apply, mps_tune_result *out)
         out->solve_ms[q] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         if (!rc && (q == 0 || out->solve_ms[q] < out->solve_ms[best_c])) best_c = q;
     }
+    // the particle phases (predictor to assembly) at each block size of the
+    // per-particle kernels: one untimed pass, then three timed ones
+    const int pb[3] = {128, 256, 512};
+    int best_p = 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = MPS_ERR_HIP;
+    for (int q = 0; q < 3 && !rc; q++) {
+        g_opt.pblk = pb[q];
+        rc = step_pre_launch(c, k);
+        if (!rc && hipEventRecord(e0, c->stream) != hipSuccess) rc = MPS_ERR_HIP;
+        for (int r = 0; r < 3 && !rc; r++) rc = step_pre_launch(c, k);
+        float ms = 0.0f;
+        if (!rc && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = MPS_ERR_HIP;
+        out->pblk_ms[q] = ms / 3.0;
+        if (!rc && (q == 0 || out->pblk_ms[q] < out->pblk_ms[best_p])) best_p = q;
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
     g_opt = saved;
     mps_hip_release(&shadow);
     if (rc) return g_last_error = rc;
     out->spmv = best_v;
     out->check_every = ce[best_c];
     out->iters = sr.iters;
+    out->pblk = pb[best_p];
     if (apply) {
         g_opt.spmv = best_v;
         g_opt.check_every = ce[best_c];
+        mps_hip_set_option("pblk", pb[best_p]);
     }
     return MPS_OK;
 }

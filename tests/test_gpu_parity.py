@@ -335,6 +335,10 @@ def test_autotune_keeps_bits(lib3):
                 assert min(res.spmv_ms) > 0 and min(res.solve_ms) > 0 and res.iters > 0
                 assert lib3.L.mps_hip_get_option(b"spmv") == res.spmv
                 assert lib3.L.mps_hip_get_option(b"check_every") == res.check_every
+                say("autotune: pblk %d (%s ms)" % (res.pblk, ["%.3f" % v for v in res.pblk_ms]))
+                assert res.pblk in (128, 256, 512) and min(res.pblk_ms) > 0
+                assert res.pblk_ms[(128, 256, 512).index(res.pblk)] == min(res.pblk_ms)
+                assert lib3.L.mps_hip_get_option(b"pblk") == res.pblk
             for _ in range(2):
                 assert g.step() == 0
             out.append([np.ascontiguousarray(g.scene.arr[f]).tobytes() for f in H.FIELDS])
@@ -342,8 +346,38 @@ def test_autotune_keeps_bits(lib3):
     finally:
         lib3.set_option("spmv", 0)
         lib3.set_option("check_every", 16)
+        lib3.set_option("pblk", 256)
     for f, a, b in zip(H.FIELDS, out[0], out[1]):
         assert a == b, f
+
+
+@pytest.mark.parametrize("graph", [0, 1])
+def test_particle_block_sizes_same_bits(lib3, graph):
+    """The per-particle kernels at block sizes 128, 256 and 512 (option
+    "pblk") give the same bits over three steps, with plain launches and
+    with captured graphs (a new size drops the captures); sizes other
+    than these are refused."""
+    dim, cfg, pos, vel, typ = H.build("dam3d_small", cg_tolerance=1e-10)
+    lib3.set_option("tol_mode", 1)
+    lib3.set_option("graph", graph)
+    out = []
+    try:
+        assert lib3.mps_hip_set_option(b"pblk", 192.0) != 0
+        for pb in (256, 128, 512):
+            lib3.set_option("pblk", pb)
+            say("pblk", pb, "graph", graph)
+            g = H.GpuRun(lib3, cfg, pos, vel, typ)
+            lib3.check(lib3.mps_hip_set_resident(g.scene.ptr(), 1), "resident")
+            lib3.check(lib3.mps_hip_run_steps(g.scene.ptr(), 3), "run_steps")
+            lib3.check(lib3.mps_hip_download(g.scene.ptr(), None), "download")
+            out.append([np.ascontiguousarray(g.scene.arr[f]).tobytes() for f in H.FIELDS])
+            g.close()
+    finally:
+        lib3.set_option("pblk", 256)
+        lib3.set_option("graph", 0)
+    for o in out[1:]:
+        for f, a, b in zip(H.FIELDS, out[0], o):
+            assert a == b, f
 
 
 @pytest.mark.parametrize("precision", [0, 1, 2, 3])
