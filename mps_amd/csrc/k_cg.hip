@@ -22,6 +22,7 @@
 #include "ctx.hpp"
 
 #define CG_BLK 256
+#define VEC_U 4     // rows per thread in flight in the grid-stride vector kernels
 
 struct CgCtrl {
     double rr[2];          // r.r entering iteration t is rr[t & 1]
@@ -508,7 +509,26 @@ k_cg_iter_b(int n, int t, const double *__restrict__ p, const double *__restrict
     if (sh_stop) return;
     const double alpha = sh_alpha;
     double v = 0.0;
-    for (int i = blockIdx.x * CG_BLK + threadIdx.x; i < n; i += gridDim.x * CG_BLK) {
+    const int st = gridDim.x * CG_BLK;
+    int i = blockIdx.x * CG_BLK + threadIdx.x;
+    for (; i + (VEC_U - 1) * st < n; i += VEC_U * st) {   // VEC_U rows in flight, same order
+        double pv[VEC_U], av[VEC_U], xv[VEC_U], rv[VEC_U];
+#pragma unroll
+        for (int u = 0; u < VEC_U; u++) {
+            pv[u] = p[i + u * st];
+            av[u] = Ap[i + u * st];
+            xv[u] = x[i + u * st];
+            rv[u] = r[i + u * st];
+        }
+#pragma unroll
+        for (int u = 0; u < VEC_U; u++) {
+            x[i + u * st] = xv[u] + alpha * pv[u];
+            const double ri = rv[u] - alpha * av[u];
+            r[i + u * st] = ri;
+            v += ri * ri;
+        }
+    }
+    for (; i < n; i += st) {
         x[i] += alpha * p[i];
         const double ri = r[i] - alpha * Ap[i];
         r[i] = ri;
@@ -565,7 +585,19 @@ k_cg3_pupdate(int n, int t, int max_iter, const double *__restrict__ r, double *
     __syncthreads();
     if (sh_stop) return;
     const double beta = sh_beta;
-    for (int i = blockIdx.x * CG_BLK + threadIdx.x; i < n; i += gridDim.x * CG_BLK) p[i] = r[i] + beta * p[i];
+    const int st = gridDim.x * CG_BLK;
+    int i = blockIdx.x * CG_BLK + threadIdx.x;
+    for (; i + (VEC_U - 1) * st < n; i += VEC_U * st) {
+        double rv[VEC_U], pv[VEC_U];
+#pragma unroll
+        for (int u = 0; u < VEC_U; u++) {
+            rv[u] = r[i + u * st];
+            pv[u] = p[i + u * st];
+        }
+#pragma unroll
+        for (int u = 0; u < VEC_U; u++) p[i + u * st] = rv[u] + beta * pv[u];
+    }
+    for (; i < n; i += st) p[i] = r[i] + beta * p[i];
 }
 
 int cg_alloc_ctrl(Ctx *c)
@@ -1149,7 +1181,27 @@ k_mx3_axpy(int n, int t, const V *__restrict__ p, const V *__restrict__ Ap, doub
     const double alpha = sh_alpha;
     const V av = (V)alpha;
     double v = 0.0;
-    for (int i = blockIdx.x * CG_BLK + threadIdx.x; i < n; i += gridDim.x * CG_BLK) {
+    const int st = gridDim.x * CG_BLK;
+    int i = blockIdx.x * CG_BLK + threadIdx.x;
+    for (; i + (VEC_U - 1) * st < n; i += VEC_U * st) {   // VEC_U rows in flight, same order
+        V pv[VEC_U], apv[VEC_U], rv[VEC_U];
+        double xv[VEC_U];
+#pragma unroll
+        for (int u = 0; u < VEC_U; u++) {
+            pv[u] = p[i + u * st];
+            apv[u] = Ap[i + u * st];
+            xv[u] = x[i + u * st];
+            rv[u] = r[i + u * st];
+        }
+#pragma unroll
+        for (int u = 0; u < VEC_U; u++) {
+            x[i + u * st] = xv[u] + alpha * (double)pv[u];
+            const V ri = rv[u] - av * apv[u];
+            r[i + u * st] = ri;
+            v += (double)ri * (double)ri;
+        }
+    }
+    for (; i < n; i += st) {
         x[i] += alpha * (double)p[i];
         const V ri = r[i] - av * Ap[i];
         r[i] = ri;
@@ -1490,7 +1542,26 @@ k_b_axpy(const BScene *__restrict__ sc, const int *__restrict__ axs, double *par
     const int j = blockIdx.x - S.ax_off;
     const double alpha = ctrl[si].alpha;
     double v = 0.0;
-    for (int i = j * CG_BLK + threadIdx.x; i < S.n_eq; i += S.nax * CG_BLK) {
+    const int st = S.nax * CG_BLK;
+    int i = j * CG_BLK + threadIdx.x;
+    for (; i + (VEC_U - 1) * st < S.n_eq; i += VEC_U * st) {
+        double pv[VEC_U], av[VEC_U], xv[VEC_U], rv[VEC_U];
+#pragma unroll
+        for (int u = 0; u < VEC_U; u++) {
+            pv[u] = S.p[i + u * st];
+            av[u] = S.Ap[i + u * st];
+            xv[u] = S.x[i + u * st];
+            rv[u] = S.r[i + u * st];
+        }
+#pragma unroll
+        for (int u = 0; u < VEC_U; u++) {
+            S.x[i + u * st] = xv[u] + alpha * pv[u];
+            const double ri = rv[u] - alpha * av[u];
+            S.r[i + u * st] = ri;
+            v += ri * ri;
+        }
+    }
+    for (; i < S.n_eq; i += st) {
         S.x[i] += alpha * S.p[i];
         const double ri = S.r[i] - alpha * S.Ap[i];
         S.r[i] = ri;
@@ -1540,7 +1611,19 @@ k_b_pupdate(const BScene *__restrict__ sc, const int *__restrict__ axs, const Cg
     const BScene &S = sc[si];
     const int j = blockIdx.x - S.ax_off;
     const double beta = ctrl[si].beta;
-    for (int i = j * CG_BLK + threadIdx.x; i < S.n_eq; i += S.nax * CG_BLK) S.p[i] = S.r[i] + beta * S.p[i];
+    const int st = S.nax * CG_BLK;
+    int i = j * CG_BLK + threadIdx.x;
+    for (; i + (VEC_U - 1) * st < S.n_eq; i += VEC_U * st) {
+        double rv[VEC_U], pv[VEC_U];
+#pragma unroll
+        for (int u = 0; u < VEC_U; u++) {
+            rv[u] = S.r[i + u * st];
+            pv[u] = S.p[i + u * st];
+        }
+#pragma unroll
+        for (int u = 0; u < VEC_U; u++) S.p[i + u * st] = rv[u] + beta * pv[u];
+    }
+    for (; i < S.n_eq; i += st) S.p[i] = S.r[i] + beta * S.p[i];
 }
 
 struct BatchBufs {
@@ -1844,7 +1927,27 @@ k_bm_axpy(const BMScene<V> *__restrict__ sc, const int *__restrict__ axs, double
     const double alpha = ctrl[si].alpha;
     const V av = (V)alpha;
     double v = 0.0;
-    for (int i = j * CG_BLK + threadIdx.x; i < S.n_eq; i += S.nax * CG_BLK) {
+    const int st = S.nax * CG_BLK;
+    int i = j * CG_BLK + threadIdx.x;
+    for (; i + (VEC_U - 1) * st < S.n_eq; i += VEC_U * st) {
+        V pv[VEC_U], apv[VEC_U], rv[VEC_U];
+        double xv[VEC_U];
+#pragma unroll
+        for (int u = 0; u < VEC_U; u++) {
+            pv[u] = S.p[i + u * st];
+            apv[u] = S.Ap[i + u * st];
+            xv[u] = S.x[i + u * st];
+            rv[u] = S.r[i + u * st];
+        }
+#pragma unroll
+        for (int u = 0; u < VEC_U; u++) {
+            S.x[i + u * st] = xv[u] + alpha * (double)pv[u];
+            const V ri = rv[u] - av * apv[u];
+            S.r[i + u * st] = ri;
+            v += (double)ri * (double)ri;
+        }
+    }
+    for (; i < S.n_eq; i += st) {
         S.x[i] += alpha * (double)S.p[i];
         const V ri = S.r[i] - av * S.Ap[i];
         S.r[i] = ri;
