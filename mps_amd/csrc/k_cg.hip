@@ -245,7 +245,18 @@ template <typename XT, typename XA>
 __device__ __forceinline__ void stage_halo_of(const PkView &P, int blk, XA x, XT *hx)
 {
     const int2 bm = P.bmeta[blk];
-    for (int h = threadIdx.x; h < bm.y; h += CG_BLK) hx[h] = x(P.halo[bm.x + h]);
+    int h = threadIdx.x;
+    for (; h + 3 * CG_BLK < bm.y; h += 4 * CG_BLK) {   // four index loads, then four gathers
+        int id[4];
+        XT v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) id[u] = P.halo[bm.x + h + u * CG_BLK];
+#pragma unroll
+        for (int u = 0; u < 4; u++) v[u] = x(id[u]);
+#pragma unroll
+        for (int u = 0; u < 4; u++) hx[h + u * CG_BLK] = v[u];
+    }
+    for (; h < bm.y; h += CG_BLK) hx[h] = x(P.halo[bm.x + h]);
     __syncthreads();
 }
 
