@@ -46,6 +46,17 @@ static inline unsigned cg_grid(int n) { return (unsigned)((n + CG_BLK - 1) / CG_
 // kernel: deterministic, and free of a single-address atomic ticket that
 // would serialise thousands of blocks.  Each thread issues its loads 8 at a
 // time before adding, so the partials arrive in a few L2 round trips.
+// The stop flag as the whole block sees it: read once by thread 0 and
+// shared, so that every wave of a block takes the same branch even when
+// block 0 of the same kernel sets the flag while the block starts.
+__device__ __forceinline__ int block_flag(const int *flag)
+{
+    __shared__ int sh;
+    if (threadIdx.x == 0) sh = *(const volatile int *)flag;
+    __syncthreads();
+    return sh;
+}
+
 __device__ __forceinline__ double fin_sum(const double *part, unsigned nparts)
 {
     double s = 0.0;
@@ -433,7 +444,7 @@ k_cg_iter_a(int n, PkView P, int t, int max_iter, const double *__restrict__ r, 
     double *hx = reinterpret_cast<double *>(smem_raw);
     __shared__ double sh_beta;
     __shared__ int sh_stop;
-    if (ctrl->done) return;
+    if (block_flag(&ctrl->done)) return;
     double beta = 0.0;
     if (t > 0) {
         const double rr_new = fin_sum(partB, nb);
@@ -496,7 +507,7 @@ k_cg_iter_b(int n, int t, const double *__restrict__ p, const double *__restrict
 {
     __shared__ double sh_alpha;
     __shared__ int sh_stop;
-    if (ctrl->done) return;
+    if (block_flag(&ctrl->done)) return;
     const double pap = fin_sum(partA, na);
     if (threadIdx.x == 0) {
         const double rr = ctrl->rr[t & 1];
@@ -562,7 +573,7 @@ k_cg3_pupdate(int n, int t, int max_iter, const double *__restrict__ r, double *
 {
     __shared__ double sh_beta;
     __shared__ int sh_stop;
-    if (ctrl->done) return;
+    if (block_flag(&ctrl->done)) return;
     const double rr_new = fin_sum(partB, nb);
     if (threadIdx.x == 0) {
         const double rr = ctrl->rr[t & 1];
@@ -1150,7 +1161,7 @@ k_mx3_axpy(int n, int t, const V *__restrict__ p, const V *__restrict__ Ap, doub
 {
     __shared__ double sh_alpha;
     __shared__ int sh_go;
-    if (c->done) return;
+    if (block_flag(&c->done)) return;
     const double pap = fin_sum(part, np);
     const int ru = c->f_ru[t & 1];
     double rr_true = 0.0;
@@ -1230,7 +1241,7 @@ k_mx3_pupdate(int n, unsigned nblk, int t, int max_iter, PkView P, const double 
 {
     __shared__ double sh_beta;
     __shared__ int sh_go, sh_ru;
-    if (c->done) return;
+    if (block_flag(&c->done)) return;
     const double rr_new = fin_sum(part2, np2);
     if (threadIdx.x == 0) {
         const double rr = c->m_rr, rr_ru = c->m_rr_ru;
