@@ -305,7 +305,7 @@ int  mps_hip_get_stats(ParticleSystem *ps, mps_step_stats *out);
  * device state are not touched.  With apply != 0 the fastest choices
  * become the options.  Returns 0 or an error code. */
 typedef struct {
-    int    spmv;            /* fastest "spmv" variant                      */
+    int    spmv;            /* faster of "spmv" variants 0 and 1            */
     int    check_every;     /* fastest "check_every"                       */
     double spmv_ms[3];      /* mean SpMV launch, variants 0 (8 entries per
                                batch, non-temporal), 1 (16, non-temporal),

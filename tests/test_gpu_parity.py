@@ -329,8 +329,8 @@ def test_autotune_keeps_bits(lib3):
                     % (res.spmv, ["%.4f" % v for v in res.spmv_ms], res.check_every,
                        ["%.2f" % v for v in res.solve_ms], res.iters))
                 assert [np.ascontiguousarray(g.scene.arr[f]).tobytes() for f in names] == before
-                assert res.spmv in (0, 1, 2) and res.check_every in (8, 16, 32, 64)
-                assert res.spmv_ms[res.spmv] == min(res.spmv_ms)
+                assert res.spmv in (0, 1) and res.check_every in (8, 16, 32, 64)
+                assert res.spmv_ms[res.spmv] == min(res.spmv_ms[0], res.spmv_ms[1])
                 assert res.solve_ms[(8, 16, 32, 64).index(res.check_every)] == min(res.solve_ms)
                 assert min(res.spmv_ms) > 0 and min(res.solve_ms) > 0 and res.iters > 0
                 assert lib3.L.mps_hip_get_option(b"spmv") == res.spmv
