@@ -947,36 +947,76 @@ k_mx_q16_build(int n_eq, PkView P, double neg_coeff, double re, double r_min, un
     if (k < n_eq) q16_build_row(k, P, neg_coeff, re, r_min, cq);
 }
 
-// (A' p)_i of the coded operator, the halo of p staged in hx
+// What a row of the coded operator needs before its block's halo is
+// staged: its slice descriptor, length, p_i, Dirichlet part and first U
+// words.  Loading them ahead of the staging overlaps the two latencies.
+template <int U>
+struct CodedRow {
+    int4 mt;
+    int m;
+    float pi, d;
+    unsigned w[U];
+};
+
+template <int U, bool NT>
+__device__ __forceinline__ void coded_pre(const PkView &P, const unsigned *__restrict__ cq,
+                                          const float *__restrict__ p, int i, CodedRow<U> &c)
+{
+    c.mt = P.smeta[i >> 6];
+    c.m = P.len[i];
+    c.pi = p[i];
+    c.d = (float)P.ddir[i];
+    const unsigned *qrow = cq + (size_t)c.mt.x * 64u + (i & 63);
+#pragma unroll
+    for (int u = 0; u < U; u++) c.w[u] = (c.mt.z && u < c.m) ? ldm<NT>(qrow + (size_t)u * 64u) : 0u;
+}
+
+// (A' p)_i of the coded operator, the halo of p staged in hx; the next
+// batch of words is loaded before the current one is gathered and summed
+template <int U, bool NT = false>
+__device__ __forceinline__ float coded_row_from(const PkView &P, const unsigned *__restrict__ cq, const QConsts &qk,
+                                                const float *__restrict__ p, const float *hx, int i,
+                                                const CodedRow<U> &c)
+{
+    const float pi = c.pi;
+    float s = c.d * pi;
+    const int m = c.m;
+    if (c.mt.z) {
+        const unsigned *qrow = cq + (size_t)c.mt.x * 64u + (i & 63);
+        for (int q0 = 0; q0 < m; q0 += U) {
+            unsigned w[U];
+            if (q0 == 0) {
+#pragma unroll
+                for (int u = 0; u < U; u++) w[u] = c.w[u];
+            } else {
+#pragma unroll
+                for (int u = 0; u < U; u++) w[u] = (q0 + u < m) ? ldm<NT>(qrow + (size_t)(q0 + u) * 64u) : 0u;
+            }
+            float h[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) h[u] = hx[w[u] & 0xfffu];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const float r = qk.r_lo + (float)(w[u] >> 12) * qk.dr;
+                const float a = qk.neg_coeff * (qk.re * __builtin_amdgcn_rcpf(r) - 1.0f);
+                if (q0 + u < m) s += a * (h[u] - pi);
+            }
+        }
+    } else {
+        s = row_sum_global<U, true, float, NT>(m, P.fval + (size_t)c.mt.x * 64u + (i & 63),
+                                               P.c32 + (size_t)c.mt.y * 64u + (i & 63), XPlain<float>{p}, pi, i, s);
+    }
+    return s;
+}
+
 template <int U, bool NT = false>
 __device__ __forceinline__ float coded_row(const PkView &P, const unsigned *__restrict__ cq, const QConsts &qk,
                                            const float *__restrict__ p, const float *hx, int i, float pi)
 {
-    float s = (float)P.ddir[i] * pi;
-        const int4 mt = P.smeta[i >> 6];
-        const int lane = i & 63;
-        const int m = P.len[i];
-        if (mt.z) {
-            const unsigned *qrow = cq + (size_t)mt.x * 64u + lane;
-            for (int q0 = 0; q0 < m; q0 += U) {
-                unsigned w[U];
-                float h[U];
-#pragma unroll
-                for (int u = 0; u < U; u++) w[u] = (q0 + u < m) ? ldm<NT>(qrow + (size_t)(q0 + u) * 64u) : 0u;
-#pragma unroll
-                for (int u = 0; u < U; u++) h[u] = hx[w[u] & 0xfffu];
-#pragma unroll
-                for (int u = 0; u < U; u++) {
-                    const float r = qk.r_lo + (float)(w[u] >> 12) * qk.dr;
-                    const float a = qk.neg_coeff * (qk.re * __builtin_amdgcn_rcpf(r) - 1.0f);
-                    if (q0 + u < m) s += a * (h[u] - pi);
-                }
-            }
-        } else {
-            s = row_sum_global<U, true, float, NT>(m, P.fval + (size_t)mt.x * 64u + lane,
-                                               P.c32 + (size_t)mt.y * 64u + lane, XPlain<float>{p}, pi, i, s);
-        }
-    return s;
+    CodedRow<U> c;
+    coded_pre<U, NT>(P, cq, p, i, c);
+    c.pi = pi;
+    return coded_row_from<U, NT>(P, cq, qk, p, hx, i, c);
 }
 
 template <int U, bool NT = false>
@@ -987,14 +1027,15 @@ k_mx_spmv_q(int n, PkView P, const unsigned *__restrict__ cq, QConsts qk, const 
     extern __shared__ unsigned char smem_raw[];
     float *hx = reinterpret_cast<float *>(smem_raw);
     if (c->done) return;
-    stage_halo(P, XPlain<float>{p}, hx);
     const int i = blockIdx.x * CG_BLK + threadIdx.x;
+    CodedRow<U> row;
+    if (i < n) coded_pre<U, NT>(P, cq, p, i, row);
+    stage_halo(P, XPlain<float>{p}, hx);
     double v = 0.0;
     if (i < n) {
-        const float pi = p[i];
-        const float s = coded_row<U, NT>(P, cq, qk, p, hx, i, pi);
+        const float s = coded_row_from<U, NT>(P, cq, qk, p, hx, i, row);
         Ap[i] = s;
-        v = (double)pi * (double)s;
+        v = (double)row.pi * (double)s;
     }
     v = block_sum<double, CG_BLK>(v);
     if (threadIdx.x == 0) part[blockIdx.x] = v;
