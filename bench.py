@@ -557,6 +557,55 @@ def sync_rate(lib, ps, n, cfg, a):
     lib.cell_list_free(cl)
 
 
+def snapshot_rate(lib, w, a):
+    """simulation_run (simulation.c:123-181) with CSV and VTK snapshots every
+    a.snapshots steps, timed over the whole call, once with the snapshots
+    written before the next step (async_output 0, the reference's order)
+    and once written by a host thread while the device steps on
+    (async_output 1).  Each run starts from a fresh scene; the files of the
+    two runs are compared by SHA-1, then deleted."""
+    import hashlib
+    import shutil
+    import tempfile
+    res = {}
+    digests = {}
+    n = 0
+    for mode in (0, 1):
+        out = tempfile.mkdtemp(prefix="mps_snap_", dir=os.environ.get("TMPDIR", "/tmp"))
+        cfg = make_config(lib, w, 0, a.tol)
+        cfg.t_end = (a.steps + 0.5) * cfg.dt
+        cfg.output_interval = a.snapshots
+        cfg.output_dir = out.encode()
+        lib.set_config(cfg)
+        lib.set_option("async_output", mode)
+        ps, n = build_scene(lib, w)
+        lib.particle_system_calc_initial_params(ps)
+        print("snapshots: async_output %d, %d steps, every %d" % (mode, a.steps, a.snapshots),
+              file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        lib.simulation_run(ps)
+        el = time.perf_counter() - t0
+        lib.check(lib.mps_hip_last_error(), "simulation_run")
+        lib.mps_hip_release(ps)
+        lib.particle_system_free(ps)
+        files = sorted(os.listdir(out))
+        size = sum(os.path.getsize(os.path.join(out, f)) for f in files)
+        digests[mode] = [(f, hashlib.sha1(open(os.path.join(out, f), "rb").read()).hexdigest())
+                         for f in files]
+        shutil.rmtree(out)
+        res[mode] = {"seconds": round(el, 3), "particle_steps_per_s": round(n * a.steps / el, 1),
+                     "files": len(files), "bytes": size}
+    lib.set_option("async_output", 1)
+    print(json.dumps({"metric": "particle-steps/s, simulation_run with CSV+VTK snapshots",
+                      "value": res[1]["particle_steps_per_s"], "unit": "particle-steps/s",
+                      "steps": a.steps, "snapshot_every": a.snapshots,
+                      "sync_writer": res[0], "async_writer": res[1],
+                      "speedup": round(res[0]["seconds"] / res[1]["seconds"], 3),
+                      "files_identical": digests[0] == digests[1],
+                      "config": {"workload": a.workload, "particles": n, "precision": a.precision}}),
+          flush=True)
+
+
 def calibrate_copy(nbytes=1 << 30, reps=5):
     """A plain copy of known size: each launch reads and writes exactly
     nbytes.  The HBM byte counters of its dispatches give the factor that
@@ -607,6 +656,9 @@ def main():
     ap.add_argument("--compare", action="store_true",
                     help="run the scene in FP64 and in the chosen mixed form for --steps steps each and "
                          "report both rates and the relative L2 error of pressure and positions")
+    ap.add_argument("--snapshots", type=int, default=0,
+                    help="K > 0: time simulation_run over --steps steps with CSV+VTK snapshots every K "
+                         "steps, with the synchronous and with the overlapped writer; not the headline")
     ap.add_argument("--sequential", action="store_true",
                     help="ensemble workloads: also time the scenes one after another and compare the bits")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -656,6 +708,11 @@ def main():
         return
     if a.sync:
         sync_rate(lib, ps, n, cfg, a)
+        return
+    if a.snapshots > 0:
+        lib.mps_hip_release(ps)
+        lib.particle_system_free(ps)
+        snapshot_rate(lib, w, a)
         return
     tune = None
     if a.autotune and a.precision == "fp64" and a.solver == "cg":

@@ -277,6 +277,11 @@ int  mps_hip_step_backward(ParticleSystem *ps, NeighborList *nl, const double *r
  *   "pblk"           block size of the per-particle kernels (collision,
  *                      predictor, search, assembly, gradient): 128, 256
  *                      (default) or 512; same results bit for bit
+ *   "async_output"   simulation_run's snapshots: 1 (default) each output
+ *                      step's state is copied to pinned host memory and a
+ *                      host thread writes the CSV/VTK files while the
+ *                      device runs on; 0 written before the next step, as
+ *                      the reference does; the same bytes either way
  * Returns 0, or MPS_ERR_ARG for an unknown key. */
 int    mps_hip_set_option(const char *key, double value);
 double mps_hip_get_option(const char *key);

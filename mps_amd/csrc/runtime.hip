@@ -14,6 +14,7 @@
 
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include <chrono>
@@ -43,8 +44,71 @@ void mps_clear_error(void) { g_last_error = MPS_OK; }
 struct Options {
     double tol_mode = 0, precision = 0, inner_tol = 1e-3, exit_on_error = 1, check_every = 16;
     double spmv = 0, profile = 0, verbose = 0, fused = 2, graph = 0, pblk = 256;
+    double async_output = 1;
 };
 static Options g_opt;
+
+// Snapshot output of simulation_run.  With option async_output (the
+// default) an output step's state is copied into one of two pinned host
+// buffers, and a host thread formats and writes it while the device runs
+// the following steps.  The thread reads nothing but its snapshot and the
+// output directory, so the files hold the same bytes as a synchronous
+// write.  At most one writer runs: each output step waits for the previous
+// writer before it starts the next, and a snapshot is downloaded into the
+// buffer that writer is not reading.
+struct SnapshotWriter {
+    Particle *buf[2] = {nullptr, nullptr};
+    size_t cap = 0;
+    int next = 0;
+    std::thread th;
+
+    void join()
+    {
+        if (th.joinable()) th.join();
+    }
+    int reserve(size_t n)
+    {
+        if (n < 1) n = 1;
+        if (n <= cap) return MPS_OK;
+        release();
+        for (int b = 0; b < 2; b++)
+            if (hipHostMalloc((void **)&buf[b], sizeof(Particle) * n, hipHostMallocDefault) != hipSuccess) {
+                buf[b] = nullptr;
+                release();
+                return MPS_ERR_ALLOC;
+            }
+        cap = n;
+        return MPS_OK;
+    }
+    void release()
+    {
+        join();
+        for (int b = 0; b < 2; b++) {
+            if (buf[b]) hipHostFree(buf[b]);
+            buf[b] = nullptr;
+        }
+        cap = 0;
+        next = 0;
+    }
+};
+static SnapshotWriter g_writer;
+
+static void write_snapshot(ParticleSystem snap, int step, std::string dir)
+{
+    output_csv(&snap, step, dir.c_str());
+    output_vtk(&snap, step, dir.c_str());
+}
+
+// Starts the writer thread for snap; writes synchronously where no thread
+// can be created.  The previous writer must have been joined.
+static void start_snapshot(const ParticleSystem &snap, int step, const char *dir)
+{
+    try {
+        g_writer.th = std::thread(write_snapshot, snap, step, std::string(dir));
+    } catch (...) {
+        write_snapshot(snap, step, std::string(dir));
+    }
+}
 
 int particle_block() { return (int)g_opt.pblk; }
 
@@ -516,6 +580,7 @@ static int check_overflow(Ctx *c, int stride)
         fprintf(stderr, "Error: neighbor count exceeded max_neighbors (%d) for particle %d. "
                         "Increase max_neighbors in params.txt.\n",
                 stride, c->hcounters->overflow_orig);
+        g_writer.join();   // the snapshot being written is completed first
         exit(EXIT_FAILURE);
     }
     char msg[128];
@@ -746,6 +811,7 @@ void mps_hip_finalize(void)
     }
     g_ctx.clear();
     cg_batch_free();
+    g_writer.release();
 }
 
 int mps_hip_last_error(void) { return g_last_error; }
@@ -788,6 +854,7 @@ int mps_hip_set_option(const char *key, double value)
     else if (k == "graph") g_opt.graph = value;
     else if (k == "profile") g_opt.profile = value;
     else if (k == "verbose") g_opt.verbose = value;
+    else if (k == "async_output") g_opt.async_output = value;
     else if (k == "pblk") {
         if (value != 128 && value != 256 && value != 512) return MPS_ERR_ARG;
         if (value != g_opt.pblk)
@@ -812,6 +879,7 @@ double mps_hip_get_option(const char *key)
     if (k == "profile") return g_opt.profile;
     if (k == "verbose") return g_opt.verbose;
     if (k == "pblk") return g_opt.pblk;
+    if (k == "async_output") return g_opt.async_output;
     return NAN;
 }
 
@@ -1116,8 +1184,19 @@ void simulation_run(ParticleSystem *ps)
     StepConsts k = make_consts(g_config, ps, g, re);
     k.stride = c->stride;
     if (search_only(c, k)) { fprintf(stderr, "Error: %s\n", g_last_msg); return; }
-    output_csv(ps, 0, out_dir);
-    output_vtk(ps, 0, out_dir);
+    // The step-0 writer reads ps->particles itself: nothing writes them
+    // before the final download, which waits for every writer.
+    bool async_out = g_opt.async_output > 0.5;
+    if (async_out && out_interval > 0 && total_steps >= out_interval &&
+        g_writer.reserve((size_t)ps->num) != MPS_OK)
+        async_out = false;
+    g_writer.join();
+    if (async_out) {
+        start_snapshot(*ps, 0, out_dir);
+    } else {
+        output_csv(ps, 0, out_dir);
+        output_vtk(ps, 0, out_dir);
+    }
     printf("Starting simulation (%dD, HIP %s): %d steps, dt = %.2e\n", D_, g_arch, total_steps, dt);
     for (int step = 1; step <= total_steps; step++) {
         if (run_step(c, g_config)) {
@@ -1125,15 +1204,27 @@ void simulation_run(ParticleSystem *ps)
             break;
         }
         if (out_interval > 0 && step % out_interval == 0) {
-            download(c, ps, nullptr);
+            ParticleSystem snap = *ps;
+            if (async_out) snap.particles = g_writer.buf[g_writer.next];
+            if (download(c, &snap, nullptr)) {
+                fprintf(stderr, "Error at step %d: %s\n", step, g_last_msg);
+                break;
+            }
             int fluid = 0;
-            for (int i = 0; i < ps->num; i++) fluid += ps->particles[i].type == FLUID_PARTICLE;
+            for (int i = 0; i < snap.num; i++) fluid += snap.particles[i].type == FLUID_PARTICLE;
             printf("Step %6d / %d  (t = %.4f s)  fluid particles: %d\n", step, total_steps,
                    step * dt, fluid);
-            output_csv(ps, step, out_dir);
-            output_vtk(ps, step, out_dir);
+            if (async_out) {
+                g_writer.join();
+                start_snapshot(snap, step, out_dir);
+                g_writer.next ^= 1;
+            } else {
+                output_csv(ps, step, out_dir);
+                output_vtk(ps, step, out_dir);
+            }
         }
     }
+    g_writer.release();
     download(c, ps, nl);
     c->resident = false;
     printf("Simulation complete.\n");
