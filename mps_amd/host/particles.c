@@ -12,6 +12,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/stat.h>
 
 #include "mps_hip.h"
 
@@ -83,6 +84,70 @@ int mps_count_particles(const char *path)
     return n;
 }
 
+/* One fgets-sized piece of the file after the header, parsed. */
+typedef struct {
+    double pos[MPS_DIM], vel[MPS_DIM];
+    int type;
+    int blank;   /* blank or comment line */
+    int got;     /* sscanf's return */
+} RowParse;
+
+#define LINE_CAP 512          /* the reference's fgets buffer */
+#define LOAD_BATCH 65536      /* pieces parsed per parallel pass */
+
+static void parse_piece(const char *text, size_t len, RowParse *r)
+{
+    char line[LINE_CAP];
+    memcpy(line, text, len);
+    line[len] = '\0';
+    const char *s = content(line);
+    r->blank = s == NULL;
+    if (r->blank) return;
+#if MPS_DIM == 3
+    r->got = sscanf(s, "%lf %lf %lf %lf %lf %lf %d", &r->pos[0], &r->pos[1], &r->pos[2],
+                    &r->vel[0], &r->vel[1], &r->vel[2], &r->type);
+#else
+    r->got = sscanf(s, "%lf %lf %lf %lf %d", &r->pos[0], &r->pos[1], &r->vel[0], &r->vel[1],
+                    &r->type);
+#endif
+}
+
+/* The rest of fp, NUL-terminated, or NULL where it does not fit in memory. */
+static char *read_rest(FILE *fp, size_t *size)
+{
+    size_t cap = (size_t)1 << 20, n = 0;
+    struct stat st;
+    if (fstat(fileno(fp), &st) == 0 && S_ISREG(st.st_mode) && st.st_size > 0)
+        cap = (size_t)st.st_size;   /* the whole file; the rest is no larger */
+    char *buf = (char *)malloc(cap + 1);
+    while (buf) {
+        if (n == cap) {
+            char *b = (char *)realloc(buf, 2 * cap + 1);
+            if (!b) {
+                free(buf);
+                return NULL;
+            }
+            buf = b;
+            cap *= 2;
+        }
+        size_t got = fread(buf + n, 1, cap - n, fp);
+        n += got;
+        if (got == 0) break;
+    }
+    if (buf) buf[n] = '\0';
+    *size = n;
+    return buf;
+}
+
+/*
+ * The rows after the header are read whole and cut into the pieces the
+ * reference's fgets loop sees (up to and including a newline, at most
+ * LINE_CAP - 1 bytes).  Each batch of pieces is parsed in parallel with
+ * the same sscanf call, then added in file order by one thread, which also
+ * prints the warnings; the particles, counts and messages are those of the
+ * line-by-line loop.  sscanf gets a short NUL-terminated copy of its piece,
+ * since glibc's sscanf measures the whole string it is given.
+ */
 int mps_load_particles(const char *path, ParticleSystem *ps)
 {
     FILE *fp = fopen(path, "r");
@@ -96,31 +161,60 @@ int mps_load_particles(const char *path, ParticleSystem *ps)
         fclose(fp);
         return -1;
     }
-    int loaded = 0, n_fluid = 0, n_wall = 0;
-    char line[512];
-    while (loaded < expected && fgets(line, sizeof line, fp)) {
-        const char *s = content(line);
-        if (!s) continue;
-        double pos[MPS_DIM], vel[MPS_DIM];
-        int type;
 #if MPS_DIM == 3
-        int got = sscanf(s, "%lf %lf %lf %lf %lf %lf %d", &pos[0], &pos[1], &pos[2],
-                         &vel[0], &vel[1], &vel[2], &type);
-        const int want = 7;
+    const int want = 7;
 #else
-        int got = sscanf(s, "%lf %lf %lf %lf %d", &pos[0], &pos[1], &vel[0], &vel[1], &type);
-        const int want = 5;
+    const int want = 5;
 #endif
-        if (got != want) {
-            fprintf(stderr, "Warning: skipping malformed line: %s", line);
-            continue;
-        }
-        particle_system_add(ps, pos, vel, type);
-        loaded++;
-        if (type == FLUID_PARTICLE) n_fluid++;
-        else if (type == WALL_PARTICLE) n_wall++;
-    }
+    size_t size = 0;
+    char *text = read_rest(fp, &size);
     fclose(fp);
+    RowParse *rows = (RowParse *)malloc(sizeof(RowParse) * LOAD_BATCH);
+    size_t *start = (size_t *)malloc(sizeof(size_t) * LOAD_BATCH);
+    int *len = (int *)malloc(sizeof(int) * LOAD_BATCH);
+    if (!text || !rows || !start || !len) {
+        fprintf(stderr, "Error: out of memory reading '%s'\n", path);
+        free(text);
+        free(rows);
+        free(start);
+        free(len);
+        return -1;
+    }
+    int loaded = 0, n_fluid = 0, n_wall = 0;
+    size_t at = 0;
+    while (loaded < expected && at < size) {
+        int m = 0;
+        while (m < LOAD_BATCH && at < size) {
+            size_t lim = size - at < LINE_CAP - 1 ? size - at : LINE_CAP - 1;
+            const char *nl = (const char *)memchr(text + at, '\n', lim);
+            size_t l = nl ? (size_t)(nl - (text + at)) + 1 : lim;
+            start[m] = at;
+            len[m] = (int)l;
+            m++;
+            at += l;
+        }
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < m; i++) parse_piece(text + start[i], (size_t)len[i], &rows[i]);
+        for (int i = 0; i < m && loaded < expected; i++) {
+            const RowParse *r = &rows[i];
+            if (r->blank) continue;
+            if (r->got != want) {
+                char line[LINE_CAP];
+                memcpy(line, text + start[i], (size_t)len[i]);
+                line[len[i]] = '\0';
+                fprintf(stderr, "Warning: skipping malformed line: %s", line);
+                continue;
+            }
+            particle_system_add(ps, (double *)r->pos, (double *)r->vel, r->type);
+            loaded++;
+            if (r->type == FLUID_PARTICLE) n_fluid++;
+            else if (r->type == WALL_PARTICLE) n_wall++;
+        }
+    }
+    free(text);
+    free(rows);
+    free(start);
+    free(len);
     if (loaded != expected)
         fprintf(stderr, "Warning: expected %d particles but loaded %d\n", expected, loaded);
     printf("Loaded particles: %d fluid, %d wall, %d total\n", n_fluid, n_wall, ps->num);

@@ -1412,3 +1412,83 @@ void orc_init_params(ParticleSystem *ps, const SimConfig *c)
         }
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* particles.txt loader                                                */
+/* ------------------------------------------------------------------ */
+
+/* particle.c:38-58: capacity-bounded append */
+static void add_particle(ParticleSystem *ps, const double pos[D], const double vel[D], int type)
+{
+    if (ps->num >= ps->capacity) {
+        fprintf(stderr, "Error: particle capacity exceeded (%d)\n", ps->capacity);
+        return;
+    }
+    Particle *p = &ps->particles[ps->num];
+    for (int d = 0; d < D; d++) {
+        p->pos[d] = pos[d];
+        p->vel[d] = vel[d];
+        p->acc[d] = 0.0;
+    }
+    p->pressure = 0.0;
+    p->n = 0.0;
+    p->type = type;
+    p->on_surface = 0;
+    ps->num++;
+}
+
+/* main.c:20-85 (2-D: mps_fluid2D/src/main.c:20-85, five columns), a line
+ * at a time through a 512-byte fgets buffer: blanks and tabs skipped, '#',
+ * newline, CR and empty lines ignored; the first line sscanf reads an int
+ * from is the count; every row sscanf fills completely is appended, others
+ * are reported and skipped; reading stops once the count is loaded. */
+int orc_load_particles(const char *path, ParticleSystem *ps)
+{
+    FILE *fp = fopen(path, "r");
+    if (!fp) {
+        fprintf(stderr, "Error: cannot open particle file '%s'\n", path);
+        return -1;
+    }
+    char line[512];
+    int count = 0, have = 0;
+    while (fgets(line, sizeof line, fp)) {
+        char *s = line;
+        while (*s == ' ' || *s == '\t') s++;
+        if (*s == '#' || *s == '\n' || *s == '\r' || *s == '\0') continue;
+        if (sscanf(s, "%d", &count) == 1) {
+            have = 1;
+            break;
+        }
+    }
+    if (!have || count <= 0) {
+        fprintf(stderr, "Error: invalid particle count in '%s'\n", path);
+        fclose(fp);
+        return -1;
+    }
+    int loaded = 0, n_fluid = 0, n_wall = 0;
+    while (fgets(line, sizeof line, fp) && loaded < count) {
+        char *s = line;
+        while (*s == ' ' || *s == '\t') s++;
+        if (*s == '#' || *s == '\n' || *s == '\r' || *s == '\0') continue;
+        double pos[D], vel[D];
+        int type;
+#if D == 3
+        if (sscanf(s, "%lf %lf %lf %lf %lf %lf %d", &pos[0], &pos[1], &pos[2],
+                   &vel[0], &vel[1], &vel[2], &type) != 7) {
+#else
+        if (sscanf(s, "%lf %lf %lf %lf %d", &pos[0], &pos[1], &vel[0], &vel[1], &type) != 5) {
+#endif
+            fprintf(stderr, "Warning: skipping malformed line: %s", line);
+            continue;
+        }
+        add_particle(ps, pos, vel, type);
+        loaded++;
+        if (type == FLUID_PARTICLE) n_fluid++;
+        else if (type == WALL_PARTICLE) n_wall++;
+    }
+    fclose(fp);
+    if (loaded != count)
+        fprintf(stderr, "Warning: expected %d particles but loaded %d\n", count, loaded);
+    printf("Loaded particles: %d fluid, %d wall, %d total\n", n_fluid, n_wall, ps->num);
+    return 0;
+}

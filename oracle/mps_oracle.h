@@ -138,6 +138,10 @@ int  orc_solve_csr(int n, const long *row_ptr, const int *col, const double *val
                    const double *b, double *x, int solver, int dense_ic,
                    int max_iter, double tol, const orc_opts *o, double *res_out);
 
+/* The particles.txt loader of main.c:20-85, line by line; messages as
+ * the reference prints them.  Returns 0, or -1 for a missing file or count. */
+int  orc_load_particles(const char *path, ParticleSystem *ps);
+
 /* accessors for language bindings */
 int  orc_sim_nlist(const orc_sim *s, const long **start, const int **idx);
 const orc_ppe *orc_sim_ppe(const orc_sim *s);

@@ -82,6 +82,7 @@ class Oracle:
                                     C.POINTER(C.c_double), C.c_int, C.c_int, C.c_int,
                                     C.c_double, vp, C.POINTER(C.c_double)]
         L.orc_solve_pressure.argtypes = [vp, PS, CF]
+        L.orc_load_particles.argtypes = [C.c_char_p, PS]
         if friction:
             L.orc_apply_friction.argtypes = [PS, CF]
 
