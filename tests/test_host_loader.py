@@ -65,6 +65,8 @@ def _load_both(dim, path, cap, capfd):
     lib = mps_amd.load(dim)
     orc = Oracle(dim)
     abi = A.types(dim)
+    _flush()              # C stdio output of earlier tests is not ours
+    capfd.readouterr()
     ps_l = lib.particle_system_create(cap)
     rc_l = lib.mps_load_particles(str(path).encode(), ps_l)
     _flush()
@@ -138,6 +140,8 @@ def test_oracle_loader_reads_shipped_inputs(rel, dim, capfd):
     pos, vel, typ = scenes.load_particles_txt(path, dim)
     abi = A.types(dim)
     ps, keep = _empty_system(abi, len(typ))
+    _flush()
+    capfd.readouterr()
     assert Oracle(dim).L.orc_load_particles(path.encode(), C.byref(ps)) == 0
     _flush()
     out = capfd.readouterr().out
