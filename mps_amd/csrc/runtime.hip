@@ -62,6 +62,9 @@ struct SnapshotWriter {
     int next = 0;
     std::thread th;
 
+    // a writer still running at exit finishes its files (the pinned
+    // buffers are left to the process teardown)
+    ~SnapshotWriter() { join(); }
     void join()
     {
         if (th.joinable()) th.join();
