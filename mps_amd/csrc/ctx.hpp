@@ -116,8 +116,8 @@ struct Packed {
     double *ddir = nullptr;          // mixed precision: Dirichlet part of each row
     unsigned *cq = nullptr;          // precision 3: (distance code << 12) | halo index, per entry
     size_t cap_cq = 0;
-    int *dmax = nullptr;             // device: largest halo
-    int *htot = nullptr;             // pinned: scan totals
+    int *dmax = nullptr;             // device, 8 ints: [0] largest halo, [1..4] scan totals
+    int *htot = nullptr;             // pinned: [0] rows, [1..5] a copy of dmax[0..4]
 };
 
 struct CgCtrl;   // device control block of the solvers (k_cg.hip)
@@ -184,6 +184,15 @@ void ctx_free(Ctx *c);
 int  launch_fill_int(int *p, int v, size_t n, hipStream_t s);
 int  launch_fill_double(double *p, double v, size_t n, hipStream_t s);
 int  launch_scan_excl(const int *in, int *out, int n, int *blk, hipStream_t s);
+// up to four independent exclusive scans in three launches; blk needs
+// 4 x 2050 ints, tot[k] receives array k's total (device memory)
+struct ScanSet {
+    const int *in[4];
+    int *out[4];
+    int n[4];
+    int count;
+};
+int  launch_scan_excl_multi(const ScanSet &ss, int *blk, int *tot, hipStream_t s);
 int  launch_unpack(Ctx *c, const void *stage_dev, int with_list);
 int  launch_pack(Ctx *c, void *stage_dev);
 int  launch_visc_predict(Ctx *c, const StepConsts &k);

@@ -389,7 +389,7 @@ static int pk_reserve_rows(Packed *p, int n)
         HIP_OK(hipMalloc(&p->bmeta, sizeof(int2) * nb));
         p->cap_rows = rows;
     }
-    if (!p->dmax) HIP_OK(hipMalloc(&p->dmax, sizeof(int)));
+    if (!p->dmax) HIP_OK(hipMalloc(&p->dmax, sizeof(int) * 8));
     if (!p->htot) HIP_OK(hipHostMalloc(&p->htot, sizeof(int) * 8, hipHostMallocDefault));
     return 0;
 }
@@ -442,16 +442,14 @@ int pack_stage_b(Ctx *c)
     hipLaunchKernelGGL(k_pk_pass1, dim3(nb), dim3(PKB), 0, s, n_eq, ns, c->stride, p->slot_of_row,
                        p->row_of_slot, c->Alen, c->Acol, p->sw, p->whw, p->w32, p->bh, p->bmode, p->dmax);
     LAUNCH_CHECK();
-    rc = launch_scan_excl(p->sw, p->voff, ns, c->scan_blk, s);
-    if (!rc) rc = launch_scan_excl(p->whw, p->chwoff, ns, c->scan_blk, s);
-    if (!rc) rc = launch_scan_excl(p->w32, p->c32off, ns, c->scan_blk, s);
-    if (!rc) rc = launch_scan_excl(p->bh, p->hoff, nb, c->scan_blk, s);
+    // value, halo-word and 32-bit column offsets per slice, halo offsets
+    // per block: one scan set, whose totals land beside the largest halo
+    // so that one copy brings all five back
+    const ScanSet ss = {{p->sw, p->whw, p->w32, p->bh}, {p->voff, p->chwoff, p->c32off, p->hoff},
+                        {ns, ns, ns, nb}, 4};
+    rc = launch_scan_excl_multi(ss, c->scan_blk, p->dmax + 1, s);
     if (rc) return rc;
-    HIP_OK(hipMemcpyAsync(&p->htot[1], p->voff + ns, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_OK(hipMemcpyAsync(&p->htot[2], p->chwoff + ns, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_OK(hipMemcpyAsync(&p->htot[3], p->c32off + ns, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_OK(hipMemcpyAsync(&p->htot[4], p->hoff + nb, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_OK(hipMemcpyAsync(&p->htot[5], p->dmax, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(&p->htot[1], p->dmax, sizeof(int) * 5, hipMemcpyDeviceToHost, s));
     return 0;
 }
 
@@ -462,11 +460,11 @@ int pack_stage_c(Ctx *c, int with_f32)
     const int n_eq = p->n_eq;
     if (n_eq == 0) return 0;
     const int ns = p->n_slices, nb = p->n_blocks;
-    p->tot_val = (size_t)p->htot[1];
-    p->tot_hw = (size_t)p->htot[2];
-    p->tot_c32 = (size_t)p->htot[3];
-    p->tot_halo = (size_t)p->htot[4];
-    p->hmax = p->htot[5];
+    p->hmax = p->htot[1];
+    p->tot_val = (size_t)p->htot[2];
+    p->tot_hw = (size_t)p->htot[3];
+    p->tot_c32 = (size_t)p->htot[4];
+    p->tot_halo = (size_t)p->htot[5];
     GROW(p->val, p->cap_val, p->tot_val * 64);
     GROW(p->chw, p->cap_hw, p->tot_hw * 64);
     GROW(p->c32, p->cap_c32, p->tot_c32 * 64);

@@ -164,6 +164,105 @@ int launch_scan_excl(const int *in, int *out, int n, int *blk, hipStream_t s)
     return 0;
 }
 
+// Up to four independent exclusive scans in the three launches of one:
+// grid.y selects the array, and array k's tile sums use its own segment of
+// the tile-sum buffer (MSCAN_SEG ints each: up to MSCAN_TILES tile sums and
+// the total).  out[k][0..n[k]] as launch_scan_excl writes it, and tot[k] =
+// its total.
+#define MSCAN_SEG 2050
+#define MSCAN_TILES 2048   // the tile-sum scan's 1024 threads x 2
+
+__global__ void __launch_bounds__(BLK) k_mscan_tiles(ScanSet ss, int *blk)
+{
+    const int k = blockIdx.y, n = ss.n[k];
+    if ((int)blockIdx.x * SCAN_TILE >= n) return;   // whole block: past this array's tiles
+    const int *in = ss.in[k];
+    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; q++)
+        if (base + q < n) s += in[base + q];
+    int tot;
+    (void)block_scan_excl<BLK>(s, &tot);
+    if (threadIdx.x == 0) blk[k * MSCAN_SEG + blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(1024) k_mscan_tile_sums(ScanSet ss, int *blk, int *tot_out)
+{
+    const int k = blockIdx.x, n = ss.n[k];
+    const int ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    int *tile_sum = blk + k * MSCAN_SEG;
+    const int base = threadIdx.x * 2;   // 1024 threads x 2 tiles cover MSCAN_TILES
+    int v[2];
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        v[q] = (base + q < ntiles) ? tile_sum[base + q] : 0;
+        s += v[q];
+    }
+    int tot;
+    int run = block_scan_excl<1024>(s, &tot);
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        if (base + q < ntiles) tile_sum[base + q] = run;
+        run += v[q];
+    }
+    if (threadIdx.x == 0) {
+        tile_sum[ntiles] = tot;
+        tot_out[k] = tot;
+        if (n == 0) ss.out[k][0] = 0;
+    }
+}
+
+__global__ void __launch_bounds__(BLK) k_mscan_apply(ScanSet ss, const int *blk)
+{
+    const int k = blockIdx.y, n = ss.n[k];
+    if ((int)blockIdx.x * SCAN_TILE >= n) return;
+    const int ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    const int *in = ss.in[k], *tile_sum = blk + k * MSCAN_SEG;
+    int *out = ss.out[k];
+    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    int v[SCAN_ITEMS];
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; q++) {
+        v[q] = (base + q < n) ? in[base + q] : 0;
+        s += v[q];
+    }
+    int tot;
+    int run = block_scan_excl<BLK>(s, &tot) + tile_sum[blockIdx.x];
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; q++) {
+        if (base + q < n) out[base + q] = run;
+        run += v[q];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = tile_sum[ntiles];
+}
+
+int launch_scan_excl_multi(const ScanSet &ss, int *blk, int *tot, hipStream_t s)
+{
+    if (ss.count < 1 || ss.count > 4) {
+        mps_set_error(MPS_ERR_ARG, __FILE__, __LINE__, "scan set of 1 to 4 arrays");
+        return MPS_ERR_ARG;
+    }
+    int max_tiles = 0;
+    for (int k = 0; k < ss.count; k++) {
+        const int t = (ss.n[k] + SCAN_TILE - 1) / SCAN_TILE;
+        if (ss.n[k] < 0 || t > MSCAN_TILES) {
+            mps_set_error(MPS_ERR_UNSUPPORTED, __FILE__, __LINE__, "scan set array longer than 4.19M elements");
+            return MPS_ERR_UNSUPPORTED;
+        }
+        if (t > max_tiles) max_tiles = t;
+    }
+    if (max_tiles > 0)
+        hipLaunchKernelGGL(k_mscan_tiles, dim3(max_tiles, ss.count), dim3(BLK), 0, s, ss, blk);
+    hipLaunchKernelGGL(k_mscan_tile_sums, dim3(ss.count), dim3(1024), 0, s, ss, blk, tot);
+    if (max_tiles > 0)
+        hipLaunchKernelGGL(k_mscan_apply, dim3(max_tiles, ss.count), dim3(BLK), 0, s, ss, blk);
+    LAUNCH_CHECK();
+    return 0;
+}
+
 // ---------------------------------------------------------------------
 // host AoS <-> device SoA
 // ---------------------------------------------------------------------
