@@ -62,3 +62,60 @@ def test_async_snapshots_same_bytes(request, tmp_path, libname, name):
         (tmp_path / "sync" / "output_000004.csv").read_bytes()
     for f in H.FIELDS:
         assert np.ascontiguousarray(a.arr[f]).tobytes() == np.ascontiguousarray(b.arr[f]).tobytes(), f
+
+
+def test_driver_matches_library_run(lib3, tmp_path):
+    """mps_amd/mps_sim3d (the reference main.c's flow: cal/params files,
+    loader, initial parameters, simulation_run) writes the same snapshot
+    bytes as the same entry points called through ctypes, and every
+    snapshot is complete when the process exits."""
+    import ctypes as C
+    import subprocess
+    import scenes
+    sc = scenes.CONFIGS["dam3d_small"]
+    case = tmp_path / "case"
+    case.mkdir()
+    d, w = list(sc["domain"]), list(sc["water"])
+    dims = (C.c_double * 6)(*(d + w))
+    n = lib3.mps_gen_dambreak(dims, sc["l0"], sc["wl"], sc["dl"], None, str(case / "particles.txt").encode())
+    assert n > 0
+    dt = 5e-4
+    (case / "params.txt").write_text(
+        "particle_distance %r\nwall_layers %d\ndummy_layers %d\ngravity_x 0\ngravity_y -9.81\ngravity_z 0\n"
+        "dt %r\nt_end %r\noutput_interval 2\ncg_tolerance 1e-8\nmax_neighbors 300\n"
+        "domain_x_min 0\ndomain_x_max %r\ndomain_y_min 0\ndomain_y_max %r\ndomain_z_min 0\ndomain_z_max %r\n"
+        "output_dir out\n" % (sc["l0"], sc["wl"], sc["dl"], dt, 6.5 * dt, d[0], d[1], d[2]))
+    (case / "cal.txt").write_text("particle_file particles.txt\nparam_file params.txt\n")
+    import mps_amd
+    driver = os.path.join(os.path.dirname(mps_amd.lib_path(3)), "mps_sim3d")
+    say("driver", driver, "particles", n)
+    p = subprocess.run([driver, "cal.txt"], cwd=str(case), capture_output=True, text=True, timeout=120)
+    say(p.stdout[-3000:])
+    say(p.stderr[-3000:])
+    assert p.returncode == 0, p.stderr[-2000:]
+    # the same flow through the library, into another directory
+    abi = lib3.abi
+    cfg = abi.SimConfig()
+    lib3.config_set_defaults(C.byref(cfg))
+    assert lib3.config_load_cal(str(case / "cal.txt").encode(), C.byref(cfg)) == 0
+    assert lib3.config_load_params(cfg.param_file, C.byref(cfg)) == 0
+    out_b = tmp_path / "lib_out"
+    out_b.mkdir()
+    cfg.output_dir = str(out_b).encode()
+    lib3.set_config(cfg)
+    lib3.set_option("tol_mode", 0)
+    lib3.set_option("precision", 0)
+    ps = lib3.particle_system_create(n)
+    assert lib3.mps_load_particles(cfg.particle_file, ps) == 0
+    lib3.particle_system_calc_initial_params(ps)
+    say("simulation_run through ctypes")
+    lib3.simulation_run(ps)
+    assert lib3.mps_hip_last_error() == 0
+    lib3.particle_system_free(ps)
+    fa = sorted(os.listdir(case / "out"))
+    assert fa == sorted(os.listdir(out_b)), (p.stdout[-1500:], p.stderr[-1500:])
+    assert fa == sorted("output_%06d.%s" % (s, e) for s in (0, 2, 4, 6) for e in ("csv", "vtk"))
+    assert sorted(os.listdir(out_b)) == fa
+    for f in fa:
+        x = (case / "out" / f).read_bytes()
+        assert len(x) > 0 and x == (out_b / f).read_bytes(), f

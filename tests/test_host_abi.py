@@ -221,3 +221,28 @@ def test_kernel_weight_host(lib):
     for r in (0.0, 1e-5, 1e-4, 0.005, 0.0309999, 0.031, 0.05):
         want = 0.0 if r >= re else re / max(r, 0.01 * 0.01) - 1.0
         assert lib.kernel_weight(r, re) == want
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+def test_driver_sees_its_config(dim, tmp_path):
+    """The stand-alone driver (the reference main.c's flow, linked against
+    the library) sets g_config in the executable; the library must read
+    that same variable, so simulation_run starts (it prints the device line
+    first, before any GPU work) instead of returning for want of a
+    config.  Runs without a GPU: the run stops at the device check."""
+    import subprocess
+    import mps_amd
+    sc = scenes.CONFIGS["dam3d_small" if dim == 3 else "dam2d_l0.01"]
+    lib = mps_amd.load(dim)
+    d = list(sc["domain"]) + ([0.0] if dim == 2 else [])
+    w = list(sc["water"]) + ([0.0] if dim == 2 else [])
+    dims = (C.c_double * 6)(*(d[:3] + w[:3]))
+    assert lib.mps_gen_dambreak(dims, sc["l0"], sc["wl"], sc["dl"], None,
+                                str(tmp_path / "particles.txt").encode()) > 0
+    (tmp_path / "params.txt").write_text("particle_distance %r\ndt 5e-4\nt_end 1e-3\noutput_interval 1\n"
+                                         "output_dir out\n" % sc["l0"])
+    (tmp_path / "cal.txt").write_text("particle_file particles.txt\nparam_file params.txt\n")
+    driver = os.path.join(os.path.dirname(mps_amd.lib_path(dim)), "mps_sim%dd" % dim)
+    p = subprocess.run([driver, "cal.txt"], cwd=str(tmp_path), capture_output=True, text=True, timeout=300)
+    assert "Loaded particles" in p.stdout
+    assert "HIP device:" in p.stdout, (p.stdout[-800:], p.stderr[-800:])
