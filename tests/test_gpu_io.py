@@ -64,7 +64,8 @@ def test_async_snapshots_same_bytes(request, tmp_path, libname, name):
         assert np.ascontiguousarray(a.arr[f]).tobytes() == np.ascontiguousarray(b.arr[f]).tobytes(), f
 
 
-def test_driver_matches_library_run(lib3, tmp_path):
+@pytest.mark.parametrize("libname,name", [("lib3", "dam3d_small"), ("lib2", "dam2d_l0.01")])
+def test_driver_matches_library_run(request, tmp_path, libname, name):
     """mps_amd/mps_sim3d (the reference main.c's flow: cal/params files,
     loader, initial parameters, simulation_run) writes the same snapshot
     bytes as the same entry points called through ctypes, and every
@@ -72,10 +73,13 @@ def test_driver_matches_library_run(lib3, tmp_path):
     import ctypes as C
     import subprocess
     import scenes
-    sc = scenes.CONFIGS["dam3d_small"]
+    lib3 = request.getfixturevalue(libname)     # the library of this dimension
+    sc = scenes.CONFIGS[name]
+    dim = sc["dim"]
     case = tmp_path / "case"
     case.mkdir()
-    d, w = list(sc["domain"]), list(sc["water"])
+    d = list(sc["domain"]) + [0.0] * (3 - dim)
+    w = list(sc["water"]) + [0.0] * (3 - dim)
     dims = (C.c_double * 6)(*(d + w))
     n = lib3.mps_gen_dambreak(dims, sc["l0"], sc["wl"], sc["dl"], None, str(case / "particles.txt").encode())
     assert n > 0
@@ -87,7 +91,7 @@ def test_driver_matches_library_run(lib3, tmp_path):
         "output_dir out\n" % (sc["l0"], sc["wl"], sc["dl"], dt, 6.5 * dt, d[0], d[1], d[2]))
     (case / "cal.txt").write_text("particle_file particles.txt\nparam_file params.txt\n")
     import mps_amd
-    driver = os.path.join(os.path.dirname(mps_amd.lib_path(3)), "mps_sim3d")
+    driver = os.path.join(os.path.dirname(mps_amd.lib_path(dim)), "mps_sim%dd" % dim)
     say("driver", driver, "particles", n)
     p = subprocess.run([driver, "cal.txt"], cwd=str(case), capture_output=True, text=True, timeout=120)
     say(p.stdout[-3000:])
