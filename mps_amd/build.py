@@ -73,7 +73,7 @@ def build_dim(dim, jobs=8, verbose=False, serial=False):
                           text=True).stdout.strip()
     tmp = lib_path(dim, serial) + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs +
-         [gomp, "-pthread", "-Wl,-Bsymbolic-functions"])
+         [gomp, "-pthread", "-Wl,--dynamic-list=" + os.path.join(HERE, "csrc", "dynamic.list")])
     os.replace(tmp, lib_path(dim, serial))
     # stand-alone driver, same CLI as the reference's mps_sim
     exe = os.path.join(HERE, "mps_sim%s" % tag)
