@@ -248,6 +248,8 @@ int cg_time_spmv(Ctx *c, int variant, int reps, double *ms);   // k_cg.hip
 // ---- runtime accessors (runtime.hip) -----------------------------------
 Ctx *mps_ctx_for(ParticleSystem *ps, int create);
 int mps_ctx_upload(Ctx *c, ParticleSystem *ps, const NeighborList *nl, const Grid &g);
+// particles only, device rows of `stride` entries (a caller's list capacity)
+int mps_ctx_upload_rows(Ctx *c, ParticleSystem *ps, int stride, const Grid &g);
 int mps_ctx_download(Ctx *c, ParticleSystem *ps, NeighborList *nl);
 int mps_ctx_search(Ctx *c, const StepConsts &k);
 StepConsts mps_consts(const SimConfig *cf, const ParticleSystem *ps, const Grid &g, double re);

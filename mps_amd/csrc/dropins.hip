@@ -164,13 +164,13 @@ void neighbor_search_cell_linked_list(NeighborList *nl, ParticleSystem *ps, Cell
     Grid g = cl ? mps_grid_from_cells(cl) : mps_grid_for(re, g_config->domain_min, g_config->domain_max);
     bool synced = false;
     if (!c->resident || !c->has_state || c->n != ps->num) {
-        if (mps_ctx_upload(c, ps, nullptr, g)) return;
+        // rows of nl's capacity: an overflow is found against nl->max_neighbors
+        if (mps_ctx_upload_rows(c, ps, nl->max_neighbors, g)) return;
         synced = true;
     }
     c->grid = g;
     StepConsts k = mps_consts(g_config, ps, g, re);
     k.stride = c->stride;
-    if (nl->max_neighbors < k.stride) k.stride = c->stride;   // device stride >= nl's
     if (mps_ctx_search(c, k)) return;
     if (synced) {
         // list into nl; the particle data itself is unchanged
@@ -210,7 +210,8 @@ void neighbor_search_brute_force(NeighborList *nl, ParticleSystem *ps, double re
         c->resident = false;
     }
     const double rs = mps_search_radius(g_config);
-    if (mps_ctx_upload(c, ps, nullptr, mps_grid_for(rs, g_config->domain_min, g_config->domain_max))) return;
+    if (mps_ctx_upload_rows(c, ps, nl->max_neighbors, mps_grid_for(rs, g_config->domain_min, g_config->domain_max)))
+        return;
     DevCounters z{};
     z.overflow_orig = INT_MAX;
     if (hipMemcpy(c->counters, &z, sizeof z, hipMemcpyHostToDevice) != hipSuccess) return;

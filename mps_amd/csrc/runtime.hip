@@ -466,17 +466,21 @@ static StepConsts make_consts(const SimConfig *cf, const ParticleSystem *ps, con
 // host <-> device
 // ---------------------------------------------------------------------
 
+// Capacity of a device neighbour row: the caller's list when there is one
+// (the reference's overflow test is count >= nl->max_neighbors,
+// neighbor_search.c:221-227), else the configuration's.
 static int stride_for(const SimConfig *cf, const NeighborList *nl)
 {
-    int s = cf ? cf->max_neighbors : 512;
-    if (nl && nl->max_neighbors > s) s = nl->max_neighbors;
+    int s = nl ? nl->max_neighbors : (cf ? cf->max_neighbors : 512);
     if (s < 1) s = 1;
     return s;
 }
 
-static int upload(Ctx *c, ParticleSystem *ps, const NeighborList *nl, const Grid &g)
+// Copies ps (and nl's lists, when nl is given) to the device; rows hold
+// `stride` entries (0: as stride_for).
+static int upload_rows(Ctx *c, ParticleSystem *ps, const NeighborList *nl, int stride, const Grid &g)
 {
-    int rc = ctx_alloc(c, ps->num, stride_for(g_config, nl), g.total);
+    int rc = ctx_alloc(c, ps->num, stride > 0 ? stride : stride_for(g_config, nl), g.total);
     if (rc) return rc;
     c->grid = g;
     if (ps->num > 0) {
@@ -519,6 +523,11 @@ static int upload(Ctx *c, ParticleSystem *ps, const NeighborList *nl, const Grid
     }
     c->has_state = true;
     return 0;
+}
+
+static int upload(Ctx *c, ParticleSystem *ps, const NeighborList *nl, const Grid &g)
+{
+    return upload_rows(c, ps, nl, 0, g);
 }
 
 static int download(Ctx *c, ParticleSystem *ps, NeighborList *nl)
@@ -1179,14 +1188,20 @@ void simulation_run(ParticleSystem *ps)
         cell_list_free(cl);
         return;
     }
+    // every early return frees the lists and leaves the error code set
+    auto fail = [&](const char *what) {
+        fprintf(stderr, "Error: %s%s\n", what, mps_hip_error_string(g_last_error));
+        cell_list_free(cl);
+        neighbor_list_free(nl);
+    };
     Ctx *c = get_ctx(ps);
-    if (!c) { fprintf(stderr, "Error: %s\n", mps_hip_error_string(g_last_error)); return; }
+    if (!c) { fail(""); return; }
     Grid g = grid_from_cells(cl);
-    if (upload(c, ps, nullptr, g)) { fprintf(stderr, "Error: upload failed: %s\n", g_last_msg); return; }
+    if (upload(c, ps, nullptr, g)) { fail("upload failed: "); return; }
     c->resident = true;
     StepConsts k = make_consts(g_config, ps, g, re);
     k.stride = c->stride;
-    if (search_only(c, k)) { fprintf(stderr, "Error: %s\n", g_last_msg); return; }
+    if (search_only(c, k)) { c->resident = false; fail(""); return; }
     // The step-0 writer reads ps->particles itself: nothing writes them
     // before the final download, which waits for every writer.
     bool async_out = g_opt.async_output > 0.5;
@@ -1241,6 +1256,7 @@ void simulation_run(ParticleSystem *ps)
 Ctx *mps_ctx_for(ParticleSystem *ps, int create) { return create ? get_ctx(ps) : find_ctx(ps); }
 int mps_ctx_step(Ctx *c, const SimConfig *cf) { return run_step(c, cf); }
 int mps_ctx_upload(Ctx *c, ParticleSystem *ps, const NeighborList *nl, const Grid &g) { return upload(c, ps, nl, g); }
+int mps_ctx_upload_rows(Ctx *c, ParticleSystem *ps, int stride, const Grid &g) { return upload_rows(c, ps, nullptr, stride, g); }
 int mps_ctx_download(Ctx *c, ParticleSystem *ps, NeighborList *nl) { return download(c, ps, nl); }
 int mps_ctx_search(Ctx *c, const StepConsts &k) { return search_only(c, k); }
 StepConsts mps_consts(const SimConfig *cf, const ParticleSystem *ps, const Grid &g, double re)

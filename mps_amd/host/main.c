@@ -43,8 +43,13 @@ int main(int argc, char *argv[])
         return 1;
     }
     particle_system_calc_initial_params(ps);
-    simulation_run(ps);
+    int rc = mps_hip_last_error();
+    if (rc == 0) {
+        simulation_run(ps);
+        rc = mps_hip_last_error();
+    }
+    if (rc != 0) fprintf(stderr, "Error: %s\n", mps_hip_error_string(rc));
     particle_system_free(ps);
     mps_hip_finalize();
-    return 0;
+    return rc != 0;
 }

@@ -7,6 +7,7 @@ stopping rule, so their outputs can be compared field by field.
 import ctypes as C
 import json
 import os
+import sys
 
 import numpy as np
 
@@ -120,3 +121,61 @@ def lists_equal(la, lb):
     if len(la) != len(lb):
         return False
     return all(np.array_equal(x, y) for x, y in zip(la, lb))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def step_compare(lib, name, steps, tol=1e-12, **over):
+    """Runs `steps` steps of scene `name` (with config overrides) through the
+    GPU library and two oracle runs that differ only in their dot-product
+    order (forward and backward serial sums), all with relative CG
+    tolerance `tol`.  Returns (gpu run, oracle run, per-step records) with
+    the GPU's relative error against the oracle and the oracle's own
+    dot-order spread for pressure, positions and velocities, and whether
+    n, flags and types are bit-identical."""
+    dim, cfg, pos, vel, typ = build(name, cg_tolerance=tol, **over)
+    lib.set_option("tol_mode", 1)
+    say("case", name, "N=%d" % len(typ), "steps", steps, over)
+    o = OracleRun(cfg, pos, vel, typ, dim, tol_mode=1)
+    ob = OracleRun(cfg, pos, vel, typ, dim, tol_mode=1, dot_order=1)
+    g = GpuRun(lib, cfg, pos, vel, typ, n0=o.scene.ps.n0, lam=o.scene.ps.lambda_)
+    out = []
+    for s in range(1, steps + 1):
+        assert g.step() == 0
+        o.step()
+        ob.step()
+        out.append(compare_states(s, g.scene.arr, o.scene.arr, ob.scene.arr))
+        say(out[-1])
+    return g, o, out
+
+
+def compare_states(step, a, b, c):
+    """Record of GPU state a against oracle state b, with c the oracle run
+    of the other dot-product order."""
+    return dict(step=step,
+                p_gpu=rel_err(a["pressure"], b["pressure"]),
+                p_spread=rel_err(c["pressure"], b["pressure"]),
+                x_gpu=rel_err(a["pos"], b["pos"]),
+                x_spread=rel_err(c["pos"], b["pos"]),
+                v_gpu=rel_err(a["vel"], b["vel"]),
+                v_spread=rel_err(c["vel"], b["vel"]),
+                n_exact=bool(np.array_equal(a["n"].view(np.uint64), b["n"].view(np.uint64))),
+                flags_exact=bool(np.array_equal(a["on_surface"], b["on_surface"])),
+                type_exact=bool(np.array_equal(a["type"], b["type"])))
+
+
+def assert_ppe_exact(lib, g, o):
+    """The GPU's last PPE, exported in the reference's layout, equals the
+    oracle's bit for bit: unknown numbering, pattern, values, RHS."""
+    q = lib.export_ppe(g.scene)
+    oq = o.ppe()
+    assert q["n_eq"] == oq["n_eq"]
+    assert np.array_equal(q["eq_idx"], oq["eq_idx"])
+    assert np.array_equal(q["row_ptr"], oq["row_ptr"].astype(np.int32))
+    assert np.array_equal(q["col"], oq["col"])
+    assert np.array_equal(q["val"].view(np.uint64), oq["val"].view(np.uint64))
+    assert np.array_equal(q["b"].view(np.uint64), oq["b"].view(np.uint64))
+    return q

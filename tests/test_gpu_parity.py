@@ -57,31 +57,7 @@ def test_initial_params_bitexact(lib3, name):
 
 
 def _step_compare(lib, name, steps, tol=1e-12, **over):
-    dim, cfg, pos, vel, typ = H.build(name, cg_tolerance=tol, **over)
-    lib.set_option("tol_mode", 1)
-    say("case", name, "N=%d" % len(typ), "steps", steps, over)
-    o = H.OracleRun(cfg, pos, vel, typ, dim, tol_mode=1)
-    ob = H.OracleRun(cfg, pos, vel, typ, dim, tol_mode=1, dot_order=1)
-    g = H.GpuRun(lib, cfg, pos, vel, typ, n0=o.scene.ps.n0, lam=o.scene.ps.lambda_)
-    out = []
-    for s in range(1, steps + 1):
-        assert g.step() == 0
-        o.step()
-        ob.step()
-        a, b, c = g.scene.arr, o.scene.arr, ob.scene.arr
-        rec = dict(step=s,
-                   p_gpu=H.rel_err(a["pressure"], b["pressure"]),
-                   p_spread=H.rel_err(c["pressure"], b["pressure"]),
-                   x_gpu=H.rel_err(a["pos"], b["pos"]),
-                   x_spread=H.rel_err(c["pos"], b["pos"]),
-                   v_gpu=H.rel_err(a["vel"], b["vel"]),
-                   v_spread=H.rel_err(c["vel"], b["vel"]),
-                   n_exact=bool(np.array_equal(a["n"].view(np.uint64), b["n"].view(np.uint64))),
-                   flags_exact=bool(np.array_equal(a["on_surface"], b["on_surface"])),
-                   type_exact=bool(np.array_equal(a["type"], b["type"])))
-        out.append(rec)
-        say(rec)
-    return g, o, out
+    return H.step_compare(lib, name, steps, tol, **over)
 
 
 def test_one_step_3d_bitexact_parts(lib3):
@@ -90,14 +66,7 @@ def test_one_step_3d_bitexact_parts(lib3):
     assert r["n_exact"] and r["flags_exact"] and r["type_exact"]
     assert H.lists_equal(g.lists(), o.lists())
     # PPE in the reference's layout
-    q = lib3.export_ppe(g.scene)
-    oq = o.ppe()
-    assert q["n_eq"] == oq["n_eq"]
-    assert np.array_equal(q["eq_idx"], oq["eq_idx"])
-    assert np.array_equal(q["row_ptr"], oq["row_ptr"].astype(np.int32))
-    assert np.array_equal(q["col"], oq["col"])
-    assert np.array_equal(q["val"].view(np.uint64), oq["val"].view(np.uint64))
-    assert np.array_equal(q["b"].view(np.uint64), oq["b"].view(np.uint64))
+    H.assert_ppe_exact(lib3, g, o)
     # FP64 solve: within 100x the reference's own dot-order spread (or 1e-10)
     assert r["p_gpu"] <= max(100 * r["p_spread"], 1e-10)
     assert r["x_gpu"] <= max(100 * r["x_spread"], 1e-14)
