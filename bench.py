@@ -760,7 +760,7 @@ def main():
         lib.check(lib.mps_hip_run_steps(ps, 1), "profiled step")
         lib.set_option("profile", 0)
     for name in ("visc_predict", "collide_bin", "sort", "search", "assemble", "spmv", "axpy",
-                 "pupdate", "grad_correct", "ic"):
+                 "pupdate", "grad_correct", "ic_factor", "ic_fwd", "ic_bwd"):
         if lib.mps_hip_profile_get(name.encode(), C.byref(ms), C.byref(cnt)) == 0 and cnt.value:
             kernels[name] = dict(total_ms=round(ms.value, 3), launches=cnt.value)
     lib.mps_hip_release(ps)
@@ -802,6 +802,7 @@ def main():
                    "dt": w["dt"], "solver": a.solver, "precision": a.precision,
                    "tolerance": "relative %g" % a.tol, "mean_iters_per_step": round(iters, 1),
                    "n_eq": st.n_eq, "nnz": st.nnz, "pairs": st.n_pairs, "parallelism": "1 GPU",
+                   "ic_levels": st.ic_levels if a.solver == "iccg" else None,
                    "spmv_variant": int(lib.L.mps_hip_get_option(b"spmv")),
                    "check_every": int(lib.L.mps_hip_get_option(b"check_every")), "graph": a.graph},
         "autotune": tune,

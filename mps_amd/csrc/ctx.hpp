@@ -266,6 +266,6 @@ int mps_ctx_step(Ctx *c, const SimConfig *cf);    // one resident timestep (runt
 
 // ---- timing (runtime.hip) ----------------------------------------------
 enum ProfKernel { PK_VISC, PK_COLLIDE, PK_SORT, PK_SEARCH, PK_ASSEMBLE, PK_SPMV, PK_AXPY,
-                  PK_PUPDATE, PK_GRAD, PK_IC, PK_PACK, PK_COUNT };
+                  PK_PUPDATE, PK_GRAD, PK_IC, PK_PACK, PK_ICF, PK_ICB, PK_ICL, PK_COUNT };
 void prof_begin(int k, hipStream_t s);
 void prof_end(int k, hipStream_t s);

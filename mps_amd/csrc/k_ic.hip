@@ -6,33 +6,56 @@
 // columns sorted (the reference's CSR, :437-551), then:
 //   factor    IC(0) row by row (pressure_solver.c:156-232 restated up-
 //             looking; same operations in the same order): entry (e,k) =
-//             (A_ek - sum_j L_ej L_kj) * dinv_k, d_e = A_ee - sum L_ep^2,
-//             d <= 0 -> A_ee (if > 0) or 1e-30, dinv_e = 1/sqrt(d)
+//             (A_ek - sum_j L_ej L_kj) * dinv_k over the common columns j in
+//             ascending order, d_e = A_ee - sum L_ep^2, d <= 0 -> A_ee (if
+//             > 0) or 1e-30, dinv_e = 1/sqrt(d)
 //   forward   y_e = (r_e - sum_p L_ep y_col) * dinv_e         (:235-243)
 //   backward  z_j = (y_j - sum_{i>j, descending} L_ij z_i) * dinv_j, the
 //             reference's column scatter (:246-255) as a gather over the
-//             upper half of row j (the pattern is symmetric)
+//             columns of L (the pattern is symmetric)
 // The 2-D program's dense formulas (mps_fluid2D/src/pressure_solver.c:
 // 86-141: sums first, divide by L_ii) are selected by `dense`.
 //
-// All three are dependency-bound (about 9*nx + 3*ny + nz levels for the
-// reference's lattice order).  They run "sync-free": persistent waves claim
-// 64-row chunks in row order from an atomic ticket, so every row a wave
-// waits on belongs to a wave that is already running; each lane publishes
-// its row's value with an agent-scope atomic store (the value itself, or
-// 1/sqrt(d) for the factor, replaces a sentinel NaN pattern) and polls the
-// values it needs with agent-scope loads, which bypass the per-XCD L2.
-// Every polling loop is bounded: past the bound the row is abandoned and an
-// error flag set, so the kernel always drains.
+// All three are dependency chains: a row's level is 1 + the largest level of
+// its strictly-lower columns, about 9*nx + 3*ny + nz levels in the
+// reference's lattice order (2,307 for the 1.08M-particle dam break).  A
+// level pass computes them, and every kernel claims rows in level order, so
+// the rows in flight sit on the dependency frontier.  All run sync-free,
+// with every wait bounded:
+//
+//   factor   one wavefront per row, rows claimed in level order from an
+//            atomic ticket (so a row only waits on rows of waves already
+//            running).  Lane t owns entry t of the row: it waits for row
+//            k = col_t (its 1/sqrt(d) published), streams row k's entries
+//            in batches, finds each column in the row's own columns (binary
+//            search in LDS) and folds the products in ascending column order
+//            as the entries of its own row it needs are finished by lower
+//            lanes.  The row's values live in LDS, so the in-row chain costs
+//            LDS latency, not a memory round trip.
+//   solves   one wavefront per row, claimed in level order (the backward
+//            solve in the reverse order).  Lane j loads entry j and the
+//            value it multiplies in one round trip, and the products are
+//            folded across the lanes in entry order up to the first value
+//            not yet published, which its lane polls for.
+// Values published to other waves use agent-scope stores and loads (they
+// bypass the per-XCD L2) over a sentinel NaN pattern: a published value is
+// its own ready flag, so no fence orders a flag after its data.  Every wait
+// counts its unsuccessful polls; past the option "ic_wait_limit" the wait
+// gives up (the term is dropped), an error flag is set and the solve reports
+// MPS_ERR_SOLVER, so every kernel always drains.
 #include <limits.h>
+#include <math.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "ctx.hpp"
 
 #define IC_BLK 256
-#define SPIN_LIMIT (1LL << 22)
+#define FB 16                 // row-k entries per batch in the factor
+#define SB 16                 // row entries per batch in the solves
+#define LDS_ROW_MAX 5400      // longest strictly-lower row the factor holds in LDS
 
 static inline unsigned nblk(size_t n, unsigned b = IC_BLK) { return (unsigned)((n + b - 1) / b); }
 
@@ -51,34 +74,90 @@ static inline unsigned nblk(size_t n, unsigned b = IC_BLK) { return (unsigned)((
 __device__ __forceinline__ bool is_sent(double v) { return __double_as_longlong(v) == (long long)SENT_BITS; }
 __device__ __forceinline__ double sent() { return __longlong_as_double((long long)SENT_BITS); }
 
-__device__ __forceinline__ double ld_acq(const double *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
 __device__ __forceinline__ double ld_rlx(const double *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_rel(double *p, double v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void st_rlx(double *p, double v)
 {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Orders LDS accesses of the lanes of one wavefront around this point (a
+// compiler barrier plus the wave's LDS counter).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ unsigned long long ballot64(bool p) { return (unsigned long long)__ballot(p); }
+
+// LDS values another wave (or lane) writes: workgroup-scope atomic accesses,
+// so the compiler keeps them as LDS operations and never caches them.
+template <typename T>
+__device__ __forceinline__ T lds_ld(const T *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <typename T>
+__device__ __forceinline__ void lds_st(T *p, T v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Sleep of a wave that made no progress in `idle` iterations in a row: about
+// 64 cycles at first, up to about 2,000 (1 us) for a long wait, so that
+// waves far from the dependency frontier poll rarely.
+__device__ __forceinline__ void backoff(int idle, int cap)
+{
+    if (idle < 4 || cap <= 0) __builtin_amdgcn_s_sleep(1);
+    else if (idle < 16 || cap == 1) __builtin_amdgcn_s_sleep(4);
+    else if (idle < 64 || cap == 2) __builtin_amdgcn_s_sleep(12);
+    else __builtin_amdgcn_s_sleep(32);
+}
+
 struct IcData {
     int n_eq = 0, cap_n = 0;
     long long nnz = 0, cap_nnz = 0;
+    // eq-order CSR of A (sorted columns, diagonal included) and the factor
     int *rp = nullptr, *col = nullptr, *dpos = nullptr, *tpos = nullptr;
     int *eq_of_slot = nullptr, *slot_of_eq = nullptr, *flag = nullptr, *eq_of_orig = nullptr, *len = nullptr;
-    double *aval = nullptr, *lval = nullptr, *dinv = nullptr;
+    double *aval = nullptr, *lval = nullptr, *dinv = nullptr, *beq = nullptr, *xeq = nullptr;
+    // the level order (rows by dependency level) and its inverse; the
+    // rows' levels
+    int *order = nullptr, *opos = nullptr;
+    double *lev = nullptr;
+    int *lhist = nullptr, *lstart = nullptr;
+    // the solve order (rows by bucket of levels, index order inside) and
+    // its inverse
+    int *sperm = nullptr, *spos = nullptr;
+    size_t cap_bh = 0;
+    int *bh = nullptr, *boff = nullptr;
+    const int *perm = nullptr;       // the order the vectors below follow (NULL: eq order)
+    // PCG vectors, by position of that order
     double *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *Ap = nullptr, *y = nullptr, *b = nullptr;
+    // A in sliced ELL by position (64-position slices, each row's entries in
+    // CSR order, columns renamed to positions)
+    int cap_sl = 0;
+    int *wA = nullptr, *oA = nullptr;
+    size_t cap_ea = 0;
+    int *acol = nullptr;
+    double *aell = nullptr;
+    // L and its columns (U) by position, rows contiguous (loff, uoff)
+    int *cntl = nullptr, *cntu = nullptr, *loff = nullptr, *uoff = nullptr;
+    size_t cap_lu = 0;
+    int *lcol = nullptr, *ucol = nullptr;
+    double *lv = nullptr, *uv = nullptr;
+    // device: [0] A's ELL total, [3] longest lower row, [4] largest level
+    int *hdr = nullptr;
+    int *hhdr = nullptr;                                 // pinned host copy
+    int *scan_blk = nullptr;
     unsigned *ticket = nullptr;   // [4]
     int *err = nullptr;
     double *part = nullptr;
-    int levels = 0;
+    int max_lower = 0, levels = 0, bucket = 0;
 };
 
 struct IcCtrl {
@@ -109,40 +188,57 @@ __global__ void k_ic_maps(int n, const int *O, const int *unk, const int *Alen, 
     }
 }
 
-// Row e of the eq-order CSR: off-diagonals then the diagonal, sorted by
-// column in place (insertion sort; columns of a row are distinct).
-__global__ void k_ic_fill(int n_eq, int stride, const int *slot_of_eq, const int *eq_of_slot,
-                          const int *Alen, const double *Aval, const int *Acol, const double *diag,
-                          const double *bslot, const int *rp, int *col, double *aval, int *dpos, double *beq)
+// Row e of the eq-order CSR: the off-diagonals of its unknown slot and the
+// diagonal, each placed at its rank among the row's columns (columns of a
+// row are distinct), so the row comes out sorted.  One wavefront per row;
+// the row's columns are staged in LDS 512 at a time and every lane counts
+// the smaller ones for its own entries.
+#define FILL_KCHUNK 512
+__global__ void __launch_bounds__(IC_BLK)
+k_ic_fill_sorted(int n_eq, int stride, const int *slot_of_eq, const int *eq_of_slot, const int *Alen,
+                 const double *Aval, const int *Acol, const double *diag, const double *bslot, const int *rp,
+                 int *col, double *aval, int *dpos, double *beq)
 {
-    const int e = blockIdx.x * IC_BLK + threadIdx.x;
-    if (e >= n_eq) return;
+    __shared__ int skey_all[IC_BLK / WAVE][FILL_KCHUNK];
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+    const int e = blockIdx.x * (IC_BLK / WAVE) + w;
+    if (e >= n_eq) return;   // whole wavefront
+    int *skey = skey_all[w];
     const int s = slot_of_eq[e];
-    const int m = Alen[s];
+    const int m = Alen[s];   // off-diagonals; entry m is the diagonal
+    const int tot = m + 1;
     const int base = rp[e];
-    for (int q = 0; q < m; q++) {
-        const int cq = eq_of_slot[Acol[sell_off(s, q, stride)]];
-        const double vq = Aval[sell_off(s, q, stride)];
-        int at = base + q;
-        while (at > base && col[at - 1] > cq) {
-            col[at] = col[at - 1];
-            aval[at] = aval[at - 1];
-            at--;
+    for (int q0 = 0; q0 < tot; q0 += WAVE) {
+        const int q = q0 + lane;
+        int key = INT_MAX;
+        double v = 0.0;
+        if (q < m) {
+            const size_t o = sell_off(s, q, stride);
+            key = eq_of_slot[Acol[o]];
+            v = Aval[o];
+        } else if (q == m) {
+            key = e;
+            v = diag[s];
         }
-        col[at] = cq;
-        aval[at] = vq;
+        int rank = 0;
+        for (int c0 = 0; c0 < tot; c0 += FILL_KCHUNK) {
+            const int cn = min(FILL_KCHUNK, tot - c0);
+            wave_sync();
+            for (int t = lane; t < cn; t += WAVE) {
+                const int qq = c0 + t;
+                skey[t] = qq < m ? eq_of_slot[Acol[sell_off(s, qq, stride)]] : e;
+            }
+            wave_sync();
+            if (q < tot)
+                for (int t = 0; t < cn; t++) rank += skey[t] < key;
+        }
+        if (q < tot) {
+            col[base + rank] = key;
+            aval[base + rank] = v;
+            if (q == m) dpos[e] = rank;
+        }
     }
-    // diagonal
-    int at = base + m;
-    while (at > base && col[at - 1] > e) {
-        col[at] = col[at - 1];
-        aval[at] = aval[at - 1];
-        at--;
-    }
-    col[at] = e;
-    aval[at] = diag[s];
-    dpos[e] = at - base;
-    beq[e] = bslot[s];
+    if (lane == 0) beq[e] = bslot[s];
 }
 
 // Position of entry (i, j) in row i for every upper entry (j, i), i > j.
@@ -163,186 +259,632 @@ __global__ void k_ic_tpos(int n_eq, const int *rp, const int *col, const int *dp
     }
 }
 
-// ---------------------------------------------------------------------
-// sync-free factor and triangular solves
-// ---------------------------------------------------------------------
-
-// Claims the next 64-row chunk for the calling wave (lane 0 takes the
-// ticket, the others read it from the wave).
-__device__ __forceinline__ int claim_chunk(unsigned *ticket)
+// Diagonal position of each row for a caller's CSR (columns sorted).
+__global__ void k_ic_dpos(int n, const int *rp, const int *col, int *dpos)
 {
-    int base = 0;
-    if ((threadIdx.x & (WAVE - 1)) == 0) base = (int)atomicAdd(ticket, (unsigned)WAVE);
-    return __shfl(base, 0, WAVE);
+    const int i = blockIdx.x * IC_BLK + threadIdx.x;
+    if (i >= n) return;
+    int q = rp[i];
+    while (q < rp[i + 1] && col[q] < i) q++;
+    dpos[i] = q - rp[i];
 }
 
-__global__ void __launch_bounds__(IC_BLK)
-k_ic_factor(int n, const int *__restrict__ rp, const int *__restrict__ col, const int *__restrict__ dpos,
-            const double *__restrict__ aval, double *lval, double *dinv, unsigned *ticket, int *err, int dense)
+// ---------------------------------------------------------------------
+// sliced-ELL copies
+// ---------------------------------------------------------------------
+
+__global__ void k_ic_hdr_reset(int *hdr, int count)
 {
+    if ((int)threadIdx.x < count) hdr[threadIdx.x] = 0;
+}
+
+// Rows are processed in a permuted order: position t holds row perm[t]
+// (perm NULL: the identity), and pos[r] is the position of row r.  The
+// sliced-ELL copies are laid out by position, with columns renamed to
+// positions; each row keeps its entries in the CSR's order, so every sum
+// runs as before.
+__device__ __forceinline__ int row_at(const int *perm, int t) { return perm ? perm[t] : t; }
+__device__ __forceinline__ int pos_of(const int *pos, int r) { return pos ? pos[r] : r; }
+
+// Width of each 64-position slice of A's sliced-ELL copy; hdr[3] = longest
+// strictly-lower row.
+__global__ void __launch_bounds__(IC_BLK)
+k_ic_slice_w(int n, const int *perm, const int *rp, const int *dpos, int *wA, int *hdr)
+{
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+    const int sl = blockIdx.x * (IC_BLK / WAVE) + w;
+    if (sl * WAVE >= n) return;
+    const int t = sl * WAVE + lane;
+    int la = 0, ll = 0;
+    if (t < n) {
+        const int r = row_at(perm, t);
+        la = rp[r + 1] - rp[r];
+        ll = dpos[r];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        la = max(la, __shfl_xor(la, o, WAVE));
+        ll = max(ll, __shfl_xor(ll, o, WAVE));
+    }
+    if (lane == 0) {
+        wA[sl] = la;
+        atomicMax(&hdr[3], ll);
+    }
+}
+
+// A in sliced ELL by position (for the SpMV).
+__global__ void k_ic_fill_a(int n, const int *perm, const int *pos, const int *rp, const int *col,
+                            const double *aval, const int *oA, int *acol, double *aell)
+{
+    const int t = blockIdx.x * IC_BLK + threadIdx.x;
+    if (t >= n) return;
+    const int r = row_at(perm, t);
+    const int r0 = rp[r], m = rp[r + 1] - r0;
+    const size_t b = (size_t)oA[t >> 6] * WAVE + (t & (WAVE - 1));
+    for (int q = 0; q < m; q++) {
+        acol[b + (size_t)q * WAVE] = pos_of(pos, col[r0 + q]);
+        aell[b + (size_t)q * WAVE] = aval[r0 + q];
+    }
+}
+
+// The factor's values (strictly lower entries, 1/sqrt(d)) reset to "not
+// computed".
+__global__ void k_ic_reset_factor(int n, const int *rp, const int *dpos, double *lval, double *dinv)
+{
+    const int e = blockIdx.x * IC_BLK + threadIdx.x;
+    if (e >= n) return;
+    for (int q = 0; q < dpos[e]; q++) lval[rp[e] + q] = sent();
+    dinv[e] = sent();
+}
+
+// L and its columns (U) by position from the finished factor, each row's
+// entries contiguous (a wavefront reads a row at once): row t of L at
+// loff[t], its columns in ascending eq order; row t of U (L_ij, i > j) at
+// uoff[t], descending i for the sparse formulas, ascending for the dense
+// ones; columns renamed to positions.
+__global__ void k_ic_fill_lu(int n, const int *perm, const int *pos, const int *rp, const int *col,
+                             const int *dpos, const int *tpos, const double *lval, const int *loff, const int *uoff,
+                             int *lcol, double *lv, int *ucol, double *uv, int dense)
+{
+    const int t = blockIdx.x * IC_BLK + threadIdx.x;
+    if (t >= n) return;
+    const int j = row_at(perm, t);
+    const int r0 = rp[j], m = dpos[j], nu = rp[j + 1] - r0 - m - 1;
+    const int bl = loff[t], bu = uoff[t];
+    for (int q = 0; q < m; q++) {
+        lcol[bl + q] = pos_of(pos, col[r0 + q]);
+        lv[bl + q] = lval[r0 + q];
+    }
+    for (int u = 0; u < nu; u++) {
+        const int p = r0 + m + 1 + u;   // (j, i), i ascending
+        const int tp = tpos[p];
+        const int slot = dense ? u : nu - 1 - u;
+        ucol[bu + slot] = pos_of(pos, col[p]);
+        uv[bu + slot] = tp >= 0 ? lval[tp] : 0.0;   // asymmetric pattern: no (i, j)
+    }
+}
+
+__global__ void k_ic_gather(int n, const int *perm, const double *src, double *dst)
+{
+    const int t = blockIdx.x * IC_BLK + threadIdx.x;
+    if (t < n) dst[t] = src[row_at(perm, t)];
+}
+
+__global__ void k_ic_scatter(int n, const int *perm, const double *src, double *dst)
+{
+    const int t = blockIdx.x * IC_BLK + threadIdx.x;
+    if (t < n) dst[row_at(perm, t)] = src[t];
+}
+
+// ---------------------------------------------------------------------
+// the level order
+// ---------------------------------------------------------------------
+
+__global__ void k_ic_lev_hist(int n, const double *lev, int *hist, int *hdr)
+{
+    const int e = blockIdx.x * IC_BLK + threadIdx.x;
+    if (e >= n) return;
+    const int L = (int)lev[e];
+    atomicAdd(&hist[L], 1);
+    atomicMax(&hdr[4], L);
+}
+
+__global__ void k_ic_lev_scatter(int n, const double *lev, int *cursor, int *fperm)
+{
+    const int e = blockIdx.x * IC_BLK + threadIdx.x;
+    if (e >= n) return;
+    fperm[atomicAdd(&cursor[(int)lev[e]], 1)] = e;
+}
+
+// Inverse of the level order: the position of each row.
+__global__ void k_ic_inverse(int n, const int *perm, int *pos)
+{
+    const int t = blockIdx.x * IC_BLK + threadIdx.x;
+    if (t < n) pos[perm[t]] = t;
+}
+
+// The solve order: rows by bucket of 2^gshift consecutive levels, ascending
+// index inside a bucket.  Dependencies point to lower indices and lower
+// levels, so this is a topological order for any bucket size, and inside a
+// bucket a lattice column contributes a run of consecutive rows, whose
+// dependencies along the column then stay inside one chunk of the solves.
+// Per block of 256 rows (index order): the block's rows in each bucket,
+// bh[b * nb + block].
+__global__ void __launch_bounds__(IC_BLK)
+k_ic_bucket_hist(int n, const double *lev, int gshift, int nbk, int *bh)
+{
+    __shared__ int h[IC_BLK];
+    const int t = threadIdx.x, nb = gridDim.x;
+    h[t] = 0;
+    __syncthreads();
+    const int e = blockIdx.x * IC_BLK + t;
+    if (e < n) atomicAdd(&h[(int)lev[e] >> gshift], 1);
+    __syncthreads();
+    if (t < nbk) bh[(size_t)t * nb + blockIdx.x] = h[t];
+}
+
+// Stable scatter: row e goes to its bucket's offset for this block plus its
+// rank among the block's rows of the same bucket.
+__global__ void __launch_bounds__(IC_BLK)
+k_ic_bucket_scatter(int n, const double *lev, int gshift, const int *boff, int *sperm, int *spos)
+{
+    __shared__ int bk[IC_BLK];
+    const int t = threadIdx.x, nb = gridDim.x;
+    const int e = blockIdx.x * IC_BLK + t;
+    const int b = e < n ? ((int)lev[e] >> gshift) : -1;
+    bk[t] = b;
+    __syncthreads();
+    if (e >= n) return;
+    int rank = 0;
+    for (int u = 0; u < t; u++) rank += bk[u] == b;
+    const int p = boff[(size_t)b * nb + blockIdx.x] + rank;
+    sperm[p] = e;
+    spos[e] = p;
+}
+
+// Entry counts by position: strictly lower (L) and strictly upper (L's
+// columns) of the row at each position.
+__global__ void k_ic_counts(int n, const int *perm, const int *rp, const int *dpos, int *cl, int *cu)
+{
+    const int t = blockIdx.x * IC_BLK + threadIdx.x;
+    if (t >= n) return;
+    const int r = row_at(perm, t);
+    cl[t] = dpos[r];
+    cu[t] = rp[r + 1] - rp[r] - dpos[r] - 1;
+}
+
+// ---------------------------------------------------------------------
+// factor: one wavefront per row
+// ---------------------------------------------------------------------
+
+__global__ void __launch_bounds__(IC_BLK)
+k_ic_factor(int n, const int *__restrict__ order, const int *__restrict__ rp, const int *__restrict__ col,
+            const int *__restrict__ dpos, const double *__restrict__ aval, double *lval, double *dinv,
+            unsigned *ticket, int *err, int dense, int lcap, long long wait_limit, int sleep_cap)
+{
+    extern __shared__ int f_smem[];
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x & (WAVE - 1);
+    int *scol = f_smem + (size_t)w * lcap * 3;
+    double *sval = (double *)(f_smem + (size_t)w * lcap * 3 + lcap);
+    bool failed = false;
     for (;;) {
-        const int base = claim_chunk(ticket);
-        if (base >= n) return;
-        const int e = base + (threadIdx.x & (WAVE - 1));
-        const bool mine = e < n;
-        int p = 0, pend = 0, row0 = 0;
-        if (mine) {
-            row0 = rp[e];
-            p = row0;
-            pend = row0 + dpos[e];
-        }
-        bool done = !mine;
-        long long spins = 0;
-        while (__any(!done)) {
-            if (!done) {
-                if (p < pend) {
-                    const int k = col[p];
-                    const double dk = ld_acq(&dinv[k]);
-                    if (!is_sent(dk)) {
-                        const int kb = rp[k], ke = rp[k] + dpos[k];
-                        if (!dense) {
-                            double s = aval[p];
-                            int a = row0, bq = kb;
-                            while (a < p && bq < ke) {
-                                const int ja = col[a], jb = col[bq];
-                                if (ja < jb) a++;
-                                else if (ja > jb) bq++;
-                                else { s -= lval[a] * lval[bq]; a++; bq++; }
-                            }
-                            lval[p] = s * dk;
-                        } else {
-                            const double aik = aval[p];
-                            if (fabs(aik) < 1.0e-30) {
-                                lval[p] = 0.0;
-                            } else {
-                                double s = 0.0;
-                                int a = row0, bq = kb;
-                                while (a < p && bq < ke) {
-                                    const int ja = col[a], jb = col[bq];
-                                    if (ja < jb) a++;
-                                    else if (ja > jb) bq++;
-                                    else { s += lval[a] * lval[bq]; a++; bq++; }
-                                }
-                                lval[p] = (aik - s) * (1.0 / dk);   // dk holds sqrt(d) here
+        int tk = 0;
+        if (lane == 0) tk = (int)atomicAdd(ticket, 1u);
+        tk = __shfl(tk, 0, WAVE);
+        if (tk >= n) break;
+        const int e = order[tk];   // rows in level order: every row waited on is claimed earlier
+        const int r0 = rp[e], m = dpos[e];
+        wave_sync();
+        for (int t = lane; t < m; t += WAVE) scol[t] = col[r0 + t];
+        wave_sync();
+        const int pstep = m > 0 ? (1 << (31 - __clz(m))) : 0;
+        for (int g0 = 0; g0 < m; g0 += WAVE) {
+            const int T = g0 + lane;
+            const bool act = T < m;
+            int k = 0, kb = 0, mk = 0;
+            double a = 0.0;
+            if (act) {
+                k = scol[T];
+                a = aval[r0 + T];
+                kb = rp[k];
+                mk = dpos[k];
+            }
+            int stage = act ? 0 : 2;   // 0: waiting for row k, 1: folding, 2: done
+            double dk = 0.0, s = dense ? 0.0 : a;
+            int u = 0, bn = 0, bi = 0;
+            int bpos[FB];
+            double bval[FB];
+#pragma unroll
+            for (int j = 0; j < FB; j++) { bpos[j] = -1; bval[j] = 0.0; }
+            long long spins = 0;
+            bool poll_all = true;
+            int idle = 0;
+            for (;;) {
+                if (ballot64(stage != 2) == 0ull) break;
+                const unsigned long long donem = ballot64(stage == 2);
+                const unsigned long long waitm = ballot64(stage == 0);
+                bool prog = false;
+                if (waitm) {
+                    const int lo = __ffsll((long long)waitm) - 1;
+                    if (stage == 0 && (poll_all || lane == lo)) {
+                        const double d = ld_rlx(&dinv[k]);
+                        if (!is_sent(d)) {
+                            dk = d;
+                            stage = 1;
+                            prog = true;
+                            spins = 0;
+                        } else if (++spins > wait_limit) {
+                            failed = true;   // give up on row k: its terms read as 0
+                            stage = 1;
+                        }
+                    }
+                    poll_all = __shfl((int)(stage != 0), lo, WAVE) != 0;
+                }
+                if (stage == 1 && bi == bn) {
+                    if (u >= mk) {
+                        double L;
+                        if (!dense) L = s * dk;
+                        else L = (fabs(a) < 1.0e-30) ? 0.0 : (a - s) * (1.0 / dk);   // dk holds sqrt(d) here
+                        lds_st(&sval[T], L);
+                        st_rlx(&lval[r0 + T], L);
+                        stage = 2;
+                        prog = true;
+                    } else {
+                        bn = min(FB, mk - u);
+                        int bc[FB];
+#pragma unroll
+                        for (int j = 0; j < FB; j++) {
+                            bc[j] = INT_MAX;
+                            if (j < bn) bc[j] = col[kb + u + j];
+                        }
+#pragma unroll
+                        for (int j = 0; j < FB; j++) bval[j] = j < bn ? ld_rlx(&lval[kb + u + j]) : 0.0;
+                        // row k's columns are all below k = scol[T]: find each
+                        // among this row's columns 0..T-1 (last index < c)
+                        int ps[FB];
+#pragma unroll
+                        for (int j = 0; j < FB; j++) ps[j] = -1;
+                        for (int st = pstep; st > 0; st >>= 1) {
+#pragma unroll
+                            for (int j = 0; j < FB; j++) {
+                                const int cand = ps[j] + st;
+                                if (cand < T && scol[cand] < bc[j]) ps[j] = cand;
                             }
                         }
-                        p++;
-                        spins = 0;
-                    } else if (++spins > SPIN_LIMIT) {
-                        atomicExch(err, 1);
-                        p = pend;   // abandon: finish the row with what is there
+#pragma unroll
+                        for (int j = 0; j < FB; j++) {
+                            const int cand = ps[j] + 1;
+                            bpos[j] = (j < bn && cand < T && scol[cand] == bc[j]) ? cand : -1;
+                        }
+                        bi = 0;
+                        u += bn;
+                        prog = true;
                     }
+                }
+                if (stage == 1 && bi < bn) {
+                    bool stop = false;
+#pragma unroll
+                    for (int j = 0; j < FB; j++) {
+                        if (!stop && j == bi && j < bn) {
+                            const int v = bpos[j];
+                            if (v < 0) {
+                                bi++;
+                                prog = true;
+                            } else {
+                                double lk = bval[j];
+                                if (is_sent(lk)) {
+                                    lk = ld_rlx(&lval[kb + u - bn + j]);
+                                    bval[j] = lk;
+                                }
+                                const bool ready = !is_sent(lk) && (v < g0 || ((donem >> (v - g0)) & 1ull));
+                                if (ready) {
+                                    const double lv = lds_ld(&sval[v]);
+                                    if (!dense) s -= lv * lk;
+                                    else s += lv * lk;
+                                    bi++;
+                                    prog = true;
+                                    spins = 0;
+                                } else if (++spins > wait_limit) {
+                                    failed = true;   // give up on this term
+                                    bi++;
+                                } else {
+                                    stop = true;
+                                }
+                            }
+                        }
+                    }
+                }
+                if (__any(prog)) idle = 0;
+                else backoff(idle++, sleep_cap);
+            }
+            wave_sync();
+        }
+        // d_e from the finished row, in column order (every lane computes it)
+        const double akk = aval[r0 + m];
+        double d;
+        if (!dense) {
+            d = akk;
+            for (int q = 0; q < m; q++) {
+                const double l = lds_ld(&sval[q]);
+                d -= l * l;
+            }
+        } else {
+            double sq = 0.0;
+            for (int q = 0; q < m; q++) {
+                const double l = lds_ld(&sval[q]);
+                sq += l * l;
+            }
+            d = akk - sq;
+        }
+        if (d <= 0.0) d = (akk > 0.0) ? akk : 1.0e-30;
+        const double out = dense ? sqrt(d) : 1.0 / sqrt(d);
+        // the row's values are stored before its 1/sqrt(d) (readers check
+        // each value anyway)
+        __builtin_amdgcn_s_waitcnt(0);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        if (lane == 0) st_rlx(&dinv[e], out);
+        wave_sync();
+    }
+    if (ballot64(failed) && lane == 0) atomicOr(err, 1);
+}
+
+// ---------------------------------------------------------------------
+// triangular solves: one workgroup per chunk of 256 rows
+// ---------------------------------------------------------------------
+
+// Dependency levels, sync-free over chunks of 256 consecutive rows (index
+// order, chunks claimed in order): lev_e = 1 + the largest level among row
+// e's strictly-lower columns (0 without any).  Levels of rows of the same
+// chunk pass through LDS, so a lattice column's chain stays on chip; other
+// levels are read from HBM, where each row publishes its level over the
+// sentinel.  A row's columns are read from the eq-order CSR in batches whose
+// outside levels are all requested at once.
+__global__ void __launch_bounds__(IC_BLK)
+k_ic_levels(int n, const int *__restrict__ rp, const int *__restrict__ dpos, const int *__restrict__ col,
+            double *lev, unsigned *ticket, int *err, long long wait_limit, int sleep_cap)
+{
+    __shared__ double sy[IC_BLK];
+    __shared__ int sflag[IC_BLK];
+    __shared__ int sclaim;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1);
+    lds_st(&sflag[tid], -1);
+    const int nch = (n + IC_BLK - 1) / IC_BLK;
+    bool failed = false;
+    for (;;) {
+        __syncthreads();
+        if (tid == 0) sclaim = (int)atomicAdd(ticket, 1u);
+        __syncthreads();
+        const int tk = sclaim;
+        if (tk >= nch) break;
+        const int base = tk * IC_BLK;
+        const int e = base + tid;
+        bool done = e >= n;
+        int m = 0;
+        size_t eb = 0;
+        double s = -1.0;
+        if (!done) {
+            m = dpos[e];
+            eb = (size_t)rp[e];
+        }
+        int q = 0, bn = 0, bi = 0;
+        int bc[SB];
+        double bv[SB];
+#pragma unroll
+        for (int j = 0; j < SB; j++) { bc[j] = 0; bv[j] = 0.0; }
+        long long spins = 0;
+        int idle = 0;
+        for (;;) {
+            if (ballot64(!done) == 0ull) break;
+            bool prog = false;
+            if (!done && bi == bn) {
+                if (q >= m) {
+                    const double out = s + 1.0;
+                    lds_st(&sy[tid], out);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");   // value before flag
+                    lds_st(&sflag[tid], tk);
+                    st_rlx(&lev[e], out);
+                    done = true;
+                    prog = true;
                 } else {
-                    const double akk = aval[pend];
-                    double d;
-                    if (!dense) {
-                        d = akk;
-                        for (int q = row0; q < pend; q++) d -= lval[q] * lval[q];
-                    } else {
-                        double sq = 0.0;
-                        for (int q = row0; q < pend; q++) sq += lval[q] * lval[q];
-                        d = akk - sq;
+                    bn = min(SB, m - q);
+#pragma unroll
+                    for (int j = 0; j < SB; j++)
+                        if (j < bn) bc[j] = col[eb + (size_t)(q + j)];
+#pragma unroll
+                    for (int j = 0; j < SB; j++) bv[j] = (j < bn && bc[j] < base) ? ld_rlx(&lev[bc[j]]) : 0.0;
+                    q += bn;
+                    bi = 0;
+                    prog = true;
+                }
+            }
+            bool blocked = false;   // waits on a level outside the chunk
+            if (!done && bi < bn) {
+                bool stop = false;
+#pragma unroll
+                for (int j = 0; j < SB; j++) {
+                    if (!stop && j == bi && j < bn) {
+                        const int c = bc[j];
+                        const bool ext = c < base;
+                        if (ext && is_sent(bv[j])) bv[j] = ld_rlx(&lev[c]);
+                        bool ready;
+                        double v;
+                        if (ext) {
+                            v = bv[j];
+                            ready = !is_sent(v);
+                        } else {
+                            ready = lds_ld(&sflag[c - base]) == tk;
+                            v = 0.0;
+                            if (ready) {
+                                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                                v = lds_ld(&sy[c - base]);
+                            }
+                        }
+                        if (ready) {
+                            s = fmax(s, v);
+                            bi++;
+                            prog = true;
+                            spins = 0;
+                        } else if (++spins > wait_limit) {
+                            failed = true;   // give up on this column
+                            bi++;
+                        } else {
+                            stop = true;
+                            blocked = ext;
+                        }
                     }
-                    if (d <= 0.0) d = (akk > 0.0) ? akk : 1.0e-30;
-                    st_rel(&dinv[e], dense ? sqrt(d) : 1.0 / sqrt(d));
-                    done = true;
                 }
             }
+            if (__any(prog)) idle = 0;
+            else backoff(idle++, sleep_cap);
+            (void)blocked;
         }
     }
+    if (ballot64(failed) && lane == 0) atomicOr(err, 1);
 }
 
-// y = L^-1 r (sparse: (r - sum) * dinv; dense: (r - sum) / Lkk)
-__global__ void __launch_bounds__(IC_BLK)
-k_ic_forward(int n, const int *__restrict__ rp, const int *__restrict__ col, const int *__restrict__ dpos,
-             const double *__restrict__ lval, const double *__restrict__ dinv, const double *__restrict__ rin,
-             double *y, unsigned *ticket, int *err, int dense, const IcCtrl *ctrl)
+// Both halves of a 64-bit value from lane k (k uniform).
+__device__ __forceinline__ double readlane_d(double v, int k)
 {
-    if (ctrl && ctrl->done) return;
-    for (;;) {
-        const int base = claim_chunk(ticket);
-        if (base >= n) return;
-        const int e = base + (threadIdx.x & (WAVE - 1));
-        const bool mine = e < n;
-        int p = 0, pend = 0;
-        double s = 0.0;
-        if (mine) {
-            p = rp[e];
-            pend = p + dpos[e];
-            s = dense ? 0.0 : rin[e];
-        }
-        bool done = !mine;
-        long long spins = 0;
-        while (__any(!done)) {
-            if (!done) {
-                while (p < pend) {
-                    const double yc = ld_rlx(&y[col[p]]);
-                    if (is_sent(yc)) break;
-                    if (dense) s += lval[p] * yc;
-                    else s -= lval[p] * yc;
-                    p++;
-                    spins = 0;
-                }
-                if (p >= pend) {
-                    st_rlx(&y[e], dense ? (rin[e] - s) / dinv[e] : s * dinv[e]);
-                    done = true;
-                } else if (++spins > SPIN_LIMIT) {
-                    atomicExch(err, 1);
-                    st_rlx(&y[e], 0.0);
-                    done = true;
-                }
-            }
-        }
-    }
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffull), k);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), k);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// z = L^-T y as a gather over the upper half of row j, rows claimed from
-// the bottom; sparse: subtract in descending i then * dinv; dense: sum in
-// ascending i then (y - sum) / Ljj.
-__global__ void __launch_bounds__(IC_BLK)
-k_ic_backward(int n, const int *__restrict__ rp, const int *__restrict__ col, const int *__restrict__ dpos,
-              const int *__restrict__ tpos, const double *__restrict__ lval, const double *__restrict__ dinv,
-              const double *__restrict__ yin, double *z, unsigned *ticket, int *err, int dense,
-              const IcCtrl *ctrl)
+// Substitution over chunks of CS_CHUNK consecutive positions of the solve
+// order, one workgroup per chunk (chunks claimed in order; BACK: from the
+// end).  The workgroup sorts its chunk's rows by dependency level (BACK:
+// descending), and its wavefronts claim rows one at a time from that list,
+// so the lattice columns of a chunk advance together, level by level, as
+// the DAG lets them.  Position t holds row perm[t] (level lev[perm[t]]); its
+// entries sit at eoff[t] .. eoff[t+1] (columns are positions).  Lane j
+// loads entry j and the value it multiplies -- from LDS when that row is in
+// the chunk, from HBM otherwise -- all in one round trip; the products are
+// then folded in entry order across the lanes (every lane holds the running
+// sum) up to the first value not yet published, which its lane polls for.
+// MODE 0 sparse: s = r, s -= l v, out = s * dinv; MODE 1 dense: s = 0,
+// s += l v, out = (r - s) / dinv (dinv holds L_ii).  rin and y are by
+// position.
+#define CS_WAVES 8
+#define CS_CHUNK 128
+template <int MODE, bool BACK>
+__global__ void __launch_bounds__(CS_WAVES * WAVE)
+k_ic_csolve(int n, const int *__restrict__ perm, const double *__restrict__ lev, const int *__restrict__ eoff,
+            const int *__restrict__ ecol, const double *__restrict__ eval, const double *__restrict__ dinv,
+            const double *__restrict__ rin, double *y, unsigned *ticket, int *err, long long wait_limit,
+            int sleep_cap, const IcCtrl *ctrl)
 {
+    __shared__ double sy[CS_CHUNK];
+    __shared__ int sflag[CS_CHUNK];
+    __shared__ int skey[CS_CHUNK], srow[CS_CHUNK];
+    __shared__ int sclaim, snext;
     if (ctrl && ctrl->done) return;
+    const int lane = threadIdx.x & (WAVE - 1);
+    if (threadIdx.x < CS_CHUNK) lds_st(&sflag[threadIdx.x], -1);
+    const int nch = (n + CS_CHUNK - 1) / CS_CHUNK;
+    bool failed = false;
     for (;;) {
-        const int claimed = claim_chunk(ticket);
-        if (claimed >= n) return;
-        const int j = n - 1 - (claimed + (threadIdx.x & (WAVE - 1)));
-        const bool mine = j >= 0;
-        int p = 0, pstop = 0, step = 0;
-        double s = 0.0;
-        if (mine) {
-            const int up0 = rp[j] + dpos[j] + 1, up1 = rp[j + 1];
-            if (!dense) { p = up1 - 1; pstop = up0 - 1; step = -1; s = yin[j]; }
-            else { p = up0; pstop = up1; step = 1; s = 0.0; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            sclaim = (int)atomicAdd(ticket, 1u);
+            snext = 0;
         }
-        bool done = !mine;
-        long long spins = 0;
-        while (__any(!done)) {
-            if (!done) {
-                while (p != pstop) {
-                    const double zc = ld_rlx(&z[col[p]]);
-                    if (is_sent(zc)) break;
-                    const int tp = tpos[p];
-                    const double l = tp >= 0 ? lval[tp] : 0.0;   // asymmetric pattern: no (i,j)
-                    if (dense) s += l * zc;
-                    else s -= l * zc;
-                    p += step;
-                    spins = 0;
+        __syncthreads();
+        const int tk = sclaim;
+        if (tk >= nch) break;
+        const int ch = BACK ? nch - 1 - tk : tk;
+        const int cbase = ch * CS_CHUNK, cnt_rows = min(CS_CHUNK, n - cbase);
+        // the chunk's rows in dependency order: by level, then position
+        // (BACK: both descending); ranks are distinct, so srow is a
+        // permutation of the chunk
+        if (threadIdx.x < cnt_rows) {
+            const int t = cbase + threadIdx.x;
+            const int L = (int)lev[perm ? perm[t] : t];
+            skey[threadIdx.x] = BACK ? -L : L;
+        }
+        __syncthreads();
+        if (threadIdx.x < cnt_rows) {
+            const int me = threadIdx.x, km = skey[me];
+            int rank = 0;
+            for (int u = 0; u < cnt_rows; u++) {
+                const int ku = skey[u];
+                rank += ku < km || (ku == km && (BACK ? u > me : u < me));
+            }
+            srow[rank] = me;
+        }
+        __syncthreads();
+        for (;;) {
+            int i = 0;
+            if (lane == 0) i = atomicAdd(&snext, 1);
+            i = __shfl(i, 0, WAVE);
+            if (i >= cnt_rows) break;
+            const int t = cbase + srow[i];
+            const int r = perm ? perm[t] : t;
+            const int o0 = eoff[t], m = eoff[t + 1] - o0;
+            const double rv = rin[t];
+            double s = MODE == 0 ? rv : 0.0;
+            for (int g0 = 0; g0 < m; g0 += WAVE) {
+                const int cnt = min(WAVE, m - g0);
+                const bool act = lane < cnt;
+                int c = 0;
+                bool in = false;
+                double l = 0.0, v = 0.0;
+                if (act) {
+                    c = ecol[o0 + g0 + lane];
+                    l = eval[o0 + g0 + lane];
+                    in = c >= cbase && c < cbase + cnt_rows;
+                    if (!in) v = ld_rlx(&y[c]);
                 }
-                if (p == pstop) {
-                    st_rlx(&z[j], dense ? (yin[j] - s) / dinv[j] : s * dinv[j]);
-                    done = true;
-                } else if (++spins > SPIN_LIMIT) {
-                    atomicExch(err, 1);
-                    st_rlx(&z[j], 0.0);
-                    done = true;
+                if (act && in) {
+                    v = sent();
+                    if (lds_ld(&sflag[c - cbase]) == tk) {
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                        v = lds_ld(&sy[c - cbase]);
+                    }
                 }
+                int k = 0;
+                long long spins = 0;
+                int idle = 0;
+                while (k < cnt) {
+                    const unsigned long long miss = ballot64(act && is_sent(v)) & (~0ull << k);
+                    const int stop = miss ? min(cnt, __ffsll((long long)miss) - 1) : cnt;
+                    const double pr = l * v;
+                    for (; k < stop; k++) {
+                        const double pk = readlane_d(pr, k);
+                        if (MODE == 0) s -= pk;
+                        else s += pk;
+                    }
+                    if (k >= cnt) break;
+                    // lane k's value (and maybe later ones) not yet there
+                    if (act && is_sent(v)) {
+                        if (!in) {
+                            v = ld_rlx(&y[c]);
+                        } else if (lds_ld(&sflag[c - cbase]) == tk) {
+                            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                            v = lds_ld(&sy[c - cbase]);
+                        }
+                    }
+                    if (!is_sent(readlane_d(v, k))) {
+                        spins = 0;
+                        idle = 0;
+                    } else if (++spins > wait_limit) {
+                        failed = true;   // give up on this term
+                        if (lane == k) v = 0.0;
+                    } else {
+                        backoff(idle++, sleep_cap);
+                    }
+                }
+            }
+            const double out = MODE == 0 ? s * dinv[r] : (rv - s) / dinv[r];
+            if (lane == 0) {
+                lds_st(&sy[t - cbase], out);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");   // value before flag
+                lds_st(&sflag[t - cbase], tk);
+                st_rlx(&y[t], out);
             }
         }
     }
+    if (ballot64(failed) && lane == 0) atomicOr(err, 1);
 }
 
 __global__ void k_ic_sentinel(int n, double *a, double *b)
@@ -351,6 +893,11 @@ __global__ void k_ic_sentinel(int n, double *a, double *b)
         if (a) a[i] = sent();
         if (b) b[i] = sent();
     }
+}
+
+__global__ void k_ic_ticket_reset(unsigned *t)
+{
+    if (threadIdx.x < 4) t[threadIdx.x] = 0u;
 }
 
 // ---------------------------------------------------------------------
@@ -371,17 +918,36 @@ static inline unsigned vgrid(int n)
     return g < 1 ? 1 : (g > 2048 ? 2048 : g);
 }
 
-// CSR y = A x, rows summed in column order (csr_mat_vec, :57-68), + partial x.y
+// y = A x from the sliced-ELL copy (by position), each row summed in its
+// CSR column order (csr_mat_vec, :57-68), + the block partials of x.y
 __global__ void __launch_bounds__(IC_BLK)
-k_csr_spmv_dot(int n, const int *__restrict__ rp, const int *__restrict__ col, const double *__restrict__ val,
-               const double *__restrict__ x, double *__restrict__ y, double *part, const IcCtrl *ctrl)
+k_ic_spmv(int n, const int *__restrict__ perm, const int *__restrict__ rp, const int *__restrict__ oA,
+          const int *__restrict__ acol, const double *__restrict__ aell, const double *__restrict__ x,
+          double *__restrict__ y, double *part, const IcCtrl *ctrl)
 {
     if (ctrl->done) return;
     const int i = blockIdx.x * IC_BLK + threadIdx.x;
     double v = 0.0;
     if (i < n) {
+        const int ri = row_at(perm, i);
+        const int m = rp[ri + 1] - rp[ri];
+        const size_t b = (size_t)oA[i >> 6] * WAVE + (i & (WAVE - 1));
         double s = 0.0;
-        for (int q = rp[i]; q < rp[i + 1]; q++) s += val[q] * x[col[q]];
+        int q = 0;
+        for (; q + 8 <= m; q += 8) {
+            int c[8];
+            double a[8], xv[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                c[j] = acol[b + (size_t)(q + j) * WAVE];
+                a[j] = aell[b + (size_t)(q + j) * WAVE];
+            }
+#pragma unroll
+            for (int j = 0; j < 8; j++) xv[j] = x[c[j]];
+#pragma unroll
+            for (int j = 0; j < 8; j++) s += a[j] * xv[j];
+        }
+        for (; q < m; q++) s += aell[b + (size_t)q * WAVE] * x[acol[b + (size_t)q * WAVE]];
         y[i] = s;
         v = x[i] * s;
     }
@@ -536,49 +1102,55 @@ __global__ void k_ic_scatter_x(int n_slots, const int *eq_of_slot, const double 
 // host side
 // ---------------------------------------------------------------------
 
-static int ic_reserve(IcData *d, int n_slots, int n_eq, long long nnz)
+template <typename T>
+static int regrow(T **p, size_t n)
+{
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    HIP_OK(hipMalloc(p, sizeof(T) * (n > 0 ? n : 1)));
+    return 0;
+}
+
+#define REGROW(p, n)                       \
+    do {                                   \
+        int rc_ = regrow(&(p), (size_t)(n)); \
+        if (rc_) return rc_;               \
+    } while (0)
+
+static int ic_reserve(IcData *d, int n_slots, long long nnz)
 {
     if (n_slots + 1 > d->cap_n) {
-        int *pi[] = {d->rp, d->dpos, d->eq_of_slot, d->slot_of_eq, d->flag, d->eq_of_orig, d->len};
-        for (int *q : pi) if (q) hipFree(q);
-        double *pd[] = {d->dinv, d->x, d->r, d->z, d->p, d->Ap, d->y, d->b};
-        for (double *q : pd) if (q) hipFree(q);
         const size_t N = (size_t)n_slots + 2;
-        HIP_OK(hipMalloc(&d->rp, sizeof(int) * N));
-        HIP_OK(hipMalloc(&d->dpos, sizeof(int) * N));
-        HIP_OK(hipMalloc(&d->eq_of_slot, sizeof(int) * N));
-        HIP_OK(hipMalloc(&d->slot_of_eq, sizeof(int) * N));
-        HIP_OK(hipMalloc(&d->flag, sizeof(int) * N));
-        HIP_OK(hipMalloc(&d->eq_of_orig, sizeof(int) * N));
-        HIP_OK(hipMalloc(&d->len, sizeof(int) * N));
-        HIP_OK(hipMalloc(&d->dinv, sizeof(double) * N));
-        HIP_OK(hipMalloc(&d->x, sizeof(double) * N));
-        HIP_OK(hipMalloc(&d->r, sizeof(double) * N));
-        HIP_OK(hipMalloc(&d->z, sizeof(double) * N));
-        HIP_OK(hipMalloc(&d->p, sizeof(double) * N));
-        HIP_OK(hipMalloc(&d->Ap, sizeof(double) * N));
-        HIP_OK(hipMalloc(&d->y, sizeof(double) * N));
-        HIP_OK(hipMalloc(&d->b, sizeof(double) * N));
+        REGROW(d->rp, N); REGROW(d->dpos, N); REGROW(d->eq_of_slot, N); REGROW(d->slot_of_eq, N);
+        REGROW(d->flag, N); REGROW(d->eq_of_orig, N); REGROW(d->len, N);
+        REGROW(d->dinv, N); REGROW(d->beq, N); REGROW(d->xeq, N);
+        REGROW(d->order, N); REGROW(d->opos, N); REGROW(d->lev, N);
+        REGROW(d->sperm, N); REGROW(d->spos, N);
+        REGROW(d->lhist, N + 1); REGROW(d->lstart, N + 1);
+        REGROW(d->cntl, N); REGROW(d->cntu, N); REGROW(d->loff, N + 1); REGROW(d->uoff, N + 1);
+        REGROW(d->x, N); REGROW(d->r, N); REGROW(d->z, N); REGROW(d->p, N);
+        REGROW(d->Ap, N); REGROW(d->y, N); REGROW(d->b, N);
         d->cap_n = n_slots + 1;
     }
+    const int nsl = (n_slots + WAVE - 1) / WAVE + 1;
+    if (nsl > d->cap_sl) {
+        REGROW(d->wA, nsl);
+        REGROW(d->oA, nsl + 1);
+        d->cap_sl = nsl;
+    }
     if (nnz > d->cap_nnz) {
-        if (d->col) hipFree(d->col);
-        if (d->tpos) hipFree(d->tpos);
-        if (d->aval) hipFree(d->aval);
-        if (d->lval) hipFree(d->lval);
         const size_t Z = (size_t)(nnz > 0 ? nnz : 1);
-        HIP_OK(hipMalloc(&d->col, sizeof(int) * Z));
-        HIP_OK(hipMalloc(&d->tpos, sizeof(int) * Z));
-        HIP_OK(hipMalloc(&d->aval, sizeof(double) * Z));
-        HIP_OK(hipMalloc(&d->lval, sizeof(double) * Z));
+        REGROW(d->col, Z); REGROW(d->tpos, Z); REGROW(d->aval, Z); REGROW(d->lval, Z);
         d->cap_nnz = nnz;
     }
     if (!d->ticket) {
         HIP_OK(hipMalloc(&d->ticket, sizeof(unsigned) * 4));
         HIP_OK(hipMalloc(&d->err, sizeof(int)));
         HIP_OK(hipMalloc(&d->part, sizeof(double) * 3 * 65536));
+        HIP_OK(hipMalloc(&d->hdr, sizeof(int) * 8));
+        HIP_OK(hipHostMalloc(&d->hhdr, sizeof(int) * 8, hipHostMallocDefault));
+        HIP_OK(hipMalloc(&d->scan_blk, sizeof(int) * 4 * 2050));
     }
-    (void)n_eq;
     return 0;
 }
 
@@ -586,58 +1158,219 @@ void ic_free(IcData *d)
 {
     if (!d) return;
     void *ps[] = {d->rp, d->col, d->dpos, d->tpos, d->eq_of_slot, d->slot_of_eq, d->flag, d->eq_of_orig, d->len,
-                  d->aval, d->lval, d->dinv, d->x, d->r, d->z, d->p, d->Ap, d->y, d->b, d->ticket, d->err, d->part};
-    for (void *q : ps) if (q) hipFree(q);
+                  d->aval, d->lval, d->dinv, d->beq, d->xeq, d->order, d->opos, d->lev, d->lhist, d->lstart,
+                  d->sperm, d->spos, d->bh, d->boff,
+                  d->x, d->r, d->z, d->p, d->Ap, d->y, d->b, d->wA, d->oA, d->acol, d->aell, d->cntl, d->cntu,
+                  d->loff, d->uoff, d->lcol, d->ucol, d->lv, d->uv, d->hdr, d->scan_blk, d->ticket, d->err,
+                  d->part};
+    for (void *q : ps) if (q) (void)hipFree(q);
+    if (d->hhdr) (void)hipHostFree(d->hhdr);
     delete d;
+}
+
+static long long wait_limit()
+{
+    const double v = mps_hip_get_option("ic_wait_limit");
+    return v >= 1.0 ? (long long)v : 1;
+}
+
+static int sleep_cap()
+{
+    const double v = mps_hip_get_option("ic_sleep");
+    return v < 0 ? 0 : (v > 3 ? 3 : (int)v);
 }
 
 static int reset_ticket(IcData *d, hipStream_t s)
 {
-    HIP_OK(hipMemsetAsync(d->ticket, 0, sizeof(unsigned) * 4, s));
+    hipLaunchKernelGGL(k_ic_ticket_reset, dim3(1), dim3(64), 0, s, d->ticket);
+    LAUNCH_CHECK();
     return 0;
 }
 
-static unsigned persistent_grid()
+static unsigned cu_count()
 {
     static unsigned g = 0;
     if (!g) {
         int dev = 0, cus = 256;
-        hipGetDevice(&dev);
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        g = (unsigned)(cus > 0 ? cus : 256) * 2;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        g = (unsigned)(cus > 0 ? cus : 256);
     }
     return g;
 }
 
-static int ic_factor(IcData *d, hipStream_t s, int dense)
+// Dependency levels of the rows (needs rp, col, dpos), then the level order
+// (rows sorted by level; within a level in any order, since a level's rows
+// are independent) and its inverse.  The factor and both solves claim rows
+// in this order, which keeps the rows in flight on the dependency frontier:
+// claimed in index order, only the few slabs of the lattice that fit the
+// resident waves overlap, and the chain they leave is far longer than the
+// DAG's depth.
+#define LEVEL_BUCKET 32   // levels per bucket of the solve order (at least)
+
+static int ic_order(IcData *d, hipStream_t s)
 {
-    if (d->n_eq == 0) return 0;
-    hipLaunchKernelGGL(k_ic_sentinel, dim3(vgrid(d->n_eq)), dim3(IC_BLK), 0, s, d->n_eq, d->dinv, (double *)nullptr);
+    const int n = d->n_eq;
     int rc = reset_ticket(d, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_ic_factor, dim3(persistent_grid()), dim3(IC_BLK), 0, s, d->n_eq, d->rp, d->col, d->dpos,
-                       d->aval, d->lval, d->dinv, d->ticket, d->err, dense);
+    hipLaunchKernelGGL(k_ic_hdr_reset, dim3(1), dim3(64), 0, s, d->hdr, 8);
+    hipLaunchKernelGGL(k_ic_sentinel, dim3(vgrid(n)), dim3(IC_BLK), 0, s, n, d->lev, (double *)nullptr);
+    prof_begin(PK_ICL, s);
+    hipLaunchKernelGGL(k_ic_levels, dim3(cu_count() * 2), dim3(IC_BLK), 0, s, n, d->rp, d->dpos, d->col, d->lev,
+                       d->ticket, d->err, wait_limit(), sleep_cap());
+    prof_end(PK_ICL, s);
+    LAUNCH_CHECK();
+    rc = launch_fill_int(d->lhist, 0, (size_t)n + 1, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_ic_lev_hist, dim3(nblk(n)), dim3(IC_BLK), 0, s, n, d->lev, d->lhist, d->hdr);
+    LAUNCH_CHECK();
+    rc = launch_scan_excl(d->lhist, d->lstart, n + 1, d->scan_blk, s);
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(d->lhist, d->lstart, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_ic_lev_scatter, dim3(nblk(n)), dim3(IC_BLK), 0, s, n, d->lev, d->lhist, d->order);
+    hipLaunchKernelGGL(k_ic_inverse, dim3(nblk(n)), dim3(IC_BLK), 0, s, n, d->order, d->opos);
+    LAUNCH_CHECK();
+    HIP_OK(hipMemcpyAsync(d->hhdr, d->hdr, sizeof(int) * 8, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    d->levels = d->hhdr[4] + 1;
+    // the solve order: buckets of 2^gshift levels, at most 256 of them
+    int gshift = 0;
+    const double want = mps_hip_get_option("ic_bucket");
+    if (want >= 0) gshift = (int)want;
+    else
+        while ((1 << gshift) < LEVEL_BUCKET) gshift++;
+    while (((d->levels - 1) >> gshift) >= IC_BLK) gshift++;
+    const int nbk = ((d->levels - 1) >> gshift) + 1;
+    d->bucket = 1 << gshift;
+    const unsigned nb = nblk(n);
+    const size_t nbh = (size_t)nbk * nb;
+    if (nbh + 1 > d->cap_bh) {
+        REGROW(d->bh, nbh + 1);
+        REGROW(d->boff, nbh + 1);
+        d->cap_bh = nbh + 1;
+    }
+    hipLaunchKernelGGL(k_ic_bucket_hist, dim3(nb), dim3(IC_BLK), 0, s, n, d->lev, gshift, nbk, d->bh);
+    LAUNCH_CHECK();
+    rc = launch_scan_excl(d->bh, d->boff, (int)nbh, d->scan_blk, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_ic_bucket_scatter, dim3(nb), dim3(IC_BLK), 0, s, n, d->lev, gshift, d->boff, d->sperm,
+                       d->spos);
     LAUNCH_CHECK();
     return 0;
 }
 
-// z = M^-1 r with y as scratch; y and z must hold the sentinel already.
-// With ctrl given, both solves return at once once the solver has stopped.
+// A's sliced-ELL copy and the right-hand side in the order perm (NULL: eq
+// order); needs rp, col, aval, dpos, beq.
+static int build_ell(IcData *d, hipStream_t s, const int *perm, const int *pos)
+{
+    const int n = d->n_eq;
+    const int nsl = (n + WAVE - 1) / WAVE;
+    d->perm = perm;
+    hipLaunchKernelGGL(k_ic_hdr_reset, dim3(1), dim3(64), 0, s, d->hdr, 4);
+    hipLaunchKernelGGL(k_ic_slice_w, dim3(nblk((size_t)nsl * WAVE)), dim3(IC_BLK), 0, s, n, perm, d->rp, d->dpos,
+                       d->wA, d->hdr);
+    LAUNCH_CHECK();
+    int rc = launch_scan_excl(d->wA, d->oA, nsl, d->scan_blk, s);
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(d->hhdr, d->hdr, sizeof(int) * 8, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(d->hhdr + 5, d->oA + nsl, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    const size_t ea = (size_t)d->hhdr[5] * WAVE;
+    d->max_lower = d->hhdr[3];
+    if (ea > d->cap_ea) {
+        REGROW(d->acol, ea); REGROW(d->aell, ea);
+        d->cap_ea = ea;
+    }
+    hipLaunchKernelGGL(k_ic_fill_a, dim3(nblk(n)), dim3(IC_BLK), 0, s, n, perm, pos, d->rp, d->col, d->aval, d->oA,
+                       d->acol, d->aell);
+    // the right-hand side by position
+    hipLaunchKernelGGL(k_ic_gather, dim3(nblk(n)), dim3(IC_BLK), 0, s, n, perm, d->beq, d->b);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+// The factor in level order, then L and its columns by position of the
+// solve order.  Needs rp, col, aval, dpos, tpos, ic_order and build_ell
+// with the solve order.
+static int ic_factor(IcData *d, hipStream_t s, int dense)
+{
+    const int n = d->n_eq;
+    if (n == 0) return 0;
+    int lcap = d->max_lower > 0 ? d->max_lower : 1;
+    lcap += lcap & 1;   // even: the values that follow the columns stay 8-byte aligned
+    if (lcap > LDS_ROW_MAX) {
+        mps_set_error(MPS_ERR_UNSUPPORTED, __FILE__, __LINE__,
+                      "IC(0): a PPE row has more strictly-lower entries than the factor holds in LDS");
+        return MPS_ERR_UNSUPPORTED;
+    }
+    // four waves (rows) per workgroup while their rows fit 64 KB of LDS
+    const int wpb = (size_t)lcap * 12 * 4 <= 65536 ? 4 : 1;
+    const size_t lds = (size_t)lcap * 12 * wpb;
+    hipLaunchKernelGGL(k_ic_reset_factor, dim3(nblk(n)), dim3(IC_BLK), 0, s, n, d->rp, d->dpos, d->lval, d->dinv);
+    int rc = reset_ticket(d, s);
+    if (rc) return rc;
+    const double og = mps_hip_get_option("ic_fgrid");
+    const unsigned grid = og >= 1 ? (unsigned)og : cu_count() * (wpb == 4 ? 4 : 8);
+    prof_begin(PK_ICF, s);
+    hipLaunchKernelGGL(k_ic_factor, dim3(grid), dim3(WAVE * wpb), lds, s, n, d->order, d->rp, d->col, d->dpos,
+                       d->aval, d->lval, d->dinv, d->ticket, d->err, dense, lcap, wait_limit(), sleep_cap());
+    prof_end(PK_ICF, s);
+    LAUNCH_CHECK();
+    // L and U by position of the solve order, rows contiguous
+    hipLaunchKernelGGL(k_ic_counts, dim3(nblk(n)), dim3(IC_BLK), 0, s, n, d->perm, d->rp, d->dpos, d->cntl, d->cntu);
+    LAUNCH_CHECK();
+    rc = launch_scan_excl(d->cntl, d->loff, n, d->scan_blk, s);
+    if (!rc) rc = launch_scan_excl(d->cntu, d->uoff, n, d->scan_blk, s);
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(d->hhdr + 6, d->loff + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(d->hhdr + 7, d->uoff + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    const size_t nl = (size_t)(d->hhdr[6] > d->hhdr[7] ? d->hhdr[6] : d->hhdr[7]);
+    if (nl > d->cap_lu) {
+        REGROW(d->lcol, nl); REGROW(d->lv, nl); REGROW(d->ucol, nl); REGROW(d->uv, nl);
+        d->cap_lu = nl;
+    }
+    hipLaunchKernelGGL(k_ic_fill_lu, dim3(nblk(n)), dim3(IC_BLK), 0, s, n, d->perm, d->spos, d->rp, d->col, d->dpos,
+                       d->tpos, d->lval, d->loff, d->uoff, d->lcol, d->lv, d->ucol, d->uv, dense);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+// z = M^-1 r with y as scratch (by position of the solve order); y and z
+// must hold the sentinel already.  With ctrl given, both solves return at
+// once once the solver has stopped.
 static int ic_apply(IcData *d, const double *r, hipStream_t s, int dense, const IcCtrl *ctrl)
 {
     int rc = reset_ticket(d, s);
     if (rc) return rc;
+    const long long wl = wait_limit();
+    const int sc = sleep_cap();
+    const double og = mps_hip_get_option("ic_grid");
+    const unsigned grid = og >= 1 ? (unsigned)og : cu_count();
+    auto fwd = dense ? k_ic_csolve<1, false> : k_ic_csolve<0, false>;
+    auto bwd = dense ? k_ic_csolve<1, true> : k_ic_csolve<0, true>;
     prof_begin(PK_IC, s);
-    hipLaunchKernelGGL(k_ic_forward, dim3(persistent_grid()), dim3(IC_BLK), 0, s, d->n_eq, d->rp, d->col, d->dpos,
-                       d->lval, d->dinv, r, d->y, d->ticket + 0, d->err, dense, ctrl);
-    hipLaunchKernelGGL(k_ic_backward, dim3(persistent_grid()), dim3(IC_BLK), 0, s, d->n_eq, d->rp, d->col,
-                       d->dpos, d->tpos, d->lval, d->dinv, d->y, d->z, d->ticket + 1, d->err, dense, ctrl);
+    hipLaunchKernelGGL(fwd, dim3(grid), dim3(CS_WAVES * WAVE), 0, s, d->n_eq, d->perm, d->lev, d->loff, d->lcol,
+                       d->lv, d->dinv, r, d->y, d->ticket + 0, d->err, wl, sc, ctrl);
     prof_end(PK_IC, s);
+    prof_begin(PK_ICB, s);
+    hipLaunchKernelGGL(bwd, dim3(grid), dim3(CS_WAVES * WAVE), 0, s, d->n_eq, d->perm, d->lev, d->uoff, d->ucol,
+                       d->uv, d->dinv, d->y, d->z, d->ticket + 1, d->err, wl, sc, ctrl);
+    prof_end(PK_ICB, s);
     LAUNCH_CHECK();
     return 0;
 }
 
-// PCG (precond 1) or CG (precond 0) on the eq-order CSR held in d.
+static int clear_err(IcData *d, hipStream_t s)
+{
+    HIP_OK(hipMemsetAsync(d->err, 0, sizeof(int), s));
+    return 0;
+}
+
+// PCG (precond 1) or CG (precond 0) on the system held in d by position of
+// d->perm (build_ell done); the solution is left in d->xeq in eq order.
+// The caller clears d->err before the level pass and the factor; it is read
+// back here, after the solves.
 static int pcg_run(IcData *d, hipStream_t s, IcCtrl *dctrl, IcCtrl *hctrl, const SolveOpts &o, int precond,
                    SolveResult *res)
 {
@@ -645,7 +1378,6 @@ static int pcg_run(IcData *d, hipStream_t s, IcCtrl *dctrl, IcCtrl *hctrl, const
     const unsigned gv = vgrid(n), gm = nblk(n);
     double *pa = d->part, *pb = d->part + 65536, *pc = d->part + 2 * 65536;
     const int dense = o.dense_ic;
-    HIP_OK(hipMemsetAsync(d->err, 0, sizeof(int), s));
     hipLaunchKernelGGL(k_pcg_init, dim3(gv), dim3(IC_BLK), 0, s, n, d->b, d->x, d->r, pa);
     if (precond) {
         hipLaunchKernelGGL(k_ic_sentinel, dim3(gv), dim3(IC_BLK), 0, s, n, d->y, d->z);
@@ -663,42 +1395,55 @@ static int pcg_run(IcData *d, hipStream_t s, IcCtrl *dctrl, IcCtrl *hctrl, const
     HIP_OK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     HIP_OK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
     volatile int *hdone = &hctrl->done;
-    int pending = -1, t = 0, k = 0;
+    int pending = -1, t = 0, k = 0, rc = 0;
     const double *zz = precond ? d->z : d->r;
-    while (t < o.max_iter) {
+    while (t < o.max_iter && !rc) {
         const int t_end = (t + chunk < o.max_iter) ? t + chunk : o.max_iter;
-        for (; t < t_end; t++) {
+        for (; t < t_end && !rc; t++) {
             prof_begin(PK_SPMV, s);
-            hipLaunchKernelGGL(k_csr_spmv_dot, dim3(gm), dim3(IC_BLK), 0, s, n, d->rp, d->col, d->aval, d->p, d->Ap,
-                               pa, dctrl);
+            hipLaunchKernelGGL(k_ic_spmv, dim3(gm), dim3(IC_BLK), 0, s, n, d->perm, d->rp, d->oA, d->acol, d->aell,
+                               d->p, d->Ap, pa, dctrl);
             prof_end(PK_SPMV, s);
             hipLaunchKernelGGL(k_pcg_fin_pap, dim3(1), dim3(IC_BLK), 0, s, pa, gm, dctrl, t);
             hipLaunchKernelGGL(k_pcg_axpy, dim3(gv), dim3(IC_BLK), 0, s, n, d->p, d->Ap, d->x, d->r, d->z, d->y, pb,
                                dctrl, precond);
             hipLaunchKernelGGL(k_pcg_fin_rr, dim3(1), dim3(IC_BLK), 0, s, pb, gv, dctrl, t, o.max_iter);
-            if (precond) {
-                // the solves return at once after the stop flag is set
-                int rc = ic_apply(d, d->r, s, dense, dctrl);
-                if (rc) return rc;
-            }
+            if (precond) rc = ic_apply(d, d->r, s, dense, dctrl);   // return at once after the stop flag
             hipLaunchKernelGGL(k_pcg_dot, dim3(gv), dim3(IC_BLK), 0, s, n, d->r, zz, pc, dctrl);
             hipLaunchKernelGGL(k_pcg_fin_rz, dim3(1), dim3(IC_BLK), 0, s, pc, gv, dctrl, t);
             hipLaunchKernelGGL(k_pcg_pupdate, dim3(gv), dim3(IC_BLK), 0, s, n, zz, d->p, dctrl);
         }
-        LAUNCH_CHECK();
-        HIP_OK(hipMemcpyAsync((void *)hdone, &dctrl->done, sizeof(int), hipMemcpyDeviceToHost, s));
-        HIP_OK(hipEventRecord(ev[k & 1], s));
+        if (rc) break;
+        const hipError_t le = hipGetLastError();
+        if (le != hipSuccess) {
+            mps_set_error(MPS_ERR_HIP, __FILE__, __LINE__, hipGetErrorString(le));
+            rc = MPS_ERR_HIP;
+            break;
+        }
+        if (hipMemcpyAsync((void *)hdone, &dctrl->done, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipEventRecord(ev[k & 1], s) != hipSuccess) {
+            mps_set_error(MPS_ERR_HIP, __FILE__, __LINE__, "PCG poll");
+            rc = MPS_ERR_HIP;
+            break;
+        }
         if (pending >= 0) {
-            HIP_OK(hipEventSynchronize(ev[pending & 1]));
+            if (hipEventSynchronize(ev[pending & 1]) != hipSuccess) {
+                mps_set_error(MPS_ERR_HIP, __FILE__, __LINE__, "PCG poll");
+                rc = MPS_ERR_HIP;
+                break;
+            }
             if (*hdone) break;
         }
         pending = k;
         k++;
     }
-    HIP_OK(hipStreamSynchronize(s));
+    if (!rc) hipLaunchKernelGGL(k_ic_scatter, dim3(nblk(n)), dim3(IC_BLK), 0, s, n, d->perm, d->x, d->xeq);
+    const hipError_t se = hipStreamSynchronize(s);
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    if (rc) return rc;
+    HIP_OK(se);
     HIP_OK(hipMemcpy(hctrl, dctrl, sizeof(IcCtrl), hipMemcpyDeviceToHost));
-    hipEventDestroy(ev[0]);
-    hipEventDestroy(ev[1]);
     int herr = 0;
     HIP_OK(hipMemcpy(&herr, d->err, sizeof(int), hipMemcpyDeviceToHost));
     res->iters = hctrl->done ? hctrl->iters : o.max_iter;
@@ -707,7 +1452,7 @@ static int pcg_run(IcData *d, hipStream_t s, IcCtrl *dctrl, IcCtrl *hctrl, const
     res->bnorm = hctrl->bnorm;
     res->outer = 1;
     if (herr) {
-        mps_set_error(MPS_ERR_SOLVER, __FILE__, __LINE__, "IC(0) dependency wait exceeded its bound");
+        mps_set_error(MPS_ERR_SOLVER, __FILE__, __LINE__, "IC(0) dependency wait exceeded its bound (ic_wait_limit)");
         return MPS_ERR_SOLVER;
     }
     return 0;
@@ -731,7 +1476,7 @@ int iccg_solve(Ctx *c, const SolveOpts &o, SolveResult *res)
     IcData *d = c->ic;
     const int n = c->n;
     hipStream_t s = c->stream;
-    rc = ic_reserve(d, n, 0, 0);
+    rc = ic_reserve(d, n, 0);
     if (rc) return rc;
     // eq numbering
     hipLaunchKernelGGL(k_ic_flag, dim3(nblk(n)), dim3(IC_BLK), 0, s, n, c->O[c->ct], c->unk, d->flag);
@@ -741,9 +1486,9 @@ int iccg_solve(Ctx *c, const SolveOpts &o, SolveResult *res)
     hipLaunchKernelGGL(k_ic_maps, dim3(nblk(n)), dim3(IC_BLK), 0, s, n, c->O[c->ct], c->unk, c->Alen, d->eq_of_orig,
                        d->eq_of_slot, d->slot_of_eq, d->len);
     LAUNCH_CHECK();
-    int n_eq = 0;
-    HIP_OK(hipMemcpyAsync(&n_eq, d->eq_of_orig + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(d->hhdr, d->eq_of_orig + n, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
+    const int n_eq = d->hhdr[0];
     d->n_eq = n_eq;
     if (n_eq == 0) {
         rc = launch_fill_double(c->x, 0.0, (size_t)n, s);
@@ -752,22 +1497,25 @@ int iccg_solve(Ctx *c, const SolveOpts &o, SolveResult *res)
     }
     rc = launch_scan_excl(d->len, d->rp, n_eq, c->scan_blk, s);
     if (rc) return rc;
-    int nnz = 0;
-    HIP_OK(hipMemcpyAsync(&nnz, d->rp + n_eq, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(d->hhdr, d->rp + n_eq, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
-    rc = ic_reserve(d, n, n_eq, nnz);
+    const long long nnz = d->hhdr[0];
+    rc = ic_reserve(d, n, nnz);
     if (rc) return rc;
     d->nnz = nnz;
-    hipLaunchKernelGGL(k_ic_fill, dim3(nblk(n_eq)), dim3(IC_BLK), 0, s, n_eq, c->stride, d->slot_of_eq, d->eq_of_slot,
-                       c->Alen, c->Aval, c->Acol, c->diag, c->b, d->rp, d->col, d->aval, d->dpos, d->b);
+    hipLaunchKernelGGL(k_ic_fill_sorted, dim3(nblk((size_t)n_eq * WAVE)), dim3(IC_BLK), 0, s, n_eq, c->stride,
+                       d->slot_of_eq, d->eq_of_slot, c->Alen, c->Aval, c->Acol, c->diag, c->b, d->rp, d->col,
+                       d->aval, d->dpos, d->beq);
     hipLaunchKernelGGL(k_ic_tpos, dim3(nblk(n_eq)), dim3(IC_BLK), 0, s, n_eq, d->rp, d->col, d->dpos, d->tpos);
     LAUNCH_CHECK();
-    HIP_OK(hipMemsetAsync(d->err, 0, sizeof(int), s));
-    rc = ic_factor(d, s, o.dense_ic);
+    rc = clear_err(d, s);
+    if (!rc) rc = ic_order(d, s);
+    if (!rc) rc = build_ell(d, s, d->sperm, d->spos);
+    if (!rc) rc = ic_factor(d, s, o.dense_ic);
+    if (!rc) rc = pcg_run(d, s, g_dctrl, g_hctrl, o, 1, res);
     if (rc) return rc;
-    rc = pcg_run(d, s, g_dctrl, g_hctrl, o, 1, res);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_ic_scatter_x, dim3(nblk(n)), dim3(IC_BLK), 0, s, n, d->eq_of_slot, d->x, c->x);
+    c->stats.ic_levels = d->levels;
+    hipLaunchKernelGGL(k_ic_scatter_x, dim3(nblk(n)), dim3(IC_BLK), 0, s, n, d->eq_of_slot, d->xeq, c->x);
     LAUNCH_CHECK();
     return 0;
 }
@@ -782,53 +1530,44 @@ int csr_cg_device(int n, long long nnz, const int *rp, const int *col, const dou
     int rc = ctrl_ready();
     if (rc) return rc;
     if (!d) d = new IcData();
-    rc = ic_reserve(d, n, n, nnz);
+    rc = ic_reserve(d, n, nnz);
     if (rc) return rc;
     d->n_eq = n;
     d->nnz = nnz;
     HIP_OK(hipMemcpyAsync(d->rp, rp, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToDevice, s));
     HIP_OK(hipMemcpyAsync(d->col, col, sizeof(int) * (size_t)nnz, hipMemcpyDeviceToDevice, s));
     HIP_OK(hipMemcpyAsync(d->aval, val, sizeof(double) * (size_t)nnz, hipMemcpyDeviceToDevice, s));
-    HIP_OK(hipMemcpyAsync(d->b, b, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
+    HIP_OK(hipMemcpyAsync(d->beq, b, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
+    // the slice widths read a diagonal position; CG needs only A's copy
+    rc = launch_fill_int(d->dpos, 0, (size_t)n, s);
+    if (!rc) rc = build_ell(d, s, nullptr, nullptr);
+    if (!rc) rc = clear_err(d, s);
+    if (rc) return rc;
     SolveOpts oo = o;
     oo.solver = 0;
     rc = pcg_run(d, s, g_dctrl, g_hctrl, oo, 0, res);
     if (rc) return rc;
-    HIP_OK(hipMemcpyAsync(x, d->x, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
+    HIP_OK(hipMemcpyAsync(x, d->xeq, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, s));
     return 0;
 }
 
-// Stand-alone CSR solve (rows with sorted columns), CG or ICCG.
-extern "C" int mps_hip_solve_csr(int n, const int *row_ptr, const int *col, const double *val, const double *b,
-                                 double *x, int solver_type, int max_iter, double tol, int *iters_out)
+// mps_hip_solve_csr's work on a fresh IcData (freed by the caller).
+static int solve_csr_on(IcData *d, int n, const int *row_ptr, const int *col, const double *val, const double *b,
+                        double *x, int solver_type, int max_iter, double tol, int *iters_out)
 {
-    int rc = mps_hip_init(0);
-    if (rc) return rc;
-    rc = ctrl_ready();
-    if (rc) return rc;
-    if (n <= 0) {
-        if (iters_out) *iters_out = 0;
-        return MPS_OK;
-    }
-    IcData *d = new IcData();
     const long long nnz = row_ptr[n];
-    rc = ic_reserve(d, n, n, nnz);
-    if (rc) { ic_free(d); return rc; }
+    int rc = ic_reserve(d, n, nnz);
+    if (rc) return rc;
     d->n_eq = n;
     d->nnz = nnz;
-    std::vector<int> dpos(n, 0);
-    for (int i = 0; i < n; i++) {
-        int q = row_ptr[i];
-        while (q < row_ptr[i + 1] && col[q] < i) q++;
-        dpos[i] = q - row_ptr[i];
-    }
     hipStream_t s = nullptr;
     HIP_OK(hipMemcpy(d->rp, row_ptr, sizeof(int) * (n + 1), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(d->col, col, sizeof(int) * nnz, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(d->aval, val, sizeof(double) * nnz, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(d->b, b, sizeof(double) * n, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(d->dpos, dpos.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(d->beq, b, sizeof(double) * n, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_ic_dpos, dim3(nblk(n)), dim3(IC_BLK), 0, s, n, d->rp, d->col, d->dpos);
     hipLaunchKernelGGL(k_ic_tpos, dim3(nblk(n)), dim3(IC_BLK), 0, s, n, d->rp, d->col, d->dpos, d->tpos);
+    LAUNCH_CHECK();
     SolveOpts o{};
     o.solver = solver_type;
     o.max_iter = max_iter;
@@ -837,18 +1576,57 @@ extern "C" int mps_hip_solve_csr(int n, const int *row_ptr, const int *col, cons
     o.check_every = (int)mps_hip_get_option("check_every");
     o.dense_ic = MPS_DIM == 2;
     SolveResult sr{};
-    HIP_OK(hipMemsetAsync(d->err, 0, sizeof(int), s));
-    if (solver_type == 1) {
-        rc = ic_factor(d, s, o.dense_ic);
-        if (!rc) rc = pcg_run(d, s, g_dctrl, g_hctrl, o, 1, &sr);
-    } else {
-        rc = pcg_run(d, s, g_dctrl, g_hctrl, o, 0, &sr);
-    }
-    if (!rc) {
-        hipError_t e = hipMemcpy(x, d->x, sizeof(double) * n, hipMemcpyDeviceToHost);
-        if (e != hipSuccess) rc = MPS_ERR_HIP;
-    }
+    const bool ic = solver_type == 1;
+    rc = clear_err(d, s);
+    if (!rc && ic) rc = ic_order(d, s);
+    if (!rc) rc = build_ell(d, s, ic ? d->sperm : nullptr, ic ? d->spos : nullptr);
+    if (!rc && ic) rc = ic_factor(d, s, o.dense_ic);
+    if (!rc) rc = pcg_run(d, s, g_dctrl, g_hctrl, o, ic, &sr);
+    if (!rc) HIP_OK(hipMemcpy(x, d->xeq, sizeof(double) * n, hipMemcpyDeviceToHost));
     if (iters_out) *iters_out = sr.iters;
+    return rc;
+}
+
+// Stand-alone CSR solve (rows with sorted columns), CG or ICCG.  ICCG needs
+// a symmetric pattern: the reference fills L_ik from the entry (k, i) of
+// row k (pressure_solver.c:196-228), the up-looking factor here from (i, k).
+extern "C" int mps_hip_solve_csr(int n, const int *row_ptr, const int *col, const double *val, const double *b,
+                                 double *x, int solver_type, int max_iter, double tol, int *iters_out)
+{
+    int rc = mps_hip_init(0);
+    if (rc) return rc;
+    rc = ctrl_ready();
+    if (rc) return rc;
+    if (iters_out) *iters_out = 0;
+    if (n <= 0) return MPS_OK;
+    if (!row_ptr || !col || !val || !b || !x || row_ptr[0] != 0 || row_ptr[n] < 0) {
+        mps_set_error(MPS_ERR_ARG, __FILE__, __LINE__, "mps_hip_solve_csr: invalid CSR");
+        return MPS_ERR_ARG;
+    }
+    for (int i = 0; i < n; i++) {
+        for (int q = row_ptr[i]; q < row_ptr[i + 1]; q++) {
+            const bool sorted = q == row_ptr[i] || col[q - 1] < col[q];
+            if (col[q] < 0 || col[q] >= n || !sorted) {
+                mps_set_error(MPS_ERR_ARG, __FILE__, __LINE__, "mps_hip_solve_csr: columns out of range or unsorted");
+                return MPS_ERR_ARG;
+            }
+        }
+    }
+    if (solver_type == 1) {
+        // pattern symmetry: every (i, j) has its (j, i)
+        for (int i = 0; i < n; i++)
+            for (int q = row_ptr[i]; q < row_ptr[i + 1]; q++) {
+                const int j = col[q];
+                const int *lo = col + row_ptr[j], *hi = col + row_ptr[j + 1];
+                const int *at = std::lower_bound(lo, hi, i);
+                if (at == hi || *at != i) {
+                    mps_set_error(MPS_ERR_ARG, __FILE__, __LINE__, "mps_hip_solve_csr: ICCG needs a symmetric pattern");
+                    return MPS_ERR_ARG;
+                }
+            }
+    }
+    IcData *d = new IcData();
+    rc = solve_csr_on(d, n, row_ptr, col, val, b, x, solver_type, max_iter, tol, iters_out);
     ic_free(d);
     return rc;
 }
@@ -880,5 +1658,3 @@ extern "C" int mps_hip_export_ic(ParticleSystem *ps, int *row_ptr, int *col, dou
     }
     return MPS_OK;
 }
-
-int ic_levels_unused;

@@ -45,6 +45,11 @@ struct Options {
     double tol_mode = 0, precision = 0, inner_tol = 1e-3, exit_on_error = 1, check_every = 16;
     double spmv = 0, profile = 0, verbose = 0, fused = 2, graph = 0, pblk = 256;
     double async_output = 1;
+    double ic_wait_limit = 1048576;   // unsuccessful polls before an IC(0) wait gives up
+    double ic_sleep = 3;               // longest backoff of an idle IC(0) wave (0-3)
+    double ic_bucket = -1;             // log2 of the levels per bucket of the IC(0) solve order (-1: automatic)
+    double ic_grid = 0;                // workgroups of the IC(0) solves (0: automatic)
+    double ic_fgrid = 0;               // workgroups of the IC(0) factor (0: automatic)
 };
 static Options g_opt;
 
@@ -126,7 +131,7 @@ struct ProfSlot {
 };
 static ProfSlot g_prof[PK_COUNT] = {{"visc_predict"}, {"collide_bin"}, {"sort"}, {"search"},
                                     {"assemble"}, {"spmv"}, {"axpy"}, {"pupdate"},
-                                    {"grad_correct"}, {"ic"}, {"pack"}};
+                                    {"grad_correct"}, {"ic_fwd"}, {"pack"}, {"ic_factor"}, {"ic_bwd"}, {"ic_levels"}};
 static std::vector<hipEvent_t> g_event_pool;
 static hipEvent_t g_open[PK_COUNT];
 
@@ -867,6 +872,14 @@ int mps_hip_set_option(const char *key, double value)
     else if (k == "profile") g_opt.profile = value;
     else if (k == "verbose") g_opt.verbose = value;
     else if (k == "async_output") g_opt.async_output = value;
+    else if (k == "ic_sleep") g_opt.ic_sleep = value;
+    else if (k == "ic_grid") g_opt.ic_grid = value;
+    else if (k == "ic_bucket") g_opt.ic_bucket = value;
+    else if (k == "ic_fgrid") g_opt.ic_fgrid = value;
+    else if (k == "ic_wait_limit") {
+        if (!(value >= 1.0)) return MPS_ERR_ARG;
+        g_opt.ic_wait_limit = value;
+    }
     else if (k == "pblk") {
         if (value != 128 && value != 256 && value != 512) return MPS_ERR_ARG;
         if (value != g_opt.pblk)
@@ -892,6 +905,11 @@ double mps_hip_get_option(const char *key)
     if (k == "verbose") return g_opt.verbose;
     if (k == "pblk") return g_opt.pblk;
     if (k == "async_output") return g_opt.async_output;
+    if (k == "ic_wait_limit") return g_opt.ic_wait_limit;
+    if (k == "ic_sleep") return g_opt.ic_sleep;
+    if (k == "ic_grid") return g_opt.ic_grid;
+    if (k == "ic_bucket") return g_opt.ic_bucket;
+    if (k == "ic_fgrid") return g_opt.ic_fgrid;
     return NAN;
 }
 

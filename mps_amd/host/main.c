@@ -43,11 +43,8 @@ int main(int argc, char *argv[])
         return 1;
     }
     particle_system_calc_initial_params(ps);
-    int rc = mps_hip_last_error();
-    if (rc == 0) {
-        simulation_run(ps);
-        rc = mps_hip_last_error();
-    }
+    simulation_run(ps);
+    const int rc = mps_hip_last_error();   /* the run's own error, if any */
     if (rc != 0) fprintf(stderr, "Error: %s\n", mps_hip_error_string(rc));
     particle_system_free(ps);
     mps_hip_finalize();

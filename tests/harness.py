@@ -16,6 +16,7 @@ from mps_amd import abi as A
 from oracle.oracle import Oracle, make_config
 
 FIELDS = ("pos", "vel", "acc", "pressure", "n", "type", "on_surface")
+INIT_FIELDS = ("n0", "lambda_", "n0_count", "C_LL", "C_SL")
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
@@ -62,7 +63,10 @@ class GpuRun:
             self.scene.ps.lambda_ = lam
         n = self.scene.num
         dim = lib.dim
+        # the search radius of simulation.c:22-24 (re_st too with surface tension)
         re = max(cfg.influence_radius_lap, cfg.influence_radius_n)
+        if cfg.surface_tension_enabled:
+            re = max(re, cfg.influence_radius_st)
         dmin = (C.c_double * dim)(*[cfg.domain_min[d] for d in range(dim)])
         dmax = (C.c_double * dim)(*[cfg.domain_max[d] for d in range(dim)])
         self.nl = lib.neighbor_list_create(n, cfg.max_neighbors)
@@ -141,7 +145,10 @@ def step_compare(lib, name, steps, tol=1e-12, **over):
     say("case", name, "N=%d" % len(typ), "steps", steps, over)
     o = OracleRun(cfg, pos, vel, typ, dim, tol_mode=1)
     ob = OracleRun(cfg, pos, vel, typ, dim, tol_mode=1, dot_order=1)
-    g = GpuRun(lib, cfg, pos, vel, typ, n0=o.scene.ps.n0, lam=o.scene.ps.lambda_)
+    g = GpuRun(lib, cfg, pos, vel, typ)
+    # the GPU's own initial parameters (particle.c:69-221), bit for bit
+    for f in INIT_FIELDS:
+        assert getattr(g.scene.ps, f) == getattr(o.scene.ps, f), f
     out = []
     for s in range(1, steps + 1):
         assert g.step() == 0

@@ -266,3 +266,32 @@ def test_finalize_then_batch_again(lib3):
         assert first == second
     finally:
         lib3.set_option("precision", 0)
+
+
+MPS_ERR_SOLVER = 8
+
+
+@pytest.mark.parametrize("dim,name", [(3, "shipped3d"), (2, "dam2d_l0.01")])
+def test_iccg_wait_bound_reports_solver_error(request, dim, name):
+    """Every wait of the IC(0) kernels is bounded by the option
+    "ic_wait_limit".  With the bound at 1 poll the factor and the solves give
+    up on whatever has not arrived at once, every kernel still runs to its
+    end, and the step returns MPS_ERR_SOLVER; with the bound restored the
+    next run is the normal one (factor bit-exact with the oracle)."""
+    lib = request.getfixturevalue("lib3" if dim == 3 else "lib2")
+    dim, cfg, pos, vel, typ = H.build(name, solver_type=1, cg_tolerance=1e-10)
+    lib.set_option("tol_mode", 1)
+    limit = lib.L.mps_hip_get_option(b"ic_wait_limit")
+    assert limit >= 1024
+    try:
+        lib.set_option("ic_wait_limit", 1)
+        g = H.GpuRun(lib, cfg, pos, vel, typ)
+        rc = g.step()
+        say("ICCG step with ic_wait_limit 1:", rc, lib.mps_hip_error_string(rc).decode())
+        assert rc == MPS_ERR_SOLVER
+        g.close()
+    finally:
+        lib.set_option("ic_wait_limit", limit)
+    g = H.GpuRun(lib, cfg, pos, vel, typ)
+    assert g.step() == 0
+    g.close()
