@@ -457,6 +457,40 @@ __global__ void k_ic_counts(int n, const int *perm, const int *rp, const int *dp
 // factor: one wavefront per row
 // ---------------------------------------------------------------------
 
+// d_e of a finished row from its values in LDS, in column order (every lane
+// computes it: d = A_ee - sum L^2, d <= 0 -> A_ee (if > 0) or 1e-30), and
+// its 1/sqrt(d) (dense formulas: sqrt(d)) published after the row's values
+// (readers check each value anyway).
+__device__ __forceinline__ void finish_row(int e, int m, double akk, const double *sval, double *dinv, int dense,
+                                           int lane)
+{
+    double d;
+    if (!dense) {
+        d = akk;
+        for (int q = 0; q < m; q++) {
+            const double l = lds_ld(&sval[q]);
+            d -= l * l;
+        }
+    } else {
+        double sq = 0.0;
+        for (int q = 0; q < m; q++) {
+            const double l = lds_ld(&sval[q]);
+            sq += l * l;
+        }
+        d = akk - sq;
+    }
+    if (d <= 0.0) d = (akk > 0.0) ? akk : 1.0e-30;
+    const double out = dense ? sqrt(d) : 1.0 / sqrt(d);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    if (lane == 0) st_rlx(&dinv[e], out);
+    wave_sync();
+}
+
+// ---------------------------------------------------------------------
+// factor, one-phase form (any row length)
+// ---------------------------------------------------------------------
+
 __global__ void __launch_bounds__(IC_BLK)
 k_ic_factor(int n, const int *__restrict__ order, const int *__restrict__ rp, const int *__restrict__ col,
             const int *__restrict__ dpos, const double *__restrict__ aval, double *lval, double *dinv,
@@ -599,31 +633,160 @@ k_ic_factor(int n, const int *__restrict__ order, const int *__restrict__ rp, co
             }
             wave_sync();
         }
-        // d_e from the finished row, in column order (every lane computes it)
-        const double akk = aval[r0 + m];
-        double d;
-        if (!dense) {
-            d = akk;
-            for (int q = 0; q < m; q++) {
-                const double l = lds_ld(&sval[q]);
-                d -= l * l;
-            }
-        } else {
-            double sq = 0.0;
-            for (int q = 0; q < m; q++) {
-                const double l = lds_ld(&sval[q]);
-                sq += l * l;
-            }
-            d = akk - sq;
-        }
-        if (d <= 0.0) d = (akk > 0.0) ? akk : 1.0e-30;
-        const double out = dense ? sqrt(d) : 1.0 / sqrt(d);
-        // the row's values are stored before its 1/sqrt(d) (readers check
-        // each value anyway)
-        __builtin_amdgcn_s_waitcnt(0);
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        if (lane == 0) st_rlx(&dinv[e], out);
+        finish_row(e, m, aval[r0 + m], sval, dinv, dense, lane);
+    }
+    if (ballot64(failed) && lane == 0) atomicOr(err, 1);
+}
+
+// Two-phase form of the factor for rows of at most 64 strictly-lower
+// entries (lcap <= 64), one wavefront per workgroup.  Lane t owns entry t,
+// (e, k): once row k is published it streams row k's entries in batches of
+// FB2 and keeps each match -- the position v in this row of a column the two
+// rows share, and L_kj -- in its own list in LDS (lane t has at most t
+// matches: list t starts at t (t - 1) / 2), without waiting on its own row.
+// Then it folds the list in order, each product as soon as lane v of this
+// row is done, all from LDS.  The row's in-lane chain then costs LDS
+// iterations instead of a memory round trip per link; the arithmetic is the
+// one-phase kernel's, term for term.
+#define FB2 32
+__global__ void __launch_bounds__(WAVE)
+k_ic_factor2(int n, const int *__restrict__ order, const int *__restrict__ rp, const int *__restrict__ col,
+             const int *__restrict__ dpos, const double *__restrict__ aval, double *lval, double *dinv,
+             unsigned *ticket, int *err, int dense, int lcap, long long wait_limit, int sleep_cap)
+{
+    extern __shared__ double f2_smem[];
+    const int lane = threadIdx.x;
+    const int tri = lcap * (lcap - 1) / 2;
+    double *sval = f2_smem;                               // lcap
+    double *mval = f2_smem + lcap;                        // tri
+    int *scol = (int *)(f2_smem + lcap + tri);            // lcap
+    unsigned short *mpos = (unsigned short *)(scol + lcap);   // tri
+    bool failed = false;
+    for (;;) {
+        int tk = 0;
+        if (lane == 0) tk = (int)atomicAdd(ticket, 1u);
+        tk = __shfl(tk, 0, WAVE);
+        if (tk >= n) break;
+        const int e = order[tk];   // rows in level order: every row waited on is claimed earlier
+        const int r0 = rp[e], m = dpos[e];
         wave_sync();
+        if (lane < m) scol[lane] = col[r0 + lane];
+        wave_sync();
+        const int pstep = m > 0 ? (1 << (31 - __clz(m))) : 0;
+        const int T = lane;
+        const bool act = T < m;
+        int k = 0, kb = 0, mk = 0;
+        double a = 0.0;
+        if (act) {
+            k = scol[T];
+            a = aval[r0 + T];
+            kb = rp[k];
+            mk = dpos[k];
+        }
+        const int lb = T * (T - 1) / 2;   // this lane's match list
+        int stage = act ? 0 : 3;          // 0 waiting for row k, 1 gathering, 2 folding, 3 done
+        double dk = 0.0, s = dense ? 0.0 : a;
+        int u = 0, cnt = 0, mi = 0;
+        long long spins = 0;
+        bool poll_all = true;
+        int idle = 0;
+        for (;;) {
+            if (ballot64(stage != 3) == 0ull) break;
+            const unsigned long long waitm = ballot64(stage == 0);
+            bool prog = false;
+            if (waitm) {
+                const int lo = __ffsll((long long)waitm) - 1;
+                if (stage == 0 && (poll_all || lane == lo)) {
+                    const double d = ld_rlx(&dinv[k]);
+                    if (!is_sent(d)) {
+                        dk = d;
+                        stage = 1;
+                        prog = true;
+                        spins = 0;
+                    } else if (++spins > wait_limit) {
+                        failed = true;   // give up on row k: its terms read as 0
+                        stage = 1;
+                    }
+                }
+                poll_all = __shfl((int)(stage != 0), lo, WAVE) != 0;
+            }
+            if (stage == 1) {
+                if (u >= mk) {
+                    stage = 2;
+                } else {
+                    const int bn = min(FB2, mk - u);
+                    int bc[FB2];
+                    double bv[FB2];
+#pragma unroll
+                    for (int j = 0; j < FB2; j++) bc[j] = j < bn ? col[kb + u + j] : INT_MAX;
+#pragma unroll
+                    for (int j = 0; j < FB2; j++) bv[j] = j < bn ? ld_rlx(&lval[kb + u + j]) : 0.0;
+                    bool all_there = true;
+#pragma unroll
+                    for (int j = 0; j < FB2; j++) all_there = all_there && !is_sent(bv[j]);
+                    if (!all_there && ++spins <= wait_limit) {
+                        // row k's 1/sqrt(d) came before some of its values:
+                        // load this batch again next time
+                    } else {
+                        if (!all_there) failed = true;   // give up: missing values read as 0
+                        int ps[FB2];
+#pragma unroll
+                        for (int j = 0; j < FB2; j++) ps[j] = -1;
+                        for (int st = pstep; st > 0; st >>= 1) {
+#pragma unroll
+                            for (int j = 0; j < FB2; j++) {
+                                const int cand = ps[j] + st;
+                                if (cand < T && scol[cand] < bc[j]) ps[j] = cand;
+                            }
+                        }
+#pragma unroll
+                        for (int j = 0; j < FB2; j++) {
+                            const int cand = ps[j] + 1;
+                            if (j < bn && cand < T && scol[cand] == bc[j]) {
+                                mpos[lb + cnt] = (unsigned short)cand;
+                                mval[lb + cnt] = is_sent(bv[j]) ? 0.0 : bv[j];
+                                cnt++;
+                            }
+                        }
+                        u += bn;
+                        spins = 0;
+                        prog = true;
+                    }
+                }
+            }
+            // the fold chain from LDS, as many links as are ready, before
+            // the next poll or gather
+            for (;;) {
+                const unsigned long long donem = ballot64(stage == 3);
+                bool fprog = false;
+                if (stage == 2) {
+                    while (mi < cnt) {
+                        const int v = mpos[lb + mi];
+                        if (!((donem >> v) & 1ull)) break;
+                        const double lv = lds_ld(&sval[v]);
+                        if (!dense) s -= lv * mval[lb + mi];
+                        else s += lv * mval[lb + mi];
+                        mi++;
+                        fprog = true;
+                    }
+                    if (mi == cnt) {
+                        double L;
+                        if (!dense) L = s * dk;
+                        else L = (fabs(a) < 1.0e-30) ? 0.0 : (a - s) * (1.0 / dk);   // dk holds sqrt(d) here
+                        lds_st(&sval[T], L);
+                        st_rlx(&lval[r0 + T], L);
+                        stage = 3;
+                        fprog = true;
+                    }
+                }
+                if (!__any(fprog)) break;
+                prog = true;
+            }
+            if (__any(prog)) idle = 0;
+            else backoff(idle++, sleep_cap);
+        }
+        wave_sync();
+        finish_row(e, m, aval[r0 + m], sval, dinv, dense, lane);
     }
     if (ballot64(failed) && lane == 0) atomicOr(err, 1);
 }
@@ -765,9 +928,7 @@ __device__ __forceinline__ double readlane_d(double v, int k)
 // MODE 0 sparse: s = r, s -= l v, out = s * dinv; MODE 1 dense: s = 0,
 // s += l v, out = (r - s) / dinv (dinv holds L_ii).  rin and y are by
 // position.
-#define CS_WAVES 8
-#define CS_CHUNK 128
-template <int MODE, bool BACK>
+template <int MODE, bool BACK, int CS_CHUNK, int CS_WAVES>
 __global__ void __launch_bounds__(CS_WAVES * WAVE)
 k_ic_csolve(int n, const int *__restrict__ perm, const double *__restrict__ lev, const int *__restrict__ eoff,
             const int *__restrict__ ecol, const double *__restrict__ eval, const double *__restrict__ dinv,
@@ -822,6 +983,7 @@ k_ic_csolve(int n, const int *__restrict__ perm, const double *__restrict__ lev,
             const int r = perm ? perm[t] : t;
             const int o0 = eoff[t], m = eoff[t + 1] - o0;
             const double rv = rin[t];
+            const double dr = dinv[r];
             double s = MODE == 0 ? rv : 0.0;
             for (int g0 = 0; g0 < m; g0 += WAVE) {
                 const int cnt = min(WAVE, m - g0);
@@ -875,7 +1037,7 @@ k_ic_csolve(int n, const int *__restrict__ perm, const double *__restrict__ lev,
                     }
                 }
             }
-            const double out = MODE == 0 ? s * dinv[r] : (rv - s) / dinv[r];
+            const double out = MODE == 0 ? s * dr : (rv - s) / dr;
             if (lane == 0) {
                 lds_st(&sy[t - cbase], out);
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");   // value before flag
@@ -1312,8 +1474,16 @@ static int ic_factor(IcData *d, hipStream_t s, int dense)
     const double og = mps_hip_get_option("ic_fgrid");
     const unsigned grid = og >= 1 ? (unsigned)og : cu_count() * (wpb == 4 ? 4 : 8);
     prof_begin(PK_ICF, s);
-    hipLaunchKernelGGL(k_ic_factor, dim3(grid), dim3(WAVE * wpb), lds, s, n, d->order, d->rp, d->col, d->dpos,
-                       d->aval, d->lval, d->dinv, d->ticket, d->err, dense, lcap, wait_limit(), sleep_cap());
+    if (lcap <= WAVE && mps_hip_get_option("ic_factor_form") != 1) {
+        // two-phase form: one wave per workgroup, match lists in LDS
+        const size_t lds2 = (size_t)lcap * 12 + (size_t)(lcap * (lcap - 1) / 2) * 10 + 16;
+        const unsigned grid2 = og >= 1 ? (unsigned)og : cu_count() * 8;
+        hipLaunchKernelGGL(k_ic_factor2, dim3(grid2), dim3(WAVE), lds2, s, n, d->order, d->rp, d->col, d->dpos,
+                           d->aval, d->lval, d->dinv, d->ticket, d->err, dense, lcap, wait_limit(), sleep_cap());
+    } else {
+        hipLaunchKernelGGL(k_ic_factor, dim3(grid), dim3(WAVE * wpb), lds, s, n, d->order, d->rp, d->col, d->dpos,
+                           d->aval, d->lval, d->dinv, d->ticket, d->err, dense, lcap, wait_limit(), sleep_cap());
+    }
     prof_end(PK_ICF, s);
     LAUNCH_CHECK();
     // L and U by position of the solve order, rows contiguous
@@ -1346,15 +1516,27 @@ static int ic_apply(IcData *d, const double *r, hipStream_t s, int dense, const 
     const long long wl = wait_limit();
     const int sc = sleep_cap();
     const double og = mps_hip_get_option("ic_grid");
-    const unsigned grid = og >= 1 ? (unsigned)og : cu_count();
-    auto fwd = dense ? k_ic_csolve<1, false> : k_ic_csolve<0, false>;
-    auto bwd = dense ? k_ic_csolve<1, true> : k_ic_csolve<0, true>;
+    const unsigned grid = og >= 1 ? (unsigned)og : cu_count() * 2;
+    // chunks of "ic_chunk" positions (32, 64, 128), "ic_waves" wavefronts
+    // (4 or 8) per workgroup
+    const int ck = (int)mps_hip_get_option("ic_chunk");
+    const int nw = (int)mps_hip_get_option("ic_waves") == 4 ? 4 : 8;
+    decltype(&k_ic_csolve<0, false, 64, 8>) fwd, bwd;
+#define IC_PICK(C, W)                                                                  \
+    do {                                                                               \
+        fwd = dense ? k_ic_csolve<1, false, C, W> : k_ic_csolve<0, false, C, W>;       \
+        bwd = dense ? k_ic_csolve<1, true, C, W> : k_ic_csolve<0, true, C, W>;         \
+    } while (0)
+    if (ck == 64) { if (nw == 4) IC_PICK(64, 4); else IC_PICK(64, 8); }
+    else if (ck == 128) { if (nw == 4) IC_PICK(128, 4); else IC_PICK(128, 8); }
+    else { if (nw == 4) IC_PICK(32, 4); else IC_PICK(32, 8); }
+#undef IC_PICK
     prof_begin(PK_IC, s);
-    hipLaunchKernelGGL(fwd, dim3(grid), dim3(CS_WAVES * WAVE), 0, s, d->n_eq, d->perm, d->lev, d->loff, d->lcol,
+    hipLaunchKernelGGL(fwd, dim3(grid), dim3(nw * WAVE), 0, s, d->n_eq, d->perm, d->lev, d->loff, d->lcol,
                        d->lv, d->dinv, r, d->y, d->ticket + 0, d->err, wl, sc, ctrl);
     prof_end(PK_IC, s);
     prof_begin(PK_ICB, s);
-    hipLaunchKernelGGL(bwd, dim3(grid), dim3(CS_WAVES * WAVE), 0, s, d->n_eq, d->perm, d->lev, d->uoff, d->ucol,
+    hipLaunchKernelGGL(bwd, dim3(grid), dim3(nw * WAVE), 0, s, d->n_eq, d->perm, d->lev, d->uoff, d->ucol,
                        d->uv, d->dinv, d->y, d->z, d->ticket + 1, d->err, wl, sc, ctrl);
     prof_end(PK_ICB, s);
     LAUNCH_CHECK();

@@ -48,6 +48,9 @@ struct Options {
     double ic_wait_limit = 1048576;   // unsuccessful polls before an IC(0) wait gives up
     double ic_sleep = 3;               // longest backoff of an idle IC(0) wave (0-3)
     double ic_bucket = -1;             // log2 of the levels per bucket of the IC(0) solve order (-1: automatic)
+    double ic_chunk = 32;              // positions per workgroup chunk of the IC(0) solves (32, 64, 128)
+    double ic_waves = 8;               // wavefronts per workgroup of the IC(0) solves (4, 8)
+    double ic_factor_form = 2;         // IC(0) factor: 2 two-phase (rows of at most 64 entries), 1 one-phase
     double ic_grid = 0;                // workgroups of the IC(0) solves (0: automatic)
     double ic_fgrid = 0;               // workgroups of the IC(0) factor (0: automatic)
 };
@@ -875,6 +878,16 @@ int mps_hip_set_option(const char *key, double value)
     else if (k == "ic_sleep") g_opt.ic_sleep = value;
     else if (k == "ic_grid") g_opt.ic_grid = value;
     else if (k == "ic_bucket") g_opt.ic_bucket = value;
+    else if (k == "ic_chunk") {
+        if (value != 32 && value != 64 && value != 128) return MPS_ERR_ARG;
+        g_opt.ic_chunk = value;
+    } else if (k == "ic_factor_form") {
+        if (value != 1 && value != 2) return MPS_ERR_ARG;
+        g_opt.ic_factor_form = value;
+    } else if (k == "ic_waves") {
+        if (value != 4 && value != 8) return MPS_ERR_ARG;
+        g_opt.ic_waves = value;
+    }
     else if (k == "ic_fgrid") g_opt.ic_fgrid = value;
     else if (k == "ic_wait_limit") {
         if (!(value >= 1.0)) return MPS_ERR_ARG;
@@ -909,6 +922,9 @@ double mps_hip_get_option(const char *key)
     if (k == "ic_sleep") return g_opt.ic_sleep;
     if (k == "ic_grid") return g_opt.ic_grid;
     if (k == "ic_bucket") return g_opt.ic_bucket;
+    if (k == "ic_chunk") return g_opt.ic_chunk;
+    if (k == "ic_waves") return g_opt.ic_waves;
+    if (k == "ic_factor_form") return g_opt.ic_factor_form;
     if (k == "ic_fgrid") return g_opt.ic_fgrid;
     return NAN;
 }

@@ -9,7 +9,9 @@ iteration at a time:
   factor  one wave per row, rows claimed in level order, lane t owning entry
           t: waiting / folding / done stages, batches of 16 of row k's
           entries, the binary search over the row's columns, "only the
-          lowest waiting lane polls until its row has come";
+          lowest waiting lane polls until its row has come"; and the
+          two-phase form (rows of at most 64 entries), whose lanes gather
+          their matches into lists in batches of 32 before folding them;
   solves  chunks of the solve order (buckets of levels, index order inside)
           claimed from a ticket, each chunk's rows sorted by level and
           claimed by its waves from an LDS ticket, one wave per row folding
@@ -445,6 +447,124 @@ def _matrix(name, **over):
     return dict(n=n, rp=rp, col=col, aval=aval, dpos=dpos, q=q, dense=dim == 2)
 
 
+FB2 = 32
+
+
+def factor2_wave(A, st, mem, dense, wait_limit, ordered=True, order=None):
+    """One wave of k_ic_factor2 (rows of at most 64 entries): lane t waits
+    for row k, gathers its matches (position v in this row, L_kj) into its
+    own list batch by batch -- a batch with a value not yet visible is loaded
+    again -- then folds the list in order as lanes v finish."""
+    rp, col, dpos, aval = A["rp"], A["col"], A["dpos"], A["aval"]
+    lval, dinv = st["lval"], st["dinv"]
+    me = object()
+    while True:
+        tk = st["ticket"]
+        st["ticket"] += 1
+        if tk >= A["n"]:
+            return
+        e = order[tk] if order is not None else tk
+        r0, m = rp[e], dpos[e]
+        assert m <= WAVE
+        scol = list(col[r0:r0 + m])
+        sval = [None] * m
+        pstep = (1 << (m.bit_length() - 1)) if m > 0 else 0
+        L = []
+        for T in range(WAVE):
+            act = T < m
+            k = scol[T] if act else 0
+            L.append(dict(T=T, k=k, a=aval[r0 + T] if act else 0.0, kb=rp[k], mk=dpos[k], stage=0 if act else 3,
+                          dk=0.0, s=0.0 if dense else (aval[r0 + T] if act else 0.0), u=0, lst=[], mi=0, spins=0))
+        poll_all = True
+        while True:
+            if all(x["stage"] == 3 for x in L):
+                break
+            waitm = [i for i, x in enumerate(L) if x["stage"] == 0]
+            if waitm:
+                lo = waitm[0]
+                for i in waitm:
+                    x = L[i]
+                    if poll_all or i == lo:
+                        d = dinv[x["k"]]
+                        if d is not None:
+                            x["dk"], x["stage"], x["spins"] = d, 1, 0
+                        else:
+                            x["spins"] += 1
+                            if x["spins"] > wait_limit:
+                                st["err"] = True
+                                x["stage"] = 1
+                poll_all = L[lo]["stage"] != 0
+            for x in L:
+                if x["stage"] == 1:
+                    if x["u"] >= x["mk"]:
+                        x["stage"] = 2
+                    else:
+                        bn = min(FB2, x["mk"] - x["u"])
+                        base = x["kb"] + x["u"]
+                        bc = [col[base + j] for j in range(bn)]
+                        bv = [lval[base + j] for j in range(bn)]
+                        there = all(v is not None for v in bv)
+                        if not there:
+                            x["spins"] += 1
+                            if x["spins"] <= wait_limit:
+                                continue
+                            st["err"] = True
+                        for j in range(bn):
+                            ps, stp = -1, pstep
+                            while stp > 0:
+                                cand = ps + stp
+                                if cand < x["T"] and scol[cand] < bc[j]:
+                                    ps = cand
+                                stp >>= 1
+                            if ps + 1 < x["T"] and scol[ps + 1] == bc[j]:
+                                x["lst"].append((ps + 1, 0.0 if bv[j] is None else bv[j]))
+                        x["u"] += bn
+                        x["spins"] = 0
+            # the fold chain, as many links as are ready
+            while True:
+                donem = [x["stage"] == 3 for x in L]
+                fprog = False
+                for x in L:
+                    if x["stage"] != 2:
+                        continue
+                    while x["mi"] < len(x["lst"]):
+                        v, lk = x["lst"][x["mi"]]
+                        if not donem[v]:
+                            break
+                        x["s"] = x["s"] - sval[v] * lk if not dense else x["s"] + sval[v] * lk
+                        x["mi"] += 1
+                        fprog = True
+                    if x["mi"] == len(x["lst"]):
+                        if not dense:
+                            v = x["s"] * x["dk"]
+                        else:
+                            v = 0.0 if abs(x["a"]) < 1.0e-30 else (x["a"] - x["s"]) * (1.0 / x["dk"])
+                        sval[x["T"]] = v
+                        mem.store(lval, r0 + x["T"], v, me)
+                        x["stage"] = 3
+                        fprog = True
+                if not fprog:
+                    break
+            yield
+        akk = aval[r0 + m]
+        if not dense:
+            d = akk
+            for q in range(m):
+                d = d - sval[q] * sval[q]
+        else:
+            sq = 0.0
+            for q in range(m):
+                sq = sq + sval[q] * sval[q]
+            d = akk - sq
+        if d <= 0.0:
+            d = akk if akk > 0.0 else 1.0e-30
+        out = np.sqrt(d) if dense else 1.0 / np.sqrt(d)
+        if ordered:
+            mem.flush_wave(me)
+        mem.store(dinv, e, float(out), me)
+        yield
+
+
 def _levels(A):
     """Dependency levels, serially: 1 + the largest level of the row's
     strictly-lower columns, 0 without any."""
@@ -474,12 +594,13 @@ def _orders(A, rng):
     return lev, order, pos
 
 
-def _factor(A, seed, n_waves, max_delay, wait_limit, ordered=True):
+def _factor(A, seed, n_waves, max_delay, wait_limit, ordered=True, form=1):
     rng = random.Random(seed)
     mem = Memory(rng, max_delay)
     _, fperm, _ = _orders(A, rng)
     st = dict(ticket=0, lval=[None] * len(A["col"]), dinv=[None] * A["n"], err=False)
-    procs = [factor_wave(A, st, mem, A["dense"], wait_limit, ordered, fperm) for _ in range(n_waves)]
+    wave = factor_wave if form == 1 else factor2_wave
+    procs = [wave(A, st, mem, A["dense"], wait_limit, ordered, fperm) for _ in range(n_waves)]
     run_waves(procs, mem, rng)
     return st
 
@@ -608,10 +729,13 @@ SOLVE_CASES = [("dam2d_l0.01", 4, 4, 0), ("dam2d_l0.01", 5, 12, 30), ("dam2d_l0.
                ("fric3d_tiny", 7, 6, 100)]
 
 
+@pytest.mark.parametrize("form", [1, 2])
 @pytest.mark.parametrize("name,seed,n_waves,max_delay,ordered", FACTOR_CASES)
-def test_factor_protocol_terminates_and_matches_oracle(name, seed, n_waves, max_delay, ordered):
+def test_factor_protocol_terminates_and_matches_oracle(name, seed, n_waves, max_delay, ordered, form):
+    """Both forms of the factor: one-phase (k_ic_factor) and two-phase
+    (k_ic_factor2, match lists first)."""
     A = _matrix(name)
-    st = _factor(A, seed, n_waves, max_delay, wait_limit=10**12, ordered=ordered)
+    st = _factor(A, seed, n_waves, max_delay, wait_limit=10**12, ordered=ordered, form=form)
     assert not st["err"]
     q = A["q"]
     lv = [st["lval"][A["rp"][e] + t] for e in range(A["n"]) for t in range(A["dpos"][e])]
@@ -647,9 +771,10 @@ def test_level_pass_protocol(name, seed, groups, lanes, waves, max_delay):
 
 def test_wait_bound_gives_up_and_terminates():
     A = _matrix("fric3d_tiny")
-    st = _factor(A, 7, 16, 50, wait_limit=1)
-    assert st["err"]
-    assert all(d is not None for d in st["dinv"])
+    for form in (1, 2):
+        st = _factor(A, 7, 16, 50, wait_limit=1, form=form)
+        assert st["err"]
+        assert all(d is not None for d in st["dinv"])
     rng = np.random.default_rng(0)
     r = [float(v) for v in rng.standard_normal(A["n"])]
     st2 = _factor(A, 11, 8, 0, wait_limit=10**12)
