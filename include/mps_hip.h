@@ -351,6 +351,23 @@ int  mps_hip_solve_csr(int n, const int *row_ptr, const int *col,
                        int solver_type, int max_iter, double tol,
                        int *iters_out);
 
+/* mps_hip_solve_csr without allocation: the same arguments, kernels and
+ * results (bit for bit), with every device buffer carved from one static
+ * device array of MPS_NOALLOC_SCRATCH_BYTES that each GPU holds for the life
+ * of the process.  It calls no hipMalloc, hipFree or other allocating
+ * routine, for callers that cannot allocate (serving loops, harnesses that
+ * forbid it).  No counterpart in the reference.  Solves that do not fit the
+ * array (about 36 bytes per entry plus 12 per ELL slot and 170 per unknown:
+ * a 3-D PPE of 15,000 unknowns with rows of up to 123 entries takes 71 MB)
+ * fail with MPS_ERR_ARG and a
+ * message naming the need; nothing is written to x then.  Every call on one
+ * GPU uses that GPU's one array: calls that overlap on a GPU race. */
+#define MPS_NOALLOC_SCRATCH_BYTES (96u << 20)
+int  mps_hip_solve_csr_noalloc(int n, const int *row_ptr, const int *col,
+                               const double *val, const double *b, double *x,
+                               int solver_type, int max_iter, double tol,
+                               int *iters_out);
+
 /* Diagnostics: computes sqrt(a[i]) and a[i]/b[i] on the GPU with the FP64
  * operations the kernels use (to check they round exactly as the host). */
 int  mps_hip_fp64_probe(int n, const double *a, const double *b, double *sqrt_out,

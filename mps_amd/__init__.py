@@ -117,6 +117,7 @@ class Lib:
             "mps_hip_profile_get": (C.c_int, [C.c_char_p, D, C.POINTER(C.c_longlong)]),
             "mps_hip_profile_reset": (None, []),
             "mps_hip_solve_csr": (C.c_int, [C.c_int, I, I, D, D, D, C.c_int, C.c_int, C.c_double, I]),
+            "mps_hip_solve_csr_noalloc": (C.c_int, [C.c_int, I, I, D, D, D, C.c_int, C.c_int, C.c_double, I]),
             "mps_hip_fp64_probe": (C.c_int, [C.c_int, D, D, D, D]),
         }
         self.missing = []

@@ -158,6 +158,13 @@ struct IcData {
     int *err = nullptr;
     double *part = nullptr;
     int max_lower = 0, levels = 0, bucket = 0;
+    // arena mode: every buffer above is carved from [arena, arena +
+    // arena_cap) instead of hipMalloc'd, the host header is hhdr_host and
+    // the PCG poll uses the events ev (created beforehand)
+    char *arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
+    int hhdr_host[8] = {0};
+    hipEvent_t ev[2] = {nullptr, nullptr};
 };
 
 struct IcCtrl {
@@ -1264,19 +1271,36 @@ __global__ void k_ic_scatter_x(int n_slots, const int *eq_of_slot, const double 
 // host side
 // ---------------------------------------------------------------------
 
-template <typename T>
-static int regrow(T **p, size_t n)
+// Bytes one arena carve of n elements of size sz takes (256-byte aligned).
+static size_t carve_bytes(size_t sz, size_t n)
 {
+    return (sz * (n > 0 ? n : 1) + 255) & ~(size_t)255;
+}
+
+template <typename T>
+static int regrow(IcData *d, T **p, size_t n)
+{
+    if (d->arena) {
+        // arena buffers are never freed: a regrow takes fresh room
+        const size_t need = carve_bytes(sizeof(T), n);
+        if (d->arena_used + need > d->arena_cap) {
+            mps_set_error(MPS_ERR_ARG, __FILE__, __LINE__, "solve does not fit the no-allocation scratch");
+            return MPS_ERR_ARG;
+        }
+        *p = (T *)(d->arena + d->arena_used);
+        d->arena_used += need;
+        return 0;
+    }
     if (*p) (void)hipFree(*p);
     *p = nullptr;
     HIP_OK(hipMalloc(p, sizeof(T) * (n > 0 ? n : 1)));
     return 0;
 }
 
-#define REGROW(p, n)                       \
-    do {                                   \
-        int rc_ = regrow(&(p), (size_t)(n)); \
-        if (rc_) return rc_;               \
+#define REGROW(p, n)                          \
+    do {                                      \
+        int rc_ = regrow(d, &(p), (size_t)(n)); \
+        if (rc_) return rc_;                  \
     } while (0)
 
 static int ic_reserve(IcData *d, int n_slots, long long nnz)
@@ -1306,12 +1330,13 @@ static int ic_reserve(IcData *d, int n_slots, long long nnz)
         d->cap_nnz = nnz;
     }
     if (!d->ticket) {
-        HIP_OK(hipMalloc(&d->ticket, sizeof(unsigned) * 4));
-        HIP_OK(hipMalloc(&d->err, sizeof(int)));
-        HIP_OK(hipMalloc(&d->part, sizeof(double) * 3 * 65536));
-        HIP_OK(hipMalloc(&d->hdr, sizeof(int) * 8));
-        HIP_OK(hipHostMalloc(&d->hhdr, sizeof(int) * 8, hipHostMallocDefault));
-        HIP_OK(hipMalloc(&d->scan_blk, sizeof(int) * 4 * 2050));
+        REGROW(d->ticket, 4);
+        REGROW(d->err, 1);
+        REGROW(d->part, 3 * 65536);
+        REGROW(d->hdr, 8);
+        REGROW(d->scan_blk, 4 * 2050);
+        if (d->arena) d->hhdr = d->hhdr_host;
+        else HIP_OK(hipHostMalloc(&d->hhdr, sizeof(int) * 8, hipHostMallocDefault));
     }
     return 0;
 }
@@ -1325,8 +1350,10 @@ void ic_free(IcData *d)
                   d->x, d->r, d->z, d->p, d->Ap, d->y, d->b, d->wA, d->oA, d->acol, d->aell, d->cntl, d->cntu,
                   d->loff, d->uoff, d->lcol, d->ucol, d->lv, d->uv, d->hdr, d->scan_blk, d->ticket, d->err,
                   d->part};
-    for (void *q : ps) if (q) (void)hipFree(q);
-    if (d->hhdr) (void)hipHostFree(d->hhdr);
+    if (!d->arena) {
+        for (void *q : ps) if (q) (void)hipFree(q);
+        if (d->hhdr) (void)hipHostFree(d->hhdr);
+    }
     delete d;
 }
 
@@ -1545,8 +1572,7 @@ static int ic_apply(IcData *d, const double *r, hipStream_t s, int dense, const 
 
 static int clear_err(IcData *d, hipStream_t s)
 {
-    HIP_OK(hipMemsetAsync(d->err, 0, sizeof(int), s));
-    return 0;
+    return launch_fill_int(d->err, 0, 1, s);
 }
 
 // PCG (precond 1) or CG (precond 0) on the system held in d by position of
@@ -1573,9 +1599,12 @@ static int pcg_run(IcData *d, hipStream_t s, IcCtrl *dctrl, IcCtrl *hctrl, const
     LAUNCH_CHECK();
     int chunk = o.check_every > 0 ? o.check_every : 16;
     if (precond && chunk > 4) chunk = 4;
-    hipEvent_t ev[2];
-    HIP_OK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-    HIP_OK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    hipEvent_t ev[2] = {d->ev[0], d->ev[1]};
+    const bool own_ev = !ev[0];
+    if (own_ev) {
+        HIP_OK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+        HIP_OK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    }
     volatile int *hdone = &hctrl->done;
     int pending = -1, t = 0, k = 0, rc = 0;
     const double *zz = precond ? d->z : d->r;
@@ -1621,8 +1650,10 @@ static int pcg_run(IcData *d, hipStream_t s, IcCtrl *dctrl, IcCtrl *hctrl, const
     }
     if (!rc) hipLaunchKernelGGL(k_ic_scatter, dim3(nblk(n)), dim3(IC_BLK), 0, s, n, d->perm, d->x, d->xeq);
     const hipError_t se = hipStreamSynchronize(s);
-    (void)hipEventDestroy(ev[0]);
-    (void)hipEventDestroy(ev[1]);
+    if (own_ev) {
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+    }
     if (rc) return rc;
     HIP_OK(se);
     HIP_OK(hipMemcpy(hctrl, dctrl, sizeof(IcCtrl), hipMemcpyDeviceToHost));
@@ -1733,9 +1764,11 @@ int csr_cg_device(int n, long long nnz, const int *rp, const int *col, const dou
     return 0;
 }
 
-// mps_hip_solve_csr's work on a fresh IcData (freed by the caller).
-static int solve_csr_on(IcData *d, int n, const int *row_ptr, const int *col, const double *val, const double *b,
-                        double *x, int solver_type, int max_iter, double tol, int *iters_out)
+// mps_hip_solve_csr's work on a fresh IcData (freed by the caller), with
+// the PCG control block dctrl (device) and its host copy hctrl.
+static int solve_csr_on(IcData *d, IcCtrl *dctrl, IcCtrl *hctrl, int n, const int *row_ptr, const int *col,
+                        const double *val, const double *b, double *x, int solver_type, int max_iter, double tol,
+                        int *iters_out)
 {
     const long long nnz = row_ptr[n];
     int rc = ic_reserve(d, n, nnz);
@@ -1763,33 +1796,36 @@ static int solve_csr_on(IcData *d, int n, const int *row_ptr, const int *col, co
     if (!rc && ic) rc = ic_order(d, s);
     if (!rc) rc = build_ell(d, s, ic ? d->sperm : nullptr, ic ? d->spos : nullptr);
     if (!rc && ic) rc = ic_factor(d, s, o.dense_ic);
-    if (!rc) rc = pcg_run(d, s, g_dctrl, g_hctrl, o, ic, &sr);
+    if (!rc) rc = pcg_run(d, s, dctrl, hctrl, o, ic, &sr);
     if (!rc) HIP_OK(hipMemcpy(x, d->xeq, sizeof(double) * n, hipMemcpyDeviceToHost));
     if (iters_out) *iters_out = sr.iters;
     return rc;
 }
 
-// Stand-alone CSR solve (rows with sorted columns), CG or ICCG.  ICCG needs
-// a symmetric pattern: the reference fills L_ik from the entry (k, i) of
-// row k (pressure_solver.c:196-228), the up-looking factor here from (i, k).
-extern "C" int mps_hip_solve_csr(int n, const int *row_ptr, const int *col, const double *val, const double *b,
-                                 double *x, int solver_type, int max_iter, double tol, int *iters_out)
+// Checks mps_hip_solve_csr's arguments: row_ptr from 0 and non-decreasing,
+// columns in range and ascending in each row, and for ICCG a symmetric
+// pattern.  The reference fills L_ik from the entry (k, i) of row k
+// (pressure_solver.c:196-228), the up-looking factor here from (i, k).
+static int check_csr(const char *who, int n, const int *row_ptr, const int *col, const double *val,
+                     const double *b, const double *x, int solver_type)
 {
-    int rc = mps_hip_init(0);
-    if (rc) return rc;
-    rc = ctrl_ready();
-    if (rc) return rc;
-    if (iters_out) *iters_out = 0;
-    if (n <= 0) return MPS_OK;
-    if (!row_ptr || !col || !val || !b || !x || row_ptr[0] != 0 || row_ptr[n] < 0) {
-        mps_set_error(MPS_ERR_ARG, __FILE__, __LINE__, "mps_hip_solve_csr: invalid CSR");
+    char msg[160];
+    if (!row_ptr || !col || !val || !b || !x || row_ptr[0] != 0) {
+        snprintf(msg, sizeof msg, "%s: invalid CSR", who);
+        mps_set_error(MPS_ERR_ARG, __FILE__, __LINE__, msg);
         return MPS_ERR_ARG;
     }
     for (int i = 0; i < n; i++) {
+        if (row_ptr[i + 1] < row_ptr[i]) {
+            snprintf(msg, sizeof msg, "%s: row_ptr decreases", who);
+            mps_set_error(MPS_ERR_ARG, __FILE__, __LINE__, msg);
+            return MPS_ERR_ARG;
+        }
         for (int q = row_ptr[i]; q < row_ptr[i + 1]; q++) {
             const bool sorted = q == row_ptr[i] || col[q - 1] < col[q];
             if (col[q] < 0 || col[q] >= n || !sorted) {
-                mps_set_error(MPS_ERR_ARG, __FILE__, __LINE__, "mps_hip_solve_csr: columns out of range or unsorted");
+                snprintf(msg, sizeof msg, "%s: columns out of range or unsorted", who);
+                mps_set_error(MPS_ERR_ARG, __FILE__, __LINE__, msg);
                 return MPS_ERR_ARG;
             }
         }
@@ -1802,14 +1838,149 @@ extern "C" int mps_hip_solve_csr(int n, const int *row_ptr, const int *col, cons
                 const int *lo = col + row_ptr[j], *hi = col + row_ptr[j + 1];
                 const int *at = std::lower_bound(lo, hi, i);
                 if (at == hi || *at != i) {
-                    mps_set_error(MPS_ERR_ARG, __FILE__, __LINE__, "mps_hip_solve_csr: ICCG needs a symmetric pattern");
+                    snprintf(msg, sizeof msg, "%s: ICCG needs a symmetric pattern", who);
+                    mps_set_error(MPS_ERR_ARG, __FILE__, __LINE__, msg);
                     return MPS_ERR_ARG;
                 }
             }
     }
+    return 0;
+}
+
+// Stand-alone CSR solve (rows with sorted columns), CG or ICCG.
+extern "C" int mps_hip_solve_csr(int n, const int *row_ptr, const int *col, const double *val, const double *b,
+                                 double *x, int solver_type, int max_iter, double tol, int *iters_out)
+{
+    int rc = mps_hip_init(0);
+    if (rc) return rc;
+    rc = ctrl_ready();
+    if (rc) return rc;
+    if (iters_out) *iters_out = 0;
+    if (n <= 0) return MPS_OK;
+    rc = check_csr("mps_hip_solve_csr", n, row_ptr, col, val, b, x, solver_type);
+    if (rc) return rc;
     IcData *d = new IcData();
-    rc = solve_csr_on(d, n, row_ptr, col, val, b, x, solver_type, max_iter, tol, iters_out);
+    rc = solve_csr_on(d, g_dctrl, g_hctrl, n, row_ptr, col, val, b, x, solver_type, max_iter, tol, iters_out);
     ic_free(d);
+    return rc;
+}
+
+// ---------------------------------------------------------------------
+// the CSR solve without allocation
+// ---------------------------------------------------------------------
+
+// Every buffer of mps_hip_solve_csr_noalloc lives in this array: the CSR
+// staged from the host, the orders, A's sliced ELL, the factor, the PCG
+// vectors, partial sums, counters and the control block.  Each device
+// holds its own copy for the life of the process; host code uses only the
+// address hipGetSymbolAddress returns for the current device.
+#define NOALLOC_DEVICES 16
+static __device__ __align__(256) unsigned char scratch[MPS_NOALLOC_SCRATCH_BYTES];
+
+static void *g_na_ptr[NOALLOC_DEVICES];            // scratch's device address, per device
+static int g_na_rc[NOALLOC_DEVICES];               // set-up status, per device
+static hipEvent_t g_na_ev[NOALLOC_DEVICES][2];     // the PCG poll's events, per device
+static IcData g_na_ic[NOALLOC_DEVICES];            // the solve's buffer table (pointers into scratch)
+static IcCtrl g_na_hctrl[NOALLOC_DEVICES];         // host copy of the control block
+
+// Set-up of the no-allocation solve on device dev (current): scratch's
+// address and the two poll events.  Run once per device from the library's
+// initialisation; a failure is kept and reported by the entry point.
+int ic_noalloc_setup(int dev)
+{
+    if (dev < 0 || dev >= NOALLOC_DEVICES) return MPS_OK;   // the entry point rejects such a device
+    if (g_na_ptr[dev] || g_na_rc[dev]) return g_na_rc[dev];
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e == hipSuccess && !ptr) e = hipErrorInvalidSymbol;
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&g_na_ev[dev][0], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&g_na_ev[dev][1], hipEventDisableTiming);
+    if (e != hipSuccess) {
+        mps_set_error(MPS_ERR_HIP, __FILE__, __LINE__, hipGetErrorString(e));
+        g_na_rc[dev] = MPS_ERR_HIP;
+        return MPS_ERR_HIP;
+    }
+    g_na_ptr[dev] = ptr;
+    return MPS_OK;
+}
+
+// Scratch bytes the solve takes, as the carves of ic_reserve, ic_order,
+// build_ell and ic_factor take them: n unknowns, nnz entries, rows of at
+// most maxlen entries, nl the larger of the strictly-lower and
+// strictly-upper counts (ICCG only).
+static size_t noalloc_bytes(int n, long long nnz, int maxlen, long long nl, bool ic)
+{
+    const size_t N = (size_t)n + 2, Z = (size_t)(nnz > 0 ? nnz : 1);
+    const size_t nsl = (size_t)(n + WAVE - 1) / WAVE + 1;
+    size_t t = 0;
+    t += 13 * carve_bytes(sizeof(int), N) + 4 * carve_bytes(sizeof(int), N + 1);
+    t += 11 * carve_bytes(sizeof(double), N);
+    t += carve_bytes(sizeof(int), nsl) + carve_bytes(sizeof(int), nsl + 1);
+    t += 2 * carve_bytes(sizeof(int), Z) + 2 * carve_bytes(sizeof(double), Z);
+    t += carve_bytes(sizeof(unsigned), 4) + carve_bytes(sizeof(int), 1) + carve_bytes(sizeof(double), 3 * 65536);
+    t += carve_bytes(sizeof(int), 8) + carve_bytes(sizeof(int), 4 * 2050);
+    t += carve_bytes(sizeof(IcCtrl), 1);
+    const size_t ea = (size_t)((n + WAVE - 1) / WAVE) * WAVE * (size_t)(maxlen > 0 ? maxlen : 1);
+    t += carve_bytes(sizeof(int), ea) + carve_bytes(sizeof(double), ea);
+    if (ic) {
+        const size_t nbh = (size_t)IC_BLK * nblk((size_t)n) + 1;
+        t += 2 * carve_bytes(sizeof(int), nbh);
+        const size_t L = (size_t)(nl > 0 ? nl : 1);
+        t += 2 * carve_bytes(sizeof(int), L) + 2 * carve_bytes(sizeof(double), L);
+    }
+    return t;
+}
+
+// mps_hip_solve_csr with every device buffer in scratch: same arguments,
+// same kernels and launch shapes, so the same bits.  No hipMalloc, hipFree
+// or other allocation.  Calls that overlap on one device race on scratch.
+extern "C" int mps_hip_solve_csr_noalloc(int n, const int *row_ptr, const int *col, const double *val,
+                                         const double *b, double *x, int solver_type, int max_iter, double tol,
+                                         int *iters_out)
+{
+    int rc = mps_hip_init(0);
+    if (rc) return rc;
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_DEVICES) {
+        mps_set_error(MPS_ERR_UNSUPPORTED, __FILE__, __LINE__, "mps_hip_solve_csr_noalloc: device id above 15");
+        return MPS_ERR_UNSUPPORTED;
+    }
+    rc = ic_noalloc_setup(dev);
+    if (rc) return rc;
+    if (iters_out) *iters_out = 0;
+    if (n <= 0) return MPS_OK;
+    rc = check_csr("mps_hip_solve_csr_noalloc", n, row_ptr, col, val, b, x, solver_type);
+    if (rc) return rc;
+    int maxlen = 0;
+    long long lo = 0, up = 0;
+    for (int i = 0; i < n; i++) {
+        const int len = row_ptr[i + 1] - row_ptr[i];
+        if (len > maxlen) maxlen = len;
+        for (int q = row_ptr[i]; q < row_ptr[i + 1]; q++) {
+            if (col[q] < i) lo++;
+            else if (col[q] > i) up++;
+        }
+    }
+    const size_t need = noalloc_bytes(n, row_ptr[n], maxlen, lo > up ? lo : up, solver_type == 1);
+    if (need > (size_t)MPS_NOALLOC_SCRATCH_BYTES) {
+        char msg[200];
+        snprintf(msg, sizeof msg,
+                 "mps_hip_solve_csr_noalloc: %d unknowns, %d entries, rows up to %d need %zu bytes of scratch, "
+                 "above its %zu", n, row_ptr[n], maxlen, need, (size_t)MPS_NOALLOC_SCRATCH_BYTES);
+        mps_set_error(MPS_ERR_ARG, __FILE__, __LINE__, msg);
+        return MPS_ERR_ARG;
+    }
+    IcData *d = &g_na_ic[dev];
+    *d = IcData();
+    d->arena = (char *)g_na_ptr[dev];
+    d->arena_cap = MPS_NOALLOC_SCRATCH_BYTES;
+    d->ev[0] = g_na_ev[dev][0];
+    d->ev[1] = g_na_ev[dev][1];
+    IcCtrl *dctrl = nullptr;
+    REGROW(dctrl, 1);
+    rc = solve_csr_on(d, dctrl, &g_na_hctrl[dev], n, row_ptr, col, val, b, x, solver_type, max_iter, tol,
+                      iters_out);
     return rc;
 }
 

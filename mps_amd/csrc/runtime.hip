@@ -200,6 +200,7 @@ static void prof_collect()
 // ---------------------------------------------------------------------
 
 static int g_device = -1;
+int ic_noalloc_setup(int dev);   // k_ic.hip
 static char g_arch[64] = "";
 static std::map<ParticleSystem *, Ctx *> g_ctx;
 
@@ -220,6 +221,7 @@ static int ensure_device()
         return MPS_ERR_NO_DEVICE;
     }
     g_device = 0;
+    (void)ic_noalloc_setup(0);   // its status is kept for mps_hip_solve_csr_noalloc
     return MPS_OK;
 }
 
