@@ -54,6 +54,12 @@ def make_config(lib, w, solver, tol, max_iter=20000):
     return bench_config(lib, w, solver, tol, max_iter)
 
 
+def fluid_count(ps, n):
+    """Fluid particles (type 0) of a ParticleSystem made by build_scene."""
+    arr = np.ctypeslib.as_array(ps.contents.particles, shape=(n,))
+    return int((arr["type"] == 0).sum())
+
+
 def build_scene(lib, w):
     dims = (C.c_double * 6)(*w["dims"])
     n = lib.mps_gen_dambreak(dims, w["l0"], 3, 2, None, None)
@@ -65,6 +71,13 @@ def build_scene(lib, w):
 
 
 PRECISIONS = {"fp64": 0, "mixed": 1, "mixed32": 2, "mixed32q": 3}
+
+
+def ic_solve_bytes(st, s=8):
+    """Algorithmic bytes of one triangular solve (forward or backward): the
+    strictly-lower entries (value + 4-byte column), 4 bytes of row offset
+    per row, the right-hand side read and the result written."""
+    return (st.nnz - st.n_eq) // 2 * (s + 4) + 4 * st.n_eq + 2 * s * st.n_eq
 
 
 def coded_bytes(st):
@@ -134,9 +147,9 @@ class GpuBnorms:
     """sqrt(b.b) of the PPE step by step from rest, from a GPU run of the
     same scene; its right-hand side is bit-exact with the reference's."""
 
-    def __init__(self, lib, w, tol):
+    def __init__(self, lib, w, tol, solver=0):
         self.lib = lib
-        self.cfg = make_config(lib, w, 0, tol)
+        self.cfg = make_config(lib, w, solver, tol)
         lib.set_config(self.cfg)
         self.ps, _ = build_scene(lib, w)
         lib.particle_system_calc_initial_params(self.ps)
@@ -154,7 +167,7 @@ class GpuBnorms:
         self.lib.particle_system_free(self.ps)
 
 
-def cpu_baseline_reference(lib, w, tol, seconds=20.0, max_steps=300):
+def cpu_baseline_reference(lib, w, tol, solver=0, seconds=20.0, max_steps=300):
     """The reference itself, compiled from its own sources with its OpenMP
     flags (oracle/_ref/libref3dp.so, built by oracle/Makefile), on this
     host: simulation_step of the same workload from rest, for as many steps
@@ -170,9 +183,9 @@ def cpu_baseline_reference(lib, w, tol, seconds=20.0, max_steps=300):
         return None
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import scenes  # the reference generator's lattice rules
-    gb = GpuBnorms(lib, w, tol)
+    gb = GpuBnorms(lib, w, tol, solver)
     ref = Reference("3dp")
-    cfg_o = make_config(lib, w, 0, tol)
+    cfg_o = make_config(lib, w, solver, tol)
     cfg_r = ref.config_from(cfg_o)
     d = w["dims"]
     pos, vel, typ = scenes.dambreak(3, d[:3], d[3:], w["l0"], 3, 2)
@@ -196,21 +209,22 @@ def cpu_baseline_reference(lib, w, tol, seconds=20.0, max_steps=300):
     ref.free_lists(nl, cl)
     gi = gb.iters
     gb.close()
+    sname = "ICCG" if solver == 1 else "CG"
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return dict(value=len(typ) * steps / el, unit="particle-steps/s", cores=threads, kind="reference",
                 sample="%d step(s) from rest of the %d-particle workload through the reference's own "
-                       "simulation_step (OpenMP, %d threads), CG to tol*|b| per step (the GPU's CG took "
+                       "simulation_step (OpenMP, %d threads), %s to tol*|b| per step (the GPU's %s took "
                        "%.0f iterations/step on those steps), %.1f s"
-                % (steps, len(typ), threads, sum(gi) / steps, el))
+                % (steps, len(typ), threads, sname, sname, sum(gi) / steps, el))
 
 
-def profiled_traffic(precision, workload, nnz):
+def profiled_traffic(precision, workload, nnz, prefix="cg"):
     """HBM bytes per launch of the solver's SpMV, as measured by rocprofv3
     (FETCH_SIZE + WRITE_SIZE scaled by the plain-copy calibration) and
     recorded in profiles/cg_<precision>_<workload>.md for a matrix of
     traffic_nnz nonzeros; scaled to this run's matrix by its nonzero count
     (the traffic is proportional to it).  None if no profile is recorded."""
-    path = os.path.join(ROOT, "profiles", "cg_%s_%s.md" % (precision, workload))
+    path = os.path.join(ROOT, "profiles", "%s_%s_%s.md" % (prefix, precision, workload))
     t = z = None
     try:
         for line in open(path):
@@ -724,8 +738,10 @@ def main():
                 "pblk": res.pblk, "pblk_ms": [round(v, 3) for v in res.pblk_ms],
                 "seconds": round(time.perf_counter() - t_tune, 2)}
     lib.check(lib.mps_hip_run_steps(ps, a.warmup), "warm-up steps")
-    # the timed steps time only the SpMV launches (two HIP events each)
-    lib.set_option("profile", 0 if a.no_profile else 2)
+    # the timed steps time only the dominant kernel's launches (two HIP
+    # events each): the SpMV for CG, the forward triangular solve for ICCG
+    iccg = a.solver == "iccg"
+    lib.set_option("profile", 0 if a.no_profile else (3 if iccg else 2))
     lib.mps_hip_profile_reset()
     st0 = mps_amd.StepStats()
     lib.mps_hip_get_stats(ps, C.byref(st0))
@@ -739,11 +755,19 @@ def main():
     iters = (st.total_iters - st0.total_iters) / max(a.steps, 1)
     ms = C.c_double()
     cnt = C.c_longlong()
-    lib.mps_hip_profile_get(b"spmv", C.byref(ms), C.byref(cnt))
+    lib.mps_hip_profile_get(b"ic_fwd" if iccg else b"spmv", C.byref(ms), C.byref(cnt))
     spmv_ms = ms.value / max(cnt.value, 1)
     spmv_launches = cnt.value
     roof = None
-    if cnt.value > 0:
+    if cnt.value > 0 and iccg:
+        byt = ic_solve_bytes(st)
+        ach = byt / (spmv_ms * 1e-3) / 1e9
+        traffic, tsrc = profiled_traffic("fwd", a.workload, st.nnz, prefix="iccg")
+        roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=traffic, traffic_source=tsrc,
+                    kernel="k_ic_csolve<0, false, 32, 8> (forward solve)", bytes_per_launch=int(byt),
+                    mean_launch_ms=round(spmv_ms, 5), launches=spmv_launches)
+    elif cnt.value > 0:
         byt = {"fp64": spmv_bytes(st, 8, 8), "mixed": spmv_bytes(st, 4, 8),
                "mixed32": spmv_bytes(st, 4, 4), "mixed32q": coded_bytes(st)}[a.precision]
         ach = byt / (spmv_ms * 1e-3) / 1e9
@@ -785,7 +809,8 @@ def main():
     cpu = None
     if not a.no_cpu_baseline and rank == 0:
         try:
-            cpu = cpu_baseline_reference(lib, w, a.tol) if a.cpu_kind == "reference" else None
+            cpu = (cpu_baseline_reference(lib, w, a.tol, solver=1 if a.solver == "iccg" else 0)
+                   if a.cpu_kind == "reference" else None)
             if cpu is None:
                 cpu = cpu_baseline(w, a.tol)
         except Exception as e:  # baseline is reported, never required
@@ -798,7 +823,7 @@ def main():
         "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if a.precision == "fp64" else "f32+f64",
         "data": "synthetic (reference dam-break lattice generator, fluid at rest)",
-        "config": {"workload": a.workload, "particles": n, "fluid": None, "l0": w["l0"],
+        "config": {"workload": a.workload, "particles": n, "fluid": fluid_count(ps, n), "l0": w["l0"],
                    "dt": w["dt"], "solver": a.solver, "precision": a.precision,
                    "tolerance": "relative %g" % a.tol, "mean_iters_per_step": round(iters, 1),
                    "n_eq": st.n_eq, "nnz": st.nnz, "pairs": st.n_pairs, "parallelism": "1 GPU",

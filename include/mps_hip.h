@@ -38,13 +38,13 @@ extern "C" {
 /* ===================================================================== */
 
 /* --- configuration (host C, same keys, defaults and derivations) ------ */
-/* mps_fluid3D_parallel/include/sim_config.h:86  config_set_defaults      */
+/* mps_fluid3D_parallel/include/sim_config.h:84  config_set_defaults      */
 void config_set_defaults(SimConfig *config);
-/* mps_fluid3D_parallel/include/sim_config.h:89  config_load_cal          */
+/* mps_fluid3D_parallel/include/sim_config.h:87  config_load_cal          */
 int  config_load_cal(const char *cal_path, SimConfig *config);
-/* mps_fluid3D_parallel/include/sim_config.h:92  config_load_params       */
+/* mps_fluid3D_parallel/include/sim_config.h:90  config_load_params       */
 int  config_load_params(const char *param_path, SimConfig *config);
-/* mps_fluid3D_parallel/include/sim_config.h:95  config_print             */
+/* mps_fluid3D_parallel/include/sim_config.h:93  config_print             */
 void config_print(const SimConfig *config);
 
 /* --- particle container ------------------------------------------------ */
@@ -273,7 +273,7 @@ int  mps_hip_step_backward(ParticleSystem *ps, NeighborList *nl, const double *r
  *                      bit for bit; not used while kernels are timed or a
  *                      tape records (0, default: plain launches)
  *   "profile"        1 times the main kernels with HIP events, 2 only the
- *                      CG SpMV
+ *                      CG SpMV, 3 only the ICCG forward solve
  *   "pblk"           block size of the per-particle kernels (collision,
  *                      predictor, search, assembly, gradient): 128, 256
  *                      (default) or 512; same results bit for bit

@@ -579,7 +579,7 @@ k_ic_factor(int n, const int *__restrict__ order, const int *__restrict__ rp, co
                             if (j < bn) bc[j] = col[kb + u + j];
                         }
 #pragma unroll
-                        for (int j = 0; j < FB; j++) bval[j] = j < bn ? ld_rlx(&lval[kb + u + j]) : 0.0;
+                        for (int j = 0; j < FB; j++) bval[j] = j < bn ? lval[kb + u + j] : 0.0;   // cached; see k_ic_factor2
                         // row k's columns are all below k = scol[T]: find each
                         // among this row's columns 0..T-1 (last index < c)
                         int ps[FB];
@@ -647,15 +647,17 @@ k_ic_factor(int n, const int *__restrict__ order, const int *__restrict__ rp, co
 
 // Two-phase form of the factor for rows of at most 64 strictly-lower
 // entries (lcap <= 64), one wavefront per workgroup.  Lane t owns entry t,
-// (e, k): once row k is published it streams row k's entries in batches of
-// FB2 and keeps each match -- the position v in this row of a column the two
-// rows share, and L_kj -- in its own list in LDS (lane t has at most t
-// matches: list t starts at t (t - 1) / 2), without waiting on its own row.
-// Then it folds the list in order, each product as soon as lane v of this
-// row is done, all from LDS.  The row's in-lane chain then costs LDS
-// iterations instead of a memory round trip per link; the arithmetic is the
-// one-phase kernel's, term for term.
-#define FB2 32
+// (e, k).  Once row k is published the whole wave loads row k (its at most
+// 64 lower entries, entry j in lane j: two coalesced loads), up to FG2 such
+// rows in flight, and keeps each match -- the position v in this row of a
+// column the two rows share, and L_kj -- in lane t's list in LDS, in row k's
+// entry order (lane t has at most t matches: list t starts at t (t - 1) / 2),
+// without waiting on its own row.  Lane t then folds its list in order, each
+// product as soon as lane v of this row is done, all from LDS.  A row's
+// in-lane chain costs LDS iterations instead of a memory round trip per
+// link, and a gathered row costs a few cache lines instead of one per entry
+// and lane; the arithmetic is the one-phase kernel's, term for term.
+#define FG2 4
 __global__ void __launch_bounds__(WAVE)
 k_ic_factor2(int n, const int *__restrict__ order, const int *__restrict__ rp, const int *__restrict__ col,
              const int *__restrict__ dpos, const double *__restrict__ aval, double *lval, double *dinv,
@@ -693,7 +695,7 @@ k_ic_factor2(int n, const int *__restrict__ order, const int *__restrict__ rp, c
         const int lb = T * (T - 1) / 2;   // this lane's match list
         int stage = act ? 0 : 3;          // 0 waiting for row k, 1 gathering, 2 folding, 3 done
         double dk = 0.0, s = dense ? 0.0 : a;
-        int u = 0, cnt = 0, mi = 0;
+        int cnt = 0, mi = 0;
         long long spins = 0;
         bool poll_all = true;
         int idle = 0;
@@ -704,7 +706,8 @@ k_ic_factor2(int n, const int *__restrict__ order, const int *__restrict__ rp, c
             if (waitm) {
                 const int lo = __ffsll((long long)waitm) - 1;
                 if (stage == 0 && (poll_all || lane == lo)) {
-                    const double d = ld_rlx(&dinv[k]);
+                    // the first poll through the caches (a non-sentinel is final)
+                    const double d = spins == 0 ? dinv[k] : ld_rlx(&dinv[k]);
                     if (!is_sent(d)) {
                         dk = d;
                         stage = 1;
@@ -717,50 +720,73 @@ k_ic_factor2(int n, const int *__restrict__ order, const int *__restrict__ rp, c
                 }
                 poll_all = __shfl((int)(stage != 0), lo, WAVE) != 0;
             }
-            if (stage == 1) {
-                if (u >= mk) {
-                    stage = 2;
-                } else {
-                    const int bn = min(FB2, mk - u);
-                    int bc[FB2];
-                    double bv[FB2];
+            // gather: row k of each lane in stage 1, FG2 rows at a time,
+            // the whole wave loading one row (entry j in lane j)
+            unsigned long long readym = ballot64(stage == 1);
+            while (readym) {
+                int tg[FG2], kbg[FG2], mkg[FG2], cj[FG2];
+                double vj[FG2];
 #pragma unroll
-                    for (int j = 0; j < FB2; j++) bc[j] = j < bn ? col[kb + u + j] : INT_MAX;
+                for (int g = 0; g < FG2; g++) {
+                    tg[g] = readym ? __ffsll((long long)readym) - 1 : -1;
+                    readym &= readym ? readym - 1 : 0ull;
+                }
 #pragma unroll
-                    for (int j = 0; j < FB2; j++) bv[j] = j < bn ? ld_rlx(&lval[kb + u + j]) : 0.0;
-                    bool all_there = true;
-#pragma unroll
-                    for (int j = 0; j < FB2; j++) all_there = all_there && !is_sent(bv[j]);
-                    if (!all_there && ++spins <= wait_limit) {
-                        // row k's 1/sqrt(d) came before some of its values:
-                        // load this batch again next time
-                    } else {
-                        if (!all_there) failed = true;   // give up: missing values read as 0
-                        int ps[FB2];
-#pragma unroll
-                        for (int j = 0; j < FB2; j++) ps[j] = -1;
-                        for (int st = pstep; st > 0; st >>= 1) {
-#pragma unroll
-                            for (int j = 0; j < FB2; j++) {
-                                const int cand = ps[j] + st;
-                                if (cand < T && scol[cand] < bc[j]) ps[j] = cand;
-                            }
+                for (int g = 0; g < FG2; g++) {
+                    cj[g] = INT_MAX;
+                    vj[g] = 0.0;
+                    kbg[g] = 0;
+                    mkg[g] = 0;
+                    if (tg[g] >= 0) {
+                        kbg[g] = __shfl(kb, tg[g], WAVE);
+                        mkg[g] = __shfl(mk, tg[g], WAVE);
+                        const bool again = __shfl((int)(spins != 0), tg[g], WAVE) != 0;
+                        if (lane < mkg[g]) {
+                            cj[g] = col[kbg[g] + lane];
+                            // The first load goes through the caches: a value
+                            // is written once per factor (the sentinel before
+                            // it by an earlier kernel, whose end made it
+                            // visible to every cache), so a non-sentinel read
+                            // from any cache is final.  Only a sentinel can be
+                            // stale: that row is loaded again past the caches.
+                            vj[g] = again ? ld_rlx(&lval[kbg[g] + lane]) : lval[kbg[g] + lane];
                         }
-#pragma unroll
-                        for (int j = 0; j < FB2; j++) {
-                            const int cand = ps[j] + 1;
-                            if (j < bn && cand < T && scol[cand] == bc[j]) {
-                                mpos[lb + cnt] = (unsigned short)cand;
-                                mval[lb + cnt] = is_sent(bv[j]) ? 0.0 : bv[j];
-                                cnt++;
-                            }
-                        }
-                        u += bn;
-                        spins = 0;
-                        prog = true;
                     }
                 }
+#pragma unroll
+                for (int g = 0; g < FG2; g++) {
+                    const int Tg = tg[g];
+                    if (Tg < 0) continue;
+                    if (ballot64(lane < mkg[g] && is_sent(vj[g]))) {
+                        // row k's 1/sqrt(d) came before some of its values:
+                        // load it again next time, or give up at the bound
+                        int quit = 0;
+                        if (lane == Tg) quit = ++spins > wait_limit;
+                        if (!__shfl(quit, Tg, WAVE)) continue;
+                        if (lane == Tg) failed = true;   // missing values read as 0
+                    }
+                    int ps = -1;
+                    for (int st = pstep; st > 0; st >>= 1) {
+                        const int cand = ps + st;
+                        if (cand < Tg && scol[cand] < cj[g]) ps = cand;
+                    }
+                    ps++;
+                    const bool hit = lane < mkg[g] && ps < Tg && scol[ps] == cj[g];
+                    const unsigned long long hm = ballot64(hit);
+                    if (hit) {
+                        const int q = Tg * (Tg - 1) / 2 + __popcll(hm & ((1ull << lane) - 1ull));
+                        mpos[q] = (unsigned short)ps;
+                        mval[q] = is_sent(vj[g]) ? 0.0 : vj[g];
+                    }
+                    if (lane == Tg) {
+                        cnt = __popcll(hm);
+                        stage = 2;
+                        spins = 0;
+                    }
+                    prog = true;
+                }
             }
+            wave_sync();   // the lists before their owners read them
             // the fold chain from LDS, as many links as are ready, before
             // the next poll or gather
             for (;;) {
@@ -1002,7 +1028,10 @@ k_ic_csolve(int n, const int *__restrict__ perm, const double *__restrict__ lev,
                     c = ecol[o0 + g0 + lane];
                     l = eval[o0 + g0 + lane];
                     in = c >= cbase && c < cbase + cnt_rows;
-                    if (!in) v = ld_rlx(&y[c]);
+                    // first load through the caches: y held the sentinel
+                    // before this kernel and each value is written once, so
+                    // only a sentinel can be stale (polled again past them)
+                    if (!in) v = y[c];
                 }
                 if (act && in) {
                     v = sent();

@@ -155,6 +155,8 @@ static hipEvent_t pool_get()
 // unprofiled speed).
 static bool prof_on(int k)
 {
+    // 1 every kernel class, 2 the SpMV alone, 3 the IC forward solve alone
+    if (g_opt.profile > 2.5) return k == PK_IC;
     return g_opt.profile > 1.5 ? k == PK_SPMV : g_opt.profile > 0.5;
 }
 

@@ -447,14 +447,16 @@ def _matrix(name, **over):
     return dict(n=n, rp=rp, col=col, aval=aval, dpos=dpos, q=q, dense=dim == 2)
 
 
-FB2 = 32
+FG2 = 4   # rows of ready lanes loaded together (k_ic_factor2)
 
 
 def factor2_wave(A, st, mem, dense, wait_limit, ordered=True, order=None):
     """One wave of k_ic_factor2 (rows of at most 64 entries): lane t waits
-    for row k, gathers its matches (position v in this row, L_kj) into its
-    own list batch by batch -- a batch with a value not yet visible is loaded
-    again -- then folds the list in order as lanes v finish."""
+    for row k; the wave then loads row k whole (the ready lanes' rows FG2 at
+    a time, all loaded before any is matched) and puts the matches (position
+    v in this row, L_kj) into lane t's list in row k's entry order -- a row
+    with a value not yet visible is loaded again next time -- then lane t
+    folds its list in order as lanes v finish."""
     rp, col, dpos, aval = A["rp"], A["col"], A["dpos"], A["aval"]
     lval, dinv = st["lval"], st["dinv"]
     me = object()
@@ -474,7 +476,7 @@ def factor2_wave(A, st, mem, dense, wait_limit, ordered=True, order=None):
             act = T < m
             k = scol[T] if act else 0
             L.append(dict(T=T, k=k, a=aval[r0 + T] if act else 0.0, kb=rp[k], mk=dpos[k], stage=0 if act else 3,
-                          dk=0.0, s=0.0 if dense else (aval[r0 + T] if act else 0.0), u=0, lst=[], mi=0, spins=0))
+                          dk=0.0, s=0.0 if dense else (aval[r0 + T] if act else 0.0), lst=[], mi=0, spins=0))
         poll_all = True
         while True:
             if all(x["stage"] == 3 for x in L):
@@ -494,32 +496,27 @@ def factor2_wave(A, st, mem, dense, wait_limit, ordered=True, order=None):
                                 st["err"] = True
                                 x["stage"] = 1
                 poll_all = L[lo]["stage"] != 0
-            for x in L:
-                if x["stage"] == 1:
-                    if x["u"] >= x["mk"]:
-                        x["stage"] = 2
-                    else:
-                        bn = min(FB2, x["mk"] - x["u"])
-                        base = x["kb"] + x["u"]
-                        bc = [col[base + j] for j in range(bn)]
-                        bv = [lval[base + j] for j in range(bn)]
-                        there = all(v is not None for v in bv)
-                        if not there:
-                            x["spins"] += 1
-                            if x["spins"] <= wait_limit:
-                                continue
-                            st["err"] = True
-                        for j in range(bn):
-                            ps, stp = -1, pstep
-                            while stp > 0:
-                                cand = ps + stp
-                                if cand < x["T"] and scol[cand] < bc[j]:
-                                    ps = cand
-                                stp >>= 1
-                            if ps + 1 < x["T"] and scol[ps + 1] == bc[j]:
-                                x["lst"].append((ps + 1, 0.0 if bv[j] is None else bv[j]))
-                        x["u"] += bn
-                        x["spins"] = 0
+            ready = [x for x in L if x["stage"] == 1]
+            for g0 in range(0, len(ready), FG2):
+                grp = ready[g0:g0 + FG2]
+                rows = [([col[x["kb"] + j] for j in range(x["mk"])],
+                         [lval[x["kb"] + j] for j in range(x["mk"])]) for x in grp]
+                for x, (bc, bv) in zip(grp, rows):
+                    if any(v is None for v in bv):
+                        x["spins"] += 1
+                        if x["spins"] <= wait_limit:
+                            continue
+                        st["err"] = True
+                    for j in range(x["mk"]):
+                        ps, stp = -1, pstep
+                        while stp > 0:
+                            cand = ps + stp
+                            if cand < x["T"] and scol[cand] < bc[j]:
+                                ps = cand
+                            stp >>= 1
+                        if ps + 1 < x["T"] and scol[ps + 1] == bc[j]:
+                            x["lst"].append((ps + 1, 0.0 if bv[j] is None else bv[j]))
+                    x["stage"], x["spins"] = 2, 0
             # the fold chain, as many links as are ready
             while True:
                 donem = [x["stage"] == 3 for x in L]
