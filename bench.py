@@ -218,13 +218,14 @@ def cpu_baseline_reference(lib, w, tol, solver=0, seconds=20.0, max_steps=300):
                 % (steps, len(typ), threads, sname, sname, sum(gi) / steps, el))
 
 
-def profiled_traffic(precision, workload, nnz, prefix="cg"):
-    """HBM bytes per launch of the solver's SpMV, as measured by rocprofv3
-    (FETCH_SIZE + WRITE_SIZE scaled by the plain-copy calibration) and
-    recorded in profiles/cg_<precision>_<workload>.md for a matrix of
+def profiled_traffic(precision, workload, nnz, name=None):
+    """HBM bytes per launch of the solver's dominant kernel (the SpMV; the
+    forward solve for ICCG, from profiles/iccg_<workload>.md), as measured
+    by rocprofv3 (FETCH_SIZE + WRITE_SIZE scaled by the plain-copy
+    calibration) and recorded in profiles/cg_<precision>_<workload>.md for a matrix of
     traffic_nnz nonzeros; scaled to this run's matrix by its nonzero count
     (the traffic is proportional to it).  None if no profile is recorded."""
-    path = os.path.join(ROOT, "profiles", "%s_%s_%s.md" % (prefix, precision, workload))
+    path = os.path.join(ROOT, "profiles", name or "cg_%s_%s.md" % (precision, workload))
     t = z = None
     try:
         for line in open(path):
@@ -762,7 +763,7 @@ def main():
     if cnt.value > 0 and iccg:
         byt = ic_solve_bytes(st)
         ach = byt / (spmv_ms * 1e-3) / 1e9
-        traffic, tsrc = profiled_traffic("fwd", a.workload, st.nnz, prefix="iccg")
+        traffic, tsrc = profiled_traffic("fp64", a.workload, st.nnz, name="iccg_%s.md" % a.workload)
         roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(ach / HBM_PEAK_GBS, 4), traffic=traffic, traffic_source=tsrc,
                     kernel="k_ic_csolve<0, false, 32, 8> (forward solve)", bytes_per_launch=int(byt),
