@@ -461,23 +461,25 @@ __global__ void k_adj_coll_init(int n, const int *__restrict__ typ, const dvec *
     ubs[i] = u;
 }
 
-__device__ __forceinline__ void atomic_add_vec(dvec *p, const dvec &v)
-{
-#pragma unroll
-    for (int d = 0; d < D_; d++) atomicAdd(&p->c[d], v.c[d]);
-}
-
 // Collision reverse, the firing sequence of fluid i recomputed from r*, u*
 // (candidates: the stale list, every partner closer than the collision
 // distance; 2-D in ascending index as mps_fluid2D/src/operators.c:341-379,
-// 3-D in list order as operators.c:395-432), then run backwards.
+// 3-D in list order as operators.c:395-432), then run backwards.  The own
+// terms go to i's adjoints; the partner terms are left as records
+// (partner, r-bar, u-bar) in i's COLL_MAX slots for k_adj_coll_pull, so
+// that every sum runs in a fixed order.
 __global__ void k_adj_coll(int n, const int *__restrict__ typ, const dvec *__restrict__ rs, const dvec *__restrict__ us,
                            const int *__restrict__ lst, const int *__restrict__ lidx, const dvec *__restrict__ rbss,
                            const dvec *__restrict__ ubss, StepConsts k, dvec *__restrict__ rbs, dvec *__restrict__ ubs,
-                           int *err)
+                           int *__restrict__ ck, int *__restrict__ cj, dvec *__restrict__ cr, dvec *__restrict__ cu,
+                           unsigned char *__restrict__ ctk, int *err)
 {
     const int i = blockIdx.x * AB + threadIdx.x;
-    if (i >= n || typ[i] != FLUID_PARTICLE) return;
+    if (i >= n) return;
+    if (typ[i] != FLUID_PARTICLE) {
+        ck[i] = 0;
+        return;
+    }
     int js[COLL_MAX];
     dvec vbefore[COLL_MAX];
     int K = 0;
@@ -515,7 +517,8 @@ __global__ void k_adj_coll(int n, const int *__restrict__ typ, const dvec *__res
         for (int dd = 0; dd < D_; dd++) vn += (vi.c[dd] - us[j].c[dd]) * (d.c[dd] / r);
         if (vn <= 0.0) continue;
         if (K == COLL_MAX) {
-            atomicExch(err, 1);
+            ck[i] = 0;
+            atomicOr(err, 1);
             return;
         }
         js[K] = j;
@@ -527,10 +530,13 @@ __global__ void k_adj_coll(int n, const int *__restrict__ typ, const dvec *__res
     }
     }
 reverse:
+    ck[i] = K;
+    {
     // v-bar of the final velocity: u** = v_f, r** = r* + dt (v_f - u*)
     dvec vb = ubss[i];
     daxpy(vb, k.dt, rbss[i]);
     dvec rown = dzero();
+    const size_t base = (size_t)i * COLL_MAX;
     for (int s = K - 1; s >= 0; s--) {
         const int j = js[s];
         const dvec d = dsub(rs[j], ri);
@@ -550,19 +556,67 @@ reverse:
         daxpy(dbar, -nn, nh);
 #pragma unroll
         for (int dd = 0; dd < D_; dd++) dbar.c[dd] /= r;
-        // d = r*_j - r*_i
-        if (typ[j] == FLUID_PARTICLE) {
-            atomic_add_vec(&rbs[j], dbar);
-            dvec uj = dzero();
-            daxpy(uj, k.e_half * nv, nh);
-            atomic_add_vec(&ubs[j], uj);
-        }
+        // d = r*_j - r*_i: the partner's share, when it is fluid
+        dvec uj = dzero();
+        daxpy(uj, k.e_half * nv, nh);
+        cj[base + s] = typ[j] == FLUID_PARTICLE ? j : -1;
+        cr[base + s] = dbar;
+        cu[base + s] = uj;
+        ctk[base + s] = 0;
         daxpy(rown, -1.0, dbar);
         // propagate to v_before: (I - c nh nh^T) vb
         daxpy(vb, -k.e_half * nv, nh);
     }
-    atomic_add_vec(&rbs[i], rown);
-    atomic_add_vec(&ubs[i], vb);
+    dvec a = rbs[i];
+    daxpy(a, 1.0, rown);
+    rbs[i] = a;
+    a = ubs[i];
+    daxpy(a, 1.0, vb);
+    ubs[i] = a;
+    }
+}
+
+// Collision reverse, partner terms: fluid j adds the records that name it,
+// pulled from the particles of its stale list in list order (the list is
+// symmetric, so every particle whose sequence holds j is in it; a pair
+// fires at most once per sequence).  Each consumed record is marked.
+__global__ void k_adj_coll_pull(int n, const int *__restrict__ typ, const int *__restrict__ lst,
+                                const int *__restrict__ lidx, const int *__restrict__ ck, const int *__restrict__ cj,
+                                const dvec *__restrict__ cr, const dvec *__restrict__ cu,
+                                unsigned char *__restrict__ ctk, dvec *__restrict__ rbs, dvec *__restrict__ ubs)
+{
+    const int j = blockIdx.x * AB + threadIdx.x;
+    if (j >= n || typ[j] != FLUID_PARTICLE) return;
+    dvec ra = rbs[j], ua = ubs[j];
+    for (int q = lst[j]; q < lst[j + 1]; q++) {
+        const int i = lidx[q];
+        const int K = ck[i];
+        const size_t base = (size_t)i * COLL_MAX;
+        for (int s = 0; s < K; s++) {
+            if (cj[base + s] != j) continue;
+            daxpy(ra, 1.0, cr[base + s]);
+            daxpy(ua, 1.0, cu[base + s]);
+            ctk[base + s] = 1;
+            break;
+        }
+    }
+    rbs[j] = ra;
+    ubs[j] = ua;
+}
+
+// Every record with a fluid partner was consumed: a record left over means
+// the partner's list does not hold the particle (an asymmetric list).
+__global__ void k_adj_coll_check(int n, const int *__restrict__ ck, const int *__restrict__ cj,
+                                 const unsigned char *__restrict__ ctk, int *err)
+{
+    const int i = blockIdx.x * AB + threadIdx.x;
+    if (i >= n) return;
+    const size_t base = (size_t)i * COLL_MAX;
+    for (int s = 0; s < ck[i]; s++)
+        if (cj[base + s] >= 0 && !ctk[base + s]) {
+            atomicOr(err, 2);
+            return;
+        }
 }
 
 // Predictor reverse, step 1: a-bar of the predictor for every fluid i.
@@ -866,11 +920,30 @@ int mps_ctx_step(Ctx *c, const SimConfig *cf);                     // runtime.hi
 int csr_cg_device(int n, long long nnz, const int *rp, const int *col, const double *val, const double *b,
                   double *x, const SolveOpts &o, SolveResult *res, hipStream_t s);   // k_ic.hip
 
+// The collision reverse's error bits: 1, a particle collided more often in
+// one step than its COLL_MAX record slots hold; 2, a record's partner did
+// not find the particle in its own list.
+static int coll_error(int bits)
+{
+    if (bits & 1)
+        mps_set_error(MPS_ERR_UNSUPPORTED, __FILE__, __LINE__,
+                      "more collisions of one particle in one step than the reverse sweep holds");
+    else
+        mps_set_error(MPS_ERR_ARG, __FILE__, __LINE__,
+                      "a collision partner's neighbour list does not hold the particle (asymmetric list)");
+    return (bits & 1) ? MPS_ERR_UNSUPPORTED : MPS_ERR_ARG;
+}
+
 struct AdjWork {
     int n = 0;
     dvec *rb = nullptr, *ub = nullptr, *rbss = nullptr, *ubss = nullptr, *rbs = nullptr, *ubs = nullptr;
     dvec *abar = nullptr, *F = nullptr, *apbar = nullptr, *ust = nullptr;
     int *mu = nullptr, *len = nullptr, *roff = nullptr, *eq = nullptr, *col = nullptr, *err = nullptr, *scan_blk = nullptr;
+    // collision records: count per particle, COLL_MAX (partner, r-bar,
+    // u-bar, consumed) slots per particle
+    int *ck = nullptr, *cj = nullptr;
+    dvec *cr = nullptr, *cu = nullptr;
+    unsigned char *ctk = nullptr;
     double *xbar = nullptr, *lam = nullptr, *val = nullptr, *rhs = nullptr, *sol = nullptr, *part = nullptr,
            *params = nullptr;
     size_t cap_nnz = 0;
@@ -879,7 +952,8 @@ struct AdjWork {
 static void adj_free(AdjWork &w)
 {
     void *b[] = {w.rb, w.ub, w.rbss, w.ubss, w.rbs, w.ubs, w.abar, w.F, w.apbar, w.ust, w.mu, w.len, w.roff, w.eq,
-                 w.col, w.err, w.scan_blk, w.xbar, w.lam, w.val, w.rhs, w.sol, w.part, w.params};
+                 w.col, w.err, w.scan_blk, w.xbar, w.lam, w.val, w.rhs, w.sol, w.part, w.params,
+                 w.ck, w.cj, w.cr, w.cu, w.ctk};
     for (void *p : b)
         if (p) hipFree(p);
     w = AdjWork();
@@ -902,6 +976,11 @@ static int adj_alloc(AdjWork &w, int n, hipStream_t s)
     HIP_OK(hipMalloc(&w.sol, sizeof(double) * n));
     HIP_OK(hipMalloc(&w.part, sizeof(double) * 4 * ((size_t)ab(n) + 1)));
     HIP_OK(hipMalloc(&w.params, sizeof(double) * 4));
+    HIP_OK(hipMalloc(&w.ck, sizeof(int) * n));
+    HIP_OK(hipMalloc(&w.cj, sizeof(int) * COLL_MAX * (size_t)n));
+    HIP_OK(hipMalloc(&w.cr, sizeof(dvec) * COLL_MAX * (size_t)n));
+    HIP_OK(hipMalloc(&w.cu, sizeof(dvec) * COLL_MAX * (size_t)n));
+    HIP_OK(hipMalloc(&w.ctk, COLL_MAX * (size_t)n));
     // on the stream the reverse kernels run on (a non-blocking stream does
     // not wait for null-stream work)
     HIP_OK(hipMemsetAsync(w.params, 0, sizeof(double) * 4, s));
@@ -991,7 +1070,10 @@ static int adj_step(AdjWork &w, const TapeStep &t, const int *ls, const int *lsi
                        w.rbss);
     hipLaunchKernelGGL(k_adj_coll_init, dim3(g), dim3(AB), 0, s, n, t.typ, w.rbss, k, w.rbs, w.ubs);
     hipLaunchKernelGGL(k_adj_coll, dim3(g), dim3(AB), 0, s, n, t.typ, t.rs, t.us, ls, lsi, w.rbss, w.ubss, k, w.rbs,
-                       w.ubs, w.err);
+                       w.ubs, w.ck, w.cj, w.cr, w.cu, w.ctk, w.err);
+    hipLaunchKernelGGL(k_adj_coll_pull, dim3(g), dim3(AB), 0, s, n, t.typ, ls, lsi, w.ck, w.cj, w.cr, w.cu, w.ctk,
+                       w.rbs, w.ubs);
+    hipLaunchKernelGGL(k_adj_coll_check, dim3(g), dim3(AB), 0, s, n, w.ck, w.cj, w.ctk, w.err);
     LAUNCH_CHECK();
     hipLaunchKernelGGL(k_adj_pred_pre, dim3(g), dim3(AB), 0, s, n, t.typ, w.rbs, w.ubs, k, w.apbar, w.ust);
     hipLaunchKernelGGL(k_adj_pred, dim3(g), dim3(AB), 0, s, n, t.typ, t.r, t.u, ls, lsi, w.rbs, w.apbar, w.ust, k, w.rb,
@@ -1130,10 +1212,7 @@ extern "C" int mps_hip_adjoint(ParticleSystem *ps, int nsteps, int checkpoint_ev
             hipStreamSynchronize(s) != hipSuccess)
             rc = MPS_ERR_HIP;
     }
-    if (!rc && herr) {
-        mps_set_error(MPS_ERR_UNSUPPORTED, __FILE__, __LINE__, "more collisions of one particle in one step than the reverse sweep holds");
-        rc = MPS_ERR_UNSUPPORTED;
-    }
+    if (!rc && herr) rc = coll_error(herr);
     if (!rc) {
         if (loss_out) *loss_out = loss;
         if (grad_nu) *grad_nu = params[0];
@@ -1342,10 +1421,7 @@ extern "C" int mps_hip_step_backward(ParticleSystem *ps, NeighborList *nl, const
             hipStreamSynchronize(s) != hipSuccess)
             rc = MPS_ERR_HIP;
     }
-    if (!rc && herr) {
-        mps_set_error(MPS_ERR_UNSUPPORTED, __FILE__, __LINE__, "more collisions of one particle in one step than the reverse sweep holds");
-        rc = MPS_ERR_UNSUPPORTED;
-    }
+    if (!rc && herr) rc = coll_error(herr);
     if (!rc) rc = mps_ctx_download(c, ps, nl);
     if (!rc) {
         if (grad_nu) *grad_nu = params[0];
