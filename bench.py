@@ -638,6 +638,13 @@ def calibrate_copy(nbytes=1 << 30, reps=5):
                       "GBps_read_plus_write": round(2 * nbytes * reps / el / 1e9, 1)}), flush=True)
 
 
+def apply_options(lib, opts):
+    """--option KEY=VALUE settings, applied after the defaults."""
+    for kv in opts:
+        key, _, val = kv.partition("=")
+        lib.check(lib.mps_hip_set_option(key.encode(), float(val)), "option " + kv)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -676,6 +683,8 @@ def main():
                          "steps, with the synchronous and with the overlapped writer; not the headline")
     ap.add_argument("--sequential", action="store_true",
                     help="ensemble workloads: also time the scenes one after another and compare the bits")
+    ap.add_argument("--option", action="append", default=[], metavar="KEY=VALUE",
+                    help="set a library option (mps_hip_set_option) after the defaults; repeatable")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy 5 times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
@@ -702,6 +711,7 @@ def main():
         lib.set_option("spmv", a.spmv)
         lib.set_option("graph", a.graph)
         lib.set_option("precision", PRECISIONS[a.precision])
+        apply_options(lib, a.option)
         run_ensemble(lib, a)
         return
     if a.graph is None:
@@ -714,6 +724,7 @@ def main():
     lib.set_option("spmv", a.spmv)
     lib.set_option("fused", a.fused)
     lib.set_option("graph", a.graph)
+    apply_options(lib, a.option)
     ps, n = build_scene(lib, w)
     print("bench: %s, %d particles, solver %s, %s" % (a.workload, n, a.solver, a.precision),
           file=sys.stderr, flush=True)

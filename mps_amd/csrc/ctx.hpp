@@ -35,6 +35,7 @@
 void mps_set_error(int code, const char *file, int line, const char *msg);
 void mps_clear_error(void);
 int particle_block();                       // block size of the per-particle kernels (runtime.hip)
+int search_form();                          // 1: candidates staged in LDS, 0: gathered from HBM (runtime.hip)
 
 // Step constants computed on the host in the reference's association order
 // (host and device FP64 arithmetic are both IEEE, so precomputed products

@@ -956,17 +956,88 @@ struct SearchOut {
     DevCounters *ctr;
 };
 
+// Running state of one particle's search.
+struct SearchAcc {
+    int cnt, cnt_n;
+    double nsum, sumw;
+};
+
+// Candidate j of particle i (neighbor_search.c:211-233): listed when
+// r^2 < re^2 (strict), with the number density, the diagonal's weight sum
+// and the count-based surface test folded in as the pair is listed.
+__device__ __forceinline__ void search_pair(int i, int j, const dvec &xi, const dvec &xj, int tj, SearchAcc &a,
+                                            const SearchOut &out, const StepConsts &k, int with_density)
+{
+    if (j == i) return;
+    double dr[D_];
+    const double r2 = dist2(xi, xj, dr);
+    if (!(r2 < k.re2)) return;
+    if (a.cnt < k.stride) out.nbr[sell_off(i, a.cnt, k.stride)] = j;
+    a.cnt++;
+    if (with_density) {
+        const bool need_n = r2 <= k.re_n2_hi;
+        const bool need_l = tj != DUMMY_PARTICLE;
+        if (need_n || need_l) {
+            const double r = sqrt(r2);
+            if (need_n) a.nsum += wfun(r, k.re_n, k.r_min);
+            if (need_l) a.sumw += wfun(r, k.re_lap, k.r_min);
+        }
+        a.cnt_n += (r2 < k.re_n2);
+    }
+}
+
+// The per-particle results of the search and the block's counters.
+template <int B>
+__device__ __forceinline__ void search_finish(int i, int n, int t, const int *__restrict__ O, const SearchAcc &a,
+                                              const SearchOut &out, const StepConsts &k, int with_density)
+{
+    int u = 0;
+    if (i < n) {
+        const int cnt = a.cnt;
+        out.ncnt[i] = cnt < k.stride ? cnt : k.stride;
+        if (cnt > k.stride) atomicMin(&out.ctr->overflow_orig, O[i]);
+        if (with_density) {
+            int s;
+            if (k.surf_method == 1) {
+                out.ncount[i] = a.cnt_n;
+                s = (t == FLUID_PARTICLE) ? ((double)a.cnt_n < k.beta_n0c) : 0;
+            } else {
+                s = (t == FLUID_PARTICLE) ? (a.nsum < k.thr_n0) : 0;
+            }
+            u = (t == FLUID_PARTICLE && !s) || t == WALL_PARTICLE;
+            out.nd[i] = a.nsum;
+            out.surf[i] = s;
+            out.unk[i] = u;
+            out.diag[i] = u ? k.diag_coeff * a.sumw : 0.0;
+            double bi = 0.0;
+            if (u && t == FLUID_PARTICLE && k.hs_mode == 0 && k.ppe_type == 0)
+                bi = k.src_k * (a.nsum - k.n0) / k.n0;
+            out.b[i] = bi;
+        }
+    }
+    // counters: unknowns, directed pairs, largest list
+    const long long pairs = block_sum<long long, B>((long long)(i < n ? a.cnt : 0));
+    const long long nu = block_sum<long long, B>((long long)u);
+    if (threadIdx.x == 0) {
+        atomicAdd(&out.ctr->n_pairs, (unsigned long long)pairs);
+        atomicAdd(&out.ctr->n_eq, (int)nu);
+    }
+    if (i < n && a.cnt > 0) atomicMax(&out.ctr->max_count, a.cnt);
+}
+
+// One thread per slot; candidate positions gathered from HBM/L2 (option
+// "search" 0).
 template <int B>
 __global__ void __launch_bounds__(B)
 k_search(int n, const dvec *__restrict__ X, const int *__restrict__ T, const int *__restrict__ O,
          const int *__restrict__ cstart, SearchOut out, StepConsts k, int with_density)
 {
     const int i = blockIdx.x * B + threadIdx.x;
-    int cnt = 0, cnt_n = 0, u = 0;
+    SearchAcc a{0, 0, 0.0, 0.0};
+    int t = GHOST_PARTICLE;
     if (i < n) {
-        const int t = T[i];
+        t = T[i];
         const dvec xi = X[i];
-        double nsum = 0.0, sumw = 0.0;
         int cc[3];
         if (t != GHOST_PARTICLE && cell_coords(k.grid, xi, cc) >= 0) {
             int lo[3], hi[3];
@@ -984,63 +1055,158 @@ k_search(int n, const dvec *__restrict__ X, const int *__restrict__ T, const int
                         // loaded together, tested and listed in order
                         for (int j0 = cstart[cid]; j0 < j1; j0 += NB_U) {
                             dvec xx[NB_U];
+                            int tt[NB_U];
 #pragma unroll
                             for (int u = 0; u < NB_U; u++)
-                                if (j0 + u < j1) xx[u] = X[j0 + u];
+                                if (j0 + u < j1) {
+                                    xx[u] = X[j0 + u];
+                                    tt[u] = with_density ? T[j0 + u] : 0;
+                                }
 #pragma unroll
                             for (int u = 0; u < NB_U; u++) {
-                                const int j = j0 + u;
-                                if (j >= j1) break;
-                                if (j == i) continue;
-                                double dr[D_];
-                                const double r2 = dist2(xi, xx[u], dr);
-                                if (!(r2 < k.re2)) continue;
-                                if (cnt < k.stride) out.nbr[sell_off(i, cnt, k.stride)] = j;
-                                cnt++;
-                                if (with_density) {
-                                    const int tj = T[j];
-                                    const bool need_n = r2 <= k.re_n2_hi;
-                                    const bool need_l = tj != DUMMY_PARTICLE;
-                                    if (need_n || need_l) {
-                                        const double r = sqrt(r2);
-                                        if (need_n) nsum += wfun(r, k.re_n, k.r_min);
-                                        if (need_l) sumw += wfun(r, k.re_lap, k.r_min);
-                                    }
-                                    cnt_n += (r2 < k.re_n2);
-                                }
+                                if (j0 + u >= j1) break;
+                                search_pair(i, j0 + u, xi, xx[u], tt[u], a, out, k, with_density);
                             }
                         }
                     }
         }
-        out.ncnt[i] = cnt < k.stride ? cnt : k.stride;
-        if (cnt > k.stride) atomicMin(&out.ctr->overflow_orig, O[i]);
-        if (with_density) {
-            int s;
-            if (k.surf_method == 1) {
-                out.ncount[i] = cnt_n;
-                s = (t == FLUID_PARTICLE) ? ((double)cnt_n < k.beta_n0c) : 0;
-            } else {
-                s = (t == FLUID_PARTICLE) ? (nsum < k.thr_n0) : 0;
+    }
+    search_finish<B>(i, n, t, O, a, out, k, with_density);
+}
+
+// The same search with the candidates staged in LDS (option "search" 1,
+// the default).  Slots are in cell order with x the fastest cell axis, so
+// the three cells x-1..x+1 of one (y, z) row of a particle's block are one
+// contiguous slot range, and the particle's 27 cells (9 in 2-D) are 9 (3)
+// such ranges, visited z -> y in the reference's order.  The workgroup
+// walks the 9 row offsets in that order; for each it stages the union of
+// its particles' ranges (consecutive slots, so a few cells wider than the
+// block) in chunks of SRCH_CAP positions and types, and every particle
+// scans its own range from LDS in ascending slot order, SRCH_U candidates
+// per batch of reads.  The lists and every sum are those of k_search, bit
+// for bit.  A union wider than SRCH_SPAN (a block that straddles a y layer
+// of cells) is read from HBM directly for that row offset.
+#define SRCH_CAP 512
+#define SRCH_SPAN (4 * SRCH_CAP)
+#define SRCH_U 4
+
+__device__ __forceinline__ int wave_min_int(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, WAVE));
+    return v;
+}
+__device__ __forceinline__ int wave_max_int(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, WAVE));
+    return v;
+}
+
+template <int B>
+__global__ void __launch_bounds__(B)
+k_search_lds(int n, const dvec *__restrict__ X, const int *__restrict__ T, const int *__restrict__ O,
+             const int *__restrict__ cstart, SearchOut out, StepConsts k, int with_density)
+{
+    __shared__ dvec sx[SRCH_CAP + SRCH_U];
+    __shared__ int stp[SRCH_CAP + SRCH_U];
+    __shared__ int smin[B / WAVE], smax[B / WAVE];
+    const int i = blockIdx.x * B + threadIdx.x;
+    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    SearchAcc a{0, 0, 0.0, 0.0};
+    int t = GHOST_PARTICLE;
+    dvec xi{};
+    bool act = false;
+    int cc[3] = {0, 0, 0}, lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    if (i < n) {
+        t = T[i];
+        xi = X[i];
+        act = t != GHOST_PARTICLE && cell_coords(k.grid, xi, cc) >= 0;
+        if (act) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                lo[d] = cc[d] > 0 ? cc[d] - 1 : 0;
+                hi[d] = cc[d] < k.grid.nc[d] - 1 ? cc[d] + 1 : k.grid.nc[d] - 1;
             }
-            u = (t == FLUID_PARTICLE && !s) || t == WALL_PARTICLE;
-            out.nd[i] = nsum;
-            out.surf[i] = s;
-            out.unk[i] = u;
-            out.diag[i] = u ? k.diag_coeff * sumw : 0.0;
-            double bi = 0.0;
-            if (u && t == FLUID_PARTICLE && k.hs_mode == 0 && k.ppe_type == 0)
-                bi = k.src_k * (nsum - k.n0) / k.n0;
-            out.b[i] = bi;
         }
     }
-    // counters: unknowns, directed pairs, largest list
-    const long long pairs = block_sum<long long, B>((long long)cnt);
-    const long long nu = block_sum<long long, B>((long long)u);
-    if (threadIdx.x == 0) {
-        atomicAdd(&out.ctr->n_pairs, (unsigned long long)pairs);
-        atomicAdd(&out.ctr->n_eq, (int)nu);
+    const int npz = D_ == 3 ? 3 : 1;
+    for (int pz = 0; pz < npz; pz++) {
+        for (int py = 0; py < 3; py++) {
+            // this particle's slot range for row offset (pz, py)
+            int ra = 0, rb = 0;
+            if (act) {
+                const int cz = D_ == 3 ? cc[2] + pz - 1 : 0;
+                const int cy = cc[1] + py - 1;
+                if (cz >= lo[2] && cz <= hi[2] && cy >= lo[1] && cy <= hi[1]) {
+                    ra = cstart[cell_key(k.grid, lo[0], cy, cz)];
+                    rb = cstart[cell_key(k.grid, hi[0], cy, cz) + 1];
+                }
+            }
+            // the block's union of ranges
+            const int wmin = wave_min_int(ra < rb ? ra : INT_MAX);
+            const int wmax = wave_max_int(ra < rb ? rb : INT_MIN);
+            __syncthreads();   // the previous row offset's readers are done
+            if (lane == 0) {
+                smin[w] = wmin;
+                smax[w] = wmax;
+            }
+            __syncthreads();
+            int bmin = INT_MAX, bmax = INT_MIN;
+#pragma unroll
+            for (int q = 0; q < B / WAVE; q++) {
+                bmin = min(bmin, smin[q]);
+                bmax = max(bmax, smax[q]);
+            }
+            if (bmin >= bmax) continue;   // uniform over the block
+            if (bmax - bmin > SRCH_SPAN) {
+                for (int j0 = ra; j0 < rb; j0 += NB_U) {
+                    dvec xx[NB_U];
+                    int tt[NB_U];
+#pragma unroll
+                    for (int u = 0; u < NB_U; u++)
+                        if (j0 + u < rb) {
+                            xx[u] = X[j0 + u];
+                            tt[u] = with_density ? T[j0 + u] : 0;
+                        }
+#pragma unroll
+                    for (int u = 0; u < NB_U; u++) {
+                        if (j0 + u >= rb) break;
+                        search_pair(i, j0 + u, xi, xx[u], tt[u], a, out, k, with_density);
+                    }
+                }
+                continue;
+            }
+            for (int s0 = bmin; s0 < bmax; s0 += SRCH_CAP) {
+                const int s1 = min(s0 + SRCH_CAP, bmax);
+                __syncthreads();
+                for (int q = threadIdx.x; q < s1 - s0; q += B) {
+                    sx[q] = X[s0 + q];
+                    if (with_density) stp[q] = T[s0 + q];
+                }
+                __syncthreads();
+                // reads of a batch may run up to SRCH_U - 1 past the
+                // staged entries (into the pad); those candidates are not
+                // tested
+                const int j1 = min(rb, s1);
+                for (int j0 = max(ra, s0); j0 < j1; j0 += SRCH_U) {
+                    dvec xx[SRCH_U];
+                    int tt[SRCH_U];
+#pragma unroll
+                    for (int u = 0; u < SRCH_U; u++) {
+                        xx[u] = sx[j0 + u - s0];
+                        tt[u] = with_density ? stp[j0 + u - s0] : 0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < SRCH_U; u++) {
+                        if (j0 + u >= j1) break;
+                        search_pair(i, j0 + u, xi, xx[u], tt[u], a, out, k, with_density);
+                    }
+                }
+            }
+        }
     }
-    if (i < n && cnt > 0) atomicMax(&out.ctr->max_count, cnt);
+    search_finish<B>(i, n, t, O, a, out, k, with_density);
 }
 
 int launch_search(Ctx *c, const StepConsts &k, int with_density, DevCounters *ctr)
@@ -1049,8 +1215,12 @@ int launch_search(Ctx *c, const StepConsts &k, int with_density, DevCounters *ct
     o.nbr = c->nbr; o.ncnt = c->ncnt; o.surf = c->SURF[c->ct]; o.unk = c->unk;
     o.ncount = c->NCOUNT[c->ct]; o.nd = c->ND[c->ct]; o.diag = c->diag; o.b = c->b; o.ctr = ctr;
     prof_begin(PK_SEARCH, c->stream);
-    PARTICLE_LAUNCH(k_search, c, c->n, c->X[c->cx],
-                       c->T[c->ct], c->O[c->ct], c->cstart, o, k, with_density);
+    if (search_form() == 1)
+        PARTICLE_LAUNCH(k_search_lds, c, c->n, c->X[c->cx], c->T[c->ct], c->O[c->ct], c->cstart, o, k,
+                        with_density);
+    else
+        PARTICLE_LAUNCH(k_search, c, c->n, c->X[c->cx], c->T[c->ct], c->O[c->ct], c->cstart, o, k,
+                        with_density);
     prof_end(PK_SEARCH, c->stream);
     LAUNCH_CHECK();
     return 0;

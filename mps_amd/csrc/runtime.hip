@@ -45,6 +45,7 @@ struct Options {
     double tol_mode = 0, precision = 0, inner_tol = 1e-3, exit_on_error = 1, check_every = 16;
     double spmv = 0, profile = 0, verbose = 0, fused = 2, graph = 0, pblk = 256;
     double async_output = 1;
+    double search = 1;                 // neighbour search: 1 candidates staged in LDS, 0 gathered from HBM
     double ic_wait_limit = 1048576;   // unsuccessful polls before an IC(0) wait gives up
     double ic_sleep = 3;               // longest backoff of an idle IC(0) wave (0-3)
     double ic_bucket = -1;             // log2 of the levels per bucket of the IC(0) solve order (-1: automatic)
@@ -122,6 +123,7 @@ static void start_snapshot(const ParticleSystem &snap, int step, const char *dir
 }
 
 int particle_block() { return (int)g_opt.pblk; }
+int search_form() { return (int)g_opt.search; }
 
 // Kernel timing: HIP events recorded on the launching stream around each
 // launch of a kernel class, drawn from a pool so recording costs no
@@ -893,6 +895,12 @@ int mps_hip_set_option(const char *key, double value)
         g_opt.ic_waves = value;
     }
     else if (k == "ic_fgrid") g_opt.ic_fgrid = value;
+    else if (k == "search") {
+        if (value != 0 && value != 1) return MPS_ERR_ARG;
+        if (value != g_opt.search)
+            for (auto &kv : g_ctx) graph_reset(kv.second);   // captures hold the kernel
+        g_opt.search = value;
+    }
     else if (k == "ic_wait_limit") {
         if (!(value >= 1.0)) return MPS_ERR_ARG;
         g_opt.ic_wait_limit = value;
@@ -930,6 +938,7 @@ double mps_hip_get_option(const char *key)
     if (k == "ic_waves") return g_opt.ic_waves;
     if (k == "ic_factor_form") return g_opt.ic_factor_form;
     if (k == "ic_fgrid") return g_opt.ic_fgrid;
+    if (k == "search") return g_opt.search;
     return NAN;
 }
 

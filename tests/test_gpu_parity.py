@@ -349,6 +349,41 @@ def test_particle_block_sizes_same_bits(lib3, graph):
             assert a == b, f
 
 
+@pytest.mark.parametrize("name,libname", [("dam3d_mid", "lib3"), ("dam2d_l0.01", "lib2")])
+def test_search_forms_same_bits(request, name, libname):
+    """The neighbour search with candidates staged in LDS (option "search"
+    1, the default) and gathered from HBM (0) give the same lists, in
+    order, and the same bits in every field over three steps, with plain
+    launches and with captured graphs; other values are refused."""
+    lib = request.getfixturevalue(libname)
+    dim, cfg, pos, vel, typ = H.build(name, cg_tolerance=1e-10)
+    lib.set_option("tol_mode", 1)
+    out = []
+    try:
+        assert lib.mps_hip_set_option(b"search", 2.0) != 0
+        for graph in (0, 1):
+            lib.set_option("graph", graph)
+            for form in (1, 0):
+                lib.set_option("search", form)
+                say(name, "search form", form, "graph", graph)
+                g = H.GpuRun(lib, cfg, pos, vel, typ)
+                lists0 = g.lists()
+                lib.check(lib.mps_hip_set_resident(g.scene.ptr(), 1), "resident")
+                lib.check(lib.mps_hip_run_steps(g.scene.ptr(), 3), "run_steps")
+                lib.check(lib.mps_hip_download(g.scene.ptr(), g.nl), "download")
+                out.append((lists0, g.lists(),
+                            [np.ascontiguousarray(g.scene.arr[f]).tobytes() for f in H.FIELDS]))
+                g.close()
+    finally:
+        lib.set_option("search", 1)
+        lib.set_option("graph", 0)
+    for o in out[1:]:
+        assert H.lists_equal(out[0][0], o[0]), "initial lists"
+        assert H.lists_equal(out[0][1], o[1]), "lists after three steps"
+        for f, a, b in zip(H.FIELDS, out[0][2], o[2]):
+            assert a == b, f
+
+
 @pytest.mark.parametrize("precision", [0, 1, 2, 3])
 def test_batch_equals_single_runs(lib3, precision):
     """mps_hip_run_batch (particle operators per scene, one batched CG for
