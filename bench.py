@@ -43,6 +43,8 @@ WORKLOADS = {
     "dam3d_l0.01": dict(dims=(1.0, 0.6, 0.5, 0.3, 0.5, 0.5), l0=0.01, dt=5e-4),
     # 64 seeded dam breaks of uneven size and viscosity (mps_amd.params.ensemble)
     "ensemble64": dict(ensemble=64),
+    # the largest scene of that ensemble alone (column 0.4 x 0.4 x 0.4 m)
+    "ensemble_scene": dict(dims=(0.8, 0.5, 0.4, 0.4, 0.4, 0.4), l0=0.01, dt=1e-3),
     # gradient of a 200-step 2-D dam break (tank 1.0 x 0.6, column 0.3 x 0.6)
     "adjoint2d": dict(adjoint=True, dims=(1.0, 0.6, 0.0, 0.3, 0.6, 0.0), l0=0.005, dt=5e-4, steps=200,
                       checkpoint=20, nu=1e-6, nu_target=5e-5, sigma_u0=0.01, seed=4),

@@ -179,6 +179,64 @@ struct Ctx {
     } sg[2];
 };
 
+// ---- batches of scenes stepped by one launch per operator ---------------
+// One scene of such a batch: its device buffers (both halves of each
+// ping-pong pair), step constants and sizes.  The scenes of a batch flip
+// their buffers together, so the one parity (cx, ct, cp) passed to each
+// launch holds for all of them.  A launch covers the concatenated block
+// ranges of every scene; first[s] is the first block of scene s (first[ns]
+// the total) and a block finds its scene by bisection (flat_scene).
+struct FlatScene {
+    int n, total;                    // particles, grid cells
+    dvec *X[2], *V[2], *ACC[2];
+    int *T[2], *O[2], *SURF[2], *NCOUNT[2];
+    double *P[2], *ND[2];
+    int *unk;
+    double *pmin;
+    int *cell, *ccount, *cstart, *cfill, *tmp_key, *sflag, *sscan;
+    int *nbr, *ncnt;
+    double *diag, *b, *Aval;
+    int *Acol, *Alen;
+    double *x;
+    DevCounters *ctr;
+    StepConsts k;
+};
+
+// Segmented exclusive scan: segment g scans in[0..n) into out[0..n]
+// (out[n] = its total, also written to *tot when tot is set).
+struct ScanSeg {
+    const int *in;
+    int *out;
+    int n;
+    int *tot;
+};
+// A set of segments on the device: segs[0..nseg), tile0[0..nseg] (the
+// segments' first 2048-element tiles; tile0[nseg] = all tiles).
+struct ScanSegs {
+    ScanSeg *segs = nullptr;
+    int *tile0 = nullptr;
+    int nseg = 0, ntiles = 0;
+};
+
+// Block offsets of one launch shape over the scenes of a batch (device).
+struct FlatGrid {
+    int *first = nullptr;
+    int blocks = 0, cap = 0;
+};
+
+struct FlatLaunch {
+    const FlatScene *tab;            // device
+    int ns;
+    FlatGrid gp;                     // per-particle kernels at the block size of option "pblk"
+    FlatGrid gb;                     // 256-thread kernels over the particles
+    FlatGrid gc;                     // 256-thread kernels over the cells (total + 1)
+    ScanSegs sc_cells, sc_sflag;     // the counting sort's two scans
+    int *scan_blk;                   // tile sums of the segmented scans
+    int any_cmps1;                   // some scene takes p_min from its neighbours' minimum
+    int cx, ct, cp;
+    hipStream_t s;
+};
+
 // ---- launchers (k_particles.hip) -------------------------------------
 int  ctx_alloc(Ctx *c, int n, int stride, int grid_cells);
 void ctx_free(Ctx *c);
@@ -209,6 +267,16 @@ int  launch_density_only(Ctx *c, const StepConsts &k);
 // neighbour list <-> compact CSR over original indices (host interchange)
 int  launch_list_export(Ctx *c, long long *total_out);
 int  launch_list_import(Ctx *c, const int *counts_dev, const int *offsets_dev);
+
+// one launch per operator over every scene of a batch (k_particles.hip)
+// host block counts -> device offsets (copied on stream s, which is then
+// synchronised)
+int  flat_build_grid(FlatGrid *g, const int *blocks, int ns, hipStream_t s);
+int  launch_scan_seg(const ScanSegs &ss, int *blk, hipStream_t s);
+int  scan_segs_set(ScanSegs *ss, const ScanSeg *h, int nseg, hipStream_t s);   // host table -> device
+int  flat_step_pre(FlatLaunch &f);      // predictor .. assembly (3-D)
+int  flat_step_post(FlatLaunch &f);     // pressure, gradient, corrector, boundary
+int  flat_counters(FlatLaunch &f, DevCounters *out_dev);        // every scene's counters, in scene order
 
 // ---- solvers (k_cg.hip, k_ic.hip) ------------------------------------
 struct SolveOpts {

@@ -112,6 +112,20 @@ __device__ __forceinline__ int xcd_block(int b, int nb)
     return x * q + (x < rem ? x : rem) + k;
 }
 
+// The scene of block b of a launch over several scenes' block ranges:
+// the s with first[s] <= b < first[s + 1] (first[0] = 0, first[ns] = the
+// launch's blocks, b below it).  Scenes without blocks are passed over.
+__device__ __forceinline__ int flat_scene(const int *__restrict__ first, int ns, int b)
+{
+    int lo = 0, hi = ns;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (first[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // ---------------------------------------------------------------------
 // wavefront / block primitives (wave64)
 // ---------------------------------------------------------------------

@@ -24,6 +24,8 @@
 // at most PK_HALO_CAP = 4096 rows), eight to three 32-bit words.
 #include <limits.h>
 
+#include <vector>
+
 #include "ctx.hpp"
 
 #define PKB 256
@@ -42,11 +44,18 @@ static inline unsigned pk_blocks(size_t n) { return (unsigned)((n + PKB - 1) / P
         }                                                                    \
     } while (0)
 
-__global__ void k_pk_slots(int n, const int *__restrict__ unk, const int *__restrict__ row_of_slot,
+__device__ __forceinline__ void pk_slots_body(int lb, int n, const int *__restrict__ unk, const int *__restrict__ row_of_slot,
                            int *__restrict__ slot_of_row)
 {
-    const int s = blockIdx.x * PKB + threadIdx.x;
+    const int s = lb * PKB + threadIdx.x;
     if (s < n && unk[s]) slot_of_row[row_of_slot[s]] = s;
+}
+
+__global__ void __launch_bounds__(PKB)
+k_pk_slots(int n, const int *__restrict__ unk, const int *__restrict__ row_of_slot,
+                           int *__restrict__ slot_of_row)
+{
+    pk_slots_body(blockIdx.x, n, unk, row_of_slot, slot_of_row);
 }
 
 __device__ __forceinline__ int wave_max_i(int v)
@@ -82,12 +91,12 @@ struct BlockRows {
     int k, slot, m, bmn, bmx;
 };
 
-__device__ __forceinline__ BlockRows block_rows(int n_eq, int stride, const int *__restrict__ slot_of_row,
+__device__ __forceinline__ BlockRows block_rows(int lb, int n_eq, int stride, const int *__restrict__ slot_of_row,
                                                 const int *__restrict__ row_of_slot, const int *__restrict__ Alen,
                                                 const int *__restrict__ Acol, int *red)
 {
     BlockRows br;
-    br.k = blockIdx.x * PKB + threadIdx.x;
+    br.k = lb * PKB + threadIdx.x;
     br.slot = -1;
     br.m = 0;
     int mn = INT_MAX, mx = -1;
@@ -147,15 +156,14 @@ __device__ __forceinline__ void block_bitmap(const BlockRows &br, int stride, co
 }
 
 // Pass 1, one workgroup per block: slice widths, halo size and mode.
-__global__ void __launch_bounds__(PKB)
-k_pk_pass1(int n_eq, int n_slices, int stride, const int *__restrict__ slot_of_row,
+__device__ __forceinline__ void pk_pass1_body(int lb, int n_eq, int n_slices, int stride, const int *__restrict__ slot_of_row,
            const int *__restrict__ row_of_slot, const int *__restrict__ Alen, const int *__restrict__ Acol,
            int *__restrict__ sw, int *__restrict__ whw, int *__restrict__ w32, int *__restrict__ bh,
            int *__restrict__ bmode, int *dmax)
 {
     __shared__ unsigned bits[BM_WORDS];
     __shared__ int red[2 * PKB / WAVE];
-    const BlockRows br = block_rows(n_eq, stride, slot_of_row, row_of_slot, Alen, Acol, red);
+    const BlockRows br = block_rows(lb, n_eq, stride, slot_of_row, row_of_slot, Alen, Acol, red);
     const bool span_ok = br.bmx < 0 || (br.bmx - br.bmn) < BM_WORDS * 32;
     int H = 0;
     if (span_ok && br.bmx >= 0) {           // uniform over the block
@@ -169,23 +177,31 @@ k_pk_pass1(int n_eq, int n_slices, int stride, const int *__restrict__ slot_of_r
     }
     const int mode = span_ok && H <= PK_HALO_CAP;
     const int wmax = wave_max_i(br.m);
-    const int sl = blockIdx.x * (PKB / WAVE) + threadIdx.x / WAVE;
+    const int sl = lb * (PKB / WAVE) + threadIdx.x / WAVE;
     if (sl < n_slices && (threadIdx.x & (WAVE - 1)) == 0) {
         sw[sl] = wmax;
         whw[sl] = mode ? (wmax + 7) / 8 * 3 : 0;     // three words per 8 entries
         w32[sl] = mode ? 0 : wmax;
     }
     if (threadIdx.x == 0) {
-        bh[blockIdx.x] = mode ? H : 0;
-        bmode[blockIdx.x] = mode;
+        bh[lb] = mode ? H : 0;
+        bmode[lb] = mode;
         if (mode) atomicMax(dmax, H);
     }
 }
 
+__global__ void __launch_bounds__(PKB)
+k_pk_pass1(int n_eq, int n_slices, int stride, const int *__restrict__ slot_of_row,
+           const int *__restrict__ row_of_slot, const int *__restrict__ Alen, const int *__restrict__ Acol,
+           int *__restrict__ sw, int *__restrict__ whw, int *__restrict__ w32, int *__restrict__ bh,
+           int *__restrict__ bmode, int *dmax)
+{
+    pk_pass1_body(blockIdx.x, n_eq, n_slices, stride, slot_of_row, row_of_slot, Alen, Acol, sw, whw, w32, bh, bmode, dmax);
+}
+
 // Pass 2, one workgroup per block: halo list, values, columns, slice and
 // block descriptors.
-__global__ void __launch_bounds__(PKB)
-k_pk_pass2(int n_eq, int n_slices, int stride, const int *__restrict__ slot_of_row,
+__device__ __forceinline__ void pk_pass2_body(int lb, int n_eq, int n_slices, int stride, const int *__restrict__ slot_of_row,
            const int *__restrict__ row_of_slot, const int *__restrict__ Alen, const double *__restrict__ Aval,
            const int *__restrict__ Acol, const double *__restrict__ diag, const double *__restrict__ b,
            const int *__restrict__ bmode, const int *__restrict__ bh, const int *__restrict__ hoff,
@@ -197,8 +213,8 @@ k_pk_pass2(int n_eq, int n_slices, int stride, const int *__restrict__ slot_of_r
     __shared__ unsigned bits[BM_WORDS];
     __shared__ int pre[BM_WORDS];
     __shared__ int red[2 * PKB / WAVE];
-    const BlockRows br = block_rows(n_eq, stride, slot_of_row, row_of_slot, Alen, Acol, red);
-    const int mode = bmode[blockIdx.x];
+    const BlockRows br = block_rows(lb, n_eq, stride, slot_of_row, row_of_slot, Alen, Acol, red);
+    const int mode = bmode[lb];
     if (mode && br.bmx >= 0) {              // uniform over the block
         block_bitmap(br, stride, row_of_slot, Acol, bits);
         const int w0 = threadIdx.x * (BM_WORDS / PKB);
@@ -216,7 +232,7 @@ k_pk_pass2(int n_eq, int n_slices, int stride, const int *__restrict__ slot_of_r
             pre[w0 + j] = run;
             run += cnt[j];
         }
-        const int ho = hoff[blockIdx.x];
+        const int ho = hoff[lb];
 #pragma unroll
         for (int j = 0; j < BM_WORDS / PKB; j++) {
             unsigned word = bits[w0 + j];
@@ -230,10 +246,10 @@ k_pk_pass2(int n_eq, int n_slices, int stride, const int *__restrict__ slot_of_r
         __syncthreads();
     }
     const int lane = threadIdx.x & (WAVE - 1);
-    const int sl = blockIdx.x * (PKB / WAVE) + threadIdx.x / WAVE;
+    const int sl = lb * (PKB / WAVE) + threadIdx.x / WAVE;
     if (sl < n_slices && lane == 0)
         smeta[sl] = make_int4(voff[sl], mode ? chwoff[sl] : c32off[sl], mode, 0);
-    if (threadIdx.x == 0) bmeta[blockIdx.x] = make_int2(mode ? hoff[blockIdx.x] : 0, bh[blockIdx.x]);
+    if (threadIdx.x == 0) bmeta[lb] = make_int2(mode ? hoff[lb] : 0, bh[lb]);
     if (br.slot < 0) return;
     const int vo = voff[sl];
     const int co = mode ? chwoff[sl] : c32off[sl];
@@ -289,13 +305,26 @@ k_pk_pass2(int n_eq, int n_slices, int stride, const int *__restrict__ slot_of_r
     pb[br.k] = b[br.slot];
 }
 
+__global__ void __launch_bounds__(PKB)
+k_pk_pass2(int n_eq, int n_slices, int stride, const int *__restrict__ slot_of_row,
+           const int *__restrict__ row_of_slot, const int *__restrict__ Alen, const double *__restrict__ Aval,
+           const int *__restrict__ Acol, const double *__restrict__ diag, const double *__restrict__ b,
+           const int *__restrict__ bmode, const int *__restrict__ bh, const int *__restrict__ hoff,
+           const int *__restrict__ voff, const int *__restrict__ chwoff, const int *__restrict__ c32off,
+           int *__restrict__ halo, double *__restrict__ val, unsigned *__restrict__ chw,
+           int *__restrict__ c32, int *__restrict__ len, double *__restrict__ pdiag, double *__restrict__ pb,
+           int4 *__restrict__ smeta, int2 *__restrict__ bmeta)
+{
+    pk_pass2_body(blockIdx.x, n_eq, n_slices, stride, slot_of_row, row_of_slot, Alen, Aval, Acol, diag, b, bmode, bh, hoff, voff, chwoff, c32off, halo, val, chw, c32, len, pdiag, pb, smeta, bmeta);
+}
+
 // FP32 copy of the off-diagonals and each row's Dirichlet part
 // d_k = A_kk + sum_j A_kj in FP64 (see the mixed solver in k_cg.hip).
-__global__ void k_pk_f32(int n_eq, const int *__restrict__ len, const int4 *__restrict__ smeta,
+__device__ __forceinline__ void pk_f32_body(int lb, int n_eq, const int *__restrict__ len, const int4 *__restrict__ smeta,
                          const double *__restrict__ val, const double *__restrict__ pdiag,
                          float *__restrict__ fval, double *__restrict__ ddir)
 {
-    const int k = blockIdx.x * PKB + threadIdx.x;
+    const int k = lb * PKB + threadIdx.x;
     if (k >= n_eq) return;
     const int m = len[k];
     const size_t base = (size_t)smeta[k >> 6].x * 64u + (size_t)(k & 63);
@@ -314,12 +343,27 @@ __global__ void k_pk_f32(int n_eq, const int *__restrict__ len, const int4 *__re
     ddir[k] = d;
 }
 
-__global__ void k_pk_unpack_x(int n, const int *__restrict__ unk, const int *__restrict__ row_of_slot,
+__global__ void __launch_bounds__(PKB)
+k_pk_f32(int n_eq, const int *__restrict__ len, const int4 *__restrict__ smeta,
+                         const double *__restrict__ val, const double *__restrict__ pdiag,
+                         float *__restrict__ fval, double *__restrict__ ddir)
+{
+    pk_f32_body(blockIdx.x, n_eq, len, smeta, val, pdiag, fval, ddir);
+}
+
+__device__ __forceinline__ void pk_unpack_x_body(int lb, int n, const int *__restrict__ unk, const int *__restrict__ row_of_slot,
                               const double *__restrict__ xp, double *__restrict__ x)
 {
-    const int s = blockIdx.x * PKB + threadIdx.x;
+    const int s = lb * PKB + threadIdx.x;
     if (s >= n) return;
     x[s] = unk[s] ? xp[row_of_slot[s]] : 0.0;
+}
+
+__global__ void __launch_bounds__(PKB)
+k_pk_unpack_x(int n, const int *__restrict__ unk, const int *__restrict__ row_of_slot,
+                              const double *__restrict__ xp, double *__restrict__ x)
+{
+    pk_unpack_x_body(blockIdx.x, n, unk, row_of_slot, xp, x);
 }
 
 template <typename T>
@@ -501,6 +545,255 @@ int unpack_solution(Ctx *c)
     if (!p || p->n_eq == 0) return launch_fill_double(c->x, 0.0, (size_t)n, c->stream);
     hipLaunchKernelGGL(k_pk_unpack_x, dim3(pk_blocks(n)), dim3(PKB), 0, c->stream, n, c->unk, p->row_of_slot,
                        p->x, c->x);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+// ---------------------------------------------------------------------
+// Packing for a batch of scenes: each stage as one launch over the
+// scenes' block ranges (block -> scene by bisection), the scans as one
+// segmented scan, and the sizes the host needs (unknowns, then slice,
+// halo and block totals) brought back for all scenes in one copy each.
+// Every scene's packed system is the one pack_ppe builds alone.
+// ---------------------------------------------------------------------
+
+struct FlatPack {
+    int n, n_eq, n_slices, stride;
+    const int *unk, *Alen, *Acol;
+    const double *Aval, *diag, *b;
+    int *row_of_slot, *slot_of_row;
+    int *sw, *whw, *w32, *bh, *bmode, *dmax, *voff, *chwoff, *c32off, *hoff, *halo;
+    double *val;
+    unsigned *chw;
+    int *c32, *len;
+    double *pdiag, *pb;
+    int4 *smeta;
+    int2 *bmeta;
+    float *fval;
+    double *ddir, *px, *x;
+};
+
+struct FlatPackBufs {
+    std::vector<FlatPack> h;
+    FlatPack *tab = nullptr;
+    int cap = 0;
+    ScanSegs sc_unk, sc_set;
+    std::vector<ScanSeg> hseg;
+    FlatGrid g_rows;
+    int cap_rows_first = 0;
+    int *tot = nullptr, *htot = nullptr;   // 8 ints per scene: [0] unknowns / largest halo, [1..4] scan totals
+    int *blk = nullptr;
+    size_t cap_blk = 0;
+};
+static FlatPackBufs g_fpk;
+
+void flat_pack_free()
+{
+    FlatPackBufs &F = g_fpk;
+    void *d[] = {F.tab, F.sc_unk.segs, F.sc_unk.tile0, F.sc_set.segs, F.sc_set.tile0, F.g_rows.first, F.tot, F.blk};
+    for (void *p : d)
+        if (p) hipFree(p);
+    if (F.htot) hipHostFree(F.htot);
+    F = FlatPackBufs();
+}
+
+static int fp_reserve(int ns)
+{
+    FlatPackBufs &F = g_fpk;
+    if (F.cap >= ns) return 0;
+    if (F.tab) hipFree(F.tab);
+    if (F.tot) hipFree(F.tot);
+    if (F.htot) hipHostFree(F.htot);
+    F.tab = nullptr;
+    F.tot = F.htot = nullptr;
+    HIP_OK(hipMalloc(&F.tab, sizeof(FlatPack) * ns));
+    HIP_OK(hipMalloc(&F.tot, sizeof(int) * 8 * ns));
+    HIP_OK(hipHostMalloc(&F.htot, sizeof(int) * 8 * ns, hipHostMallocDefault));
+    F.cap = ns;
+    return 0;
+}
+
+static int fp_blk(size_t need)
+{
+    FlatPackBufs &F = g_fpk;
+    if (F.blk && F.cap_blk >= need) return 0;
+    if (F.blk) hipFree(F.blk);
+    F.blk = nullptr;
+    HIP_OK(hipMalloc(&F.blk, sizeof(int) * (need + 64)));
+    F.cap_blk = need + 64;
+    return 0;
+}
+
+// The table of the scenes' packed buffers, uploaded (stream-ordered).
+static int fp_table(Ctx *const *cs, int ns, hipStream_t s)
+{
+    FlatPackBufs &F = g_fpk;
+    F.h.resize(ns);
+    for (int q = 0; q < ns; q++) {
+        const Ctx *c = cs[q];
+        const Packed *p = c->pk;
+        FlatPack &t = F.h[q];
+        t.n = c->n;
+        t.n_eq = p->n_eq;
+        t.n_slices = p->n_slices;
+        t.stride = c->stride;
+        t.unk = c->unk; t.Alen = c->Alen; t.Acol = c->Acol; t.Aval = c->Aval; t.diag = c->diag; t.b = c->b;
+        t.row_of_slot = p->row_of_slot; t.slot_of_row = p->slot_of_row;
+        t.sw = p->sw; t.whw = p->whw; t.w32 = p->w32; t.bh = p->bh; t.bmode = p->bmode; t.dmax = p->dmax;
+        t.voff = p->voff; t.chwoff = p->chwoff; t.c32off = p->c32off; t.hoff = p->hoff; t.halo = p->halo;
+        t.val = p->val; t.chw = p->chw; t.c32 = p->c32; t.len = p->len; t.pdiag = p->diag; t.pb = p->b;
+        t.smeta = p->smeta; t.bmeta = p->bmeta; t.fval = p->fval; t.ddir = p->ddir; t.px = p->x; t.x = c->x;
+    }
+    HIP_OK(hipMemcpyAsync(F.tab, F.h.data(), sizeof(FlatPack) * ns, hipMemcpyHostToDevice, s));
+    // the host table is rewritten by the next call: wait for the copy
+    HIP_OK(hipStreamSynchronize(s));
+    return 0;
+}
+
+#define FLAT_PACK()                                         \
+    const int s_ = flat_scene(first, ns, blockIdx.x);       \
+    const FlatPack &P = tab[s_];                            \
+    const int lb = blockIdx.x - first[s_]
+
+__global__ void __launch_bounds__(PKB)
+k_pk_slots_flat(const FlatPack *__restrict__ tab, const int *__restrict__ first, int ns)
+{
+    FLAT_PACK();
+    pk_slots_body(lb, P.n, P.unk, P.row_of_slot, P.slot_of_row);
+}
+
+__global__ void k_pk_dmax_zero_flat(const FlatPack *__restrict__ tab, int ns)
+{
+    for (int s = threadIdx.x; s < ns; s += blockDim.x) tab[s].dmax[0] = 0;
+}
+
+__global__ void k_pk_dmax_gather_flat(const FlatPack *__restrict__ tab, int ns, int *out)
+{
+    for (int s = threadIdx.x; s < ns; s += blockDim.x)
+        for (int j = 0; j < 5; j++) out[8 * s + j] = tab[s].dmax[j];
+}
+
+__global__ void __launch_bounds__(PKB)
+k_pk_pass1_flat(const FlatPack *__restrict__ tab, const int *__restrict__ first, int ns)
+{
+    FLAT_PACK();
+    pk_pass1_body(lb, P.n_eq, P.n_slices, P.stride, P.slot_of_row, P.row_of_slot, P.Alen, P.Acol, P.sw, P.whw,
+                  P.w32, P.bh, P.bmode, P.dmax);
+}
+
+__global__ void __launch_bounds__(PKB)
+k_pk_pass2_flat(const FlatPack *__restrict__ tab, const int *__restrict__ first, int ns)
+{
+    FLAT_PACK();
+    pk_pass2_body(lb, P.n_eq, P.n_slices, P.stride, P.slot_of_row, P.row_of_slot, P.Alen, P.Aval, P.Acol,
+                  P.diag, P.b, P.bmode, P.bh, P.hoff, P.voff, P.chwoff, P.c32off, P.halo, P.val, P.chw, P.c32,
+                  P.len, P.pdiag, P.pb, P.smeta, P.bmeta);
+}
+
+__global__ void __launch_bounds__(PKB)
+k_pk_f32_flat(const FlatPack *__restrict__ tab, const int *__restrict__ first, int ns)
+{
+    FLAT_PACK();
+    pk_f32_body(lb, P.n_eq, P.len, P.smeta, P.val, P.pdiag, P.fval, P.ddir);
+}
+
+__global__ void __launch_bounds__(PKB)
+k_pk_unpack_x_flat(const FlatPack *__restrict__ tab, const int *__restrict__ first, int ns)
+{
+    FLAT_PACK();
+    pk_unpack_x_body(lb, P.n, P.unk, P.row_of_slot, P.px, P.x);
+}
+
+// Packs the assembled PPE of every scene (as pack_ppe does each alone);
+// gb holds the scenes' 256-thread block offsets over their particles.
+int flat_pack(Ctx *const *cs, int ns, const FlatGrid &gb, int with_f32, hipStream_t s)
+{
+    FlatPackBufs &F = g_fpk;
+    int rc = fp_reserve(ns);
+    if (rc) return rc;
+    for (int q = 0; q < ns; q++) {
+        if (!cs[q]->pk) cs[q]->pk = new Packed();
+        rc = pk_reserve_rows(cs[q]->pk, cs[q]->n);
+        if (rc) return rc;
+    }
+    prof_begin(PK_PACK, s);
+    // stage a: the unknowns' row numbers, the counts to the host
+    F.hseg.resize(4 * (size_t)ns);
+    for (int q = 0; q < ns; q++)
+        F.hseg[q] = ScanSeg{cs[q]->unk, cs[q]->pk->row_of_slot, cs[q]->n, F.tot + 8 * q};
+    rc = scan_segs_set(&F.sc_unk, F.hseg.data(), ns, s);
+    if (!rc) rc = fp_blk((size_t)F.sc_unk.ntiles + ns);
+    if (!rc) rc = launch_scan_seg(F.sc_unk, F.blk, s);
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(F.htot, F.tot, sizeof(int) * 8 * ns, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    std::vector<int> nb(ns);
+    for (int q = 0; q < ns; q++) {
+        Packed *p = cs[q]->pk;
+        p->n_eq = F.htot[8 * q];
+        p->n_slices = (p->n_eq + 63) / 64;
+        p->n_blocks = (p->n_eq + PKB - 1) / PKB;
+        p->hmax = 0;
+        nb[q] = p->n_blocks;
+    }
+    rc = fp_table(cs, ns, s);
+    if (!rc) rc = flat_build_grid(&F.g_rows, nb.data(), ns, s);
+    if (rc) return rc;
+    // stage b: slice widths, halos and modes, their scans
+    if (gb.blocks > 0) hipLaunchKernelGGL(k_pk_slots_flat, dim3(gb.blocks), dim3(PKB), 0, s, F.tab, gb.first, ns);
+    hipLaunchKernelGGL(k_pk_dmax_zero_flat, dim3(1), dim3(64), 0, s, F.tab, ns);
+    if (F.g_rows.blocks > 0)
+        hipLaunchKernelGGL(k_pk_pass1_flat, dim3(F.g_rows.blocks), dim3(PKB), 0, s, F.tab, F.g_rows.first, ns);
+    LAUNCH_CHECK();
+    for (int q = 0; q < ns; q++) {
+        Packed *p = cs[q]->pk;
+        F.hseg[4 * q + 0] = ScanSeg{p->sw, p->voff, p->n_slices, p->dmax + 1};
+        F.hseg[4 * q + 1] = ScanSeg{p->whw, p->chwoff, p->n_slices, p->dmax + 2};
+        F.hseg[4 * q + 2] = ScanSeg{p->w32, p->c32off, p->n_slices, p->dmax + 3};
+        F.hseg[4 * q + 3] = ScanSeg{p->bh, p->hoff, p->n_blocks, p->dmax + 4};
+    }
+    rc = scan_segs_set(&F.sc_set, F.hseg.data(), 4 * ns, s);
+    if (!rc) rc = fp_blk((size_t)F.sc_set.ntiles + 4 * ns);
+    if (!rc) rc = launch_scan_seg(F.sc_set, F.blk, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_pk_dmax_gather_flat, dim3(1), dim3(64), 0, s, F.tab, ns, F.tot);
+    LAUNCH_CHECK();
+    HIP_OK(hipMemcpyAsync(F.htot, F.tot, sizeof(int) * 8 * ns, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    // stage c: values, indices, halos, descriptors
+    for (int q = 0; q < ns; q++) {
+        Packed *p = cs[q]->pk;
+        if (p->n_eq == 0) continue;
+        const int *t = F.htot + 8 * q;
+        p->hmax = t[0];
+        p->tot_val = (size_t)t[1];
+        p->tot_hw = (size_t)t[2];
+        p->tot_c32 = (size_t)t[3];
+        p->tot_halo = (size_t)t[4];
+        GROW(p->val, p->cap_val, p->tot_val * 64);
+        GROW(p->chw, p->cap_hw, p->tot_hw * 64);
+        GROW(p->c32, p->cap_c32, p->tot_c32 * 64);
+        GROW(p->halo, p->cap_halo, p->tot_halo);
+        if (with_f32) GROW(p->fval, p->cap_fval, p->tot_val * 64);
+    }
+    rc = fp_table(cs, ns, s);
+    if (rc) return rc;
+    if (F.g_rows.blocks > 0) {
+        hipLaunchKernelGGL(k_pk_pass2_flat, dim3(F.g_rows.blocks), dim3(PKB), 0, s, F.tab, F.g_rows.first, ns);
+        if (with_f32)
+            hipLaunchKernelGGL(k_pk_f32_flat, dim3(F.g_rows.blocks), dim3(PKB), 0, s, F.tab, F.g_rows.first, ns);
+    }
+    LAUNCH_CHECK();
+    prof_end(PK_PACK, s);
+    return 0;
+}
+
+// Every scene's packed solution into its slot-ordered x (unpack_solution).
+int flat_unpack(Ctx *const *cs, int ns, const FlatGrid &gb, hipStream_t s)
+{
+    (void)cs;
+    if (gb.blocks > 0)
+        hipLaunchKernelGGL(k_pk_unpack_x_flat, dim3(gb.blocks), dim3(PKB), 0, s, g_fpk.tab, gb.first, ns);
     LAUNCH_CHECK();
     return 0;
 }

@@ -17,6 +17,8 @@
 // FP64 everywhere here, association order of the reference, no contraction.
 #include <limits.h>
 
+#include <vector>
+
 #include "ctx.hpp"
 
 #define BLK 256
@@ -549,12 +551,11 @@ int launch_friction(Ctx *c, const StepConsts &k)
 #define NB_U 4     // of the viscosity and pressure-gradient sums
 
 template <int B>
-__global__ void __launch_bounds__(B)
-k_visc_predict(int n, const dvec *__restrict__ X, const dvec *__restrict__ V, const int *__restrict__ T,
+__device__ __forceinline__ void visc_predict_body(int lb, int n, const dvec *__restrict__ X, const dvec *__restrict__ V, const int *__restrict__ T,
                const int *__restrict__ nbr, const int *__restrict__ ncnt, dvec *__restrict__ Xo,
                dvec *__restrict__ Vo, StepConsts k)
 {
-    const int i = blockIdx.x * B + threadIdx.x;
+    const int i = lb * B + threadIdx.x;
     if (i >= n) return;
     const dvec xi = X[i], vi = V[i];
     if (T[i] != FLUID_PARTICLE) {
@@ -619,6 +620,15 @@ k_visc_predict(int n, const dvec *__restrict__ X, const dvec *__restrict__ V, co
     Vo[i] = vo;
 }
 
+template <int B>
+__global__ void __launch_bounds__(B)
+k_visc_predict(int n, const dvec *__restrict__ X, const dvec *__restrict__ V, const int *__restrict__ T,
+               const int *__restrict__ nbr, const int *__restrict__ ncnt, dvec *__restrict__ Xo,
+               dvec *__restrict__ Vo, StepConsts k)
+{
+    visc_predict_body<B>(blockIdx.x, n, X, V, T, nbr, ncnt, Xo, Vo, k);
+}
+
 int launch_visc_predict(Ctx *c, const StepConsts &k)
 {
     prof_begin(PK_VISC, c->stream);
@@ -637,13 +647,12 @@ int launch_visc_predict(Ctx *c, const StepConsts &k)
 // ---------------------------------------------------------------------
 
 template <int B>
-__global__ void __launch_bounds__(B)
-k_collide_bin(int n, const dvec *__restrict__ X, const dvec *__restrict__ V, const int *__restrict__ T,
+__device__ __forceinline__ void collide_bin_body(int lb, int n, const dvec *__restrict__ X, const dvec *__restrict__ V, const int *__restrict__ T,
               const int *__restrict__ nbr, const int *__restrict__ ncnt, dvec *__restrict__ Xo,
               dvec *__restrict__ Vo, int *__restrict__ cell, int *__restrict__ ccount, StepConsts k,
               int collide)
 {
-    const int i = blockIdx.x * B + threadIdx.x;
+    const int i = lb * B + threadIdx.x;
     if (i >= n) return;
     const int t = T[i];
     const dvec xi = X[i], vi = V[i];
@@ -697,6 +706,16 @@ k_collide_bin(int n, const dvec *__restrict__ X, const dvec *__restrict__ V, con
     if (ci < 0) ci = k.grid.total;
     cell[i] = ci;
     atomicAdd(&ccount[ci], 1);
+}
+
+template <int B>
+__global__ void __launch_bounds__(B)
+k_collide_bin(int n, const dvec *__restrict__ X, const dvec *__restrict__ V, const int *__restrict__ T,
+              const int *__restrict__ nbr, const int *__restrict__ ncnt, dvec *__restrict__ Xo,
+              dvec *__restrict__ Vo, int *__restrict__ cell, int *__restrict__ ccount, StepConsts k,
+              int collide)
+{
+    collide_bin_body<B>(blockIdx.x, n, X, V, T, nbr, ncnt, Xo, Vo, cell, ccount, k, collide);
 }
 
 int launch_collide_bin(Ctx *c, const StepConsts &k, int collide)
@@ -864,11 +883,10 @@ int collide_2d(Ctx *c, const StepConsts &k)
 // 3. counting sort by (cell, descending original index)
 // ---------------------------------------------------------------------
 
-__global__ void __launch_bounds__(BLK)
-k_scatter(int n, const int *__restrict__ cell, const int *__restrict__ O, const int *__restrict__ cstart,
+__device__ __forceinline__ void scatter_body(int lb, int n, const int *__restrict__ cell, const int *__restrict__ O, const int *__restrict__ cstart,
           int *__restrict__ cfill, int *__restrict__ tmp_key, int *__restrict__ sflag, int total)
 {
-    const int i = blockIdx.x * BLK + threadIdx.x;
+    const int i = lb * BLK + threadIdx.x;
     if (i >= n) return;
     const int ci = cell[i];
     sflag[i] = (ci == total);
@@ -877,16 +895,22 @@ k_scatter(int n, const int *__restrict__ cell, const int *__restrict__ O, const 
     tmp_key[cstart[ci] + at] = O[i];
 }
 
+__global__ void __launch_bounds__(BLK)
+k_scatter(int n, const int *__restrict__ cell, const int *__restrict__ O, const int *__restrict__ cstart,
+          int *__restrict__ cfill, int *__restrict__ tmp_key, int *__restrict__ sflag, int total)
+{
+    scatter_body(blockIdx.x, n, cell, O, cstart, cfill, tmp_key, sflag, total);
+}
+
 struct SortMove {
     SlotArrays src, dst;
     int full;
 };
 
-__global__ void __launch_bounds__(BLK)
-k_rank_move(int n, const int *__restrict__ cell, const int *__restrict__ cstart,
+__device__ __forceinline__ void rank_move_body(int lb, int n, const int *__restrict__ cell, const int *__restrict__ cstart,
             const int *__restrict__ tmp_key, const int *__restrict__ sscan, int total, SortMove mv)
 {
-    const int i = blockIdx.x * BLK + threadIdx.x;
+    const int i = lb * BLK + threadIdx.x;
     if (i >= n) return;
     const int ci = cell[i];
     const int key = mv.src.O[i];
@@ -910,6 +934,13 @@ k_rank_move(int n, const int *__restrict__ cell, const int *__restrict__ cstart,
         mv.dst.ND[dst] = mv.src.ND[i];
         mv.dst.S[dst] = mv.src.S[i];
     }
+}
+
+__global__ void __launch_bounds__(BLK)
+k_rank_move(int n, const int *__restrict__ cell, const int *__restrict__ cstart,
+            const int *__restrict__ tmp_key, const int *__restrict__ sscan, int total, SortMove mv)
+{
+    rank_move_body(blockIdx.x, n, cell, cstart, tmp_key, sscan, total, mv);
 }
 
 int launch_sort(Ctx *c, const StepConsts &k, int full)
@@ -1028,11 +1059,10 @@ __device__ __forceinline__ void search_finish(int i, int n, int t, const int *__
 // One thread per slot; candidate positions gathered from HBM/L2 (option
 // "search" 0).
 template <int B>
-__global__ void __launch_bounds__(B)
-k_search(int n, const dvec *__restrict__ X, const int *__restrict__ T, const int *__restrict__ O,
+__device__ __forceinline__ void search_hbm_body(int lb, int n, const dvec *__restrict__ X, const int *__restrict__ T, const int *__restrict__ O,
          const int *__restrict__ cstart, SearchOut out, StepConsts k, int with_density)
 {
-    const int i = blockIdx.x * B + threadIdx.x;
+    const int i = lb * B + threadIdx.x;
     SearchAcc a{0, 0, 0.0, 0.0};
     int t = GHOST_PARTICLE;
     if (i < n) {
@@ -1074,6 +1104,14 @@ k_search(int n, const dvec *__restrict__ X, const int *__restrict__ T, const int
     search_finish<B>(i, n, t, O, a, out, k, with_density);
 }
 
+template <int B>
+__global__ void __launch_bounds__(B)
+k_search(int n, const dvec *__restrict__ X, const int *__restrict__ T, const int *__restrict__ O,
+         const int *__restrict__ cstart, SearchOut out, StepConsts k, int with_density)
+{
+    search_hbm_body<B>(blockIdx.x, n, X, T, O, cstart, out, k, with_density);
+}
+
 // The same search with the candidates staged in LDS (option "search" 1,
 // the default).  Slots are in cell order with x the fastest cell axis, so
 // the three cells x-1..x+1 of one (y, z) row of a particle's block are one
@@ -1104,14 +1142,13 @@ __device__ __forceinline__ int wave_max_int(int v)
 }
 
 template <int B>
-__global__ void __launch_bounds__(B)
-k_search_lds(int n, const dvec *__restrict__ X, const int *__restrict__ T, const int *__restrict__ O,
+__device__ __forceinline__ void search_lds_body(int lb, int n, const dvec *__restrict__ X, const int *__restrict__ T, const int *__restrict__ O,
              const int *__restrict__ cstart, SearchOut out, StepConsts k, int with_density)
 {
     __shared__ dvec sx[SRCH_CAP + SRCH_U];
     __shared__ int stp[SRCH_CAP + SRCH_U];
     __shared__ int smin[B / WAVE], smax[B / WAVE];
-    const int i = blockIdx.x * B + threadIdx.x;
+    const int i = lb * B + threadIdx.x;
     const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
     SearchAcc a{0, 0, 0.0, 0.0};
     int t = GHOST_PARTICLE;
@@ -1209,6 +1246,14 @@ k_search_lds(int n, const dvec *__restrict__ X, const int *__restrict__ T, const
     search_finish<B>(i, n, t, O, a, out, k, with_density);
 }
 
+template <int B>
+__global__ void __launch_bounds__(B)
+k_search_lds(int n, const dvec *__restrict__ X, const int *__restrict__ T, const int *__restrict__ O,
+             const int *__restrict__ cstart, SearchOut out, StepConsts k, int with_density)
+{
+    search_lds_body<B>(blockIdx.x, n, X, T, O, cstart, out, k, with_density);
+}
+
 int launch_search(Ctx *c, const StepConsts &k, int with_density, DevCounters *ctr)
 {
     SearchOut o;
@@ -1232,13 +1277,12 @@ int launch_search(Ctx *c, const StepConsts &k, int with_density, DevCounters *ct
 // ---------------------------------------------------------------------
 
 template <int B>
-__global__ void __launch_bounds__(B)
-k_assemble(int n, const dvec *__restrict__ X, const dvec *__restrict__ V, const int *__restrict__ T,
+__device__ __forceinline__ void assemble_body(int lb, int n, const dvec *__restrict__ X, const dvec *__restrict__ V, const int *__restrict__ T,
            const double *__restrict__ ND, const int *__restrict__ unk, const int *__restrict__ nbr,
            const int *__restrict__ ncnt, double *__restrict__ Aval, int *__restrict__ Acol,
            int *__restrict__ Alen, double *__restrict__ b, DevCounters *ctr, StepConsts k)
 {
-    const int i = blockIdx.x * B + threadIdx.x;
+    const int i = lb * B + threadIdx.x;
     long long row_nnz = 0;
     if (i < n) {
         if (!unk[i]) {
@@ -1324,6 +1368,16 @@ k_assemble(int n, const dvec *__restrict__ X, const dvec *__restrict__ V, const 
     if (threadIdx.x == 0) atomicAdd(&ctr->nnz, (unsigned long long)tot);
 }
 
+template <int B>
+__global__ void __launch_bounds__(B)
+k_assemble(int n, const dvec *__restrict__ X, const dvec *__restrict__ V, const int *__restrict__ T,
+           const double *__restrict__ ND, const int *__restrict__ unk, const int *__restrict__ nbr,
+           const int *__restrict__ ncnt, double *__restrict__ Aval, int *__restrict__ Acol,
+           int *__restrict__ Alen, double *__restrict__ b, DevCounters *ctr, StepConsts k)
+{
+    assemble_body<B>(blockIdx.x, n, X, V, T, ND, unk, nbr, ncnt, Aval, Acol, Alen, b, ctr, k);
+}
+
 int launch_assemble(Ctx *c, const StepConsts &k, DevCounters *ctr)
 {
     prof_begin(PK_ASSEMBLE, c->stream);
@@ -1340,15 +1394,21 @@ int launch_assemble(Ctx *c, const StepConsts &k, DevCounters *ctr)
 //    (operators.c:188-199)
 // ---------------------------------------------------------------------
 
-__global__ void __launch_bounds__(BLK)
-k_pressure_out(int n, const double *__restrict__ x, const int *__restrict__ unk, const int *__restrict__ T,
+__device__ __forceinline__ void pressure_out_body(int lb, int n, const double *__restrict__ x, const int *__restrict__ unk, const int *__restrict__ T,
                double *__restrict__ P, int clamp)
 {
-    const int i = blockIdx.x * BLK + threadIdx.x;
+    const int i = lb * BLK + threadIdx.x;
     if (i >= n) return;
     double p = unk[i] ? x[i] : 0.0;
     if (clamp && T[i] == FLUID_PARTICLE && p < 0.0) p = 0.0;
     P[i] = p;
+}
+
+__global__ void __launch_bounds__(BLK)
+k_pressure_out(int n, const double *__restrict__ x, const int *__restrict__ unk, const int *__restrict__ T,
+               double *__restrict__ P, int clamp)
+{
+    pressure_out_body(blockIdx.x, n, x, unk, T, P, clamp);
 }
 
 int launch_pressure_out(Ctx *c, const StepConsts &k, int n_eq_zero)
@@ -1365,11 +1425,10 @@ int launch_pressure_out(Ctx *c, const StepConsts &k, int n_eq_zero)
 //    100-108), wall re-fix and out-of-domain ghosting (boundary.c:12-61)
 // ---------------------------------------------------------------------
 
-__global__ void __launch_bounds__(BLK)
-k_pmin(int n, const double *__restrict__ P, const int *__restrict__ nbr, const int *__restrict__ ncnt,
+__device__ __forceinline__ void pmin_body(int lb, int n, const double *__restrict__ P, const int *__restrict__ nbr, const int *__restrict__ ncnt,
        double *__restrict__ pmin, int stride)
 {
-    const int i = blockIdx.x * BLK + threadIdx.x;
+    const int i = lb * BLK + threadIdx.x;
     if (i >= n) return;
     double m = P[i];
     const int cnt = ncnt[i];
@@ -1378,6 +1437,13 @@ k_pmin(int n, const double *__restrict__ P, const int *__restrict__ nbr, const i
         if (pj < m) m = pj;
     }
     pmin[i] = m;
+}
+
+__global__ void __launch_bounds__(BLK)
+k_pmin(int n, const double *__restrict__ P, const int *__restrict__ nbr, const int *__restrict__ ncnt,
+       double *__restrict__ pmin, int stride)
+{
+    pmin_body(blockIdx.x, n, P, nbr, ncnt, pmin, stride);
 }
 
 struct GradIO {
@@ -1389,10 +1455,9 @@ struct GradIO {
 };
 
 template <int B>
-__global__ void __launch_bounds__(B)
-k_grad_correct(int n, GradIO io, const int *__restrict__ nbr, const int *__restrict__ ncnt, StepConsts k)
+__device__ __forceinline__ void grad_correct_body(int lb, int n, GradIO io, const int *__restrict__ nbr, const int *__restrict__ ncnt, StepConsts k)
 {
-    const int i = blockIdx.x * B + threadIdx.x;
+    const int i = lb * B + threadIdx.x;
     if (i >= n) return;
     int t = io.T[i];
     const dvec xi = io.X[i], vi = io.V[i];
@@ -1478,6 +1543,13 @@ k_grad_correct(int n, GradIO io, const int *__restrict__ nbr, const int *__restr
     io.Po[i] = po;
 }
 
+template <int B>
+__global__ void __launch_bounds__(B)
+k_grad_correct(int n, GradIO io, const int *__restrict__ nbr, const int *__restrict__ ncnt, StepConsts k)
+{
+    grad_correct_body<B>(blockIdx.x, n, io, nbr, ncnt, k);
+}
+
 int launch_grad_correct(Ctx *c, const StepConsts &k)
 {
     if (k.cmps == 1) {
@@ -1523,6 +1595,350 @@ int launch_density_only(Ctx *c, const StepConsts &k)
 {
     hipLaunchKernelGGL(k_density, dim3(nblk(c->n)), dim3(BLK), 0, c->stream, c->n, c->X[c->cx], c->nbr,
                        c->ncnt, c->ND[c->ct], k);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+// ---------------------------------------------------------------------
+// Batches of scenes: every operator of the step as one launch over the
+// concatenated block ranges of all scenes.  Each block finds its scene by
+// bisection over the launch's block offsets and runs the single-scene
+// kernel's body on that scene's buffers, with that scene's local block
+// index, so every scene gets the bits of a launch of its own.
+// ---------------------------------------------------------------------
+
+int flat_build_grid(FlatGrid *g, const int *blocks, int ns, hipStream_t s)
+{
+    std::vector<int> h((size_t)ns + 1);
+    h[0] = 0;
+    for (int q = 0; q < ns; q++) h[q + 1] = h[q] + blocks[q];
+    if (g->cap < ns + 1) {
+        if (g->first) hipFree(g->first);
+        g->first = nullptr;
+        HIP_OK(hipMalloc(&g->first, sizeof(int) * h.size()));
+        g->cap = ns + 1;
+    }
+    HIP_OK(hipMemcpyAsync(g->first, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, s));
+    HIP_OK(hipStreamSynchronize(s));
+    g->blocks = h[ns];
+    return 0;
+}
+
+// Segmented exclusive scans in three launches: the tiles of every segment
+// (segment g on tiles tile0[g] .. tile0[g + 1] - 1), one block per segment
+// scanning its tile sums (kept at blk[tile0[g] + g ...], the segment's
+// total after them), then the tiles again with their offsets.
+__global__ void __launch_bounds__(BLK) k_sscan_tiles(const ScanSeg *__restrict__ segs,
+                                                     const int *__restrict__ tile0, int nseg, int *blk)
+{
+    const int g = flat_scene(tile0, nseg, blockIdx.x);
+    const ScanSeg &S = segs[g];
+    const int lt = blockIdx.x - tile0[g];
+    const int base = lt * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; q++)
+        if (base + q < S.n) s += S.in[base + q];
+    int tot;
+    (void)block_scan_excl<BLK>(s, &tot);
+    if (threadIdx.x == 0) blk[tile0[g] + g + lt] = tot;
+}
+
+__global__ void __launch_bounds__(1024) k_sscan_tile_sums(const ScanSeg *__restrict__ segs,
+                                                          const int *__restrict__ tile0, int *blk)
+{
+    const int g = blockIdx.x;
+    const ScanSeg &S = segs[g];
+    const int ntiles = tile0[g + 1] - tile0[g];
+    int *tile_sum = blk + tile0[g] + g;
+    const int base = threadIdx.x * 8;   // 1024 threads x 8 tiles each
+    int v[8];
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        v[q] = (base + q < ntiles) ? tile_sum[base + q] : 0;
+        s += v[q];
+    }
+    int tot;
+    int run = block_scan_excl<1024>(s, &tot);
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        if (base + q < ntiles) tile_sum[base + q] = run;
+        run += v[q];
+    }
+    if (threadIdx.x == 0) {
+        tile_sum[ntiles] = tot;
+        if (S.tot) *S.tot = tot;
+        if (S.n == 0) S.out[0] = 0;
+    }
+}
+
+__global__ void __launch_bounds__(BLK) k_sscan_apply(const ScanSeg *__restrict__ segs,
+                                                     const int *__restrict__ tile0, int nseg, const int *blk)
+{
+    const int g = flat_scene(tile0, nseg, blockIdx.x);
+    const ScanSeg &S = segs[g];
+    const int lt = blockIdx.x - tile0[g];
+    const int ntiles = tile0[g + 1] - tile0[g];
+    const int *tile_sum = blk + tile0[g] + g;
+    const int base = lt * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    int v[SCAN_ITEMS];
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; q++) {
+        v[q] = (base + q < S.n) ? S.in[base + q] : 0;
+        s += v[q];
+    }
+    int tot;
+    int run = block_scan_excl<BLK>(s, &tot) + tile_sum[lt];
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; q++) {
+        if (base + q < S.n) S.out[base + q] = run;
+        run += v[q];
+    }
+    if (lt == 0 && threadIdx.x == 0) S.out[S.n] = tile_sum[ntiles];
+}
+
+int scan_segs_set(ScanSegs *ss, const ScanSeg *h, int nseg, hipStream_t s)
+{
+    std::vector<int> t0((size_t)nseg + 1);
+    t0[0] = 0;
+    for (int g = 0; g < nseg; g++) {
+        const int nt = (h[g].n + SCAN_TILE - 1) / SCAN_TILE;
+        if (h[g].n < 0 || nt > 8192) {
+            mps_set_error(MPS_ERR_UNSUPPORTED, __FILE__, __LINE__, "scan longer than 16.7M elements");
+            return MPS_ERR_UNSUPPORTED;
+        }
+        t0[g + 1] = t0[g] + nt;
+    }
+    if (nseg > ss->nseg) {
+        if (ss->segs) hipFree(ss->segs);
+        if (ss->tile0) hipFree(ss->tile0);
+        ss->segs = nullptr;
+        ss->tile0 = nullptr;
+        HIP_OK(hipMalloc(&ss->segs, sizeof(ScanSeg) * nseg));
+        HIP_OK(hipMalloc(&ss->tile0, sizeof(int) * (nseg + 1)));
+    }
+    ss->nseg = nseg;
+    ss->ntiles = t0[nseg];
+    // synchronous copies: the host tables may be rebuilt at once
+    HIP_OK(hipMemcpy(ss->segs, h, sizeof(ScanSeg) * nseg, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(ss->tile0, t0.data(), sizeof(int) * (nseg + 1), hipMemcpyHostToDevice));
+    (void)s;
+    return 0;
+}
+
+// blk needs ntiles + nseg ints.
+int launch_scan_seg(const ScanSegs &ss, int *blk, hipStream_t s)
+{
+    if (ss.nseg <= 0) return 0;
+    if (ss.ntiles > 0)
+        hipLaunchKernelGGL(k_sscan_tiles, dim3(ss.ntiles), dim3(BLK), 0, s, ss.segs, ss.tile0, ss.nseg, blk);
+    hipLaunchKernelGGL(k_sscan_tile_sums, dim3(ss.nseg), dim3(1024), 0, s, ss.segs, ss.tile0, blk);
+    if (ss.ntiles > 0)
+        hipLaunchKernelGGL(k_sscan_apply, dim3(ss.ntiles), dim3(BLK), 0, s, ss.segs, ss.tile0, ss.nseg, blk);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+#define FLAT_SCENE()                                        \
+    const int s_ = flat_scene(first, ns, blockIdx.x);       \
+    const FlatScene &S = tab[s_];                           \
+    const int lb = blockIdx.x - first[s_]
+
+template <int B>
+__global__ void __launch_bounds__(B)
+k_visc_predict_flat(const FlatScene *__restrict__ tab, const int *__restrict__ first, int ns, int cx, int ct)
+{
+    FLAT_SCENE();
+    visc_predict_body<B>(lb, S.n, S.X[cx], S.V[cx], S.T[ct], S.nbr, S.ncnt, S.X[1 - cx], S.V[1 - cx], S.k);
+}
+
+__global__ void __launch_bounds__(BLK)
+k_cells_zero_flat(const FlatScene *__restrict__ tab, const int *__restrict__ first, int ns)
+{
+    FLAT_SCENE();
+    const int c = lb * BLK + threadIdx.x;
+    if (c > S.total) return;
+    S.ccount[c] = 0;
+    S.cfill[c] = 0;
+}
+
+template <int B>
+__global__ void __launch_bounds__(B)
+k_collide_bin_flat(const FlatScene *__restrict__ tab, const int *__restrict__ first, int ns, int cx, int ct)
+{
+    FLAT_SCENE();
+    collide_bin_body<B>(lb, S.n, S.X[cx], S.V[cx], S.T[ct], S.nbr, S.ncnt, S.X[1 - cx], S.V[1 - cx], S.cell,
+                        S.ccount, S.k, S.k.collide);
+}
+
+__global__ void __launch_bounds__(BLK)
+k_scatter_flat(const FlatScene *__restrict__ tab, const int *__restrict__ first, int ns, int ct)
+{
+    FLAT_SCENE();
+    scatter_body(lb, S.n, S.cell, S.O[ct], S.cstart, S.cfill, S.tmp_key, S.sflag, S.total);
+}
+
+__global__ void __launch_bounds__(BLK)
+k_rank_move_flat(const FlatScene *__restrict__ tab, const int *__restrict__ first, int ns, int cx, int ct, int cp)
+{
+    FLAT_SCENE();
+    SortMove mv;
+    mv.src.X = S.X[cx]; mv.src.V = S.V[cx]; mv.src.A = S.ACC[ct]; mv.src.P = S.P[cp]; mv.src.ND = S.ND[ct];
+    mv.src.T = S.T[ct]; mv.src.O = S.O[ct]; mv.src.S = S.SURF[ct]; mv.src.NC = S.NCOUNT[ct];
+    mv.dst.X = S.X[1 - cx]; mv.dst.V = S.V[1 - cx]; mv.dst.A = S.ACC[1 - ct]; mv.dst.ND = S.ND[1 - ct];
+    mv.dst.T = S.T[1 - ct]; mv.dst.O = S.O[1 - ct]; mv.dst.S = S.SURF[1 - ct]; mv.dst.NC = S.NCOUNT[1 - ct];
+    mv.dst.P = S.P[1 - cp];
+    mv.full = 0;
+    rank_move_body(lb, S.n, S.cell, S.cstart, S.tmp_key, S.sscan, S.total, mv);
+}
+
+__global__ void k_counters_reset_flat(const FlatScene *__restrict__ tab, int ns)
+{
+    for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+        DevCounters z{};
+        z.overflow_orig = INT_MAX;
+        *tab[s].ctr = z;
+    }
+}
+
+__global__ void k_counters_gather_flat(const FlatScene *__restrict__ tab, int ns, DevCounters *out)
+{
+    for (int s = threadIdx.x; s < ns; s += blockDim.x) out[s] = *tab[s].ctr;
+}
+
+template <int B>
+__global__ void __launch_bounds__(B)
+k_search_flat(const FlatScene *__restrict__ tab, const int *__restrict__ first, int ns, int cx, int ct, int form)
+{
+    FLAT_SCENE();
+    SearchOut o;
+    o.nbr = S.nbr; o.ncnt = S.ncnt; o.surf = S.SURF[ct]; o.unk = S.unk; o.ncount = S.NCOUNT[ct];
+    o.nd = S.ND[ct]; o.diag = S.diag; o.b = S.b; o.ctr = S.ctr;
+    if (form == 1) search_lds_body<B>(lb, S.n, S.X[cx], S.T[ct], S.O[ct], S.cstart, o, S.k, 1);
+    else search_hbm_body<B>(lb, S.n, S.X[cx], S.T[ct], S.O[ct], S.cstart, o, S.k, 1);
+}
+
+template <int B>
+__global__ void __launch_bounds__(B)
+k_assemble_flat(const FlatScene *__restrict__ tab, const int *__restrict__ first, int ns, int cx, int ct)
+{
+    FLAT_SCENE();
+    assemble_body<B>(lb, S.n, S.X[cx], S.V[cx], S.T[ct], S.ND[ct], S.unk, S.nbr, S.ncnt, S.Aval, S.Acol, S.Alen,
+                     S.b, S.ctr, S.k);
+}
+
+__global__ void __launch_bounds__(BLK)
+k_pressure_out_flat(const FlatScene *__restrict__ tab, const int *__restrict__ first, int ns, int ct, int cp)
+{
+    FLAT_SCENE();
+    pressure_out_body(lb, S.n, S.x, S.unk, S.T[ct], S.P[cp], S.k.clamp);
+}
+
+__global__ void __launch_bounds__(BLK)
+k_pmin_flat(const FlatScene *__restrict__ tab, const int *__restrict__ first, int ns, int cp)
+{
+    FLAT_SCENE();
+    pmin_body(lb, S.n, S.P[cp], S.nbr, S.ncnt, S.pmin, S.k.stride);
+}
+
+template <int B>
+__global__ void __launch_bounds__(B)
+k_grad_correct_flat(const FlatScene *__restrict__ tab, const int *__restrict__ first, int ns, int cx, int ct,
+                    int cp)
+{
+    FLAT_SCENE();
+    GradIO io;
+    io.X = S.X[cx]; io.V = S.V[cx]; io.P = S.P[cp]; io.pmin = S.pmin;
+    io.Xo = S.X[1 - cx]; io.Vo = S.V[1 - cx]; io.A = S.ACC[ct];
+    io.Po = S.P[1 - cp]; io.ND = S.ND[ct]; io.T = S.T[ct];
+    grad_correct_body<B>(lb, S.n, io, S.nbr, S.ncnt, S.k);
+}
+
+// Per-particle kernel at the block size of option "pblk" over the scenes.
+#define FLAT_PARTICLE_LAUNCH(kern, f, ...)                                                            \
+    do {                                                                                              \
+        const int pb_ = particle_block();                                                             \
+        if ((f).gp.blocks > 0) {                                                                      \
+            if (pb_ == 128)                                                                           \
+                hipLaunchKernelGGL(kern<128>, dim3((f).gp.blocks), dim3(128), 0, (f).s, __VA_ARGS__); \
+            else if (pb_ == 512)                                                                      \
+                hipLaunchKernelGGL(kern<512>, dim3((f).gp.blocks), dim3(512), 0, (f).s, __VA_ARGS__); \
+            else                                                                                      \
+                hipLaunchKernelGGL(kern<256>, dim3((f).gp.blocks), dim3(256), 0, (f).s, __VA_ARGS__); \
+        }                                                                                             \
+    } while (0)
+
+// The step of every scene up to the assembled PPE: predictor, collision
+// with the cell histogram, counting sort, search with density and flags,
+// assembly (launch_visc_predict .. launch_assemble of one scene).
+int flat_step_pre(FlatLaunch &f)
+{
+    prof_begin(PK_VISC, f.s);
+    FLAT_PARTICLE_LAUNCH(k_visc_predict_flat, f, f.tab, f.gp.first, f.ns, f.cx, f.ct);
+    prof_end(PK_VISC, f.s);
+    LAUNCH_CHECK();
+    f.cx ^= 1;
+    if (f.gc.blocks > 0)
+        hipLaunchKernelGGL(k_cells_zero_flat, dim3(f.gc.blocks), dim3(BLK), 0, f.s, f.tab, f.gc.first, f.ns);
+    prof_begin(PK_COLLIDE, f.s);
+    FLAT_PARTICLE_LAUNCH(k_collide_bin_flat, f, f.tab, f.gp.first, f.ns, f.cx, f.ct);
+    prof_end(PK_COLLIDE, f.s);
+    LAUNCH_CHECK();
+    f.cx ^= 1;
+    prof_begin(PK_SORT, f.s);
+    int rc = launch_scan_seg(f.sc_cells, f.scan_blk, f.s);
+    if (rc) return rc;
+    if (f.gb.blocks > 0)
+        hipLaunchKernelGGL(k_scatter_flat, dim3(f.gb.blocks), dim3(BLK), 0, f.s, f.tab, f.gb.first, f.ns, f.ct);
+    LAUNCH_CHECK();
+    rc = launch_scan_seg(f.sc_sflag, f.scan_blk, f.s);
+    if (rc) return rc;
+    if (f.gb.blocks > 0)
+        hipLaunchKernelGGL(k_rank_move_flat, dim3(f.gb.blocks), dim3(BLK), 0, f.s, f.tab, f.gb.first, f.ns, f.cx,
+                           f.ct, f.cp);
+    prof_end(PK_SORT, f.s);
+    LAUNCH_CHECK();
+    f.cx ^= 1;
+    f.ct ^= 1;
+    hipLaunchKernelGGL(k_counters_reset_flat, dim3(1), dim3(64), 0, f.s, f.tab, f.ns);
+    prof_begin(PK_SEARCH, f.s);
+    FLAT_PARTICLE_LAUNCH(k_search_flat, f, f.tab, f.gp.first, f.ns, f.cx, f.ct, search_form());
+    prof_end(PK_SEARCH, f.s);
+    LAUNCH_CHECK();
+    prof_begin(PK_ASSEMBLE, f.s);
+    FLAT_PARTICLE_LAUNCH(k_assemble_flat, f, f.tab, f.gp.first, f.ns, f.cx, f.ct);
+    prof_end(PK_ASSEMBLE, f.s);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+// The step of every scene after the solve (x already in slot order):
+// pressure and clamp, p_min where a scene needs it, gradient, corrector
+// and boundary (launch_pressure_out, launch_grad_correct of one scene).
+int flat_step_post(FlatLaunch &f)
+{
+    if (f.gb.blocks > 0)
+        hipLaunchKernelGGL(k_pressure_out_flat, dim3(f.gb.blocks), dim3(BLK), 0, f.s, f.tab, f.gb.first, f.ns,
+                           f.ct, f.cp);
+    LAUNCH_CHECK();
+    if (f.any_cmps1 && f.gb.blocks > 0) {
+        hipLaunchKernelGGL(k_pmin_flat, dim3(f.gb.blocks), dim3(BLK), 0, f.s, f.tab, f.gb.first, f.ns, f.cp);
+        LAUNCH_CHECK();
+    }
+    prof_begin(PK_GRAD, f.s);
+    FLAT_PARTICLE_LAUNCH(k_grad_correct_flat, f, f.tab, f.gp.first, f.ns, f.cx, f.ct, f.cp);
+    prof_end(PK_GRAD, f.s);
+    LAUNCH_CHECK();
+    f.cx ^= 1;
+    f.cp ^= 1;
+    return 0;
+}
+
+int flat_counters(FlatLaunch &f, DevCounters *out_dev)
+{
+    hipLaunchKernelGGL(k_counters_gather_flat, dim3(1), dim3(64), 0, f.s, f.tab, f.ns, out_dev);
     LAUNCH_CHECK();
     return 0;
 }

@@ -46,6 +46,7 @@ struct Options {
     double spmv = 0, profile = 0, verbose = 0, fused = 2, graph = 0, pblk = 256;
     double async_output = 1;
     double search = 1;                 // neighbour search: 1 candidates staged in LDS, 0 gathered from HBM
+    double batch_flat = 1;             // mps_hip_run_batch: 1 one launch per operator for all scenes, 0 per scene
     double ic_wait_limit = 1048576;   // unsuccessful polls before an IC(0) wait gives up
     double ic_sleep = 3;               // longest backoff of an idle IC(0) wave (0-3)
     double ic_bucket = -1;             // log2 of the levels per bucket of the IC(0) solve order (-1: automatic)
@@ -759,9 +760,10 @@ static int step_post_launch(Ctx *c, const StepConsts &k)
     return 0;
 }
 
-static int step_post_finish(Ctx *c, const StepConsts &k, const SolveResult &sr)
+// The statistics of a finished step (its counters already in hcounters)
+// and the neighbour-overflow check.
+static int step_bookkeep(Ctx *c, const StepConsts &k, const SolveResult &sr)
 {
-    HIP_OK(hipStreamSynchronize(c->stream));
     c->ppe_valid = true;
     c->stats.steps++;
     c->stats.last_iters = sr.iters;
@@ -775,6 +777,12 @@ static int step_post_finish(Ctx *c, const StepConsts &k, const SolveResult &sr)
     c->stats.max_count = c->hcounters->max_count;
     if (g_opt.profile > 0.5) prof_collect();
     return check_overflow(c, k.stride);
+}
+
+static int step_post_finish(Ctx *c, const StepConsts &k, const SolveResult &sr)
+{
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return step_bookkeep(c, k, sr);
 }
 
 static int step_post(Ctx *c, const StepConsts &k, const SolveResult &sr)
@@ -817,6 +825,175 @@ static int ensure_resident(Ctx *c, ParticleSystem *ps, SimConfig *cf)
 }
 
 // ---------------------------------------------------------------------
+// Batches stepped by one launch per operator for all scenes (3-D, CG)
+// ---------------------------------------------------------------------
+
+int flat_pack(Ctx *const *cs, int ns, const FlatGrid &gb, int with_f32, hipStream_t s);   // k_pack.hip
+int flat_unpack(Ctx *const *cs, int ns, const FlatGrid &gb, hipStream_t s);
+void flat_pack_free();
+
+struct FlatBatchState {
+    std::vector<FlatScene> h;
+    FlatScene *tab = nullptr;
+    int cap = 0;
+    FlatGrid gp, gb, gc;
+    ScanSegs sc_cells, sc_sflag;
+    int *scan_blk = nullptr;
+    size_t cap_blk = 0;
+    DevCounters *ctr = nullptr, *hctr = nullptr;
+    int cap_ctr = 0;
+};
+static FlatBatchState g_flat;
+
+static void flat_free()
+{
+    FlatBatchState &F = g_flat;
+    void *d[] = {F.tab, F.gp.first, F.gb.first, F.gc.first, F.sc_cells.segs, F.sc_cells.tile0, F.sc_sflag.segs,
+                 F.sc_sflag.tile0, F.scan_blk, F.ctr};
+    for (void *p : d)
+        if (p) hipFree(p);
+    if (F.hctr) hipHostFree(F.hctr);
+    F = FlatBatchState();
+    flat_pack_free();
+}
+
+// Makes buffer index 0 current for X/V, T/O/... and P by swapping the
+// halves of each ping-pong pair (no data moves), so that scenes with
+// different histories share one parity.
+static void ctx_parity_zero(Ctx *c)
+{
+    if (c->cx) {
+        std::swap(c->X[0], c->X[1]);
+        std::swap(c->V[0], c->V[1]);
+        c->cx = 0;
+        graph_reset(c);
+    }
+    if (c->ct) {
+        std::swap(c->T[0], c->T[1]);
+        std::swap(c->O[0], c->O[1]);
+        std::swap(c->ACC[0], c->ACC[1]);
+        std::swap(c->ND[0], c->ND[1]);
+        std::swap(c->SURF[0], c->SURF[1]);
+        std::swap(c->NCOUNT[0], c->NCOUNT[1]);
+        c->ct = 0;
+        graph_reset(c);
+    }
+    if (c->cp) {
+        std::swap(c->P[0], c->P[1]);
+        c->cp = 0;
+        graph_reset(c);
+    }
+}
+
+// The scene table, block offsets, scan segments and counters of a batch.
+static int flat_setup(Ctx *const *cs, SimConfig *const *cfgs, int ns, hipStream_t s, FlatLaunch *f,
+                      std::vector<StepConsts> &ks)
+{
+    FlatBatchState &F = g_flat;
+    if (F.cap < ns) {
+        if (F.tab) hipFree(F.tab);
+        if (F.ctr) hipFree(F.ctr);
+        if (F.hctr) hipHostFree(F.hctr);
+        F.tab = nullptr;
+        F.ctr = F.hctr = nullptr;
+        HIP_OK(hipMalloc(&F.tab, sizeof(FlatScene) * ns));
+        HIP_OK(hipMalloc(&F.ctr, sizeof(DevCounters) * ns));
+        HIP_OK(hipHostMalloc(&F.hctr, sizeof(DevCounters) * ns, hipHostMallocDefault));
+        F.cap = ns;
+    }
+    F.h.resize(ns);
+    std::vector<int> bp(ns), bb(ns), bc(ns);
+    std::vector<ScanSeg> seg_c(ns), seg_f(ns);
+    const int pb = particle_block();
+    int any_cmps1 = 0;
+    for (int q = 0; q < ns; q++) {
+        Ctx *c = cs[q];
+        ctx_parity_zero(c);
+        StepConsts k = make_consts(cfgs[q], c->ps, c->grid, search_radius(cfgs[q]));
+        k.stride = c->stride;
+        ks[q] = k;
+        any_cmps1 |= k.cmps == 1;
+        FlatScene &t = F.h[q];
+        t.n = c->n;
+        t.total = k.grid.total;
+        for (int h = 0; h < 2; h++) {
+            t.X[h] = c->X[h]; t.V[h] = c->V[h]; t.ACC[h] = c->ACC[h];
+            t.T[h] = c->T[h]; t.O[h] = c->O[h]; t.SURF[h] = c->SURF[h]; t.NCOUNT[h] = c->NCOUNT[h];
+            t.P[h] = c->P[h]; t.ND[h] = c->ND[h];
+        }
+        t.unk = c->unk; t.pmin = c->pmin;
+        t.cell = c->cell; t.ccount = c->ccount; t.cstart = c->cstart; t.cfill = c->cfill;
+        t.tmp_key = c->tmp_key; t.sflag = c->sflag; t.sscan = c->sscan;
+        t.nbr = c->nbr; t.ncnt = c->ncnt;
+        t.diag = c->diag; t.b = c->b; t.Aval = c->Aval; t.Acol = c->Acol; t.Alen = c->Alen;
+        t.x = c->x;
+        t.ctr = c->counters;
+        t.k = k;
+        bp[q] = (c->n + pb - 1) / pb;
+        bb[q] = (c->n + 255) / 256;
+        bc[q] = (k.grid.total + 1 + 255) / 256;
+        seg_c[q] = ScanSeg{c->ccount, c->cstart, k.grid.total + 1, nullptr};
+        seg_f[q] = ScanSeg{c->sflag, c->sscan, c->n, nullptr};
+    }
+    HIP_OK(hipMemcpyAsync(F.tab, F.h.data(), sizeof(FlatScene) * ns, hipMemcpyHostToDevice, s));
+    int rc = flat_build_grid(&F.gp, bp.data(), ns, s);
+    if (!rc) rc = flat_build_grid(&F.gb, bb.data(), ns, s);
+    if (!rc) rc = flat_build_grid(&F.gc, bc.data(), ns, s);
+    if (!rc) rc = scan_segs_set(&F.sc_cells, seg_c.data(), ns, s);
+    if (!rc) rc = scan_segs_set(&F.sc_sflag, seg_f.data(), ns, s);
+    if (rc) return rc;
+    const size_t need = (size_t)std::max(F.sc_cells.ntiles, F.sc_sflag.ntiles) + ns + 64;
+    if (F.cap_blk < need) {
+        if (F.scan_blk) hipFree(F.scan_blk);
+        F.scan_blk = nullptr;
+        HIP_OK(hipMalloc(&F.scan_blk, sizeof(int) * need));
+        F.cap_blk = need;
+    }
+    *f = FlatLaunch{F.tab, ns, F.gp, F.gb, F.gc, F.sc_cells, F.sc_sflag, F.scan_blk, any_cmps1, 0, 0, 0, s};
+    return 0;
+}
+
+// nsteps timesteps of every scene, each operator one launch for all
+// scenes; the CG (FP64 or mixed) batched as in the per-scene path.
+static int run_batch_flat(Ctx *const *cs, SimConfig *const *cfgs, int ns, int nsteps)
+{
+    hipStream_t s = cs[0]->stream;
+    // work a caller queued on the scenes' own streams comes first
+    for (int q = 0; q < ns; q++) HIP_OK(hipStreamSynchronize(cs[q]->stream));
+    std::vector<StepConsts> ks(ns);
+    std::vector<SolveOpts> so(ns);
+    std::vector<SolveResult> sr(ns);
+    FlatLaunch f;
+    int rc = flat_setup(cs, cfgs, ns, s, &f, ks);
+    if (rc) return rc;
+    for (int q = 0; q < ns; q++) so[q] = solve_opts(cfgs[q], ks[q]);
+    const int mixed = so[0].precision >= 1 && so[0].precision <= 3;
+    for (int step = 0; step < nsteps && !rc; step++) {
+        rc = flat_step_pre(f);
+        for (int q = 0; q < ns && !rc; q++) rc = cg_alloc_ctrl(cs[q]);
+        if (!rc) rc = flat_pack(cs, ns, f.gb, mixed, s);
+        if (!rc)
+            rc = mixed ? batch_cg_mixed(cs, ns, so.data(), sr.data(), s) : batch_cg(cs, ns, so.data(), sr.data(), s);
+        if (!rc) rc = flat_unpack(cs, ns, f.gb, s);
+        if (!rc) rc = flat_step_post(f);
+        if (!rc) rc = flat_counters(f, g_flat.ctr);
+        if (rc) break;
+        HIP_OK(hipMemcpyAsync(g_flat.hctr, g_flat.ctr, sizeof(DevCounters) * ns, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        for (int q = 0; q < ns && !rc; q++) {
+            Ctx *c = cs[q];
+            c->cx = f.cx;
+            c->ct = f.ct;
+            c->cp = f.cp;
+            c->list_valid = true;
+            *c->hcounters = g_flat.hctr[q];
+            rc = step_bookkeep(c, ks[q], sr[q]);
+        }
+    }
+    return rc;
+}
+
+// ---------------------------------------------------------------------
 // C-ABI
 // ---------------------------------------------------------------------
 
@@ -837,6 +1014,7 @@ void mps_hip_finalize(void)
     }
     g_ctx.clear();
     cg_batch_free();
+    flat_free();
     g_writer.release();
 }
 
@@ -895,6 +1073,10 @@ int mps_hip_set_option(const char *key, double value)
         g_opt.ic_waves = value;
     }
     else if (k == "ic_fgrid") g_opt.ic_fgrid = value;
+    else if (k == "batch_flat") {
+        if (value != 0 && value != 1) return MPS_ERR_ARG;
+        g_opt.batch_flat = value;
+    }
     else if (k == "search") {
         if (value != 0 && value != 1) return MPS_ERR_ARG;
         if (value != g_opt.search)
@@ -939,6 +1121,7 @@ double mps_hip_get_option(const char *key)
     if (k == "ic_factor_form") return g_opt.ic_factor_form;
     if (k == "ic_fgrid") return g_opt.ic_fgrid;
     if (k == "search") return g_opt.search;
+    if (k == "batch_flat") return g_opt.batch_flat;
     return NAN;
 }
 
@@ -1021,6 +1204,15 @@ int mps_hip_run_batch(ParticleSystem *const *ps, SimConfig *const *cfgs, int nsc
     // CG (FP64 or mixed) is solved batched; ICCG runs per scene
     bool batched = true;
     for (int q = 0; q < nscenes; q++) batched = batched && cfgs[q]->solver_type == 0;
+#if MPS_DIM == 3 && !defined(MPS_FRICTION_FIELDS)
+    if (batched && g_opt.batch_flat > 0.5 && !tape_recording()) {
+        SimConfig *saved = g_config;
+        rc = run_batch_flat(cs.data(), cfgs, nscenes, nsteps);
+        g_config = saved;
+        if (rc) g_last_error = rc;
+        return rc;
+    }
+#endif
     std::vector<StepConsts> ks(nscenes);
     std::vector<SolveOpts> so(nscenes);
     std::vector<SolveResult> sr(nscenes);

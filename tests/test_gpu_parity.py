@@ -384,20 +384,32 @@ def test_search_forms_same_bits(request, name, libname):
             assert a == b, f
 
 
+@pytest.mark.parametrize("flat", [1, 0])
 @pytest.mark.parametrize("precision", [0, 1, 2, 3])
-def test_batch_equals_single_runs(lib3, precision):
-    """mps_hip_run_batch (particle operators per scene, one batched CG for
-    all scenes: FP64, or the mixed forms of precisions 1-3) leaves every
-    scene with the bits and CG iteration counts of its own single-scene
-    run: scenes of different sizes and viscosities."""
+def test_batch_equals_single_runs(lib3, precision, flat):
+    """mps_hip_run_batch leaves every scene with the bits and CG iteration
+    counts of its own single-scene run (scenes of different sizes and
+    viscosities), with one launch per operator for all scenes (option
+    "batch_flat" 1, the default) and with per-scene launches (0); one
+    batched CG for all scenes in both: FP64, or the mixed forms of
+    precisions 1-3."""
     lib3.set_option("precision", precision)
+    lib3.set_option("batch_flat", flat)
     try:
         _batch_equals_single(lib3)
     finally:
         lib3.set_option("precision", 0)
+        lib3.set_option("batch_flat", 1)
 
 
-def _batch_equals_single(lib3):
+def test_batch_after_single_steps(lib3):
+    """Scenes that took different numbers of single-scene steps first (so
+    their buffer halves differ) then run batched: each ends with the bits
+    of the same number of steps taken alone."""
+    _batch_equals_single(lib3, pre=(1, 0, 2))
+
+
+def _batch_equals_single(lib3, pre=(0, 0, 0)):
     import ctypes as C
     from mps_amd import params
     scenes_w = [dict(dims=(0.4, 0.3, 0.12, 0.12, 0.2, 0.12), l0=0.01, dt=5e-4, nu=1e-6),
@@ -425,12 +437,16 @@ def _batch_equals_single(lib3):
         cfg, ps, n = make(w)
         lib3.set_config(cfg)
         say("single scene", n, "nu", w["nu"])
-        lib3.check(lib3.mps_hip_run_steps(ps, steps), "run_steps")
+        lib3.check(lib3.mps_hip_run_steps(ps, steps + pre[len(single)]), "run_steps")
         lib3.check(lib3.mps_hip_download(ps, None), "download")
         single.append((fields(ps, n), lib3.stats(ps).total_iters))
         lib3.mps_hip_release(ps)
         lib3.particle_system_free(ps)
     made = [make(w) for w in scenes_w]
+    for (cfg, ps, n), k in zip(made, pre):
+        if k:
+            lib3.set_config(cfg)
+            lib3.check(lib3.mps_hip_run_steps(ps, k), "steps before the batch")
     cfgs = [m[0] for m in made]
     PSP = C.POINTER(lib3.abi.ParticleSystem)
     ps_arr = (PSP * len(made))(*[m[1] for m in made])
@@ -442,7 +458,7 @@ def _batch_equals_single(lib3):
         got = fields(ps, n)
         for f in H.FIELDS:
             assert got[f] == single[q][0][f], (q, f)
-        assert lib3.stats(ps).steps == steps
+        assert lib3.stats(ps).steps == steps + pre[q]
         assert lib3.stats(ps).total_iters == single[q][1]
         lib3.mps_hip_release(ps)
         lib3.particle_system_free(ps)
