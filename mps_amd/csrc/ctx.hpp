@@ -291,6 +291,7 @@ struct SolveOpts {
     int dense_ic;      // 2-D dense-formula IC(0)
     int spmv_variant;  // entries per load batch in the SpMV
     int fused;         // FP64 CG in two kernels per iteration
+    int graph;         // batched CG: replay one captured chunk of iterations
     // the PPE's off-diagonal A_ij = neg_coeff w(r_ij) (w of re_lap, r
     // clamped at r_min), for the distance-coded operator (precision 3)
     double neg_coeff, re_lap, r_min;

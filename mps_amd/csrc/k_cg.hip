@@ -1576,10 +1576,11 @@ k_b_spmv(const BScene *__restrict__ sc, const int *__restrict__ bs, double *part
 }
 
 __global__ void __launch_bounds__(CG_BLK)
-k_b_spmv_fin(const BScene *__restrict__ sc, const double *part, CgCtrl *ctrl, int t, int *ndone)
+k_b_spmv_fin(const BScene *__restrict__ sc, const double *part, CgCtrl *ctrl, int *ndone)
 {
     CgCtrl *c = ctrl + blockIdx.x;
     if (c->done) return;
+    const int t = ndone[1];
     const BScene &S = sc[blockIdx.x];
     const double pap = fin_sum(part + S.blk_off, S.nblk);
     if (threadIdx.x == 0) {
@@ -1635,10 +1636,11 @@ k_b_axpy(const BScene *__restrict__ sc, const int *__restrict__ axs, double *par
 }
 
 __global__ void __launch_bounds__(CG_BLK)
-k_b_axpy_fin(const BScene *__restrict__ sc, const double *part2, CgCtrl *ctrl, int t, int *ndone)
+k_b_axpy_fin(const BScene *__restrict__ sc, const double *part2, CgCtrl *ctrl, int *ndone)
 {
     CgCtrl *c = ctrl + blockIdx.x;
     if (c->done) return;
+    const int t = ndone[1];
     const BScene &S = sc[blockIdx.x];
     const double rr_new = fin_sum(part2 + S.ax_off, S.nax);
     if (threadIdx.x == 0) {
@@ -1667,8 +1669,11 @@ k_b_axpy_fin(const BScene *__restrict__ sc, const double *part2, CgCtrl *ctrl, i
 }
 
 __global__ void __launch_bounds__(CG_BLK)
-k_b_pupdate(const BScene *__restrict__ sc, const int *__restrict__ axs, const CgCtrl *ctrl)
+k_b_pupdate(const BScene *__restrict__ sc, const int *__restrict__ axs, const CgCtrl *ctrl, int *ndone)
 {
+    // the iteration's last kernel advances the iteration index (no other
+    // kernel of the iteration reads it while this one runs)
+    if (blockIdx.x == 0 && threadIdx.x == 0) ndone[1] += 1;
     const int si = axs[blockIdx.x];
     if (ctrl[si].done) return;
     const BScene &S = sc[si];
@@ -1715,6 +1720,63 @@ static int reserve_pair(T **d, T **h, int *cap, int need)
     return 0;
 }
 
+// The iteration loop of a batched solve: iterations in chunks of `chunk`
+// (iter() enqueues one), with the count of stopped scenes (dndone[0])
+// copied back after each chunk and read one chunk later, until all ns
+// scenes have stopped.  The kernels take the iteration index from
+// dndone[1], which each iteration's last kernel advances, so with `graph`
+// one chunk is captured once and the same graph is replayed for every
+// chunk; scenes that have stopped return at once in either form, so both
+// give the same bits.
+template <typename F>
+static int batch_loop(hipStream_t s, int max_iter, int chunk, int ns, const int *dndone, int *hndone, bool graph,
+                      F iter)
+{
+    hipGraphExec_t ge = nullptr;
+    if (graph) {
+        HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        for (int u = 0; u < chunk; u++) iter();
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        if (e != hipSuccess || hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            mps_set_error(MPS_ERR_HIP, __FILE__, __LINE__, "capture of the batched CG iterations failed");
+            return MPS_ERR_HIP;
+        }
+        hipGraphDestroy(g);
+    }
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int rc = 0;
+    if (hipEventCreateWithFlags(&ev[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ev[1], hipEventDisableTiming) != hipSuccess)
+        rc = MPS_ERR_HIP;
+    int pending = -1, t = 0, k = 0;
+    while (!rc && t < max_iter) {
+        if (ge) {
+            if (hipGraphLaunch(ge, s) != hipSuccess) rc = MPS_ERR_HIP;
+            t += chunk;
+        } else {
+            const int t_end = (t + chunk < max_iter) ? t + chunk : max_iter;
+            for (; t < t_end; t++) iter();
+        }
+        if (!rc && (hipGetLastError() != hipSuccess ||
+                    hipMemcpyAsync(hndone, dndone, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                    hipEventRecord(ev[k & 1], s) != hipSuccess))
+            rc = MPS_ERR_HIP;
+        if (!rc && pending >= 0) {
+            if (hipEventSynchronize(ev[pending & 1]) != hipSuccess) rc = MPS_ERR_HIP;
+            else if (*(volatile int *)hndone >= ns) break;
+        }
+        pending = k;
+        k++;
+    }
+    for (hipEvent_t e : ev)
+        if (e) hipEventDestroy(e);
+    if (ge) hipGraphExecDestroy(ge);
+    if (rc) mps_set_error(MPS_ERR_HIP, __FILE__, __LINE__, "batched CG iterations");
+    return rc;
+}
+
 // Solves the packed systems of cs[0..ns) (each already packed by pack_ppe)
 // with FP64 CG on stream s; solutions stay in each scene's packed x.
 int batch_cg(Ctx *const *cs, int ns, const SolveOpts *opts, SolveResult *res, hipStream_t s)
@@ -1748,7 +1810,7 @@ int batch_cg(Ctx *const *cs, int ns, const SolveOpts *opts, SolveResult *res, hi
         B.cap_part2 = B.cap_ax;
     }
     if (!B.ndone) {
-        HIP_OK(hipMalloc(&B.ndone, sizeof(int)));
+        HIP_OK(hipMalloc(&B.ndone, 2 * sizeof(int)));   // [0] stopped scenes, [1] iteration
         HIP_OK(hipHostMalloc(&B.hndone, sizeof(int), hipHostMallocDefault));
     }
     int bo = 0, ao = 0;
@@ -1778,7 +1840,7 @@ int batch_cg(Ctx *const *cs, int ns, const SolveOpts *opts, SolveResult *res, hi
     HIP_OK(hipMemcpyAsync(B.sc, B.hsc, sizeof(BScene) * ns, hipMemcpyHostToDevice, s));
     HIP_OK(hipMemcpyAsync(B.bs, B.hbs, sizeof(int) * (tot_blk + 1), hipMemcpyHostToDevice, s));
     HIP_OK(hipMemcpyAsync(B.axs, B.haxs, sizeof(int) * (tot_ax + 1), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_fill_zero_int, dim3(1), dim3(64), 0, s, B.ndone, 1);
+    hipLaunchKernelGGL(k_fill_zero_int, dim3(1), dim3(64), 0, s, B.ndone, 2);
     if (tot_ax > 0) hipLaunchKernelGGL(k_b_init, dim3(tot_ax), dim3(CG_BLK), 0, s, B.sc, B.axs, B.part2);
     hipLaunchKernelGGL(k_b_init_fin, dim3(ns), dim3(CG_BLK), 0, s, B.sc, B.part2, B.ctrl, B.ndone);
     LAUNCH_CHECK();
@@ -1786,33 +1848,17 @@ int batch_cg(Ctx *const *cs, int ns, const SolveOpts *opts, SolveResult *res, hi
     const spmv_fn_b spmv = opts[0].spmv_variant == 1 ? k_b_spmv<16, true>
                                                      : (opts[0].spmv_variant == 2 ? k_b_spmv<16, false> : k_b_spmv<8, true>);
     // the host stops when every scene has stopped: watch ndone == ns
-    hipEvent_t ev[2];
-    HIP_OK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-    HIP_OK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-    int pending = -1, t = 0, k = 0;
-    while (t < max_iter) {
-        const int t_end = (t + check_every < max_iter) ? t + check_every : max_iter;
-        for (; t < t_end; t++) {
-            prof_begin(PK_SPMV, s);
-            if (tot_blk > 0) hipLaunchKernelGGL(spmv, dim3(tot_blk), dim3(CG_BLK), lds, s, B.sc, B.bs, B.part, B.ctrl);
-            prof_end(PK_SPMV, s);
-            hipLaunchKernelGGL(k_b_spmv_fin, dim3(ns), dim3(CG_BLK), 0, s, B.sc, B.part, B.ctrl, t, B.ndone);
-            if (tot_ax > 0) hipLaunchKernelGGL(k_b_axpy, dim3(tot_ax), dim3(CG_BLK), 0, s, B.sc, B.axs, B.part2, B.ctrl);
-            hipLaunchKernelGGL(k_b_axpy_fin, dim3(ns), dim3(CG_BLK), 0, s, B.sc, B.part2, B.ctrl, t, B.ndone);
-            if (tot_ax > 0) hipLaunchKernelGGL(k_b_pupdate, dim3(tot_ax), dim3(CG_BLK), 0, s, B.sc, B.axs, B.ctrl);
-        }
-        LAUNCH_CHECK();
-        HIP_OK(hipMemcpyAsync(B.hndone, B.ndone, sizeof(int), hipMemcpyDeviceToHost, s));
-        HIP_OK(hipEventRecord(ev[k & 1], s));
-        if (pending >= 0) {
-            HIP_OK(hipEventSynchronize(ev[pending & 1]));
-            if (*(volatile int *)B.hndone >= ns) break;
-        }
-        pending = k;
-        k++;
-    }
-    hipEventDestroy(ev[0]);
-    hipEventDestroy(ev[1]);
+    int rc2 = batch_loop(s, max_iter, check_every, ns, B.ndone, B.hndone, opts[0].graph, [&]() {
+        prof_begin(PK_SPMV, s);
+        if (tot_blk > 0) hipLaunchKernelGGL(spmv, dim3(tot_blk), dim3(CG_BLK), lds, s, B.sc, B.bs, B.part, B.ctrl);
+        prof_end(PK_SPMV, s);
+        hipLaunchKernelGGL(k_b_spmv_fin, dim3(ns), dim3(CG_BLK), 0, s, B.sc, B.part, B.ctrl, B.ndone);
+        if (tot_ax > 0) hipLaunchKernelGGL(k_b_axpy, dim3(tot_ax), dim3(CG_BLK), 0, s, B.sc, B.axs, B.part2, B.ctrl);
+        hipLaunchKernelGGL(k_b_axpy_fin, dim3(ns), dim3(CG_BLK), 0, s, B.sc, B.part2, B.ctrl, B.ndone);
+        if (tot_ax > 0)
+            hipLaunchKernelGGL(k_b_pupdate, dim3(tot_ax), dim3(CG_BLK), 0, s, B.sc, B.axs, B.ctrl, B.ndone);
+    });
+    if (rc2) return rc2;
     HIP_OK(hipStreamSynchronize(s));
     HIP_OK(hipMemcpy(B.hctrl, B.ctrl, sizeof(CgCtrl) * ns, hipMemcpyDeviceToHost));
     for (int q = 0; q < ns; q++) {
@@ -1943,11 +1989,12 @@ k_bm_spmv(const BMScene<V> *__restrict__ sc, const int *__restrict__ bs, double 
 
 template <typename V>
 __global__ void __launch_bounds__(CG_BLK)
-k_bm_fin1(const BMScene<V> *__restrict__ sc, const double *part, const double *part_ru, MixedCtrl *ctrl, int t,
+k_bm_fin1(const BMScene<V> *__restrict__ sc, const double *part, const double *part_ru, MixedCtrl *ctrl,
           int *ndone)
 {
     MixedCtrl *c = ctrl + blockIdx.x;
     if (c->done) return;
+    const int t = ndone[1];
     const BMScene<V> &S = sc[blockIdx.x];
     const double pap = fin_sum(part + S.blk_off, S.nblk);
     const int ru = c->ru;
@@ -2022,10 +2069,11 @@ k_bm_axpy(const BMScene<V> *__restrict__ sc, const int *__restrict__ axs, double
 
 template <typename V>
 __global__ void __launch_bounds__(CG_BLK)
-k_bm_fin2(const BMScene<V> *__restrict__ sc, const double *part2, MixedCtrl *ctrl, int t, int *ndone)
+k_bm_fin2(const BMScene<V> *__restrict__ sc, const double *part2, MixedCtrl *ctrl, int *ndone)
 {
     MixedCtrl *c = ctrl + blockIdx.x;
     if (c->done) return;
+    const int t = ndone[1];
     const BMScene<V> &S = sc[blockIdx.x];
     const double rr_new = fin_sum(part2 + S.ax_off, S.nax);
     if (threadIdx.x == 0) {
@@ -2045,8 +2093,12 @@ k_bm_fin2(const BMScene<V> *__restrict__ sc, const double *part2, MixedCtrl *ctr
 
 template <int U, typename V>
 __global__ void __launch_bounds__(CG_BLK)
-k_bm_pupdate(const BMScene<V> *__restrict__ sc, const int *__restrict__ pbs, double *part_ru, const MixedCtrl *ctrl)
+k_bm_pupdate(const BMScene<V> *__restrict__ sc, const int *__restrict__ pbs, double *part_ru, const MixedCtrl *ctrl,
+             int *ndone)
 {
+    // the iteration's last kernel advances the iteration index (no other
+    // kernel of the iteration reads it while this one runs)
+    if (blockIdx.x == 0 && threadIdx.x == 0) ndone[1] += 1;
     const int si = pbs[blockIdx.x];
     if (ctrl[si].done) return;
     const BMScene<V> &S = sc[si];
@@ -2103,7 +2155,7 @@ static int batch_mixed_run(Ctx *const *cs, int ns, const SolveOpts *opts, SolveR
     if (!rc) rc = reserve_dev(&M.part3, &M.cap_part3, tot_ax + 1);
     if (rc) return rc;
     if (!B.ndone) {
-        HIP_OK(hipMalloc(&B.ndone, sizeof(int)));
+        HIP_OK(hipMalloc(&B.ndone, 2 * sizeof(int)));   // [0] stopped scenes, [1] iteration
         HIP_OK(hipHostMalloc(&B.hndone, sizeof(int), hipHostMallocDefault));
     }
     BMScene<V> *hsc = reinterpret_cast<BMScene<V> *>(M.hsc);
@@ -2156,7 +2208,7 @@ static int batch_mixed_run(Ctx *const *cs, int ns, const SolveOpts *opts, SolveR
     HIP_OK(hipMemcpyAsync(M.pbs, M.hpbs, sizeof(int) * (tot_pb + 1), hipMemcpyHostToDevice, s));
     HIP_OK(hipMemcpyAsync(B.bs, B.hbs, sizeof(int) * (tot_blk + 1), hipMemcpyHostToDevice, s));
     HIP_OK(hipMemcpyAsync(B.axs, B.haxs, sizeof(int) * (tot_ax + 1), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_fill_zero_int, dim3(1), dim3(64), 0, s, B.ndone, 1);
+    hipLaunchKernelGGL(k_fill_zero_int, dim3(1), dim3(64), 0, s, B.ndone, 2);
     if (coded && tot_blk > 0)
         hipLaunchKernelGGL(k_bm_q16_build, dim3(tot_blk), dim3(CG_BLK), 0, s,
                            reinterpret_cast<const BMScene<float> *>(dsc), B.bs);
@@ -2171,43 +2223,22 @@ static int batch_mixed_run(Ctx *const *cs, int ns, const SolveOpts *opts, SolveR
             spmv = sv == 1 ? k_bm_spmv<16, V, true, true> : (sv == 2 ? k_bm_spmv<16, V, true, false> : k_bm_spmv<8, V, true, true>);
     const size_t lds = sizeof(V) * (size_t)hmax;
     // the host stops when every scene has stopped: watch ndone == ns
-    hipEvent_t ev[2];
-    HIP_OK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-    HIP_OK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-    int pending = -1, t = 0, k = 0;
-    hipError_t e = hipSuccess;
-    while (t < max_iter && e == hipSuccess) {
-        const int t_end = (t + check_every < max_iter) ? t + check_every : max_iter;
-        for (; t < t_end; t++) {
-            prof_begin(PK_SPMV, s);
-            if (tot_blk > 0) hipLaunchKernelGGL(spmv, dim3(tot_blk), dim3(CG_BLK), lds, s, dsc, B.bs, B.part, M.ctrl);
-            prof_end(PK_SPMV, s);
-            hipLaunchKernelGGL(k_bm_fin1<V>, dim3(ns), dim3(CG_BLK), 0, s, dsc, B.part, M.part_ru, M.ctrl, t, B.ndone);
-            prof_begin(PK_AXPY, s);
-            if (tot_ax > 0) hipLaunchKernelGGL(k_bm_axpy<V>, dim3(tot_ax), dim3(CG_BLK), 0, s, dsc, B.axs, B.part2, M.ctrl);
-            prof_end(PK_AXPY, s);
-            hipLaunchKernelGGL(k_bm_fin2<V>, dim3(ns), dim3(CG_BLK), 0, s, dsc, B.part2, M.ctrl, t, B.ndone);
-            prof_begin(PK_PUPDATE, s);
-            if (tot_pb > 0)
-                hipLaunchKernelGGL((k_bm_pupdate<8, V>), dim3(tot_pb), dim3(CG_BLK), 0, s, dsc, M.pbs, M.part_ru, M.ctrl);
-            prof_end(PK_PUPDATE, s);
-        }
-        e = hipGetLastError();
-        if (e == hipSuccess) e = hipMemcpyAsync(B.hndone, B.ndone, sizeof(int), hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipEventRecord(ev[k & 1], s);
-        if (e == hipSuccess && pending >= 0) {
-            e = hipEventSynchronize(ev[pending & 1]);
-            if (e == hipSuccess && *(volatile int *)B.hndone >= ns) break;
-        }
-        pending = k;
-        k++;
-    }
-    hipEventDestroy(ev[0]);
-    hipEventDestroy(ev[1]);
-    if (e != hipSuccess) {
-        mps_set_error(MPS_ERR_HIP, __FILE__, __LINE__, hipGetErrorString(e));
-        return MPS_ERR_HIP;
-    }
+    rc = batch_loop(s, max_iter, check_every, ns, B.ndone, B.hndone, opts[0].graph, [&]() {
+        prof_begin(PK_SPMV, s);
+        if (tot_blk > 0) hipLaunchKernelGGL(spmv, dim3(tot_blk), dim3(CG_BLK), lds, s, dsc, B.bs, B.part, M.ctrl);
+        prof_end(PK_SPMV, s);
+        hipLaunchKernelGGL(k_bm_fin1<V>, dim3(ns), dim3(CG_BLK), 0, s, dsc, B.part, M.part_ru, M.ctrl, B.ndone);
+        prof_begin(PK_AXPY, s);
+        if (tot_ax > 0) hipLaunchKernelGGL(k_bm_axpy<V>, dim3(tot_ax), dim3(CG_BLK), 0, s, dsc, B.axs, B.part2, M.ctrl);
+        prof_end(PK_AXPY, s);
+        hipLaunchKernelGGL(k_bm_fin2<V>, dim3(ns), dim3(CG_BLK), 0, s, dsc, B.part2, M.ctrl, B.ndone);
+        prof_begin(PK_PUPDATE, s);
+        if (tot_pb > 0)
+            hipLaunchKernelGGL((k_bm_pupdate<8, V>), dim3(tot_pb), dim3(CG_BLK), 0, s, dsc, M.pbs, M.part_ru, M.ctrl,
+                               B.ndone);
+        prof_end(PK_PUPDATE, s);
+    });
+    if (rc) return rc;
     HIP_OK(hipStreamSynchronize(s));
     HIP_OK(hipMemcpy(M.hctrl, M.ctrl, sizeof(MixedCtrl) * ns, hipMemcpyDeviceToHost));
     for (int q = 0; q < ns; q++) {

@@ -739,6 +739,7 @@ static SolveOpts solve_opts(const SimConfig *cf, const StepConsts &k)
     so.dense_ic = (D_ == 2);
     so.spmv_variant = (int)g_opt.spmv;
     so.fused = (int)g_opt.fused;
+    so.graph = g_opt.graph > 0.5 && g_opt.profile < 0.5 && !tape_recording();
     so.neg_coeff = k.neg_coeff;
     so.re_lap = k.re_lap;
     so.r_min = k.r_min;
@@ -832,7 +833,15 @@ int flat_pack(Ctx *const *cs, int ns, const FlatGrid &gb, int with_f32, hipStrea
 int flat_unpack(Ctx *const *cs, int ns, const FlatGrid &gb, hipStream_t s);
 void flat_pack_free();
 
+// A captured phase of the flat step, valid from the buffer parity it was
+// captured at (every step flips the parities, so two captures alternate).
+struct FlatGraph {
+    hipGraphExec_t exec = nullptr;
+    int in[3] = {0, 0, 0}, out[3] = {0, 0, 0};
+};
+
 struct FlatBatchState {
+    FlatGraph pre[2], post[2];
     std::vector<FlatScene> h;
     FlatScene *tab = nullptr;
     int cap = 0;
@@ -845,8 +854,18 @@ struct FlatBatchState {
 };
 static FlatBatchState g_flat;
 
+static void flat_graphs_drop()
+{
+    for (FlatGraph *gs : {g_flat.pre, g_flat.post})
+        for (int q = 0; q < 2; q++) {
+            if (gs[q].exec) hipGraphExecDestroy(gs[q].exec);
+            gs[q] = FlatGraph();
+        }
+}
+
 static void flat_free()
 {
+    flat_graphs_drop();
     FlatBatchState &F = g_flat;
     void *d[] = {F.tab, F.gp.first, F.gb.first, F.gc.first, F.sc_cells.segs, F.sc_cells.tile0, F.sc_sflag.segs,
                  F.sc_sflag.tile0, F.scan_blk, F.ctr};
@@ -890,6 +909,7 @@ static int flat_setup(Ctx *const *cs, SimConfig *const *cfgs, int ns, hipStream_
                       std::vector<StepConsts> &ks)
 {
     FlatBatchState &F = g_flat;
+    flat_graphs_drop();   // their launches hold this call's tables and sizes
     if (F.cap < ns) {
         if (F.tab) hipFree(F.tab);
         if (F.ctr) hipFree(F.ctr);
@@ -953,8 +973,66 @@ static int flat_setup(Ctx *const *cs, SimConfig *const *cfgs, int ns, hipStream_
     return 0;
 }
 
+// The phase after the solve: unpacked solution, pressure, gradient,
+// corrector and boundary, and the counters gathered in scene order.
+static int flat_post_all(FlatLaunch &f)
+{
+    int rc = flat_unpack(nullptr, f.ns, f.gb, f.s);
+    if (!rc) rc = flat_step_post(f);
+    if (!rc) rc = flat_counters(f, g_flat.ctr);
+    return rc;
+}
+
+// Runs phase fn from f's parity: replays the capture made from the same
+// parity, or captures it into the slot of that parity and replays it; f's
+// parity then moves as fn's launches move it.
+static int flat_graph(FlatGraph *slots, FlatLaunch &f, int (*fn)(FlatLaunch &))
+{
+    const int in[3] = {f.cx, f.ct, f.cp};
+    FlatGraph *hit = nullptr;
+    for (int q = 0; q < 2; q++)
+        if (slots[q].exec && memcmp(slots[q].in, in, sizeof in) == 0) hit = &slots[q];
+    if (!hit) {
+        // the two parities a phase starts from in turn differ in ct
+        FlatGraph &g = slots[f.ct & 1];
+        if (g.exec) hipGraphExecDestroy(g.exec);
+        g = FlatGraph();
+        HIP_OK(hipStreamBeginCapture(f.s, hipStreamCaptureModeThreadLocal));
+        int rc = fn(f);
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(f.s, &gr);
+        const int out[3] = {f.cx, f.ct, f.cp};
+        f.cx = in[0];
+        f.ct = in[1];
+        f.cp = in[2];
+        if (!rc && e != hipSuccess) {
+            mps_set_error(MPS_ERR_HIP, __FILE__, __LINE__, hipGetErrorString(e));
+            rc = MPS_ERR_HIP;
+        }
+        if (!rc && hipGraphInstantiate(&g.exec, gr, nullptr, nullptr, 0) != hipSuccess) {
+            g.exec = nullptr;
+            mps_set_error(MPS_ERR_HIP, __FILE__, __LINE__, "hipGraphInstantiate failed");
+            rc = MPS_ERR_HIP;
+        }
+        if (gr) hipGraphDestroy(gr);
+        if (rc) return rc;
+        memcpy(g.in, in, sizeof in);
+        memcpy(g.out, out, sizeof out);
+        hit = &g;
+    }
+    HIP_OK(hipGraphLaunch(hit->exec, f.s));
+    f.cx = hit->out[0];
+    f.ct = hit->out[1];
+    f.cp = hit->out[2];
+    return 0;
+}
+
 // nsteps timesteps of every scene, each operator one launch for all
-// scenes; the CG (FP64 or mixed) batched as in the per-scene path.
+// scenes; the CG (FP64 or mixed) batched as in the per-scene path.  With
+// option "graph" the phases before and after the solve replay captured
+// graphs and the CG replays one captured chunk of iterations; only the
+// packing's two waits for sizes (which decide allocations) and the CG's
+// polls of its stop counter return to the host.
 static int run_batch_flat(Ctx *const *cs, SimConfig *const *cfgs, int ns, int nsteps)
 {
     hipStream_t s = cs[0]->stream;
@@ -968,15 +1046,14 @@ static int run_batch_flat(Ctx *const *cs, SimConfig *const *cfgs, int ns, int ns
     if (rc) return rc;
     for (int q = 0; q < ns; q++) so[q] = solve_opts(cfgs[q], ks[q]);
     const int mixed = so[0].precision >= 1 && so[0].precision <= 3;
+    const bool graph = g_opt.graph > 0.5 && g_opt.profile < 0.5;
     for (int step = 0; step < nsteps && !rc; step++) {
-        rc = flat_step_pre(f);
+        rc = graph ? flat_graph(g_flat.pre, f, flat_step_pre) : flat_step_pre(f);
         for (int q = 0; q < ns && !rc; q++) rc = cg_alloc_ctrl(cs[q]);
         if (!rc) rc = flat_pack(cs, ns, f.gb, mixed, s);
         if (!rc)
             rc = mixed ? batch_cg_mixed(cs, ns, so.data(), sr.data(), s) : batch_cg(cs, ns, so.data(), sr.data(), s);
-        if (!rc) rc = flat_unpack(cs, ns, f.gb, s);
-        if (!rc) rc = flat_step_post(f);
-        if (!rc) rc = flat_counters(f, g_flat.ctr);
+        if (!rc) rc = graph ? flat_graph(g_flat.post, f, flat_post_all) : flat_post_all(f);
         if (rc) break;
         HIP_OK(hipMemcpyAsync(g_flat.hctr, g_flat.ctr, sizeof(DevCounters) * ns, hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));

@@ -384,22 +384,25 @@ def test_search_forms_same_bits(request, name, libname):
             assert a == b, f
 
 
-@pytest.mark.parametrize("flat", [1, 0])
+@pytest.mark.parametrize("flat,graph", [(1, 0), (1, 1), (0, 0)])
 @pytest.mark.parametrize("precision", [0, 1, 2, 3])
-def test_batch_equals_single_runs(lib3, precision, flat):
+def test_batch_equals_single_runs(lib3, precision, flat, graph):
     """mps_hip_run_batch leaves every scene with the bits and CG iteration
     counts of its own single-scene run (scenes of different sizes and
-    viscosities), with one launch per operator for all scenes (option
-    "batch_flat" 1, the default) and with per-scene launches (0); one
-    batched CG for all scenes in both: FP64, or the mixed forms of
+    viscosities): with one launch per operator for all scenes (option
+    "batch_flat" 1, the default), also with the step replayed from
+    captured graphs (option "graph": the phases before and after the solve,
+    and a chunk of CG iterations), and with per-scene launches (0); one
+    batched CG for all scenes in every form: FP64, or the mixed forms of
     precisions 1-3."""
     lib3.set_option("precision", precision)
     lib3.set_option("batch_flat", flat)
     try:
-        _batch_equals_single(lib3)
+        _batch_equals_single(lib3, graph=graph)
     finally:
         lib3.set_option("precision", 0)
         lib3.set_option("batch_flat", 1)
+        lib3.set_option("graph", 0)
 
 
 def test_batch_after_single_steps(lib3):
@@ -409,13 +412,13 @@ def test_batch_after_single_steps(lib3):
     _batch_equals_single(lib3, pre=(1, 0, 2))
 
 
-def _batch_equals_single(lib3, pre=(0, 0, 0)):
+def _batch_equals_single(lib3, pre=(0, 0, 0), graph=0):
     import ctypes as C
     from mps_amd import params
     scenes_w = [dict(dims=(0.4, 0.3, 0.12, 0.12, 0.2, 0.12), l0=0.01, dt=5e-4, nu=1e-6),
                 dict(dims=(0.6, 0.4, 0.2, 0.2, 0.3, 0.2), l0=0.01, dt=5e-4, nu=3e-5),
                 dict(dims=(0.4, 0.3, 0.12, 0.2, 0.15, 0.12), l0=0.01, dt=5e-4, nu=1e-4)]
-    steps = 3
+    steps = 4
     lib3.set_option("tol_mode", 1)
 
     def make(w):
@@ -451,8 +454,15 @@ def _batch_equals_single(lib3, pre=(0, 0, 0)):
     PSP = C.POINTER(lib3.abi.ParticleSystem)
     ps_arr = (PSP * len(made))(*[m[1] for m in made])
     cf_arr = (C.POINTER(lib3.abi.SimConfig) * len(made))(*[C.pointer(c) for c in cfgs])
-    say("batch of", len(made))
-    lib3.check(lib3.mps_hip_run_batch(ps_arr, cf_arr, len(made), steps), "run_batch")
+    say("batch of", len(made), "graph", graph)
+    lib3.set_option("graph", graph)
+    try:
+        # in two calls (each captures anew); the second call's third step
+        # replays the capture of its first
+        lib3.check(lib3.mps_hip_run_batch(ps_arr, cf_arr, len(made), 1), "run_batch")
+        lib3.check(lib3.mps_hip_run_batch(ps_arr, cf_arr, len(made), steps - 1), "run_batch")
+    finally:
+        lib3.set_option("graph", 0)
     for q, (cfg, ps, n) in enumerate(made):
         lib3.check(lib3.mps_hip_download(ps, None), "download")
         got = fields(ps, n)
