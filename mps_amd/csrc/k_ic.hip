@@ -83,6 +83,13 @@ __device__ __forceinline__ void st_rlx(double *p, double v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// The error flag another wave may have raised (a wait that gave up): then
+// every other wait gives up too, so the kernel drains at once.
+__device__ __forceinline__ bool err_raised(const int *err)
+{
+    return __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
 // Orders LDS accesses of the lanes of one wavefront around this point (a
 // compiler barrier plus the wave's LDS counter).
 __device__ __forceinline__ void wave_sync()
@@ -1065,8 +1072,11 @@ k_ic_csolve(int n, const int *__restrict__ perm, const double *__restrict__ lev,
                     if (!is_sent(readlane_d(v, k))) {
                         spins = 0;
                         idle = 0;
-                    } else if (++spins > wait_limit) {
-                        failed = true;   // give up on this term
+                    } else if (++spins > wait_limit || ((spins & 31) == 0 && err_raised(err))) {
+                        // give up on this term, and make every other wait
+                        // of the kernel give up too
+                        if (!failed && lane == 0) atomicOr(err, 1);
+                        failed = true;
                         if (lane == k) v = 0.0;
                     } else {
                         backoff(idle++, sleep_cap);
