@@ -160,6 +160,11 @@ struct Ctx {
     int *nl_stage = nullptr;         // compact list staging (device)
     size_t nl_stage_elems = 0;
     int *cnt_orig = nullptr, *off_orig = nullptr;   // per original index
+    // pinned host staging of the per-call path (the caller's particles,
+    // the compact list, its counts and offsets), grown as needed
+    Particle *h_part = nullptr;
+    int *h_flat = nullptr, *h_idx = nullptr;   // h_idx: counts [0, n], then offsets [n + 1, 2n + 2)
+    size_t h_part_cap = 0, h_flat_cap = 0, h_idx_cap = 0;
     // state flags
     bool has_state = false, resident = false, list_valid = false, ppe_valid = false;
     Grid grid{};

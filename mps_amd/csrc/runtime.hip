@@ -280,6 +280,11 @@ void ctx_free(Ctx *c)
     if (c->ctrl) { hipFree(c->ctrl); c->ctrl = nullptr; }
     if (c->hctrl) { hipHostFree(c->hctrl); c->hctrl = nullptr; }
     if (c->hcounters) { hipHostFree(c->hcounters); c->hcounters = nullptr; }
+    for (void *h : {(void *)c->h_part, (void *)c->h_flat, (void *)c->h_idx})
+        if (h) hipHostFree(h);
+    c->h_part = nullptr;
+    c->h_flat = c->h_idx = nullptr;
+    c->h_part_cap = c->h_flat_cap = c->h_idx_cap = 0;
     if (c->ic) { ic_free(c->ic); c->ic = nullptr; }
     if (c->mixed) { mixed_free(c->mixed); c->mixed = nullptr; }
     if (c->pk) { packed_free(c->pk); c->pk = nullptr; }
@@ -493,25 +498,82 @@ static int stride_for(const SimConfig *cf, const NeighborList *nl)
 
 // Copies ps (and nl's lists, when nl is given) to the device; rows hold
 // `stride` entries (0: as stride_for).
+// f(lo, hi) over pieces of [0, n) on up to 16 host threads: the host side
+// of the per-call copies (gathering and scattering the caller's padded
+// list and particle array), which one thread would make the bottleneck.
+template <typename F>
+static void host_for(long long n, F f)
+{
+    unsigned nt = std::thread::hardware_concurrency();
+    if (nt > 16) nt = 16;
+    if (nt < 2 || n < (1 << 16)) {
+        f(0LL, n);
+        return;
+    }
+    const long long per = (n + nt - 1) / nt;
+    std::vector<std::thread> th;
+    long long done = per < n ? per : n;
+    try {
+        for (unsigned t = 1; t < nt && done < n; t++) {
+            const long long lo = done, hi = lo + per < n ? lo + per : n;
+            th.emplace_back(f, lo, hi);
+            done = hi;
+        }
+    } catch (...) {
+    }
+    f(0LL, per < n ? per : n);
+    for (auto &x : th) x.join();
+    if (done < n) f(done, n);   // pieces no thread could be started for
+}
+
+// Pinned host memory of at least `bytes` in *p (capacity *cap), kept.
+template <typename T>
+static int pinned_grow(T **p, size_t *cap, size_t bytes)
+{
+    if (*p && *cap >= bytes) return 0;
+    if (*p) hipHostFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    const size_t want = bytes + bytes / 8 + 64;
+    HIP_OK(hipHostMalloc((void **)p, want, hipHostMallocDefault));
+    *cap = want;
+    return 0;
+}
+
+static void host_copy(void *dst, const void *src, size_t bytes)
+{
+    host_for((long long)((bytes + 4095) / 4096), [&](long long lo, long long hi) {
+        const size_t a = (size_t)lo * 4096, b = (size_t)hi * 4096 < bytes ? (size_t)hi * 4096 : bytes;
+        if (a < b) memcpy((char *)dst + a, (const char *)src + a, b - a);
+    });
+}
+
 static int upload_rows(Ctx *c, ParticleSystem *ps, const NeighborList *nl, int stride, const Grid &g)
 {
     int rc = ctx_alloc(c, ps->num, stride > 0 ? stride : stride_for(g_config, nl), g.total);
     if (rc) return rc;
     c->grid = g;
-    if (ps->num > 0) {
-        HIP_OK(hipMemcpyAsync(c->stage, ps->particles, sizeof(Particle) * (size_t)ps->num,
-                              hipMemcpyHostToDevice, c->stream));
+    const int n = ps->num;
+    if (n > 0) {
+        // the caller's particles through pinned staging
+        rc = pinned_grow(&c->h_part, &c->h_part_cap, sizeof(Particle) * (size_t)n);
+        if (rc) return rc;
+        host_copy(c->h_part, ps->particles, sizeof(Particle) * (size_t)n);
+        HIP_OK(hipMemcpyAsync(c->stage, c->h_part, sizeof(Particle) * (size_t)n, hipMemcpyHostToDevice, c->stream));
         rc = launch_unpack(c, c->stage, 0);
         if (rc) return rc;
     } else {
         c->cx = c->ct = c->cp = 0;
     }
     c->list_valid = false;
-    if (nl && nl->count && nl->neighbors && ps->num > 0) {
-        // compact the caller's padded list to CSR and expand it on the device
-        std::vector<int> cnt(ps->num), off(ps->num + 1);
+    if (nl && nl->count && nl->neighbors && n > 0) {
+        // the caller's padded list as a compact CSR (count[i] entries of
+        // each row), expanded on the device
+        rc = pinned_grow(&c->h_idx, &c->h_idx_cap, sizeof(int) * 2 * ((size_t)n + 1));
+        if (rc) return rc;
+        int *cnt = c->h_idx, *off = c->h_idx + n + 1;
         long long tot = 0;
-        for (int i = 0; i < ps->num; i++) {
+        for (int i = 0; i < n; i++) {
             int m = nl->count[i];
             if (m < 0) m = 0;
             if (m > nl->max_neighbors) m = nl->max_neighbors;
@@ -519,23 +581,28 @@ static int upload_rows(Ctx *c, ParticleSystem *ps, const NeighborList *nl, int s
             off[i] = (int)tot;
             tot += m;
         }
-        off[ps->num] = (int)tot;
-        std::vector<int> flat((size_t)(tot > 0 ? tot : 1));
-        for (int i = 0; i < ps->num; i++)
-            memcpy(&flat[off[i]], nl->neighbors + (size_t)i * nl->max_neighbors, sizeof(int) * cnt[i]);
+        off[n] = (int)tot;
+        rc = pinned_grow(&c->h_flat, &c->h_flat_cap, sizeof(int) * (size_t)(tot > 0 ? tot : 1));
+        if (rc) return rc;
+        int *flat = c->h_flat;
+        host_for(n, [&](long long lo, long long hi) {
+            for (long long i = lo; i < hi; i++)
+                memcpy(flat + off[i], nl->neighbors + (size_t)i * nl->max_neighbors, sizeof(int) * cnt[i]);
+        });
         if ((size_t)tot > c->nl_stage_elems) {
             DFREE(c->nl_stage);
             HIP_OK(hipMalloc(&c->nl_stage, sizeof(int) * (size_t)(tot > 0 ? tot : 1)));
             c->nl_stage_elems = (size_t)(tot > 0 ? tot : 1);
         }
-        HIP_OK(hipMemcpyAsync(c->nl_stage, flat.data(), sizeof(int) * (size_t)tot, hipMemcpyHostToDevice, c->stream));
-        HIP_OK(hipMemcpyAsync(c->cnt_orig, cnt.data(), sizeof(int) * ps->num, hipMemcpyHostToDevice, c->stream));
-        HIP_OK(hipMemcpyAsync(c->off_orig, off.data(), sizeof(int) * (ps->num + 1), hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipMemcpyAsync(c->nl_stage, flat, sizeof(int) * (size_t)tot, hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipMemcpyAsync(c->cnt_orig, cnt, sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipMemcpyAsync(c->off_orig, off, sizeof(int) * (n + 1), hipMemcpyHostToDevice, c->stream));
         rc = launch_list_import(c, c->cnt_orig, c->off_orig);
         if (rc) return rc;
-        HIP_OK(hipStreamSynchronize(c->stream));   // host vectors go out of scope
         c->list_valid = true;
     }
+    // the staging is rewritten by the next call
+    HIP_OK(hipStreamSynchronize(c->stream));
     c->has_state = true;
     return 0;
 }
@@ -551,29 +618,37 @@ static int download(Ctx *c, ParticleSystem *ps, NeighborList *nl)
         mps_set_error(MPS_ERR_ARG, __FILE__, __LINE__, "no device state matching this ParticleSystem");
         return MPS_ERR_ARG;
     }
-    if (ps->num > 0) {
+    const int n = ps->num;
+    if (n > 0) {
         int rc = launch_pack(c, c->stage);
+        if (!rc) rc = pinned_grow(&c->h_part, &c->h_part_cap, sizeof(Particle) * (size_t)n);
         if (rc) return rc;
-        HIP_OK(hipMemcpyAsync(ps->particles, c->stage, sizeof(Particle) * (size_t)ps->num,
-                              hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipMemcpyAsync(c->h_part, c->stage, sizeof(Particle) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
     }
-    if (nl && nl->count && nl->neighbors && c->list_valid && ps->num > 0) {
-        long long tot = 0;
+    const bool list = nl && nl->count && nl->neighbors && c->list_valid && n > 0;
+    long long tot = 0;
+    if (list) {
+        // the device list as a compact CSR by original index
         int rc = launch_list_export(c, &tot);
+        if (!rc) rc = pinned_grow(&c->h_idx, &c->h_idx_cap, sizeof(int) * 2 * ((size_t)n + 1));
+        if (!rc) rc = pinned_grow(&c->h_flat, &c->h_flat_cap, sizeof(int) * (size_t)(tot > 0 ? tot : 1));
         if (rc) return rc;
-        std::vector<int> flat((size_t)(tot > 0 ? tot : 1)), cnt(ps->num);
-        HIP_OK(hipMemcpyAsync(flat.data(), c->nl_stage, sizeof(int) * (size_t)tot, hipMemcpyDeviceToHost, c->stream));
-        HIP_OK(hipMemcpyAsync(cnt.data(), c->cnt_orig, sizeof(int) * ps->num, hipMemcpyDeviceToHost, c->stream));
-        HIP_OK(hipStreamSynchronize(c->stream));
-        size_t at = 0;
-        for (int i = 0; i < ps->num; i++) {
-            int m = cnt[i] < nl->max_neighbors ? cnt[i] : nl->max_neighbors;
-            memcpy(nl->neighbors + (size_t)i * nl->max_neighbors, &flat[at], sizeof(int) * m);
-            nl->count[i] = m;
-            at += cnt[i];
-        }
+        HIP_OK(hipMemcpyAsync(c->h_flat, c->nl_stage, sizeof(int) * (size_t)tot, hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipMemcpyAsync(c->h_idx, c->cnt_orig, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipMemcpyAsync(c->h_idx + n + 1, c->off_orig, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
     }
     HIP_OK(hipStreamSynchronize(c->stream));
+    if (n > 0) host_copy(ps->particles, c->h_part, sizeof(Particle) * (size_t)n);
+    if (list) {
+        const int *flat = c->h_flat, *cnt = c->h_idx, *off = c->h_idx + n + 1;
+        host_for(n, [&](long long lo, long long hi) {
+            for (long long i = lo; i < hi; i++) {
+                const int m = cnt[i] < nl->max_neighbors ? cnt[i] : nl->max_neighbors;
+                memcpy(nl->neighbors + (size_t)i * nl->max_neighbors, flat + off[i], sizeof(int) * m);
+                nl->count[i] = m;
+            }
+        });
+    }
     return 0;
 }
 
