@@ -278,6 +278,11 @@ def run_ensemble(lib, a):
     PSP = C.POINTER(lib.abi.ParticleSystem)
     ps_arr = (PSP * len(scenes))(*[m[1] for m in scenes])
     cf_arr = (C.POINTER(lib.abi.SimConfig) * len(scenes))(*[C.pointer(m[0]) for m in scenes])
+    if a.autotune and lib.mps_hip_get_option(b"batch_flat") == 1:
+        # the particle kernels' block size for the launches over all scenes
+        res = mps_amd.TuneResult()
+        lib.check(lib.mps_hip_autotune_batch(ps_arr, cf_arr, len(scenes), 1, C.byref(res)), "autotune_batch")
+        tune = dict(tune or {}, batch_pblk=res.pblk, batch_pblk_ms=[round(v, 3) for v in res.pblk_ms])
     lib.check(lib.mps_hip_run_batch(ps_arr, cf_arr, len(scenes), a.warmup), "warm-up steps")
     t0 = time.perf_counter()
     lib.check(lib.mps_hip_run_batch(ps_arr, cf_arr, len(scenes), a.steps), "timed steps")

@@ -187,11 +187,15 @@ int  mps_hip_run_steps(ParticleSystem *ps, int nsteps);
 /* Ensemble of independent scenes (no counterpart in the reference, which
  * runs one scene per process): nsteps timesteps of every ps[i] under its
  * own configuration cfgs[i] (viscosity, geometry, ... may differ; same
- * dimensionality), state resident on the GPU.  The particle operators run
- * per scene; the CG (FP64, or the mixed form of option "precision") runs
- * batched -- one launch per CG phase for all scenes, per-scene reductions
- * and stop tests -- and gives every scene the bits of its own single-scene
- * solve.  Scenes that use ICCG are solved one by one.  ps[i] stays resident: read it back with
+ * dimensionality), state resident on the GPU.  In the 3-D library with
+ * option "batch_flat" 1 (the default) every particle operator, sort and
+ * packing stage is one launch over all scenes' particles; otherwise they
+ * run per scene.  The CG (FP64, or the mixed form of option "precision")
+ * runs batched -- one launch per CG phase for all scenes, per-scene
+ * reductions and stop tests.  With option "graph" the phases before and
+ * after the solve and a chunk of CG iterations replay captured HIP graphs.
+ * Every scene gets the bits of its own single-scene run.  Scenes that use
+ * ICCG are stepped one by one.  ps[i] stays resident: read it back with
  * mps_hip_download.  g_config is left as it was. */
 int  mps_hip_run_batch(ParticleSystem *const *ps, SimConfig *const *cfgs, int nscenes, int nsteps);
 
@@ -322,6 +326,14 @@ typedef struct {
                                128, 256, 512                               */
 } mps_tune_result;
 int  mps_hip_autotune(ParticleSystem *ps, int apply, mps_tune_result *out);
+/* Batch autotuner (no counterpart in the reference): the block size of the
+ * per-particle kernels (option "pblk") for mps_hip_run_batch's launches
+ * over all scenes -- the batch's particle phases timed at 128, 256 and 512
+ * on private device copies of the scenes' host particles (pblk and
+ * pblk_ms of *out; the other fields 0).  With apply != 0 the fastest
+ * becomes the option.  3-D library; MPS_ERR_UNSUPPORTED otherwise. */
+int  mps_hip_autotune_batch(ParticleSystem *const *ps, SimConfig *const *cfgs, int nscenes, int apply,
+                            mps_tune_result *out);
 
 /* Exports of the last assembled PPE in the reference's layout
  * (mps_fluid3D_parallel/src/pressure_solver.c:411-551): rows in eq order

@@ -111,6 +111,8 @@ class Lib:
             "mps_hip_get_option": (C.c_double, [C.c_char_p]),
             "mps_hip_get_stats": (C.c_int, [PS, C.POINTER(StepStats)]),
             "mps_hip_autotune": (C.c_int, [PS, C.c_int, C.POINTER(TuneResult)]),
+            "mps_hip_autotune_batch": (C.c_int, [C.POINTER(PS), C.POINTER(C.POINTER(a.SimConfig)), C.c_int, C.c_int,
+                                                 C.POINTER(TuneResult)]),
             "mps_hip_export_ppe": (C.c_int, [PS, I, I, I, D, D, D]),
             "mps_hip_export_solution": (C.c_int, [PS, D]),
             "mps_hip_export_ic": (C.c_int, [PS, I, I, D, D]),

@@ -1505,6 +1505,74 @@ int mps_hip_autotune(ParticleSystem *ps, int apply, mps_tune_result *out)
     return MPS_OK;
 }
 
+// The block size of the per-particle kernels for a batch stepped with one
+// launch per operator: the batch's particle phases (predictor to assembly,
+// all scenes) timed three times at each size after an untimed pass, on private
+// device copies of the scenes' host particles.  The caller's scenes and
+// their device state are not touched.
+int mps_hip_autotune_batch(ParticleSystem *const *ps, SimConfig *const *cfgs, int nscenes, int apply,
+                           mps_tune_result *out)
+{
+    if (!ps || !cfgs || nscenes <= 0 || !out) return g_last_error = MPS_ERR_ARG;
+    int rc = ensure_device();
+    if (rc) return rc;
+    memset(out, 0, sizeof *out);
+#if MPS_DIM == 3 && !defined(MPS_FRICTION_FIELDS)
+    for (int q = 0; q < nscenes; q++)
+        if (!ps[q] || !cfgs[q]) return g_last_error = MPS_ERR_ARG;
+    std::vector<ParticleSystem> shadow(nscenes);
+    std::vector<Ctx *> cs(nscenes, nullptr);
+    const Options saved = g_opt;
+    SimConfig *saved_cfg = g_config;
+    g_opt.profile = 0;
+    g_opt.graph = 0;
+    for (int q = 0; q < nscenes && !rc; q++) {
+        shadow[q] = *ps[q];   // same host particles, a context of its own
+        cs[q] = get_ctx(&shadow[q]);
+        rc = cs[q] ? ensure_resident(cs[q], &shadow[q], cfgs[q]) : g_last_error;
+    }
+    const int pb[3] = {128, 256, 512};
+    int best = 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = MPS_ERR_HIP;
+    for (int k = 0; k < 3 && !rc; k++) {
+        g_opt.pblk = pb[k];
+        std::vector<StepConsts> ks(nscenes);
+        FlatLaunch f;
+        hipStream_t s = cs[0]->stream;
+        rc = flat_setup(cs.data(), cfgs, nscenes, s, &f, ks);
+        if (!rc) rc = flat_step_pre(f);
+        if (!rc && hipEventRecord(e0, s) != hipSuccess) rc = MPS_ERR_HIP;
+        for (int r = 0; r < 3 && !rc; r++) rc = flat_step_pre(f);
+        float ms = 0.0f;
+        if (!rc && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = MPS_ERR_HIP;
+        for (int q = 0; q < nscenes; q++) {
+            cs[q]->cx = f.cx;
+            cs[q]->ct = f.ct;
+            cs[q]->cp = f.cp;
+        }
+        out->pblk_ms[k] = ms / 3.0;
+        if (!rc && k > 0 && out->pblk_ms[k] < out->pblk_ms[best]) best = k;
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    g_opt = saved;
+    g_config = saved_cfg;
+    for (int q = 0; q < nscenes; q++)
+        if (cs[q]) mps_hip_release(&shadow[q]);
+    if (rc) return g_last_error = rc;
+    out->pblk = pb[best];
+    if (apply) mps_hip_set_option("pblk", pb[best]);
+    return MPS_OK;
+#else
+    (void)apply;
+    mps_set_error(MPS_ERR_UNSUPPORTED, __FILE__, __LINE__, "batch tuning is for the 3-D library");
+    return g_last_error = MPS_ERR_UNSUPPORTED;
+#endif
+}
+
 int mps_hip_get_stats(ParticleSystem *ps, mps_step_stats *out)
 {
     Ctx *c = ps ? find_ctx(ps) : nullptr;

@@ -405,6 +405,17 @@ def test_batch_equals_single_runs(lib3, precision, flat, graph):
         lib3.set_option("graph", 0)
 
 
+def test_autotune_batch_keeps_bits(lib3):
+    """mps_hip_autotune_batch times the batch's particle phases at each
+    block size on private copies of the scenes and picks one; the scenes
+    it was given are left untouched, and the batch run with its choice has
+    the bits of the single-scene runs."""
+    try:
+        _batch_equals_single(lib3, tune=True)
+    finally:
+        lib3.set_option("pblk", 256)
+
+
 def test_batch_after_single_steps(lib3):
     """Scenes that took different numbers of single-scene steps first (so
     their buffer halves differ) then run batched: each ends with the bits
@@ -412,7 +423,7 @@ def test_batch_after_single_steps(lib3):
     _batch_equals_single(lib3, pre=(1, 0, 2))
 
 
-def _batch_equals_single(lib3, pre=(0, 0, 0), graph=0):
+def _batch_equals_single(lib3, pre=(0, 0, 0), graph=0, tune=False):
     import ctypes as C
     from mps_amd import params
     scenes_w = [dict(dims=(0.4, 0.3, 0.12, 0.12, 0.2, 0.12), l0=0.01, dt=5e-4, nu=1e-6),
@@ -454,6 +465,13 @@ def _batch_equals_single(lib3, pre=(0, 0, 0), graph=0):
     PSP = C.POINTER(lib3.abi.ParticleSystem)
     ps_arr = (PSP * len(made))(*[m[1] for m in made])
     cf_arr = (C.POINTER(lib3.abi.SimConfig) * len(made))(*[C.pointer(c) for c in cfgs])
+    if tune:
+        import mps_amd
+        res = mps_amd.TuneResult()
+        lib3.check(lib3.mps_hip_autotune_batch(ps_arr, cf_arr, len(made), 1, C.byref(res)), "autotune_batch")
+        say("batch tuner: pblk", res.pblk, "ms", list(res.pblk_ms))
+        assert res.pblk in (128, 256, 512) and lib3.mps_hip_get_option(b"pblk") == res.pblk
+        assert all(v > 0 for v in res.pblk_ms)
     say("batch of", len(made), "graph", graph)
     lib3.set_option("graph", graph)
     try:
