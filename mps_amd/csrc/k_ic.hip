@@ -25,24 +25,29 @@
 //
 //   factor   one wavefront per row, rows claimed in level order from an
 //            atomic ticket (so a row only waits on rows of waves already
-//            running).  Lane t owns entry t of the row: it waits for row
-//            k = col_t (its 1/sqrt(d) published), streams row k's entries
-//            in batches, finds each column in the row's own columns (binary
-//            search in LDS) and folds the products in ascending column order
-//            as the entries of its own row it needs are finished by lower
-//            lanes.  The row's values live in LDS, so the in-row chain costs
-//            LDS latency, not a memory round trip.
-//   solves   one wavefront per row, claimed in level order (the backward
-//            solve in the reverse order).  Lane j loads entry j and the
-//            value it multiplies in one round trip, and the products are
-//            folded across the lanes in entry order up to the first value
-//            not yet published, which its lane polls for.
+//            running).  Lane t owns entry t of the row, (e, k).  Two forms:
+//            one-phase (any row length: lane t streams row k once it is
+//            published and folds the products in ascending column order)
+//            and two-phase (rows of at most 64 lower entries: the wave
+//            loads each published row k whole, keeps every shared column
+//            in lane t's match list in LDS, then lane t folds its list in
+//            order).  The row's values live in LDS, so the in-row chain
+//            costs LDS latency, not a memory round trip.
+//   solves   chunks of consecutive positions of the solve order (buckets
+//            of levels) claimed by workgroups in order (the backward
+//            solve from the end); inside a chunk the rows go by level,
+//            one wavefront per row.  Lane j loads entry j and the value it
+//            multiplies (from LDS for rows of the chunk) in one round
+//            trip, and the products are folded across the lanes in entry
+//            order up to the first value not yet published, which its
+//            lane polls for.
 // Values published to other waves use agent-scope stores and loads (they
 // bypass the per-XCD L2) over a sentinel NaN pattern: a published value is
 // its own ready flag, so no fence orders a flag after its data.  Every wait
 // counts its unsuccessful polls; past the option "ic_wait_limit" the wait
 // gives up (the term is dropped), an error flag is set and the solve reports
-// MPS_ERR_SOLVER, so every kernel always drains.
+// MPS_ERR_SOLVER, so every kernel always drains (in the solves the flag
+// makes every other wait give up as well).
 #include <limits.h>
 #include <math.h>
 #include <stdint.h>
