@@ -1129,14 +1129,18 @@ static int run_batch_flat(Ctx *const *cs, SimConfig *const *cfgs, int ns, int ns
         if (!rc)
             rc = mixed ? batch_cg_mixed(cs, ns, so.data(), sr.data(), s) : batch_cg(cs, ns, so.data(), sr.data(), s);
         if (!rc) rc = graph ? flat_graph(g_flat.post, f, flat_post_all) : flat_post_all(f);
+        // the scenes' buffer indices follow the launches made, also when a
+        // later part of the step failed
+        for (int q = 0; q < ns; q++) {
+            cs[q]->cx = f.cx;
+            cs[q]->ct = f.ct;
+            cs[q]->cp = f.cp;
+        }
         if (rc) break;
         HIP_OK(hipMemcpyAsync(g_flat.hctr, g_flat.ctr, sizeof(DevCounters) * ns, hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
         for (int q = 0; q < ns && !rc; q++) {
             Ctx *c = cs[q];
-            c->cx = f.cx;
-            c->ct = f.ct;
-            c->cp = f.cp;
             c->list_valid = true;
             *c->hcounters = g_flat.hctr[q];
             rc = step_bookkeep(c, ks[q], sr[q]);
