@@ -508,6 +508,21 @@ k_cg_iter_b(int n, int t, const double *__restrict__ p, const double *__restrict
     __shared__ double sh_alpha;
     __shared__ int sh_stop;
     if (block_flag(&ctrl->done)) return;
+    // the first VEC_U rows' operands, loaded before the partials are summed
+    // so that the two latencies overlap
+    const int st = gridDim.x * CG_BLK;
+    const int i0 = blockIdx.x * CG_BLK + threadIdx.x;
+    const bool pre = i0 + (VEC_U - 1) * st < n;
+    double pv0[VEC_U], av0[VEC_U], xv0[VEC_U], rv0[VEC_U];
+    if (pre) {
+#pragma unroll
+        for (int u = 0; u < VEC_U; u++) {
+            pv0[u] = p[i0 + u * st];
+            av0[u] = Ap[i0 + u * st];
+            xv0[u] = x[i0 + u * st];
+            rv0[u] = r[i0 + u * st];
+        }
+    }
     const double pap = fin_sum(partA, na);
     if (threadIdx.x == 0) {
         const double rr = ctrl->rr[t & 1];
@@ -531,8 +546,17 @@ k_cg_iter_b(int n, int t, const double *__restrict__ p, const double *__restrict
     if (sh_stop) return;
     const double alpha = sh_alpha;
     double v = 0.0;
-    const int st = gridDim.x * CG_BLK;
-    int i = blockIdx.x * CG_BLK + threadIdx.x;
+    int i = i0;
+    if (pre) {
+#pragma unroll
+        for (int u = 0; u < VEC_U; u++) {
+            x[i + u * st] = xv0[u] + alpha * pv0[u];
+            const double ri = rv0[u] - alpha * av0[u];
+            r[i + u * st] = ri;
+            v += ri * ri;
+        }
+        i += VEC_U * st;
+    }
     for (; i + (VEC_U - 1) * st < n; i += VEC_U * st) {   // VEC_U rows in flight, same order
         double pv[VEC_U], av[VEC_U], xv[VEC_U], rv[VEC_U];
 #pragma unroll
@@ -574,6 +598,18 @@ k_cg3_pupdate(int n, int t, int max_iter, const double *__restrict__ r, double *
     __shared__ double sh_beta;
     __shared__ int sh_stop;
     if (block_flag(&ctrl->done)) return;
+    // the first VEC_U rows, loaded before the partials are summed
+    const int st = gridDim.x * CG_BLK;
+    const int i0 = blockIdx.x * CG_BLK + threadIdx.x;
+    const bool pre = i0 + (VEC_U - 1) * st < n;
+    double rv0[VEC_U], pv0[VEC_U];
+    if (pre) {
+#pragma unroll
+        for (int u = 0; u < VEC_U; u++) {
+            rv0[u] = r[i0 + u * st];
+            pv0[u] = p[i0 + u * st];
+        }
+    }
     const double rr_new = fin_sum(partB, nb);
     if (threadIdx.x == 0) {
         const double rr = ctrl->rr[t & 1];
@@ -607,8 +643,12 @@ k_cg3_pupdate(int n, int t, int max_iter, const double *__restrict__ r, double *
     __syncthreads();
     if (sh_stop) return;
     const double beta = sh_beta;
-    const int st = gridDim.x * CG_BLK;
-    int i = blockIdx.x * CG_BLK + threadIdx.x;
+    int i = i0;
+    if (pre) {
+#pragma unroll
+        for (int u = 0; u < VEC_U; u++) p[i + u * st] = rv0[u] + beta * pv0[u];
+        i += VEC_U * st;
+    }
     for (; i + (VEC_U - 1) * st < n; i += VEC_U * st) {
         double rv[VEC_U], pv[VEC_U];
 #pragma unroll
@@ -1124,16 +1164,18 @@ k_mx_fin2(const double *part, unsigned np, MixedCtrl *c, int t, int max_iter)
 // 64.2 ms per mixed step at 1.08M particles; 173.6, 176.1, 179.3 ms per
 // step of the 64-scene ensemble): the reliable-update SpMV wants rows
 // spread over threads more than the plain update wants rows per thread.
-#define PU_R 1
-template <int U, typename V>
+#define PU_R 1       // 256-row chunks per p-update block, one scene
+#define PU_RB 1      // the same in the batched solve (more chunks per block serialise the
+                     // reliable updates' FP64 residual rows: 4 measured 3 % slower per step)
+template <int U, typename V, int R>
 __device__ __forceinline__ void pupdate_chunks(const PkView &P, const double *__restrict__ b,
                                                const double *__restrict__ x, V *__restrict__ r, V *__restrict__ p,
                                                int n, int c0, int nch, V beta, int ru, double *part_ru)
 {
     if (!ru) {
-        V rv[PU_R], pv[PU_R];
+        V rv[R], pv[R];
 #pragma unroll
-        for (int c = 0; c < PU_R; c++) {
+        for (int c = 0; c < R; c++) {
             const int i = (c0 + c) * CG_BLK + threadIdx.x;
             if (c < nch && i < n) {
                 rv[c] = r[i];
@@ -1141,7 +1183,7 @@ __device__ __forceinline__ void pupdate_chunks(const PkView &P, const double *__
             }
         }
 #pragma unroll
-        for (int c = 0; c < PU_R; c++) {
+        for (int c = 0; c < R; c++) {
             const int i = (c0 + c) * CG_BLK + threadIdx.x;
             if (c < nch && i < n) p[i] = rv[c] + beta * pv[c];
         }
@@ -1149,7 +1191,7 @@ __device__ __forceinline__ void pupdate_chunks(const PkView &P, const double *__
     }
     // each chunk's sum as block_sum forms it: wave sums, then the waves'
     // sums in wave order; one barrier for all chunks
-    __shared__ double wpart[PU_R][CG_BLK / WAVE];
+    __shared__ double wpart[R][CG_BLK / WAVE];
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
     for (int c = 0; c < nch; c++) {
         const int i = (c0 + c) * CG_BLK + threadIdx.x;
@@ -1175,6 +1217,7 @@ __device__ __forceinline__ void pupdate_chunks(const PkView &P, const double *__
 }
 
 static inline unsigned pu_grid(unsigned nblk) { return (nblk + PU_R - 1) / PU_R; }
+static inline unsigned pu_grid_b(unsigned nblk) { return (nblk + PU_RB - 1) / PU_RB; }
 
 // the p update of the mixed iteration over nblk = cg_grid(n) chunks
 template <int U, typename V>
@@ -1185,7 +1228,7 @@ k_mx_pupdate_pk(int n, unsigned nblk, PkView P, const double *__restrict__ b, co
     if (c->done) return;
     const int c0 = blockIdx.x * PU_R;
     const int nch = min(PU_R, (int)nblk - c0);
-    pupdate_chunks<U, V>(P, b, x, r, p, n, c0, nch, (V)c->beta, c->ru, part_ru);
+    pupdate_chunks<U, V, PU_R>(P, b, x, r, p, n, c0, nch, (V)c->beta, c->ru, part_ru);
 }
 
 // ---- the mixed iteration in three kernels (option fused = 2) ----------
@@ -1203,6 +1246,21 @@ k_mx3_axpy(int n, int t, const V *__restrict__ p, const V *__restrict__ Ap, doub
     __shared__ double sh_alpha;
     __shared__ int sh_go;
     if (block_flag(&c->done)) return;
+    // the first VEC_U rows' operands, loaded before the partials are summed
+    const int st = gridDim.x * CG_BLK;
+    const int i0 = blockIdx.x * CG_BLK + threadIdx.x;
+    const bool pre = i0 + (VEC_U - 1) * st < n;
+    V pv0[VEC_U], apv0[VEC_U], rv0[VEC_U];
+    double xv0[VEC_U];
+    if (pre) {
+#pragma unroll
+        for (int u = 0; u < VEC_U; u++) {
+            pv0[u] = p[i0 + u * st];
+            apv0[u] = Ap[i0 + u * st];
+            xv0[u] = x[i0 + u * st];
+            rv0[u] = r[i0 + u * st];
+        }
+    }
     const double pap = fin_sum(part, np);
     const int ru = c->f_ru[t & 1];
     double rr_true = 0.0;
@@ -1244,8 +1302,17 @@ k_mx3_axpy(int n, int t, const V *__restrict__ p, const V *__restrict__ Ap, doub
     const double alpha = sh_alpha;
     const V av = (V)alpha;
     double v = 0.0;
-    const int st = gridDim.x * CG_BLK;
-    int i = blockIdx.x * CG_BLK + threadIdx.x;
+    int i = i0;
+    if (pre) {
+#pragma unroll
+        for (int u = 0; u < VEC_U; u++) {
+            x[i + u * st] = xv0[u] + alpha * (double)pv0[u];
+            const V ri = rv0[u] - av * apv0[u];
+            r[i + u * st] = ri;
+            v += (double)ri * (double)ri;
+        }
+        i += VEC_U * st;
+    }
     for (; i + (VEC_U - 1) * st < n; i += VEC_U * st) {   // VEC_U rows in flight, same order
         V pv[VEC_U], apv[VEC_U], rv[VEC_U];
         double xv[VEC_U];
@@ -1283,6 +1350,15 @@ k_mx3_pupdate(int n, unsigned nblk, int t, int max_iter, PkView P, const double 
     __shared__ double sh_beta;
     __shared__ int sh_go, sh_ru;
     if (block_flag(&c->done)) return;
+    // this block's row of r and p, loaded before the partials are summed
+    // (used unless the iteration takes a reliable update)
+    static_assert(PU_R == 1, "one chunk per p-update block");
+    const int i0 = blockIdx.x * CG_BLK + threadIdx.x;
+    V rpre = (V)0, ppre = (V)0;
+    if (i0 < n) {
+        rpre = r[i0];
+        ppre = p[i0];
+    }
     const double rr_new = fin_sum(part2, np2);
     if (threadIdx.x == 0) {
         const double rr = c->m_rr, rr_ru = c->m_rr_ru;
@@ -1313,9 +1389,13 @@ k_mx3_pupdate(int n, unsigned nblk, int t, int max_iter, PkView P, const double 
     }
     __syncthreads();
     if (!sh_go) return;
+    if (!sh_ru) {
+        if (i0 < n) p[i0] = rpre + (V)sh_beta * ppre;
+        return;
+    }
     const int c0 = blockIdx.x * PU_R;
     const int nch = min(PU_R, (int)nblk - c0);
-    pupdate_chunks<U, V>(P, b, x, r, p, n, c0, nch, (V)sh_beta, sh_ru, part_ru);
+    pupdate_chunks<U, V, PU_R>(P, b, x, r, p, n, c0, nch, (V)sh_beta, sh_ru, part_ru);
 }
 
 struct MixedBufs {
@@ -2102,10 +2182,10 @@ k_bm_pupdate(const BMScene<V> *__restrict__ sc, const int *__restrict__ pbs, dou
     const int si = pbs[blockIdx.x];
     if (ctrl[si].done) return;
     const BMScene<V> &S = sc[si];
-    const int c0 = (blockIdx.x - S.pb_off) * PU_R;
-    const int nch = min(PU_R, S.nblk - c0);
-    pupdate_chunks<U, V>(S.P, S.b, S.x, S.r, S.p, S.n_eq, c0, nch, (V)ctrl[si].beta, ctrl[si].ru,
-                         part_ru + S.blk_off);
+    const int c0 = (blockIdx.x - S.pb_off) * PU_RB;
+    const int nch = min(PU_RB, S.nblk - c0);
+    pupdate_chunks<U, V, PU_RB>(S.P, S.b, S.x, S.r, S.p, S.n_eq, c0, nch, (V)ctrl[si].beta, ctrl[si].ru,
+                                part_ru + S.blk_off);
 }
 
 struct BatchMixedBufs {
@@ -2138,7 +2218,7 @@ static int batch_mixed_run(Ctx *const *cs, int ns, const SolveOpts *opts, SolveR
     for (int k = 0; k < ns; k++) {
         const int n = cs[k]->pk->n_eq;
         tot_blk += (int)cg_grid(n);
-        tot_pb += (int)pu_grid(cg_grid(n));
+        tot_pb += (int)pu_grid_b(cg_grid(n));
         tot_ax += n > 0 ? (int)vec_grid(n) : 0;
         if (cs[k]->pk->hmax > hmax) hmax = cs[k]->pk->hmax;
         if (opts[k].max_iter > max_iter) max_iter = opts[k].max_iter;
@@ -2187,7 +2267,7 @@ static int batch_mixed_run(Ctx *const *cs, int ns, const SolveOpts *opts, SolveR
         S.blk_off = bo;
         S.nblk = (int)cg_grid(pk->n_eq);
         S.pb_off = po;
-        S.npb = (int)pu_grid(cg_grid(pk->n_eq));
+        S.npb = (int)pu_grid_b(cg_grid(pk->n_eq));
         S.ax_off = ao;
         S.nax = pk->n_eq > 0 ? (int)vec_grid(pk->n_eq) : 0;
         S.max_iter = o.max_iter;

@@ -416,6 +416,20 @@ def test_autotune_batch_keeps_bits(lib3):
         lib3.set_option("pblk", 256)
 
 
+@pytest.mark.parametrize("graph", [0, 1])
+@pytest.mark.parametrize("precision", [0, 3])
+def test_batch_equals_single_runs_2d(lib2, precision, graph):
+    """The 2-D library's batches (particle phases per scene, one batched
+    CG, its iterations replayed from a captured chunk with option "graph"):
+    every scene keeps the bits and CG iteration counts of its own run."""
+    lib2.set_option("precision", precision)
+    try:
+        _batch_equals_single(lib2, graph=graph, scenes_w=SCENES_2D)
+    finally:
+        lib2.set_option("precision", 0)
+        lib2.set_option("graph", 0)
+
+
 def test_batch_after_single_steps(lib3):
     """Scenes that took different numbers of single-scene steps first (so
     their buffer halves differ) then run batched: each ends with the bits
@@ -423,12 +437,17 @@ def test_batch_after_single_steps(lib3):
     _batch_equals_single(lib3, pre=(1, 0, 2))
 
 
-def _batch_equals_single(lib3, pre=(0, 0, 0), graph=0, tune=False):
+SCENES_3D = [dict(dims=(0.4, 0.3, 0.12, 0.12, 0.2, 0.12), l0=0.01, dt=5e-4, nu=1e-6),
+             dict(dims=(0.6, 0.4, 0.2, 0.2, 0.3, 0.2), l0=0.01, dt=5e-4, nu=3e-5),
+             dict(dims=(0.4, 0.3, 0.12, 0.2, 0.15, 0.12), l0=0.01, dt=5e-4, nu=1e-4)]
+SCENES_2D = [dict(dims=(0.6, 0.4, 0.0, 0.15, 0.3, 0.0), l0=0.005, dt=5e-4, nu=1e-6),
+             dict(dims=(1.0, 0.6, 0.0, 0.3, 0.5, 0.0), l0=0.01, dt=5e-4, nu=3e-5),
+             dict(dims=(0.8, 0.5, 0.0, 0.25, 0.25, 0.0), l0=0.005, dt=5e-4, nu=1e-4)]
+
+
+def _batch_equals_single(lib3, pre=(0, 0, 0), graph=0, tune=False, scenes_w=SCENES_3D):
     import ctypes as C
     from mps_amd import params
-    scenes_w = [dict(dims=(0.4, 0.3, 0.12, 0.12, 0.2, 0.12), l0=0.01, dt=5e-4, nu=1e-6),
-                dict(dims=(0.6, 0.4, 0.2, 0.2, 0.3, 0.2), l0=0.01, dt=5e-4, nu=3e-5),
-                dict(dims=(0.4, 0.3, 0.12, 0.2, 0.15, 0.12), l0=0.01, dt=5e-4, nu=1e-4)]
     steps = 4
     lib3.set_option("tol_mode", 1)
 
