@@ -1126,7 +1126,7 @@ k_search(int n, const dvec *__restrict__ X, const int *__restrict__ T, const int
 // of cells) is read from HBM directly for that row offset.
 #define SRCH_CAP 512
 #define SRCH_SPAN (4 * SRCH_CAP)
-#define SRCH_U 4
+#define SRCH_FB 1   // candidates per batch on the HBM path of a wide row offset
 
 __device__ __forceinline__ int wave_min_int(int v)
 {
@@ -1141,11 +1141,11 @@ __device__ __forceinline__ int wave_max_int(int v)
     return v;
 }
 
-template <int B>
+template <int B, int SRCH_U>
 __device__ __forceinline__ void search_lds_body(int lb, int n, const dvec *__restrict__ X, const int *__restrict__ T, const int *__restrict__ O,
              const int *__restrict__ cstart, SearchOut out, StepConsts k, int with_density)
 {
-    __shared__ dvec sx[SRCH_CAP + SRCH_U];
+    __shared__ double sxc[D_][SRCH_CAP + SRCH_U];   // staged positions, one array per axis
     __shared__ int stp[SRCH_CAP + SRCH_U];
     __shared__ int smin[B / WAVE], smax[B / WAVE];
     const int i = lb * B + threadIdx.x;
@@ -1197,17 +1197,17 @@ __device__ __forceinline__ void search_lds_body(int lb, int n, const dvec *__res
             }
             if (bmin >= bmax) continue;   // uniform over the block
             if (bmax - bmin > SRCH_SPAN) {
-                for (int j0 = ra; j0 < rb; j0 += NB_U) {
-                    dvec xx[NB_U];
-                    int tt[NB_U];
+                for (int j0 = ra; j0 < rb; j0 += SRCH_FB) {
+                    dvec xx[SRCH_FB];
+                    int tt[SRCH_FB];
 #pragma unroll
-                    for (int u = 0; u < NB_U; u++)
+                    for (int u = 0; u < SRCH_FB; u++)
                         if (j0 + u < rb) {
                             xx[u] = X[j0 + u];
                             tt[u] = with_density ? T[j0 + u] : 0;
                         }
 #pragma unroll
-                    for (int u = 0; u < NB_U; u++) {
+                    for (int u = 0; u < SRCH_FB; u++) {
                         if (j0 + u >= rb) break;
                         search_pair(i, j0 + u, xi, xx[u], tt[u], a, out, k, with_density);
                     }
@@ -1218,7 +1218,9 @@ __device__ __forceinline__ void search_lds_body(int lb, int n, const dvec *__res
                 const int s1 = min(s0 + SRCH_CAP, bmax);
                 __syncthreads();
                 for (int q = threadIdx.x; q < s1 - s0; q += B) {
-                    sx[q] = X[s0 + q];
+                    const dvec xq = X[s0 + q];
+#pragma unroll
+                    for (int d = 0; d < D_; d++) sxc[d][q] = xq.c[d];
                     if (with_density) stp[q] = T[s0 + q];
                 }
                 __syncthreads();
@@ -1231,7 +1233,8 @@ __device__ __forceinline__ void search_lds_body(int lb, int n, const dvec *__res
                     int tt[SRCH_U];
 #pragma unroll
                     for (int u = 0; u < SRCH_U; u++) {
-                        xx[u] = sx[j0 + u - s0];
+#pragma unroll
+                        for (int d = 0; d < D_; d++) xx[u].c[d] = sxc[d][j0 + u - s0];
                         tt[u] = with_density ? stp[j0 + u - s0] : 0;
                     }
 #pragma unroll
@@ -1251,7 +1254,17 @@ __global__ void __launch_bounds__(B)
 k_search_lds(int n, const dvec *__restrict__ X, const int *__restrict__ T, const int *__restrict__ O,
              const int *__restrict__ cstart, SearchOut out, StepConsts k, int with_density)
 {
-    search_lds_body<B>(blockIdx.x, n, X, T, O, cstart, out, k, with_density);
+    search_lds_body<B, 4>(blockIdx.x, n, X, T, O, cstart, out, k, with_density);
+}
+
+// The same with two candidates per batch of LDS reads (option "search" 2):
+// fewer registers, so more waves per SIMD.
+template <int B>
+__global__ void __launch_bounds__(B)
+k_search_lds2(int n, const dvec *__restrict__ X, const int *__restrict__ T, const int *__restrict__ O,
+              const int *__restrict__ cstart, SearchOut out, StepConsts k, int with_density)
+{
+    search_lds_body<B, 2>(blockIdx.x, n, X, T, O, cstart, out, k, with_density);
 }
 
 int launch_search(Ctx *c, const StepConsts &k, int with_density, DevCounters *ctr)
@@ -1262,6 +1275,9 @@ int launch_search(Ctx *c, const StepConsts &k, int with_density, DevCounters *ct
     prof_begin(PK_SEARCH, c->stream);
     if (search_form() == 1)
         PARTICLE_LAUNCH(k_search_lds, c, c->n, c->X[c->cx], c->T[c->ct], c->O[c->ct], c->cstart, o, k,
+                        with_density);
+    else if (search_form() == 2)
+        PARTICLE_LAUNCH(k_search_lds2, c, c->n, c->X[c->cx], c->T[c->ct], c->O[c->ct], c->cstart, o, k,
                         with_density);
     else
         PARTICLE_LAUNCH(k_search, c, c->n, c->X[c->cx], c->T[c->ct], c->O[c->ct], c->cstart, o, k,
@@ -1808,16 +1824,38 @@ __global__ void k_counters_gather_flat(const FlatScene *__restrict__ tab, int ns
     for (int s = threadIdx.x; s < ns; s += blockDim.x) out[s] = *tab[s].ctr;
 }
 
+// The search over all scenes, one kernel per form (each holds only the
+// registers its form needs).
+#define SEARCH_FLAT_ARGS                                                                                  \
+    SearchOut o;                                                                                          \
+    o.nbr = S.nbr; o.ncnt = S.ncnt; o.surf = S.SURF[ct]; o.unk = S.unk; o.ncount = S.NCOUNT[ct];          \
+    o.nd = S.ND[ct]; o.diag = S.diag; o.b = S.b; o.ctr = S.ctr
+
 template <int B>
 __global__ void __launch_bounds__(B)
-k_search_flat(const FlatScene *__restrict__ tab, const int *__restrict__ first, int ns, int cx, int ct, int form)
+k_search_flat(const FlatScene *__restrict__ tab, const int *__restrict__ first, int ns, int cx, int ct)
 {
     FLAT_SCENE();
-    SearchOut o;
-    o.nbr = S.nbr; o.ncnt = S.ncnt; o.surf = S.SURF[ct]; o.unk = S.unk; o.ncount = S.NCOUNT[ct];
-    o.nd = S.ND[ct]; o.diag = S.diag; o.b = S.b; o.ctr = S.ctr;
-    if (form == 1) search_lds_body<B>(lb, S.n, S.X[cx], S.T[ct], S.O[ct], S.cstart, o, S.k, 1);
-    else search_hbm_body<B>(lb, S.n, S.X[cx], S.T[ct], S.O[ct], S.cstart, o, S.k, 1);
+    SEARCH_FLAT_ARGS;
+    search_lds_body<B, 4>(lb, S.n, S.X[cx], S.T[ct], S.O[ct], S.cstart, o, S.k, 1);
+}
+
+template <int B>
+__global__ void __launch_bounds__(B)
+k_search_flat2(const FlatScene *__restrict__ tab, const int *__restrict__ first, int ns, int cx, int ct)
+{
+    FLAT_SCENE();
+    SEARCH_FLAT_ARGS;
+    search_lds_body<B, 2>(lb, S.n, S.X[cx], S.T[ct], S.O[ct], S.cstart, o, S.k, 1);
+}
+
+template <int B>
+__global__ void __launch_bounds__(B)
+k_search_flat_hbm(const FlatScene *__restrict__ tab, const int *__restrict__ first, int ns, int cx, int ct)
+{
+    FLAT_SCENE();
+    SEARCH_FLAT_ARGS;
+    search_hbm_body<B>(lb, S.n, S.X[cx], S.T[ct], S.O[ct], S.cstart, o, S.k, 1);
 }
 
 template <int B>
@@ -1904,7 +1942,9 @@ int flat_step_pre(FlatLaunch &f)
     f.ct ^= 1;
     hipLaunchKernelGGL(k_counters_reset_flat, dim3(1), dim3(64), 0, f.s, f.tab, f.ns);
     prof_begin(PK_SEARCH, f.s);
-    FLAT_PARTICLE_LAUNCH(k_search_flat, f, f.tab, f.gp.first, f.ns, f.cx, f.ct, search_form());
+    if (search_form() == 1) FLAT_PARTICLE_LAUNCH(k_search_flat, f, f.tab, f.gp.first, f.ns, f.cx, f.ct);
+    else if (search_form() == 2) FLAT_PARTICLE_LAUNCH(k_search_flat2, f, f.tab, f.gp.first, f.ns, f.cx, f.ct);
+    else FLAT_PARTICLE_LAUNCH(k_search_flat_hbm, f, f.tab, f.gp.first, f.ns, f.cx, f.ct);
     prof_end(PK_SEARCH, f.s);
     LAUNCH_CHECK();
     prof_begin(PK_ASSEMBLE, f.s);

@@ -45,7 +45,7 @@ struct Options {
     double tol_mode = 0, precision = 0, inner_tol = 1e-3, exit_on_error = 1, check_every = 16;
     double spmv = 0, profile = 0, verbose = 0, fused = 2, graph = 0, pblk = 256;
     double async_output = 1;
-    double search = 1;                 // neighbour search: 1 candidates staged in LDS, 0 gathered from HBM
+    double search = 1;                 // neighbour search: 1 (2) candidates staged in LDS, read 4 (2) at a time; 0 gathered from HBM
     double batch_flat = 1;             // mps_hip_run_batch: 1 one launch per operator for all scenes, 0 per scene
     double ic_wait_limit = 1048576;   // unsuccessful polls before an IC(0) wait gives up
     double ic_sleep = 3;               // longest backoff of an idle IC(0) wave (0-3)
@@ -1234,7 +1234,7 @@ int mps_hip_set_option(const char *key, double value)
         g_opt.batch_flat = value;
     }
     else if (k == "search") {
-        if (value != 0 && value != 1) return MPS_ERR_ARG;
+        if (value != 0 && value != 1 && value != 2) return MPS_ERR_ARG;
         if (value != g_opt.search)
             for (auto &kv : g_ctx) graph_reset(kv.second);   // captures hold the kernel
         g_opt.search = value;

@@ -352,18 +352,19 @@ def test_particle_block_sizes_same_bits(lib3, graph):
 @pytest.mark.parametrize("name,libname", [("dam3d_mid", "lib3"), ("dam2d_l0.01", "lib2")])
 def test_search_forms_same_bits(request, name, libname):
     """The neighbour search with candidates staged in LDS (option "search"
-    1, the default) and gathered from HBM (0) give the same lists, in
-    order, and the same bits in every field over three steps, with plain
-    launches and with captured graphs; other values are refused."""
+    1, the default, and 2, reading two candidates at a time) and gathered
+    from HBM (0) give the same lists, in order, and the same bits in every
+    field over three steps, with plain launches and with captured graphs;
+    other values are refused."""
     lib = request.getfixturevalue(libname)
     dim, cfg, pos, vel, typ = H.build(name, cg_tolerance=1e-10)
     lib.set_option("tol_mode", 1)
     out = []
     try:
-        assert lib.mps_hip_set_option(b"search", 2.0) != 0
+        assert lib.mps_hip_set_option(b"search", 3.0) != 0
         for graph in (0, 1):
             lib.set_option("graph", graph)
-            for form in (1, 0):
+            for form in (1, 2, 0):
                 lib.set_option("search", form)
                 say(name, "search form", form, "graph", graph)
                 g = H.GpuRun(lib, cfg, pos, vel, typ)
