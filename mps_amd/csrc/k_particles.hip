@@ -549,6 +549,8 @@ int launch_friction(Ctx *c, const StepConsts &k)
 // before summing in list order: the gathers of a batch overlap.
 #define ASM_U 8    // neighbours per load batch of the assembly
 #define NB_U 4     // of the viscosity and pressure-gradient sums
+#define COL_U 8    // of the collision (4 here and in the assembly, for fewer
+                   // registers and more waves, measured 24-32 % slower)
 
 template <int B>
 __device__ __forceinline__ void visc_predict_body(int lb, int n, const dvec *__restrict__ X, const dvec *__restrict__ V, const int *__restrict__ T,
@@ -662,18 +664,18 @@ __device__ __forceinline__ void collide_bin_body(int lb, int n, const dvec *__re
 #pragma unroll
         for (int d = 0; d < D_; d++) w[d] = vi.c[d];
         const int m = ncnt[i];
-        for (int q0 = 0; q0 < m; q0 += NB_U * 2) {
-            int jj[NB_U * 2], tt[NB_U * 2];
-            dvec xx[NB_U * 2];
+        for (int q0 = 0; q0 < m; q0 += COL_U) {
+            int jj[COL_U], tt[COL_U];
+            dvec xx[COL_U];
 #pragma unroll
-            for (int u = 0; u < NB_U * 2; u++) jj[u] = (q0 + u < m) ? nbr[sell_off(i, q0 + u, k.stride)] : i;
+            for (int u = 0; u < COL_U; u++) jj[u] = (q0 + u < m) ? nbr[sell_off(i, q0 + u, k.stride)] : i;
 #pragma unroll
-            for (int u = 0; u < NB_U * 2; u++) {
+            for (int u = 0; u < COL_U; u++) {
                 tt[u] = T[jj[u]];
                 xx[u] = X[jj[u]];
             }
 #pragma unroll
-            for (int u = 0; u < NB_U * 2; u++) {
+            for (int u = 0; u < COL_U; u++) {
                 if (q0 + u >= m) break;
                 const int j = jj[u];
                 if (tt[u] == GHOST_PARTICLE) continue;
