@@ -274,7 +274,8 @@ def run_ensemble(lib, a):
         lib.check(lib.mps_hip_autotune(big[1], 1, C.byref(res)), "autotune")
         tune = {"scene_particles": big[2], "spmv": res.spmv, "check_every": res.check_every,
                 "spmv_ms": [round(v, 5) for v in res.spmv_ms], "solve_ms": [round(v, 2) for v in res.solve_ms],
-                "pblk": res.pblk, "pblk_ms": [round(v, 3) for v in res.pblk_ms]}
+                "pblk": res.pblk, "pblk_ms": [round(v, 3) for v in res.pblk_ms],
+                "search": res.search, "search_ms": [round(v, 3) for v in res.search_ms]}
     PSP = C.POINTER(lib.abi.ParticleSystem)
     ps_arr = (PSP * len(scenes))(*[m[1] for m in scenes])
     cf_arr = (C.POINTER(lib.abi.SimConfig) * len(scenes))(*[C.pointer(m[0]) for m in scenes])
@@ -282,7 +283,8 @@ def run_ensemble(lib, a):
         # the particle kernels' block size for the launches over all scenes
         res = mps_amd.TuneResult()
         lib.check(lib.mps_hip_autotune_batch(ps_arr, cf_arr, len(scenes), 1, C.byref(res)), "autotune_batch")
-        tune = dict(tune or {}, batch_pblk=res.pblk, batch_pblk_ms=[round(v, 3) for v in res.pblk_ms])
+        tune = dict(tune or {}, batch_pblk=res.pblk, batch_pblk_ms=[round(v, 3) for v in res.pblk_ms],
+                    batch_search=res.search, batch_search_ms=[round(v, 3) for v in res.search_ms])
     lib.check(lib.mps_hip_run_batch(ps_arr, cf_arr, len(scenes), a.warmup), "warm-up steps")
     t0 = time.perf_counter()
     lib.check(lib.mps_hip_run_batch(ps_arr, cf_arr, len(scenes), a.steps), "timed steps")
@@ -755,6 +757,7 @@ def main():
         tune = {"spmv": res.spmv, "check_every": res.check_every,
                 "spmv_ms": [round(v, 5) for v in res.spmv_ms], "solve_ms": [round(v, 2) for v in res.solve_ms],
                 "pblk": res.pblk, "pblk_ms": [round(v, 3) for v in res.pblk_ms],
+                "search": res.search, "search_ms": [round(v, 3) for v in res.search_ms],
                 "seconds": round(time.perf_counter() - t_tune, 2)}
     lib.check(lib.mps_hip_run_steps(ps, a.warmup), "warm-up steps")
     # the timed steps time only the dominant kernel's launches (two HIP

@@ -307,8 +307,9 @@ int  mps_hip_get_stats(ParticleSystem *ps, mps_step_stats *out);
 
 /* Autotuner (no counterpart in the reference): times the choices that
  * leave every result bit unchanged -- the CG SpMV kernel (option "spmv"),
- * the host's polling interval (option "check_every") and the block size of
- * the per-particle kernels (option "pblk") -- on a private device copy of
+ * the host's polling interval (option "check_every"), the block size of
+ * the per-particle kernels (option "pblk") and the search's read batch
+ * (option "search" 1 or 2) -- on a private device copy of
  * ps's host particles under g_config (one step's PPE, solved once per
  * candidate; the particle phases three times per block size).  ps and its
  * device state are not touched.  With apply != 0 the fastest choices
@@ -323,15 +324,21 @@ typedef struct {
     int    iters;           /* CG iterations of that solve                 */
     int    pblk;            /* fastest "pblk"                              */
     double pblk_ms[3];      /* particle phases of a step at block sizes
-                               128, 256, 512                               */
+                               128, 256, 512 (search option 1)             */
+    int    search;          /* faster "search" form, 1 or 2 (staged
+                               candidates read 4 or 2 at a time)           */
+    double search_ms[2];    /* particle phases at the chosen block size
+                               with search 1 and 2                         */
 } mps_tune_result;
 int  mps_hip_autotune(ParticleSystem *ps, int apply, mps_tune_result *out);
 /* Batch autotuner (no counterpart in the reference): the block size of the
- * per-particle kernels (option "pblk") for mps_hip_run_batch's launches
- * over all scenes -- the batch's particle phases timed at 128, 256 and 512
- * on private device copies of the scenes' host particles (pblk and
- * pblk_ms of *out; the other fields 0).  With apply != 0 the fastest
- * becomes the option.  3-D library; MPS_ERR_UNSUPPORTED otherwise. */
+ * per-particle kernels (option "pblk") and the search's read batch (option
+ * "search") for mps_hip_run_batch's launches over all scenes -- the
+ * batch's particle phases timed at 128, 256 and 512, then with search 2 at
+ * the best size, on private device copies of the scenes' host particles
+ * (pblk, pblk_ms, search and search_ms of *out; the other fields 0).  With
+ * apply != 0 the fastest choices become the options.  3-D library;
+ * MPS_ERR_UNSUPPORTED otherwise. */
 int  mps_hip_autotune_batch(ParticleSystem *const *ps, SimConfig *const *cfgs, int nscenes, int apply,
                             mps_tune_result *out);
 

@@ -31,7 +31,7 @@ class StepStats(C.Structure):
 class TuneResult(C.Structure):
     _fields_ = [("spmv", C.c_int), ("check_every", C.c_int), ("spmv_ms", C.c_double * 3),
                 ("solve_ms", C.c_double * 4), ("iters", C.c_int), ("pblk", C.c_int),
-                ("pblk_ms", C.c_double * 3)]
+                ("pblk_ms", C.c_double * 3), ("search", C.c_int), ("search_ms", C.c_double * 2)]
 
 
 _libs = {}

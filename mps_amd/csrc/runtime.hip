@@ -1480,6 +1480,7 @@ int mps_hip_autotune(ParticleSystem *ps, int apply, mps_tune_result *out)
     int best_p = 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = MPS_ERR_HIP;
+    g_opt.search = 1;
     for (int q = 0; q < 3 && !rc; q++) {
         g_opt.pblk = pb[q];
         rc = step_pre_launch(c, k);
@@ -1492,6 +1493,23 @@ int mps_hip_autotune(ParticleSystem *ps, int apply, mps_tune_result *out)
         out->pblk_ms[q] = ms / 3.0;
         if (!rc && (q == 0 || out->pblk_ms[q] < out->pblk_ms[best_p])) best_p = q;
     }
+    // at that block size, the search reading 4 (option "search" 1) or 2
+    // (2) staged candidates per batch
+    int best_s = 1;
+    out->search_ms[0] = out->pblk_ms[best_p];
+    if (!rc) {
+        g_opt.pblk = pb[best_p];
+        g_opt.search = 2;
+        rc = step_pre_launch(c, k);
+        if (!rc && hipEventRecord(e0, c->stream) != hipSuccess) rc = MPS_ERR_HIP;
+        for (int r = 0; r < 3 && !rc; r++) rc = step_pre_launch(c, k);
+        float ms = 0.0f;
+        if (!rc && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = MPS_ERR_HIP;
+        out->search_ms[1] = ms / 3.0;
+        if (!rc && out->search_ms[1] < out->search_ms[0]) best_s = 2;
+    }
     if (e0) hipEventDestroy(e0);
     if (e1) hipEventDestroy(e1);
     g_opt = saved;
@@ -1501,10 +1519,12 @@ int mps_hip_autotune(ParticleSystem *ps, int apply, mps_tune_result *out)
     out->check_every = ce[best_c];
     out->iters = sr.iters;
     out->pblk = pb[best_p];
+    out->search = best_s;
     if (apply) {
         g_opt.spmv = best_v;
         g_opt.check_every = ce[best_c];
         mps_hip_set_option("pblk", pb[best_p]);
+        mps_hip_set_option("search", best_s);
     }
     return MPS_OK;
 }
@@ -1536,11 +1556,18 @@ int mps_hip_autotune_batch(ParticleSystem *const *ps, SimConfig *const *cfgs, in
         rc = cs[q] ? ensure_resident(cs[q], &shadow[q], cfgs[q]) : g_last_error;
     }
     const int pb[3] = {128, 256, 512};
-    int best = 0;
+    int best = 0, best_s = 1;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = MPS_ERR_HIP;
-    for (int k = 0; k < 3 && !rc; k++) {
-        g_opt.pblk = pb[k];
+    // block sizes with search form 1, then form 2 at the best size
+    g_opt.search = 1;
+    for (int k = 0; k < 4 && !rc; k++) {
+        if (k == 3) {
+            g_opt.search = 2;
+            g_opt.pblk = pb[best];
+        } else {
+            g_opt.pblk = pb[k];
+        }
         std::vector<StepConsts> ks(nscenes);
         FlatLaunch f;
         hipStream_t s = cs[0]->stream;
@@ -1557,8 +1584,14 @@ int mps_hip_autotune_batch(ParticleSystem *const *ps, SimConfig *const *cfgs, in
             cs[q]->ct = f.ct;
             cs[q]->cp = f.cp;
         }
-        out->pblk_ms[k] = ms / 3.0;
-        if (!rc && k > 0 && out->pblk_ms[k] < out->pblk_ms[best]) best = k;
+        if (k < 3) {
+            out->pblk_ms[k] = ms / 3.0;
+            if (!rc && k > 0 && out->pblk_ms[k] < out->pblk_ms[best]) best = k;
+        } else {
+            out->search_ms[0] = out->pblk_ms[best];
+            out->search_ms[1] = ms / 3.0;
+            if (!rc && out->search_ms[1] < out->search_ms[0]) best_s = 2;
+        }
     }
     if (e0) hipEventDestroy(e0);
     if (e1) hipEventDestroy(e1);
@@ -1568,7 +1601,11 @@ int mps_hip_autotune_batch(ParticleSystem *const *ps, SimConfig *const *cfgs, in
         if (cs[q]) mps_hip_release(&shadow[q]);
     if (rc) return g_last_error = rc;
     out->pblk = pb[best];
-    if (apply) mps_hip_set_option("pblk", pb[best]);
+    out->search = best_s;
+    if (apply) {
+        mps_hip_set_option("pblk", pb[best]);
+        mps_hip_set_option("search", best_s);
+    }
     return MPS_OK;
 #else
     (void)apply;

@@ -308,6 +308,9 @@ def test_autotune_keeps_bits(lib3):
                 assert res.pblk in (128, 256, 512) and min(res.pblk_ms) > 0
                 assert res.pblk_ms[(128, 256, 512).index(res.pblk)] == min(res.pblk_ms)
                 assert lib3.L.mps_hip_get_option(b"pblk") == res.pblk
+                assert res.search in (1, 2) and min(res.search_ms) > 0
+                assert res.search_ms[res.search - 1] == min(res.search_ms)
+                assert lib3.L.mps_hip_get_option(b"search") == res.search
             for _ in range(2):
                 assert g.step() == 0
             out.append([np.ascontiguousarray(g.scene.arr[f]).tobytes() for f in H.FIELDS])
@@ -316,6 +319,7 @@ def test_autotune_keeps_bits(lib3):
         lib3.set_option("spmv", 0)
         lib3.set_option("check_every", 16)
         lib3.set_option("pblk", 256)
+        lib3.set_option("search", 1)
     for f, a, b in zip(H.FIELDS, out[0], out[1]):
         assert a == b, f
 
@@ -415,6 +419,7 @@ def test_autotune_batch_keeps_bits(lib3):
         _batch_equals_single(lib3, tune=True)
     finally:
         lib3.set_option("pblk", 256)
+        lib3.set_option("search", 1)
 
 
 @pytest.mark.parametrize("graph", [0, 1])
@@ -489,9 +494,11 @@ def _batch_equals_single(lib3, pre=(0, 0, 0), graph=0, tune=False, scenes_w=SCEN
         import mps_amd
         res = mps_amd.TuneResult()
         lib3.check(lib3.mps_hip_autotune_batch(ps_arr, cf_arr, len(made), 1, C.byref(res)), "autotune_batch")
-        say("batch tuner: pblk", res.pblk, "ms", list(res.pblk_ms))
+        say("batch tuner: pblk", res.pblk, "ms", list(res.pblk_ms), "search", res.search, list(res.search_ms))
         assert res.pblk in (128, 256, 512) and lib3.mps_hip_get_option(b"pblk") == res.pblk
         assert all(v > 0 for v in res.pblk_ms)
+        assert res.search in (1, 2) and lib3.mps_hip_get_option(b"search") == res.search
+        assert res.search_ms[res.search - 1] == min(res.search_ms)
     say("batch of", len(made), "graph", graph)
     lib3.set_option("graph", graph)
     try:
