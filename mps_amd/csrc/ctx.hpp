@@ -215,6 +215,12 @@ struct ScanSeg {
     int n;
     int *tot;
 };
+__device__ __forceinline__ ScanSeg seg_global(const ScanSeg &s)
+{
+    ScanSeg g = s;
+    g.in = gptr(s.in); g.out = gptr(s.out); g.tot = s.tot ? gptr(s.tot) : nullptr;
+    return g;
+}
 // A set of segments on the device: segs[0..nseg), tile0[0..nseg] (the
 // segments' first 2048-element tiles; tile0[nseg] = all tiles).
 struct ScanSegs {

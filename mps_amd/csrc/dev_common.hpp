@@ -95,6 +95,21 @@ __device__ __forceinline__ int cell_coords(const Grid &g, const dvec &p, int c[3
     return cell_key(g, c[0], c[1], c[2]);
 }
 
+// A pointer that a kernel reads out of a table in device memory (the
+// per-scene tables of the batched launches) is generic to the compiler, so
+// every access through it becomes a flat instruction.  Flat accesses count
+// against the LDS wait counter as well as the memory one, so each wait for
+// an LDS read then also waits for every flat load or store in flight.  The
+// tables only hold device buffers: gptr() reads the field as a global
+// pointer, and the accesses through the result compile to global ones.
+template <typename T>
+__device__ __forceinline__ T *gptr(T *const &field)
+{
+    typedef __attribute__((address_space(1))) T GT;
+    GT *g = *(GT *const *)&field;
+    return (T *)g;
+}
+
 // ---------------------------------------------------------------------
 // XCD-aware block order.  The dispatcher hands workgroups to the 8 XCDs
 // round-robin (block b runs on XCD b % 8) and each XCD has its own L2.

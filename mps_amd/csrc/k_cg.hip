@@ -97,6 +97,17 @@ struct PkView {
     const int *c32;
 };
 
+// The view with its buffers as global pointers (gptr), for views read out
+// of a per-scene table.
+__device__ __forceinline__ PkView pk_global(const PkView &s)
+{
+    PkView g;
+    g.len = gptr(s.len); g.smeta = gptr(s.smeta); g.bmeta = gptr(s.bmeta); g.halo = gptr(s.halo);
+    g.diag = gptr(s.diag); g.val = gptr(s.val); g.ddir = gptr(s.ddir); g.fval = gptr(s.fval);
+    g.chw = gptr(s.chw); g.c32 = gptr(s.c32);
+    return g;
+}
+
 static PkView pk_view(const Packed *p)
 {
     PkView v;
@@ -1593,10 +1604,19 @@ struct BScene {
     double tol;
 };
 
+// A scene of the table with its buffers as global pointers (gptr).
+__device__ __forceinline__ BScene bscene_global(const BScene &s)
+{
+    BScene g = s;
+    g.P = pk_global(s.P);
+    g.b = gptr(s.b); g.x = gptr(s.x); g.r = gptr(s.r); g.p = gptr(s.p); g.Ap = gptr(s.Ap);
+    return g;
+}
+
 __global__ void __launch_bounds__(CG_BLK)
 k_b_init(const BScene *__restrict__ sc, const int *__restrict__ axs, double *part2)
 {
-    const BScene &S = sc[axs[blockIdx.x]];
+    const BScene S = bscene_global(sc[axs[blockIdx.x]]);
     const int j = blockIdx.x - S.ax_off;
     double v = 0.0;
     for (int i = j * CG_BLK + threadIdx.x; i < S.n_eq; i += S.nax * CG_BLK) {
@@ -1613,7 +1633,7 @@ k_b_init(const BScene *__restrict__ sc, const int *__restrict__ axs, double *par
 __global__ void __launch_bounds__(CG_BLK)
 k_b_init_fin(const BScene *__restrict__ sc, const double *part2, CgCtrl *ctrl, int *ndone)
 {
-    const BScene &S = sc[blockIdx.x];
+    const BScene S = bscene_global(sc[blockIdx.x]);
     CgCtrl *c = ctrl + blockIdx.x;
     const double bb = fin_sum(part2 + S.ax_off, S.nax);
     if (threadIdx.x == 0) {
@@ -1639,7 +1659,7 @@ k_b_spmv(const BScene *__restrict__ sc, const int *__restrict__ bs, double *part
     double *hx = reinterpret_cast<double *>(smem_raw);
     const int si = bs[blockIdx.x];
     if (ctrl[si].done) return;
-    const BScene &S = sc[si];
+    const BScene S = bscene_global(sc[si]);
     const int lb = blockIdx.x - S.blk_off;
     const double *p = S.p;
     stage_halo_of(S.P, lb, XPlain<double>{p}, hx);
@@ -1661,7 +1681,7 @@ k_b_spmv_fin(const BScene *__restrict__ sc, const double *part, CgCtrl *ctrl, in
     CgCtrl *c = ctrl + blockIdx.x;
     if (c->done) return;
     const int t = ndone[1];
-    const BScene &S = sc[blockIdx.x];
+    const BScene S = bscene_global(sc[blockIdx.x]);
     const double pap = fin_sum(part + S.blk_off, S.nblk);
     if (threadIdx.x == 0) {
         c->pap = pap;
@@ -1682,7 +1702,7 @@ k_b_axpy(const BScene *__restrict__ sc, const int *__restrict__ axs, double *par
 {
     const int si = axs[blockIdx.x];
     if (ctrl[si].done) return;
-    const BScene &S = sc[si];
+    const BScene S = bscene_global(sc[si]);
     const int j = blockIdx.x - S.ax_off;
     const double alpha = ctrl[si].alpha;
     double v = 0.0;
@@ -1721,7 +1741,7 @@ k_b_axpy_fin(const BScene *__restrict__ sc, const double *part2, CgCtrl *ctrl, i
     CgCtrl *c = ctrl + blockIdx.x;
     if (c->done) return;
     const int t = ndone[1];
-    const BScene &S = sc[blockIdx.x];
+    const BScene S = bscene_global(sc[blockIdx.x]);
     const double rr_new = fin_sum(part2 + S.ax_off, S.nax);
     if (threadIdx.x == 0) {
         const double rr = c->rr[t & 1];
@@ -1756,7 +1776,7 @@ k_b_pupdate(const BScene *__restrict__ sc, const int *__restrict__ axs, const Cg
     if (blockIdx.x == 0 && threadIdx.x == 0) ndone[1] += 1;
     const int si = axs[blockIdx.x];
     if (ctrl[si].done) return;
-    const BScene &S = sc[si];
+    const BScene S = bscene_global(sc[si]);
     const int j = blockIdx.x - S.ax_off;
     const double beta = ctrl[si].beta;
     const int st = S.nax * CG_BLK;
@@ -1978,10 +1998,20 @@ struct BMScene {
     double tol, delta, neg_coeff, re, r_min;
 };
 
+template <typename V>
+__device__ __forceinline__ BMScene<V> bmscene_global(const BMScene<V> &s)
+{
+    BMScene<V> g = s;
+    g.P = pk_global(s.P);
+    g.b = gptr(s.b); g.x = gptr(s.x); g.r = gptr(s.r); g.p = gptr(s.p); g.Ap = gptr(s.Ap);
+    g.cq = gptr(s.cq);
+    return g;
+}
+
 __global__ void __launch_bounds__(CG_BLK)
 k_bm_q16_build(const BMScene<float> *__restrict__ sc, const int *__restrict__ bs)
 {
-    const BMScene<float> &S = sc[bs[blockIdx.x]];
+    const BMScene<float> S = bmscene_global(sc[bs[blockIdx.x]]);
     const int k = (blockIdx.x - S.blk_off) * CG_BLK + threadIdx.x;
     if (k < S.n_eq) q16_build_row(k, S.P, S.neg_coeff, S.re, S.r_min, S.cq);
 }
@@ -1990,7 +2020,7 @@ template <typename V>
 __global__ void __launch_bounds__(CG_BLK)
 k_bm_init(const BMScene<V> *__restrict__ sc, const int *__restrict__ axs, double *part2, double *part3)
 {
-    const BMScene<V> &S = sc[axs[blockIdx.x]];
+    const BMScene<V> S = bmscene_global(sc[axs[blockIdx.x]]);
     const int j = blockIdx.x - S.ax_off;
     double v = 0.0, w = 0.0;
     for (int i = j * CG_BLK + threadIdx.x; i < S.n_eq; i += S.nax * CG_BLK) {
@@ -2015,7 +2045,7 @@ __global__ void __launch_bounds__(CG_BLK)
 k_bm_init_fin(const BMScene<V> *__restrict__ sc, const double *part2, const double *part3, MixedCtrl *ctrl,
               int *ndone)
 {
-    const BMScene<V> &S = sc[blockIdx.x];
+    const BMScene<V> S = bmscene_global(sc[blockIdx.x]);
     MixedCtrl *c = ctrl + blockIdx.x;
     const double rr = fin_sum(part2 + S.ax_off, S.nax);
     const double bb = fin_sum(part3 + S.ax_off, S.nax);
@@ -2047,7 +2077,7 @@ k_bm_spmv(const BMScene<V> *__restrict__ sc, const int *__restrict__ bs, double 
     V *hx = reinterpret_cast<V *>(smem_raw);
     const int si = bs[blockIdx.x];
     if (ctrl[si].done) return;
-    const BMScene<V> &S = sc[si];
+    const BMScene<V> S = bmscene_global(sc[si]);
     const int lb = blockIdx.x - S.blk_off;
     const V *p = S.p;
     stage_halo_of(S.P, lb, XPlain<V>{p}, hx);
@@ -2075,7 +2105,7 @@ k_bm_fin1(const BMScene<V> *__restrict__ sc, const double *part, const double *p
     MixedCtrl *c = ctrl + blockIdx.x;
     if (c->done) return;
     const int t = ndone[1];
-    const BMScene<V> &S = sc[blockIdx.x];
+    const BMScene<V> S = bmscene_global(sc[blockIdx.x]);
     const double pap = fin_sum(part + S.blk_off, S.nblk);
     const int ru = c->ru;
     double rr_true = 0.0;
@@ -2112,7 +2142,7 @@ k_bm_axpy(const BMScene<V> *__restrict__ sc, const int *__restrict__ axs, double
 {
     const int si = axs[blockIdx.x];
     if (ctrl[si].done) return;
-    const BMScene<V> &S = sc[si];
+    const BMScene<V> S = bmscene_global(sc[si]);
     const int j = blockIdx.x - S.ax_off;
     const double alpha = ctrl[si].alpha;
     const V av = (V)alpha;
@@ -2154,7 +2184,7 @@ k_bm_fin2(const BMScene<V> *__restrict__ sc, const double *part2, MixedCtrl *ctr
     MixedCtrl *c = ctrl + blockIdx.x;
     if (c->done) return;
     const int t = ndone[1];
-    const BMScene<V> &S = sc[blockIdx.x];
+    const BMScene<V> S = bmscene_global(sc[blockIdx.x]);
     const double rr_new = fin_sum(part2 + S.ax_off, S.nax);
     if (threadIdx.x == 0) {
         const double rr = c->rr;
@@ -2181,7 +2211,7 @@ k_bm_pupdate(const BMScene<V> *__restrict__ sc, const int *__restrict__ pbs, dou
     if (blockIdx.x == 0 && threadIdx.x == 0) ndone[1] += 1;
     const int si = pbs[blockIdx.x];
     if (ctrl[si].done) return;
-    const BMScene<V> &S = sc[si];
+    const BMScene<V> S = bmscene_global(sc[si]);
     const int c0 = (blockIdx.x - S.pb_off) * PU_RB;
     const int nch = min(PU_RB, S.nblk - c0);
     pupdate_chunks<U, V, PU_RB>(S.P, S.b, S.x, S.r, S.p, S.n_eq, c0, nch, (V)ctrl[si].beta, ctrl[si].ru,

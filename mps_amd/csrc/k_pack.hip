@@ -573,6 +573,22 @@ struct FlatPack {
     double *ddir, *px, *x;
 };
 
+// A scene of the table with its buffers as global pointers (gptr).
+__device__ __forceinline__ FlatPack pack_global(const FlatPack &s)
+{
+    FlatPack g = s;
+    g.unk = gptr(s.unk); g.Alen = gptr(s.Alen); g.Acol = gptr(s.Acol);
+    g.Aval = gptr(s.Aval); g.diag = gptr(s.diag); g.b = gptr(s.b);
+    g.row_of_slot = gptr(s.row_of_slot); g.slot_of_row = gptr(s.slot_of_row);
+    g.sw = gptr(s.sw); g.whw = gptr(s.whw); g.w32 = gptr(s.w32); g.bh = gptr(s.bh);
+    g.bmode = gptr(s.bmode); g.dmax = gptr(s.dmax); g.voff = gptr(s.voff); g.chwoff = gptr(s.chwoff);
+    g.c32off = gptr(s.c32off); g.hoff = gptr(s.hoff); g.halo = gptr(s.halo);
+    g.val = gptr(s.val); g.chw = gptr(s.chw); g.c32 = gptr(s.c32); g.len = gptr(s.len);
+    g.pdiag = gptr(s.pdiag); g.pb = gptr(s.pb); g.smeta = gptr(s.smeta); g.bmeta = gptr(s.bmeta);
+    g.fval = gptr(s.fval); g.ddir = gptr(s.ddir); g.px = gptr(s.px); g.x = gptr(s.x);
+    return g;
+}
+
 struct FlatPackBufs {
     std::vector<FlatPack> h;
     FlatPack *tab = nullptr;
@@ -652,7 +668,7 @@ static int fp_table(Ctx *const *cs, int ns, hipStream_t s)
 
 #define FLAT_PACK()                                         \
     const int s_ = flat_scene(first, ns, blockIdx.x);       \
-    const FlatPack &P = tab[s_];                            \
+    const FlatPack P = pack_global(tab[s_]);                \
     const int lb = blockIdx.x - first[s_]
 
 __global__ void __launch_bounds__(PKB)

@@ -1650,7 +1650,7 @@ __global__ void __launch_bounds__(BLK) k_sscan_tiles(const ScanSeg *__restrict__
                                                      const int *__restrict__ tile0, int nseg, int *blk)
 {
     const int g = flat_scene(tile0, nseg, blockIdx.x);
-    const ScanSeg &S = segs[g];
+    const ScanSeg S = seg_global(segs[g]);
     const int lt = blockIdx.x - tile0[g];
     const int base = lt * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     int s = 0;
@@ -1666,7 +1666,7 @@ __global__ void __launch_bounds__(1024) k_sscan_tile_sums(const ScanSeg *__restr
                                                           const int *__restrict__ tile0, int *blk)
 {
     const int g = blockIdx.x;
-    const ScanSeg &S = segs[g];
+    const ScanSeg S = seg_global(segs[g]);
     const int ntiles = tile0[g + 1] - tile0[g];
     int *tile_sum = blk + tile0[g] + g;
     const int base = threadIdx.x * 8;   // 1024 threads x 8 tiles each
@@ -1695,7 +1695,7 @@ __global__ void __launch_bounds__(BLK) k_sscan_apply(const ScanSeg *__restrict__
                                                      const int *__restrict__ tile0, int nseg, const int *blk)
 {
     const int g = flat_scene(tile0, nseg, blockIdx.x);
-    const ScanSeg &S = segs[g];
+    const ScanSeg S = seg_global(segs[g]);
     const int lt = blockIdx.x - tile0[g];
     const int ntiles = tile0[g + 1] - tile0[g];
     const int *tile_sum = blk + tile0[g] + g;
@@ -1769,7 +1769,7 @@ __global__ void __launch_bounds__(B)
 k_visc_predict_flat(const FlatScene *__restrict__ tab, const int *__restrict__ first, int ns, int cx, int ct)
 {
     FLAT_SCENE();
-    visc_predict_body<B>(lb, S.n, S.X[cx], S.V[cx], S.T[ct], S.nbr, S.ncnt, S.X[1 - cx], S.V[1 - cx], S.k);
+    visc_predict_body<B>(lb, S.n, gptr(S.X[cx]), gptr(S.V[cx]), gptr(S.T[ct]), gptr(S.nbr), gptr(S.ncnt), gptr(S.X[1 - cx]), gptr(S.V[1 - cx]), S.k);
 }
 
 __global__ void __launch_bounds__(BLK)
@@ -1778,8 +1778,8 @@ k_cells_zero_flat(const FlatScene *__restrict__ tab, const int *__restrict__ fir
     FLAT_SCENE();
     const int c = lb * BLK + threadIdx.x;
     if (c > S.total) return;
-    S.ccount[c] = 0;
-    S.cfill[c] = 0;
+    gptr(S.ccount)[c] = 0;
+    gptr(S.cfill)[c] = 0;
 }
 
 template <int B>
@@ -1787,15 +1787,15 @@ __global__ void __launch_bounds__(B)
 k_collide_bin_flat(const FlatScene *__restrict__ tab, const int *__restrict__ first, int ns, int cx, int ct)
 {
     FLAT_SCENE();
-    collide_bin_body<B>(lb, S.n, S.X[cx], S.V[cx], S.T[ct], S.nbr, S.ncnt, S.X[1 - cx], S.V[1 - cx], S.cell,
-                        S.ccount, S.k, S.k.collide);
+    collide_bin_body<B>(lb, S.n, gptr(S.X[cx]), gptr(S.V[cx]), gptr(S.T[ct]), gptr(S.nbr), gptr(S.ncnt), gptr(S.X[1 - cx]), gptr(S.V[1 - cx]), gptr(S.cell),
+                        gptr(S.ccount), S.k, S.k.collide);
 }
 
 __global__ void __launch_bounds__(BLK)
 k_scatter_flat(const FlatScene *__restrict__ tab, const int *__restrict__ first, int ns, int ct)
 {
     FLAT_SCENE();
-    scatter_body(lb, S.n, S.cell, S.O[ct], S.cstart, S.cfill, S.tmp_key, S.sflag, S.total);
+    scatter_body(lb, S.n, gptr(S.cell), gptr(S.O[ct]), gptr(S.cstart), gptr(S.cfill), gptr(S.tmp_key), gptr(S.sflag), S.total);
 }
 
 __global__ void __launch_bounds__(BLK)
@@ -1803,13 +1803,13 @@ k_rank_move_flat(const FlatScene *__restrict__ tab, const int *__restrict__ firs
 {
     FLAT_SCENE();
     SortMove mv;
-    mv.src.X = S.X[cx]; mv.src.V = S.V[cx]; mv.src.A = S.ACC[ct]; mv.src.P = S.P[cp]; mv.src.ND = S.ND[ct];
-    mv.src.T = S.T[ct]; mv.src.O = S.O[ct]; mv.src.S = S.SURF[ct]; mv.src.NC = S.NCOUNT[ct];
-    mv.dst.X = S.X[1 - cx]; mv.dst.V = S.V[1 - cx]; mv.dst.A = S.ACC[1 - ct]; mv.dst.ND = S.ND[1 - ct];
-    mv.dst.T = S.T[1 - ct]; mv.dst.O = S.O[1 - ct]; mv.dst.S = S.SURF[1 - ct]; mv.dst.NC = S.NCOUNT[1 - ct];
-    mv.dst.P = S.P[1 - cp];
+    mv.src.X = gptr(S.X[cx]); mv.src.V = gptr(S.V[cx]); mv.src.A = gptr(S.ACC[ct]); mv.src.P = gptr(S.P[cp]); mv.src.ND = gptr(S.ND[ct]);
+    mv.src.T = gptr(S.T[ct]); mv.src.O = gptr(S.O[ct]); mv.src.S = gptr(S.SURF[ct]); mv.src.NC = gptr(S.NCOUNT[ct]);
+    mv.dst.X = gptr(S.X[1 - cx]); mv.dst.V = gptr(S.V[1 - cx]); mv.dst.A = gptr(S.ACC[1 - ct]); mv.dst.ND = gptr(S.ND[1 - ct]);
+    mv.dst.T = gptr(S.T[1 - ct]); mv.dst.O = gptr(S.O[1 - ct]); mv.dst.S = gptr(S.SURF[1 - ct]); mv.dst.NC = gptr(S.NCOUNT[1 - ct]);
+    mv.dst.P = gptr(S.P[1 - cp]);
     mv.full = 0;
-    rank_move_body(lb, S.n, S.cell, S.cstart, S.tmp_key, S.sscan, S.total, mv);
+    rank_move_body(lb, S.n, gptr(S.cell), gptr(S.cstart), gptr(S.tmp_key), gptr(S.sscan), S.total, mv);
 }
 
 __global__ void k_counters_reset_flat(const FlatScene *__restrict__ tab, int ns)
@@ -1830,8 +1830,8 @@ __global__ void k_counters_gather_flat(const FlatScene *__restrict__ tab, int ns
 // registers its form needs).
 #define SEARCH_FLAT_ARGS                                                                                  \
     SearchOut o;                                                                                          \
-    o.nbr = S.nbr; o.ncnt = S.ncnt; o.surf = S.SURF[ct]; o.unk = S.unk; o.ncount = S.NCOUNT[ct];          \
-    o.nd = S.ND[ct]; o.diag = S.diag; o.b = S.b; o.ctr = S.ctr
+    o.nbr = gptr(S.nbr); o.ncnt = gptr(S.ncnt); o.surf = gptr(S.SURF[ct]); o.unk = gptr(S.unk); o.ncount = gptr(S.NCOUNT[ct]);          \
+    o.nd = gptr(S.ND[ct]); o.diag = gptr(S.diag); o.b = gptr(S.b); o.ctr = gptr(S.ctr)
 
 template <int B>
 __global__ void __launch_bounds__(B)
@@ -1839,7 +1839,7 @@ k_search_flat(const FlatScene *__restrict__ tab, const int *__restrict__ first, 
 {
     FLAT_SCENE();
     SEARCH_FLAT_ARGS;
-    search_lds_body<B, 4>(lb, S.n, S.X[cx], S.T[ct], S.O[ct], S.cstart, o, S.k, 1);
+    search_lds_body<B, 4>(lb, S.n, gptr(S.X[cx]), gptr(S.T[ct]), gptr(S.O[ct]), gptr(S.cstart), o, S.k, 1);
 }
 
 template <int B>
@@ -1848,7 +1848,7 @@ k_search_flat2(const FlatScene *__restrict__ tab, const int *__restrict__ first,
 {
     FLAT_SCENE();
     SEARCH_FLAT_ARGS;
-    search_lds_body<B, 2>(lb, S.n, S.X[cx], S.T[ct], S.O[ct], S.cstart, o, S.k, 1);
+    search_lds_body<B, 2>(lb, S.n, gptr(S.X[cx]), gptr(S.T[ct]), gptr(S.O[ct]), gptr(S.cstart), o, S.k, 1);
 }
 
 template <int B>
@@ -1857,7 +1857,7 @@ k_search_flat_hbm(const FlatScene *__restrict__ tab, const int *__restrict__ fir
 {
     FLAT_SCENE();
     SEARCH_FLAT_ARGS;
-    search_hbm_body<B>(lb, S.n, S.X[cx], S.T[ct], S.O[ct], S.cstart, o, S.k, 1);
+    search_hbm_body<B>(lb, S.n, gptr(S.X[cx]), gptr(S.T[ct]), gptr(S.O[ct]), gptr(S.cstart), o, S.k, 1);
 }
 
 template <int B>
@@ -1865,22 +1865,22 @@ __global__ void __launch_bounds__(B)
 k_assemble_flat(const FlatScene *__restrict__ tab, const int *__restrict__ first, int ns, int cx, int ct)
 {
     FLAT_SCENE();
-    assemble_body<B>(lb, S.n, S.X[cx], S.V[cx], S.T[ct], S.ND[ct], S.unk, S.nbr, S.ncnt, S.Aval, S.Acol, S.Alen,
-                     S.b, S.ctr, S.k);
+    assemble_body<B>(lb, S.n, gptr(S.X[cx]), gptr(S.V[cx]), gptr(S.T[ct]), gptr(S.ND[ct]), gptr(S.unk), gptr(S.nbr), gptr(S.ncnt), gptr(S.Aval), gptr(S.Acol), gptr(S.Alen),
+                     gptr(S.b), gptr(S.ctr), S.k);
 }
 
 __global__ void __launch_bounds__(BLK)
 k_pressure_out_flat(const FlatScene *__restrict__ tab, const int *__restrict__ first, int ns, int ct, int cp)
 {
     FLAT_SCENE();
-    pressure_out_body(lb, S.n, S.x, S.unk, S.T[ct], S.P[cp], S.k.clamp);
+    pressure_out_body(lb, S.n, gptr(S.x), gptr(S.unk), gptr(S.T[ct]), gptr(S.P[cp]), S.k.clamp);
 }
 
 __global__ void __launch_bounds__(BLK)
 k_pmin_flat(const FlatScene *__restrict__ tab, const int *__restrict__ first, int ns, int cp)
 {
     FLAT_SCENE();
-    pmin_body(lb, S.n, S.P[cp], S.nbr, S.ncnt, S.pmin, S.k.stride);
+    pmin_body(lb, S.n, gptr(S.P[cp]), gptr(S.nbr), gptr(S.ncnt), gptr(S.pmin), S.k.stride);
 }
 
 template <int B>
@@ -1890,10 +1890,10 @@ k_grad_correct_flat(const FlatScene *__restrict__ tab, const int *__restrict__ f
 {
     FLAT_SCENE();
     GradIO io;
-    io.X = S.X[cx]; io.V = S.V[cx]; io.P = S.P[cp]; io.pmin = S.pmin;
-    io.Xo = S.X[1 - cx]; io.Vo = S.V[1 - cx]; io.A = S.ACC[ct];
-    io.Po = S.P[1 - cp]; io.ND = S.ND[ct]; io.T = S.T[ct];
-    grad_correct_body<B>(lb, S.n, io, S.nbr, S.ncnt, S.k);
+    io.X = gptr(S.X[cx]); io.V = gptr(S.V[cx]); io.P = gptr(S.P[cp]); io.pmin = gptr(S.pmin);
+    io.Xo = gptr(S.X[1 - cx]); io.Vo = gptr(S.V[1 - cx]); io.A = gptr(S.ACC[ct]);
+    io.Po = gptr(S.P[1 - cp]); io.ND = gptr(S.ND[ct]); io.T = gptr(S.T[ct]);
+    grad_correct_body<B>(lb, S.n, io, gptr(S.nbr), gptr(S.ncnt), S.k);
 }
 
 // Per-particle kernel at the block size of option "pblk" over the scenes.
