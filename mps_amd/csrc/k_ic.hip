@@ -510,6 +510,143 @@ __device__ __forceinline__ void finish_row(int e, int m, double akk, const doubl
 // factor, one-phase form (any row length)
 // ---------------------------------------------------------------------
 
+// One row e of the factor (one-phase form, any row length), claimed by the
+// calling wave: lane T owns entries T, T + 64, ... in groups of 64; row e's
+// columns and finished values sit in scol/sval (LDS, m entries each).
+__device__ __forceinline__ void factor_row_any(int e, int lane, const int *__restrict__ rp, const int *__restrict__ col,
+                                               const int *__restrict__ dpos, const double *__restrict__ aval,
+                                               double *lval, double *dinv, int dense, long long wait_limit,
+                                               int sleep_cap, int *scol, double *sval, bool &failed)
+{
+    const int r0 = rp[e], m = dpos[e];
+    wave_sync();
+    for (int t = lane; t < m; t += WAVE) scol[t] = col[r0 + t];
+    wave_sync();
+    const int pstep = m > 0 ? (1 << (31 - __clz(m))) : 0;
+    for (int g0 = 0; g0 < m; g0 += WAVE) {
+        const int T = g0 + lane;
+        const bool act = T < m;
+        int k = 0, kb = 0, mk = 0;
+        double a = 0.0;
+        if (act) {
+            k = scol[T];
+            a = aval[r0 + T];
+            kb = rp[k];
+            mk = dpos[k];
+        }
+        int stage = act ? 0 : 2;   // 0: waiting for row k, 1: folding, 2: done
+        double dk = 0.0, s = dense ? 0.0 : a;
+        int u = 0, bn = 0, bi = 0;
+        int bpos[FB];
+        double bval[FB];
+#pragma unroll
+        for (int j = 0; j < FB; j++) { bpos[j] = -1; bval[j] = 0.0; }
+        long long spins = 0;
+        bool poll_all = true;
+        int idle = 0;
+        for (;;) {
+            if (ballot64(stage != 2) == 0ull) break;
+            const unsigned long long donem = ballot64(stage == 2);
+            const unsigned long long waitm = ballot64(stage == 0);
+            bool prog = false;
+            if (waitm) {
+                const int lo = __ffsll((long long)waitm) - 1;
+                if (stage == 0 && (poll_all || lane == lo)) {
+                    const double d = ld_rlx(&dinv[k]);
+                    if (!is_sent(d)) {
+                        dk = d;
+                        stage = 1;
+                        prog = true;
+                        spins = 0;
+                    } else if (++spins > wait_limit) {
+                        failed = true;   // give up on row k: its terms read as 0
+                        stage = 1;
+                    }
+                }
+                poll_all = __shfl((int)(stage != 0), lo, WAVE) != 0;
+            }
+            if (stage == 1 && bi == bn) {
+                if (u >= mk) {
+                    double L;
+                    if (!dense) L = s * dk;
+                    else L = (fabs(a) < 1.0e-30) ? 0.0 : (a - s) * (1.0 / dk);   // dk holds sqrt(d) here
+                    lds_st(&sval[T], L);
+                    st_rlx(&lval[r0 + T], L);
+                    stage = 2;
+                    prog = true;
+                } else {
+                    bn = min(FB, mk - u);
+                    int bc[FB];
+#pragma unroll
+                    for (int j = 0; j < FB; j++) {
+                        bc[j] = INT_MAX;
+                        if (j < bn) bc[j] = col[kb + u + j];
+                    }
+#pragma unroll
+                    for (int j = 0; j < FB; j++) bval[j] = j < bn ? lval[kb + u + j] : 0.0;   // cached; see k_ic_factor2
+                    // row k's columns are all below k = scol[T]: find each
+                    // among this row's columns 0..T-1 (last index < c)
+                    int ps[FB];
+#pragma unroll
+                    for (int j = 0; j < FB; j++) ps[j] = -1;
+                    for (int st = pstep; st > 0; st >>= 1) {
+#pragma unroll
+                        for (int j = 0; j < FB; j++) {
+                            const int cand = ps[j] + st;
+                            if (cand < T && scol[cand] < bc[j]) ps[j] = cand;
+                        }
+                    }
+#pragma unroll
+                    for (int j = 0; j < FB; j++) {
+                        const int cand = ps[j] + 1;
+                        bpos[j] = (j < bn && cand < T && scol[cand] == bc[j]) ? cand : -1;
+                    }
+                    bi = 0;
+                    u += bn;
+                    prog = true;
+                }
+            }
+            if (stage == 1 && bi < bn) {
+                bool stop = false;
+#pragma unroll
+                for (int j = 0; j < FB; j++) {
+                    if (!stop && j == bi && j < bn) {
+                        const int v = bpos[j];
+                        if (v < 0) {
+                            bi++;
+                            prog = true;
+                        } else {
+                            double lk = bval[j];
+                            if (is_sent(lk)) {
+                                lk = ld_rlx(&lval[kb + u - bn + j]);
+                                bval[j] = lk;
+                            }
+                            const bool ready = !is_sent(lk) && (v < g0 || ((donem >> (v - g0)) & 1ull));
+                            if (ready) {
+                                const double lv = lds_ld(&sval[v]);
+                                if (!dense) s -= lv * lk;
+                                else s += lv * lk;
+                                bi++;
+                                prog = true;
+                                spins = 0;
+                            } else if (++spins > wait_limit) {
+                                failed = true;   // give up on this term
+                                bi++;
+                            } else {
+                                stop = true;
+                            }
+                        }
+                    }
+                }
+            }
+            if (__any(prog)) idle = 0;
+            else backoff(idle++, sleep_cap);
+        }
+        wave_sync();
+    }
+    finish_row(e, m, aval[r0 + m], sval, dinv, dense, lane);
+}
+
 __global__ void __launch_bounds__(IC_BLK)
 k_ic_factor(int n, const int *__restrict__ order, const int *__restrict__ rp, const int *__restrict__ col,
             const int *__restrict__ dpos, const double *__restrict__ aval, double *lval, double *dinv,
@@ -526,139 +663,16 @@ k_ic_factor(int n, const int *__restrict__ order, const int *__restrict__ rp, co
         tk = __shfl(tk, 0, WAVE);
         if (tk >= n) break;
         const int e = order[tk];   // rows in level order: every row waited on is claimed earlier
-        const int r0 = rp[e], m = dpos[e];
-        wave_sync();
-        for (int t = lane; t < m; t += WAVE) scol[t] = col[r0 + t];
-        wave_sync();
-        const int pstep = m > 0 ? (1 << (31 - __clz(m))) : 0;
-        for (int g0 = 0; g0 < m; g0 += WAVE) {
-            const int T = g0 + lane;
-            const bool act = T < m;
-            int k = 0, kb = 0, mk = 0;
-            double a = 0.0;
-            if (act) {
-                k = scol[T];
-                a = aval[r0 + T];
-                kb = rp[k];
-                mk = dpos[k];
-            }
-            int stage = act ? 0 : 2;   // 0: waiting for row k, 1: folding, 2: done
-            double dk = 0.0, s = dense ? 0.0 : a;
-            int u = 0, bn = 0, bi = 0;
-            int bpos[FB];
-            double bval[FB];
-#pragma unroll
-            for (int j = 0; j < FB; j++) { bpos[j] = -1; bval[j] = 0.0; }
-            long long spins = 0;
-            bool poll_all = true;
-            int idle = 0;
-            for (;;) {
-                if (ballot64(stage != 2) == 0ull) break;
-                const unsigned long long donem = ballot64(stage == 2);
-                const unsigned long long waitm = ballot64(stage == 0);
-                bool prog = false;
-                if (waitm) {
-                    const int lo = __ffsll((long long)waitm) - 1;
-                    if (stage == 0 && (poll_all || lane == lo)) {
-                        const double d = ld_rlx(&dinv[k]);
-                        if (!is_sent(d)) {
-                            dk = d;
-                            stage = 1;
-                            prog = true;
-                            spins = 0;
-                        } else if (++spins > wait_limit) {
-                            failed = true;   // give up on row k: its terms read as 0
-                            stage = 1;
-                        }
-                    }
-                    poll_all = __shfl((int)(stage != 0), lo, WAVE) != 0;
-                }
-                if (stage == 1 && bi == bn) {
-                    if (u >= mk) {
-                        double L;
-                        if (!dense) L = s * dk;
-                        else L = (fabs(a) < 1.0e-30) ? 0.0 : (a - s) * (1.0 / dk);   // dk holds sqrt(d) here
-                        lds_st(&sval[T], L);
-                        st_rlx(&lval[r0 + T], L);
-                        stage = 2;
-                        prog = true;
-                    } else {
-                        bn = min(FB, mk - u);
-                        int bc[FB];
-#pragma unroll
-                        for (int j = 0; j < FB; j++) {
-                            bc[j] = INT_MAX;
-                            if (j < bn) bc[j] = col[kb + u + j];
-                        }
-#pragma unroll
-                        for (int j = 0; j < FB; j++) bval[j] = j < bn ? lval[kb + u + j] : 0.0;   // cached; see k_ic_factor2
-                        // row k's columns are all below k = scol[T]: find each
-                        // among this row's columns 0..T-1 (last index < c)
-                        int ps[FB];
-#pragma unroll
-                        for (int j = 0; j < FB; j++) ps[j] = -1;
-                        for (int st = pstep; st > 0; st >>= 1) {
-#pragma unroll
-                            for (int j = 0; j < FB; j++) {
-                                const int cand = ps[j] + st;
-                                if (cand < T && scol[cand] < bc[j]) ps[j] = cand;
-                            }
-                        }
-#pragma unroll
-                        for (int j = 0; j < FB; j++) {
-                            const int cand = ps[j] + 1;
-                            bpos[j] = (j < bn && cand < T && scol[cand] == bc[j]) ? cand : -1;
-                        }
-                        bi = 0;
-                        u += bn;
-                        prog = true;
-                    }
-                }
-                if (stage == 1 && bi < bn) {
-                    bool stop = false;
-#pragma unroll
-                    for (int j = 0; j < FB; j++) {
-                        if (!stop && j == bi && j < bn) {
-                            const int v = bpos[j];
-                            if (v < 0) {
-                                bi++;
-                                prog = true;
-                            } else {
-                                double lk = bval[j];
-                                if (is_sent(lk)) {
-                                    lk = ld_rlx(&lval[kb + u - bn + j]);
-                                    bval[j] = lk;
-                                }
-                                const bool ready = !is_sent(lk) && (v < g0 || ((donem >> (v - g0)) & 1ull));
-                                if (ready) {
-                                    const double lv = lds_ld(&sval[v]);
-                                    if (!dense) s -= lv * lk;
-                                    else s += lv * lk;
-                                    bi++;
-                                    prog = true;
-                                    spins = 0;
-                                } else if (++spins > wait_limit) {
-                                    failed = true;   // give up on this term
-                                    bi++;
-                                } else {
-                                    stop = true;
-                                }
-                            }
-                        }
-                    }
-                }
-                if (__any(prog)) idle = 0;
-                else backoff(idle++, sleep_cap);
-            }
-            wave_sync();
-        }
-        finish_row(e, m, aval[r0 + m], sval, dinv, dense, lane);
+        factor_row_any(e, lane, rp, col, dpos, aval, lval, dinv, dense, wait_limit, sleep_cap, scol, sval, failed);
     }
     if (ballot64(failed) && lane == 0) atomicOr(err, 1);
 }
 
 // Two-phase form of the factor for rows of at most 64 strictly-lower
-// entries (lcap <= 64), one wavefront per workgroup.  Lane t owns entry t,
+// entries (lcap <= 64, its LDS layout), one wavefront per workgroup; a
+// longer row (up to lcap_all entries) takes the one-phase form's row in the
+// same wave, with the same arithmetic, its columns and values in the same
+// LDS.  Lane t owns entry t,
 // (e, k).  Once row k is published the whole wave loads row k (its at most
 // 64 lower entries, entry j in lane j: two coalesced loads), up to FG2 such
 // rows in flight, and keeps each match -- the position v in this row of a
@@ -673,7 +687,7 @@ k_ic_factor(int n, const int *__restrict__ order, const int *__restrict__ rp, co
 __global__ void __launch_bounds__(WAVE)
 k_ic_factor2(int n, const int *__restrict__ order, const int *__restrict__ rp, const int *__restrict__ col,
              const int *__restrict__ dpos, const double *__restrict__ aval, double *lval, double *dinv,
-             unsigned *ticket, int *err, int dense, int lcap, long long wait_limit, int sleep_cap)
+             unsigned *ticket, int *err, int dense, int lcap, int lcap_all, long long wait_limit, int sleep_cap)
 {
     extern __shared__ double f2_smem[];
     const int lane = threadIdx.x;
@@ -690,6 +704,12 @@ k_ic_factor2(int n, const int *__restrict__ order, const int *__restrict__ rp, c
         if (tk >= n) break;
         const int e = order[tk];   // rows in level order: every row waited on is claimed earlier
         const int r0 = rp[e], m = dpos[e];
+        if (m > lcap) {
+            int *gcol = (int *)f2_smem;
+            factor_row_any(e, lane, rp, col, dpos, aval, lval, dinv, dense, wait_limit, sleep_cap, gcol,
+                           (double *)(gcol + lcap_all), failed);
+            continue;
+        }
         wave_sync();
         if (lane < m) scol[lane] = col[r0 + lane];
         wave_sync();
@@ -769,29 +789,53 @@ k_ic_factor2(int n, const int *__restrict__ order, const int *__restrict__ rp, c
                 for (int g = 0; g < FG2; g++) {
                     const int Tg = tg[g];
                     if (Tg < 0) continue;
-                    if (ballot64(lane < mkg[g] && is_sent(vj[g]))) {
+                    // row k in pieces of 64 entries (a row k past 64 entries
+                    // is one of the long rows the one-phase form takes; its
+                    // later pieces are loaded here, one at a time); each
+                    // piece's matches follow the earlier ones in lane Tg's
+                    // list, so the list stays in row k's entry order
+                    const int mkk = mkg[g];
+                    const bool again = mkk > WAVE && __shfl((int)(spins != 0), Tg, WAVE) != 0;
+                    bool missing = false;
+                    int base = 0;
+                    for (int c0 = 0; c0 < mkk; c0 += WAVE) {
+                        int cjc = cj[g];
+                        double vjc = vj[g];
+                        if (c0 > 0) {
+                            cjc = INT_MAX;
+                            vjc = 0.0;
+                            if (c0 + lane < mkk) {
+                                cjc = col[kbg[g] + c0 + lane];
+                                vjc = again ? ld_rlx(&lval[kbg[g] + c0 + lane]) : lval[kbg[g] + c0 + lane];
+                            }
+                        }
+                        if (ballot64(c0 + lane < mkk && is_sent(vjc))) missing = true;
+                        int ps = -1;
+                        for (int st = pstep; st > 0; st >>= 1) {
+                            const int cand = ps + st;
+                            if (cand < Tg && scol[cand] < cjc) ps = cand;
+                        }
+                        ps++;
+                        const bool hit = c0 + lane < mkk && ps < Tg && scol[ps] == cjc;
+                        const unsigned long long hm = ballot64(hit);
+                        if (hit) {
+                            const int q = Tg * (Tg - 1) / 2 + base + __popcll(hm & ((1ull << lane) - 1ull));
+                            mpos[q] = (unsigned short)ps;
+                            mval[q] = is_sent(vjc) ? 0.0 : vjc;
+                        }
+                        base += __popcll(hm);
+                    }
+                    if (missing) {
                         // row k's 1/sqrt(d) came before some of its values:
-                        // load it again next time, or give up at the bound
+                        // load it again next time (the list is rewritten
+                        // then), or give up at the bound
                         int quit = 0;
                         if (lane == Tg) quit = ++spins > wait_limit;
                         if (!__shfl(quit, Tg, WAVE)) continue;
                         if (lane == Tg) failed = true;   // missing values read as 0
                     }
-                    int ps = -1;
-                    for (int st = pstep; st > 0; st >>= 1) {
-                        const int cand = ps + st;
-                        if (cand < Tg && scol[cand] < cj[g]) ps = cand;
-                    }
-                    ps++;
-                    const bool hit = lane < mkg[g] && ps < Tg && scol[ps] == cj[g];
-                    const unsigned long long hm = ballot64(hit);
-                    if (hit) {
-                        const int q = Tg * (Tg - 1) / 2 + __popcll(hm & ((1ull << lane) - 1ull));
-                        mpos[q] = (unsigned short)ps;
-                        mval[q] = is_sent(vj[g]) ? 0.0 : vj[g];
-                    }
                     if (lane == Tg) {
-                        cnt = __popcll(hm);
+                        cnt = base;
                         stage = 2;
                         spins = 0;
                     }
@@ -1545,12 +1589,15 @@ static int ic_factor(IcData *d, hipStream_t s, int dense)
     const double og = mps_hip_get_option("ic_fgrid");
     const unsigned grid = og >= 1 ? (unsigned)og : cu_count() * (wpb == 4 ? 4 : 8);
     prof_begin(PK_ICF, s);
-    if (lcap <= WAVE && mps_hip_get_option("ic_factor_form") != 1) {
-        // two-phase form: one wave per workgroup, match lists in LDS
-        const size_t lds2 = (size_t)lcap * 12 + (size_t)(lcap * (lcap - 1) / 2) * 10 + 16;
+    if (mps_hip_get_option("ic_factor_form") != 1) {
+        // two-phase form: one wave per workgroup, match lists in LDS for
+        // rows of up to 64 entries; longer rows in the one-phase form
+        const int l2 = lcap < WAVE ? lcap : WAVE;
+        size_t lds2 = (size_t)l2 * 12 + (size_t)(l2 * (l2 - 1) / 2) * 10 + 16;
+        if ((size_t)lcap * 12 > lds2) lds2 = (size_t)lcap * 12;
         const unsigned grid2 = og >= 1 ? (unsigned)og : cu_count() * 8;
         hipLaunchKernelGGL(k_ic_factor2, dim3(grid2), dim3(WAVE), lds2, s, n, d->order, d->rp, d->col, d->dpos,
-                           d->aval, d->lval, d->dinv, d->ticket, d->err, dense, lcap, wait_limit(), sleep_cap());
+                           d->aval, d->lval, d->dinv, d->ticket, d->err, dense, l2, lcap, wait_limit(), sleep_cap());
     } else {
         hipLaunchKernelGGL(k_ic_factor, dim3(grid), dim3(WAVE * wpb), lds, s, n, d->order, d->rp, d->col, d->dpos,
                            d->aval, d->lval, d->dinv, d->ticket, d->err, dense, lcap, wait_limit(), sleep_cap());

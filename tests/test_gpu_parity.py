@@ -162,6 +162,30 @@ def test_iccg_exact_factor(lib3, lib2, dim, name):
     g.close()
 
 
+@pytest.mark.parametrize("ratio", [3.2, 4.0])
+def test_iccg_exact_factor_long_rows(lib3, ratio):
+    """ICCG with PPE rows of more than 64 strictly-lower entries (Laplacian
+    radius 3.2 and 4.0 l0: 73 and 125 in the lattice's interior, fewer at
+    the walls and the surface), as a compressed flow also makes at 3.1 l0:
+    the two-phase factor takes the long rows in the one-phase form, the
+    short ones in its own, and the factor is the reference's bit for bit."""
+    g, o, recs = _step_compare(lib3, "dam3d_small", 1, solver_type=1, influence_ratio_lap=ratio)
+    r = recs[0]
+    assert r["n_exact"] and r["flags_exact"]
+    rp, col, val, dinv = _export_ic(lib3, g.scene)
+    lower = np.diff(rp)
+    say("ratio", ratio, "rows", len(lower), "longest strictly-lower row", int(lower.max()),
+        "rows above 64:", int((lower > 64).sum()))
+    assert lower.max() > 64 and (lower <= 64).any()
+    oq = o.ppe()
+    assert np.array_equal(rp, oq["l_ptr"].astype(np.int32))
+    assert np.array_equal(col, oq["l_col"])
+    assert np.array_equal(val.view(np.uint64), oq["l_val"].view(np.uint64))
+    assert np.array_equal(dinv.view(np.uint64), oq["l_diag"].view(np.uint64))
+    assert r["p_gpu"] <= max(100 * r["p_spread"], 1e-10)
+    g.close()
+
+
 def _laplacian(nx, ny, shift=0.05):
     n = nx * ny
     rows, cols, vals = [], [], []
