@@ -432,14 +432,14 @@ __global__ void k_ic_inverse(int n, const int *perm, int *pos)
 __global__ void __launch_bounds__(IC_BLK)
 k_ic_bucket_hist(int n, const double *lev, int gshift, int nbk, int *bh)
 {
-    __shared__ int h[IC_BLK];
+    extern __shared__ int h[];   // nbk counters
     const int t = threadIdx.x, nb = gridDim.x;
-    h[t] = 0;
+    for (int q = t; q < nbk; q += IC_BLK) h[q] = 0;
     __syncthreads();
     const int e = blockIdx.x * IC_BLK + t;
     if (e < n) atomicAdd(&h[(int)lev[e] >> gshift], 1);
     __syncthreads();
-    if (t < nbk) bh[(size_t)t * nb + blockIdx.x] = h[t];
+    for (int q = t; q < nbk; q += IC_BLK) bh[(size_t)q * nb + blockIdx.x] = h[q];
 }
 
 // Stable scatter: row e goes to its bucket's offset for this block plus its
@@ -1024,6 +1024,16 @@ k_ic_csolve(int n, const int *__restrict__ perm, const double *__restrict__ lev,
             const double *__restrict__ rin, double *y, unsigned *ticket, int *err, long long wait_limit,
             int sleep_cap, const IcCtrl *ctrl)
 {
+    // the chunk's entries (columns and values, one contiguous range),
+    // right-hand side, 1/sqrt(d) and entry offsets, staged in one pass when
+    // the chunk is claimed (chunks of at most ECAP entries; a larger chunk
+    // reads its entries from HBM row by row), so that a row's loads are off
+    // the chain of rows that wait on one another
+    constexpr int ECAP = CS_CHUNK <= 32 ? 2048 : 4096;
+    __shared__ int sec[ECAP];
+    __shared__ double sev[ECAP];
+    __shared__ double srin[CS_CHUNK], sdr[CS_CHUNK];
+    __shared__ int seo[CS_CHUNK + 1];
     __shared__ double sy[CS_CHUNK];
     __shared__ int sflag[CS_CHUNK];
     __shared__ int skey[CS_CHUNK], srow[CS_CHUNK];
@@ -1049,8 +1059,21 @@ k_ic_csolve(int n, const int *__restrict__ perm, const double *__restrict__ lev,
         // permutation of the chunk
         if (threadIdx.x < cnt_rows) {
             const int t = cbase + threadIdx.x;
-            const int L = (int)lev[perm ? perm[t] : t];
+            const int r = perm ? perm[t] : t;
+            const int L = (int)lev[r];
             skey[threadIdx.x] = BACK ? -L : L;
+            srin[threadIdx.x] = rin[t];
+            sdr[threadIdx.x] = dinv[r];
+        }
+        if (threadIdx.x <= cnt_rows) seo[threadIdx.x] = eoff[cbase + threadIdx.x];
+        const int e0 = eoff[cbase], ne = eoff[cbase + cnt_rows] - e0;
+        const bool staged = ne <= ECAP;   // uniform over the block
+        if (staged) {
+            for (int q = threadIdx.x; q < ne; q += CS_WAVES * WAVE) {
+                const int c = ecol[e0 + q];
+                sec[q] = c;
+                sev[q] = eval[e0 + q];
+            }
         }
         __syncthreads();
         if (threadIdx.x < cnt_rows) {
@@ -1069,10 +1092,9 @@ k_ic_csolve(int n, const int *__restrict__ perm, const double *__restrict__ lev,
             i = __shfl(i, 0, WAVE);
             if (i >= cnt_rows) break;
             const int t = cbase + srow[i];
-            const int r = perm ? perm[t] : t;
-            const int o0 = eoff[t], m = eoff[t + 1] - o0;
-            const double rv = rin[t];
-            const double dr = dinv[r];
+            const int o0 = seo[t - cbase], m = seo[t - cbase + 1] - o0;
+            const double rv = srin[t - cbase];
+            const double dr = sdr[t - cbase];
             double s = MODE == 0 ? rv : 0.0;
             for (int g0 = 0; g0 < m; g0 += WAVE) {
                 const int cnt = min(WAVE, m - g0);
@@ -1081,8 +1103,9 @@ k_ic_csolve(int n, const int *__restrict__ perm, const double *__restrict__ lev,
                 bool in = false;
                 double l = 0.0, v = 0.0;
                 if (act) {
-                    c = ecol[o0 + g0 + lane];
-                    l = eval[o0 + g0 + lane];
+                    const int q = o0 + g0 + lane;
+                    c = staged ? sec[q - e0] : ecol[q];
+                    l = staged ? sev[q - e0] : eval[q];
                     in = c >= cbase && c < cbase + cnt_rows;
                     // first load through the caches: y held the sentinel
                     // before this kernel and each value is written once, so
@@ -1483,7 +1506,14 @@ static unsigned cu_count()
 // claimed in index order, only the few slabs of the lattice that fit the
 // resident waves overlap, and the chain they leave is far longer than the
 // DAG's depth.
-#define LEVEL_BUCKET 32   // levels per bucket of the solve order (at least)
+// levels per bucket of the solve order (at least): with 16 a 32-position
+// chunk of the solves holds two neighbouring lattice columns, so the
+// dependency on the next column stays in the chunk as well as the one along
+// the column (forward solve at 1.08M particles 3.56-3.62 ms; 32 levels
+// 4.04-4.11 ms, 8 levels 4.43-4.50 ms)
+#define LEVEL_BUCKET 16
+#define MAX_BUCKETS 1024  // buckets of the solve order (at most; their histograms are sized for it)
+#define SCAN_MAX ((size_t)8192 * 2048)   // longest exclusive scan launch_scan_excl takes
 
 static int ic_order(IcData *d, hipStream_t s)
 {
@@ -1516,17 +1546,20 @@ static int ic_order(IcData *d, hipStream_t s)
     if (want >= 0) gshift = (int)want;
     else
         while ((1 << gshift) < LEVEL_BUCKET) gshift++;
-    while (((d->levels - 1) >> gshift) >= IC_BLK) gshift++;
+    const unsigned nb = nblk(n);
+    while (((d->levels - 1) >> gshift) >= MAX_BUCKETS ||
+           ((size_t)(((d->levels - 1) >> gshift) + 1) * nb + 1 > SCAN_MAX))
+        gshift++;
     const int nbk = ((d->levels - 1) >> gshift) + 1;
     d->bucket = 1 << gshift;
-    const unsigned nb = nblk(n);
     const size_t nbh = (size_t)nbk * nb;
     if (nbh + 1 > d->cap_bh) {
         REGROW(d->bh, nbh + 1);
         REGROW(d->boff, nbh + 1);
         d->cap_bh = nbh + 1;
     }
-    hipLaunchKernelGGL(k_ic_bucket_hist, dim3(nb), dim3(IC_BLK), 0, s, n, d->lev, gshift, nbk, d->bh);
+    hipLaunchKernelGGL(k_ic_bucket_hist, dim3(nb), dim3(IC_BLK), sizeof(int) * (size_t)nbk, s, n, d->lev, gshift,
+                       nbk, d->bh);
     LAUNCH_CHECK();
     rc = launch_scan_excl(d->bh, d->boff, (int)nbh, d->scan_blk, s);
     if (rc) return rc;
@@ -2014,7 +2047,7 @@ static size_t noalloc_bytes(int n, long long nnz, int maxlen, long long nl, bool
     const size_t ea = (size_t)((n + WAVE - 1) / WAVE) * WAVE * (size_t)(maxlen > 0 ? maxlen : 1);
     t += carve_bytes(sizeof(int), ea) + carve_bytes(sizeof(double), ea);
     if (ic) {
-        const size_t nbh = (size_t)IC_BLK * nblk((size_t)n) + 1;
+        const size_t nbh = (size_t)MAX_BUCKETS * nblk((size_t)n) + 1;
         t += 2 * carve_bytes(sizeof(int), nbh);
         const size_t L = (size_t)(nl > 0 ? nl : 1);
         t += 2 * carve_bytes(sizeof(int), L) + 2 * carve_bytes(sizeof(double), L);
