@@ -376,7 +376,8 @@ int  mps_hip_solve_csr(int n, const int *row_ptr, const int *col,
  * of the process.  It calls no hipMalloc, hipFree or other allocating
  * routine, for callers that cannot allocate (serving loops, harnesses that
  * forbid it).  No counterpart in the reference.  Solves that do not fit the
- * array (about 36 bytes per entry plus 12 per ELL slot and 170 per unknown:
+ * array (about 36 bytes per entry plus 12 per ELL slot and 170 per unknown,
+ * 194 for ICCG:
  * a 3-D PPE of 15,000 unknowns with rows of up to 123 entries takes 71 MB)
  * fail with MPS_ERR_ARG and a
  * message naming the need; nothing is written to x then.  Every call on one
