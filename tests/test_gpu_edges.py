@@ -295,3 +295,33 @@ def test_iccg_wait_bound_reports_solver_error(request, dim, name):
     g = H.GpuRun(lib, cfg, pos, vel, typ)
     assert g.step() == 0
     g.close()
+
+
+def test_brute_force_search_sets(lib3):
+    """neighbor_search_brute_force (neighbor_search.c:44-81) on the GPU:
+    every particle's list is the oracle's cell-search set, in ascending
+    index order as the reference's O(N^2) loop lists it, on an off-lattice
+    scene with ghosts (which list nothing and appear in no list)."""
+    dim, cfg, pos, vel, typ = H.build("dam3d_small")
+    rng = np.random.default_rng(5)
+    pos = pos + rng.normal(0, 0.2 * 0.01, pos.shape)
+    typ = typ.copy()
+    typ[rng.choice(len(typ), 50, replace=False)] = 2
+    o = H.OracleRun(cfg, pos, vel, typ, dim)
+    want = o.lists()
+    lib3.set_config(cfg)
+    sc = A.HostScene(lib3.abi, pos, vel, typ)
+    n = sc.num
+    re_s = max(cfg.influence_radius_lap, cfg.influence_radius_n)
+    nl = lib3.neighbor_list_create(n, cfg.max_neighbors)
+    say("brute force search, N=%d" % n)
+    lib3.neighbor_search_brute_force(nl, sc.ptr(), re_s)
+    assert lib3.mps_hip_last_error() == 0
+    got = A.padded_lists(nl.contents, n)
+    for i in range(n):
+        g = np.asarray(got[i])
+        assert np.all(np.diff(g) > 0), i
+        assert np.array_equal(g, np.sort(np.asarray(want[i]))), i
+    assert all(len(got[i]) == 0 for i in np.nonzero(typ == 2)[0])
+    lib3.mps_hip_release(sc.ptr())
+    lib3.neighbor_list_free(nl)
