@@ -1759,6 +1759,9 @@ int launch_scan_seg(const ScanSegs &ss, int *blk, hipStream_t s)
     return 0;
 }
 
+// A block's scene S and its block index lb within the scene.  The kernels
+// below pass S's buffers on through gptr(): read from the table, they are
+// global pointers to the compiler (flat ones otherwise).
 #define FLAT_SCENE()                                        \
     const int s_ = flat_scene(first, ns, blockIdx.x);       \
     const FlatScene &S = tab[s_];                           \
@@ -1769,7 +1772,8 @@ __global__ void __launch_bounds__(B)
 k_visc_predict_flat(const FlatScene *__restrict__ tab, const int *__restrict__ first, int ns, int cx, int ct)
 {
     FLAT_SCENE();
-    visc_predict_body<B>(lb, S.n, gptr(S.X[cx]), gptr(S.V[cx]), gptr(S.T[ct]), gptr(S.nbr), gptr(S.ncnt), gptr(S.X[1 - cx]), gptr(S.V[1 - cx]), S.k);
+    visc_predict_body<B>(lb, S.n, gptr(S.X[cx]), gptr(S.V[cx]), gptr(S.T[ct]), gptr(S.nbr), gptr(S.ncnt),
+                         gptr(S.X[1 - cx]), gptr(S.V[1 - cx]), S.k);
 }
 
 __global__ void __launch_bounds__(BLK)
@@ -1787,7 +1791,8 @@ __global__ void __launch_bounds__(B)
 k_collide_bin_flat(const FlatScene *__restrict__ tab, const int *__restrict__ first, int ns, int cx, int ct)
 {
     FLAT_SCENE();
-    collide_bin_body<B>(lb, S.n, gptr(S.X[cx]), gptr(S.V[cx]), gptr(S.T[ct]), gptr(S.nbr), gptr(S.ncnt), gptr(S.X[1 - cx]), gptr(S.V[1 - cx]), gptr(S.cell),
+    collide_bin_body<B>(lb, S.n, gptr(S.X[cx]), gptr(S.V[cx]), gptr(S.T[ct]), gptr(S.nbr), gptr(S.ncnt),
+                        gptr(S.X[1 - cx]), gptr(S.V[1 - cx]), gptr(S.cell),
                         gptr(S.ccount), S.k, S.k.collide);
 }
 
@@ -1795,7 +1800,8 @@ __global__ void __launch_bounds__(BLK)
 k_scatter_flat(const FlatScene *__restrict__ tab, const int *__restrict__ first, int ns, int ct)
 {
     FLAT_SCENE();
-    scatter_body(lb, S.n, gptr(S.cell), gptr(S.O[ct]), gptr(S.cstart), gptr(S.cfill), gptr(S.tmp_key), gptr(S.sflag), S.total);
+    scatter_body(lb, S.n, gptr(S.cell), gptr(S.O[ct]), gptr(S.cstart), gptr(S.cfill), gptr(S.tmp_key), gptr(S.sflag),
+                 S.total);
 }
 
 __global__ void __launch_bounds__(BLK)
@@ -1803,10 +1809,13 @@ k_rank_move_flat(const FlatScene *__restrict__ tab, const int *__restrict__ firs
 {
     FLAT_SCENE();
     SortMove mv;
-    mv.src.X = gptr(S.X[cx]); mv.src.V = gptr(S.V[cx]); mv.src.A = gptr(S.ACC[ct]); mv.src.P = gptr(S.P[cp]); mv.src.ND = gptr(S.ND[ct]);
+    mv.src.X = gptr(S.X[cx]); mv.src.V = gptr(S.V[cx]); mv.src.A = gptr(S.ACC[ct]); mv.src.P = gptr(S.P[cp]);
+    mv.src.ND = gptr(S.ND[ct]);
     mv.src.T = gptr(S.T[ct]); mv.src.O = gptr(S.O[ct]); mv.src.S = gptr(S.SURF[ct]); mv.src.NC = gptr(S.NCOUNT[ct]);
-    mv.dst.X = gptr(S.X[1 - cx]); mv.dst.V = gptr(S.V[1 - cx]); mv.dst.A = gptr(S.ACC[1 - ct]); mv.dst.ND = gptr(S.ND[1 - ct]);
-    mv.dst.T = gptr(S.T[1 - ct]); mv.dst.O = gptr(S.O[1 - ct]); mv.dst.S = gptr(S.SURF[1 - ct]); mv.dst.NC = gptr(S.NCOUNT[1 - ct]);
+    mv.dst.X = gptr(S.X[1 - cx]); mv.dst.V = gptr(S.V[1 - cx]); mv.dst.A = gptr(S.ACC[1 - ct]);
+    mv.dst.ND = gptr(S.ND[1 - ct]);
+    mv.dst.T = gptr(S.T[1 - ct]); mv.dst.O = gptr(S.O[1 - ct]); mv.dst.S = gptr(S.SURF[1 - ct]);
+    mv.dst.NC = gptr(S.NCOUNT[1 - ct]);
     mv.dst.P = gptr(S.P[1 - cp]);
     mv.full = 0;
     rank_move_body(lb, S.n, gptr(S.cell), gptr(S.cstart), gptr(S.tmp_key), gptr(S.sscan), S.total, mv);
@@ -1830,8 +1839,9 @@ __global__ void k_counters_gather_flat(const FlatScene *__restrict__ tab, int ns
 // registers its form needs).
 #define SEARCH_FLAT_ARGS                                                                                  \
     SearchOut o;                                                                                          \
-    o.nbr = gptr(S.nbr); o.ncnt = gptr(S.ncnt); o.surf = gptr(S.SURF[ct]); o.unk = gptr(S.unk); o.ncount = gptr(S.NCOUNT[ct]);          \
-    o.nd = gptr(S.ND[ct]); o.diag = gptr(S.diag); o.b = gptr(S.b); o.ctr = gptr(S.ctr)
+    o.nbr = gptr(S.nbr); o.ncnt = gptr(S.ncnt); o.surf = gptr(S.SURF[ct]); o.unk = gptr(S.unk);           \
+    o.ncount = gptr(S.NCOUNT[ct]); o.nd = gptr(S.ND[ct]); o.diag = gptr(S.diag); o.b = gptr(S.b);         \
+    o.ctr = gptr(S.ctr)
 
 template <int B>
 __global__ void __launch_bounds__(B)
@@ -1865,7 +1875,8 @@ __global__ void __launch_bounds__(B)
 k_assemble_flat(const FlatScene *__restrict__ tab, const int *__restrict__ first, int ns, int cx, int ct)
 {
     FLAT_SCENE();
-    assemble_body<B>(lb, S.n, gptr(S.X[cx]), gptr(S.V[cx]), gptr(S.T[ct]), gptr(S.ND[ct]), gptr(S.unk), gptr(S.nbr), gptr(S.ncnt), gptr(S.Aval), gptr(S.Acol), gptr(S.Alen),
+    assemble_body<B>(lb, S.n, gptr(S.X[cx]), gptr(S.V[cx]), gptr(S.T[ct]), gptr(S.ND[ct]), gptr(S.unk), gptr(S.nbr),
+                     gptr(S.ncnt), gptr(S.Aval), gptr(S.Acol), gptr(S.Alen), 
                      gptr(S.b), gptr(S.ctr), S.k);
 }
 
