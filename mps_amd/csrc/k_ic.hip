@@ -1276,9 +1276,12 @@ k_pcg_axpy(int n, const double *p, const double *Ap, double *x, double *r, doubl
 
 // r.r of the new residual: the stop tests of the next loop top, and the
 // |rz| < 1e-60 break of this iteration (decided before z is computed,
-// which only feeds p)
+// which only feeds p).  Without a preconditioner (plain) z is r, and
+// k_pcg_dot(r, r) would form the same partials as k_pcg_axpy (same grid,
+// rows and block sums): r.z is this sum, and its finaliser's work (beta,
+// the next r.z) is done here instead.
 __global__ void __launch_bounds__(IC_BLK)
-k_pcg_fin_rr(const double *part, unsigned np, IcCtrl *c, int t, int max_iter)
+k_pcg_fin_rr(const double *part, unsigned np, IcCtrl *c, int t, int max_iter, int plain)
 {
     if (c->done) return;
     const double rr = fin_sum(part, np);
@@ -1294,6 +1297,9 @@ k_pcg_fin_rr(const double *part, unsigned np, IcCtrl *c, int t, int max_iter)
         } else if (sqrt(rr) < c->stop) {
             c->done = 1;
             c->iters = t + 1;
+        } else if (plain) {
+            c->beta = rr / c->rz[t & 1];
+            c->rz[(t + 1) & 1] = rr;
         }
     }
 }
@@ -1742,10 +1748,12 @@ static int pcg_run(IcData *d, hipStream_t s, IcCtrl *dctrl, IcCtrl *hctrl, const
             hipLaunchKernelGGL(k_pcg_fin_pap, dim3(1), dim3(IC_BLK), 0, s, pa, gm, dctrl, t);
             hipLaunchKernelGGL(k_pcg_axpy, dim3(gv), dim3(IC_BLK), 0, s, n, d->p, d->Ap, d->x, d->r, d->z, d->y, pb,
                                dctrl, precond);
-            hipLaunchKernelGGL(k_pcg_fin_rr, dim3(1), dim3(IC_BLK), 0, s, pb, gv, dctrl, t, o.max_iter);
-            if (precond) rc = ic_apply(d, d->r, s, dense, dctrl);   // return at once after the stop flag
-            hipLaunchKernelGGL(k_pcg_dot, dim3(gv), dim3(IC_BLK), 0, s, n, d->r, zz, pc, dctrl);
-            hipLaunchKernelGGL(k_pcg_fin_rz, dim3(1), dim3(IC_BLK), 0, s, pc, gv, dctrl, t);
+            hipLaunchKernelGGL(k_pcg_fin_rr, dim3(1), dim3(IC_BLK), 0, s, pb, gv, dctrl, t, o.max_iter, !precond);
+            if (precond) {
+                rc = ic_apply(d, d->r, s, dense, dctrl);   // return at once after the stop flag
+                hipLaunchKernelGGL(k_pcg_dot, dim3(gv), dim3(IC_BLK), 0, s, n, d->r, zz, pc, dctrl);
+                hipLaunchKernelGGL(k_pcg_fin_rz, dim3(1), dim3(IC_BLK), 0, s, pc, gv, dctrl, t);
+            }
             hipLaunchKernelGGL(k_pcg_pupdate, dim3(gv), dim3(IC_BLK), 0, s, n, zz, d->p, dctrl);
         }
         if (rc) break;
