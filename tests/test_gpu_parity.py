@@ -186,6 +186,28 @@ def test_iccg_exact_factor_long_rows(lib3, ratio):
     g.close()
 
 
+@pytest.mark.parametrize("bucket,chunk", [(0, 32), (4, 64), (5, 128)])
+def test_iccg_solve_orders(lib3, bucket, chunk):
+    """The IC(0) triangular solves with other solve orders and chunk sizes
+    (option "ic_bucket": levels per bucket 2^bucket, 0 giving the small
+    dam break a bucket per level, several hundred of them; "ic_chunk" 64
+    and 128, whose chunks can exceed the entries a chunk stages in LDS and
+    then read their rows from HBM): pressure and positions within the
+    oracle's dot-order spread over two steps."""
+    try:
+        lib3.set_option("ic_bucket", bucket)
+        lib3.set_option("ic_chunk", chunk)
+        g, o, recs = _step_compare(lib3, "dam3d_small", 2, solver_type=1)
+    finally:
+        lib3.set_option("ic_bucket", -1)
+        lib3.set_option("ic_chunk", 32)
+    for r in recs:
+        assert r["type_exact"]
+        assert r["p_gpu"] <= max(100 * r["p_spread"], 1e-10)
+        assert r["x_gpu"] <= max(100 * r["x_spread"], 1e-14)
+    g.close()
+
+
 def _laplacian(nx, ny, shift=0.05):
     n = nx * ny
     rows, cols, vals = [], [], []
