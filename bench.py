@@ -220,6 +220,14 @@ def cpu_baseline_reference(lib, w, tol, solver=0, seconds=20.0, max_steps=300):
                 % (steps, len(typ), threads, sname, sname, sum(gi) / steps, el))
 
 
+def ic_shape(lib):
+    """(positions per chunk, waves per workgroup) of the IC(0) solve kernel
+    the options select (16 waves only with 64-position chunks)."""
+    ck = int(lib.L.mps_hip_get_option(b"ic_chunk"))
+    w = int(lib.L.mps_hip_get_option(b"ic_waves"))
+    return ck, (4 if w == 4 else (16 if w == 16 and ck == 64 else 8))
+
+
 def profiled_traffic(precision, workload, nnz, name=None):
     """HBM bytes per launch of the solver's dominant kernel (the SpMV; the
     forward solve for ICCG, from profiles/iccg_<workload>.md), as measured
@@ -787,7 +795,7 @@ def main():
         traffic, tsrc = profiled_traffic("fp64", a.workload, st.nnz, name="iccg_%s.md" % a.workload)
         roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(ach / HBM_PEAK_GBS, 4), traffic=traffic, traffic_source=tsrc,
-                    kernel="k_ic_csolve<0, false, 32, 8> (forward solve)", bytes_per_launch=int(byt),
+                    kernel="k_ic_csolve<0, false, %d, %d> (forward solve)" % ic_shape(lib), bytes_per_launch=int(byt),
                     mean_launch_ms=round(spmv_ms, 5), launches=spmv_launches)
     elif cnt.value > 0:
         byt = {"fp64": spmv_bytes(st, 8, 8), "mixed": spmv_bytes(st, 4, 8),

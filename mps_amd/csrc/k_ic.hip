@@ -1512,11 +1512,12 @@ static unsigned cu_count()
 // claimed in index order, only the few slabs of the lattice that fit the
 // resident waves overlap, and the chain they leave is far longer than the
 // DAG's depth.
-// levels per bucket of the solve order (at least): with 16 a 32-position
-// chunk of the solves holds two neighbouring lattice columns, so the
-// dependency on the next column stays in the chunk as well as the one along
-// the column (forward solve at 1.08M particles 3.56-3.62 ms; 32 levels
-// 4.04-4.11 ms, 8 levels 4.43-4.50 ms)
+// levels per bucket of the solve order (at least): with 16 a chunk of the
+// solves (64 positions, 16 waves by default) holds four neighbouring
+// lattice columns, so the dependencies on the next columns stay in the
+// chunk as well as the ones along a column (forward solve at 1.08M
+// particles 3.42-3.47 ms; 32-position chunks of 8 waves 3.56-3.62 ms;
+// 32-level buckets 3.83-3.90 ms, 8-level 4.50-4.58 ms)
 #define LEVEL_BUCKET 16
 #define MAX_BUCKETS 1024  // buckets of the solve order (at most; their histograms are sized for it)
 #define SCAN_MAX ((size_t)8192 * 2048)   // longest exclusive scan launch_scan_excl takes
@@ -1675,16 +1676,17 @@ static int ic_apply(IcData *d, const double *r, hipStream_t s, int dense, const 
     const double og = mps_hip_get_option("ic_grid");
     const unsigned grid = og >= 1 ? (unsigned)og : cu_count() * 2;
     // chunks of "ic_chunk" positions (32, 64, 128), "ic_waves" wavefronts
-    // (4 or 8) per workgroup
+    // (4 or 8, and 16 for 64-position chunks) per workgroup
     const int ck = (int)mps_hip_get_option("ic_chunk");
-    const int nw = (int)mps_hip_get_option("ic_waves") == 4 ? 4 : 8;
+    const int ow = (int)mps_hip_get_option("ic_waves");
+    const int nw = ow == 4 ? 4 : (ow == 16 && ck == 64 ? 16 : 8);
     decltype(&k_ic_csolve<0, false, 64, 8>) fwd, bwd;
 #define IC_PICK(C, W)                                                                  \
     do {                                                                               \
         fwd = dense ? k_ic_csolve<1, false, C, W> : k_ic_csolve<0, false, C, W>;       \
         bwd = dense ? k_ic_csolve<1, true, C, W> : k_ic_csolve<0, true, C, W>;         \
     } while (0)
-    if (ck == 64) { if (nw == 4) IC_PICK(64, 4); else IC_PICK(64, 8); }
+    if (ck == 64) { if (nw == 4) IC_PICK(64, 4); else if (nw == 16) IC_PICK(64, 16); else IC_PICK(64, 8); }
     else if (ck == 128) { if (nw == 4) IC_PICK(128, 4); else IC_PICK(128, 8); }
     else { if (nw == 4) IC_PICK(32, 4); else IC_PICK(32, 8); }
 #undef IC_PICK

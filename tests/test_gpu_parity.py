@@ -194,13 +194,14 @@ def test_iccg_solve_orders(lib3, bucket, chunk):
     and 128, whose chunks can exceed the entries a chunk stages in LDS and
     then read their rows from HBM): pressure and positions within the
     oracle's dot-order spread over two steps."""
+    saved = {k: lib3.L.mps_hip_get_option(k.encode()) for k in ("ic_bucket", "ic_chunk")}
     try:
         lib3.set_option("ic_bucket", bucket)
         lib3.set_option("ic_chunk", chunk)
         g, o, recs = _step_compare(lib3, "dam3d_small", 2, solver_type=1)
     finally:
-        lib3.set_option("ic_bucket", -1)
-        lib3.set_option("ic_chunk", 32)
+        for k, v in saved.items():
+            lib3.set_option(k, v)
     for r in recs:
         assert r["type_exact"]
         assert r["p_gpu"] <= max(100 * r["p_spread"], 1e-10)
