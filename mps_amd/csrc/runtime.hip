@@ -48,7 +48,7 @@ struct Options {
     double search = 1;                 // neighbour search: 1 (2) candidates staged in LDS, read 4 (2) at a time; 0 gathered from HBM
     double batch_flat = 1;             // mps_hip_run_batch: 1 one launch per operator for all scenes, 0 per scene
     double ic_wait_limit = 1048576;   // unsuccessful polls before an IC(0) wait gives up
-    double ic_sleep = 3;               // longest backoff of an idle IC(0) wave (0-3)
+    double ic_sleep = 1;               // longest backoff of an idle IC(0) wave (0-3)
     double ic_bucket = -1;             // log2 of the levels per bucket of the IC(0) solve order (-1: automatic)
     double ic_chunk = 64;              // positions per workgroup chunk of the IC(0) solves (32, 64, 128)
     double ic_waves = 16;              // wavefronts per workgroup of the IC(0) solves (4, 8; 16 with 64-position chunks)
