@@ -563,7 +563,7 @@ __device__ __forceinline__ void factor_row_any(int e, int lane, const int *__res
                         stage = 1;
                     }
                 }
-                poll_all = __shfl((int)(stage != 0), lo, WAVE) != 0;
+                poll_all = __builtin_amdgcn_readlane((int)(stage != 0), lo) != 0;
             }
             if (stage == 1 && bi == bn) {
                 if (u >= mk) {
@@ -660,7 +660,7 @@ k_ic_factor(int n, const int *__restrict__ order, const int *__restrict__ rp, co
     for (;;) {
         int tk = 0;
         if (lane == 0) tk = (int)atomicAdd(ticket, 1u);
-        tk = __shfl(tk, 0, WAVE);
+        tk = __builtin_amdgcn_readlane(tk, 0);
         if (tk >= n) break;
         const int e = order[tk];   // rows in level order: every row waited on is claimed earlier
         factor_row_any(e, lane, rp, col, dpos, aval, lval, dinv, dense, wait_limit, sleep_cap, scol, sval, failed);
@@ -700,7 +700,7 @@ k_ic_factor2(int n, const int *__restrict__ order, const int *__restrict__ rp, c
     for (;;) {
         int tk = 0;
         if (lane == 0) tk = (int)atomicAdd(ticket, 1u);
-        tk = __shfl(tk, 0, WAVE);
+        tk = __builtin_amdgcn_readlane(tk, 0);
         if (tk >= n) break;
         const int e = order[tk];   // rows in level order: every row waited on is claimed earlier
         const int r0 = rp[e], m = dpos[e];
@@ -750,7 +750,7 @@ k_ic_factor2(int n, const int *__restrict__ order, const int *__restrict__ rp, c
                         stage = 1;
                     }
                 }
-                poll_all = __shfl((int)(stage != 0), lo, WAVE) != 0;
+                poll_all = __builtin_amdgcn_readlane((int)(stage != 0), lo) != 0;
             }
             // gather: row k of each lane in stage 1, FG2 rows at a time,
             // the whole wave loading one row (entry j in lane j)
@@ -770,9 +770,9 @@ k_ic_factor2(int n, const int *__restrict__ order, const int *__restrict__ rp, c
                     kbg[g] = 0;
                     mkg[g] = 0;
                     if (tg[g] >= 0) {
-                        kbg[g] = __shfl(kb, tg[g], WAVE);
-                        mkg[g] = __shfl(mk, tg[g], WAVE);
-                        const bool again = __shfl((int)(spins != 0), tg[g], WAVE) != 0;
+                        kbg[g] = __builtin_amdgcn_readlane(kb, tg[g]);
+                        mkg[g] = __builtin_amdgcn_readlane(mk, tg[g]);
+                        const bool again = __builtin_amdgcn_readlane((int)(spins != 0), tg[g]) != 0;
                         if (lane < mkg[g]) {
                             cj[g] = col[kbg[g] + lane];
                             // The first load goes through the caches: a value
@@ -795,7 +795,7 @@ k_ic_factor2(int n, const int *__restrict__ order, const int *__restrict__ rp, c
                     // piece's matches follow the earlier ones in lane Tg's
                     // list, so the list stays in row k's entry order
                     const int mkk = mkg[g];
-                    const bool again = mkk > WAVE && __shfl((int)(spins != 0), Tg, WAVE) != 0;
+                    const bool again = mkk > WAVE && __builtin_amdgcn_readlane((int)(spins != 0), Tg) != 0;
                     bool missing = false;
                     int base = 0;
                     for (int c0 = 0; c0 < mkk; c0 += WAVE) {
@@ -831,7 +831,7 @@ k_ic_factor2(int n, const int *__restrict__ order, const int *__restrict__ rp, c
                         // then), or give up at the bound
                         int quit = 0;
                         if (lane == Tg) quit = ++spins > wait_limit;
-                        if (!__shfl(quit, Tg, WAVE)) continue;
+                        if (!__builtin_amdgcn_readlane(quit, Tg)) continue;
                         if (lane == Tg) failed = true;   // missing values read as 0
                     }
                     if (lane == Tg) {
@@ -1050,7 +1050,10 @@ k_ic_csolve(int n, const int *__restrict__ perm, const double *__restrict__ lev,
             snext = 0;
         }
         __syncthreads();
-        const int tk = sclaim;
+        // the claim, the row and its entry range are uniform over the wave;
+        // read into scalar registers, the term loops below run on scalar
+        // control instead of exec masks
+        const int tk = __builtin_amdgcn_readfirstlane(sclaim);
         if (tk >= nch) break;
         const int ch = BACK ? nch - 1 - tk : tk;
         const int cbase = ch * CS_CHUNK, cnt_rows = min(CS_CHUNK, n - cbase);
@@ -1089,10 +1092,11 @@ k_ic_csolve(int n, const int *__restrict__ perm, const double *__restrict__ lev,
         for (;;) {
             int i = 0;
             if (lane == 0) i = atomicAdd(&snext, 1);
-            i = __shfl(i, 0, WAVE);
+            i = __builtin_amdgcn_readlane(i, 0);
             if (i >= cnt_rows) break;
-            const int t = cbase + srow[i];
-            const int o0 = seo[t - cbase], m = seo[t - cbase + 1] - o0;
+            const int t = cbase + __builtin_amdgcn_readfirstlane(srow[i]);
+            const int o0 = __builtin_amdgcn_readfirstlane(seo[t - cbase]);
+            const int m = __builtin_amdgcn_readfirstlane(seo[t - cbase + 1]) - o0;
             const double rv = srin[t - cbase];
             const double dr = sdr[t - cbase];
             double s = MODE == 0 ? rv : 0.0;
@@ -1126,6 +1130,17 @@ k_ic_csolve(int n, const int *__restrict__ perm, const double *__restrict__ lev,
                     const unsigned long long miss = ballot64(act && is_sent(v)) & (~0ull << k);
                     const int stop = miss ? min(cnt, __ffsll((long long)miss) - 1) : cnt;
                     const double pr = l * v;
+                    // in entry order; four products read out ahead of
+                    // their adds
+                    for (; k + 4 <= stop; k += 4) {
+                        const double p0 = readlane_d(pr, k), p1 = readlane_d(pr, k + 1);
+                        const double p2 = readlane_d(pr, k + 2), p3 = readlane_d(pr, k + 3);
+                        if (MODE == 0) {
+                            s -= p0; s -= p1; s -= p2; s -= p3;
+                        } else {
+                            s += p0; s += p1; s += p2; s += p3;
+                        }
+                    }
                     for (; k < stop; k++) {
                         const double pk = readlane_d(pr, k);
                         if (MODE == 0) s -= pk;
