@@ -1036,7 +1036,14 @@ k_ic_csolve(int n, const int *__restrict__ perm, const double *__restrict__ lev,
     __shared__ int seo[CS_CHUNK + 1];
     __shared__ double sy[CS_CHUNK];
     __shared__ int sflag[CS_CHUNK];
-    __shared__ int skey[CS_CHUNK], srow[CS_CHUNK];
+    __shared__ int skey[CS_CHUNK];
+    // by rank in dependency order: the row (chunk offset), its entry range,
+    // right-hand side and 1/sqrt(d), read together when a wave claims it
+    struct RowInfo {
+        int me, o0, o1, pad;
+        double rv, dr;
+    };
+    __shared__ RowInfo sri[CS_CHUNK];
     __shared__ int sclaim, snext;
     if (ctrl && ctrl->done) return;
     const int lane = threadIdx.x & (WAVE - 1);
@@ -1058,8 +1065,8 @@ k_ic_csolve(int n, const int *__restrict__ perm, const double *__restrict__ lev,
         const int ch = BACK ? nch - 1 - tk : tk;
         const int cbase = ch * CS_CHUNK, cnt_rows = min(CS_CHUNK, n - cbase);
         // the chunk's rows in dependency order: by level, then position
-        // (BACK: both descending); ranks are distinct, so srow is a
-        // permutation of the chunk
+        // (BACK: both descending); ranks are distinct, so sri is filled
+        // once for each row of the chunk
         if (threadIdx.x < cnt_rows) {
             const int t = cbase + threadIdx.x;
             const int r = perm ? perm[t] : t;
@@ -1086,7 +1093,7 @@ k_ic_csolve(int n, const int *__restrict__ perm, const double *__restrict__ lev,
                 const int ku = skey[u];
                 rank += ku < km || (ku == km && (BACK ? u > me : u < me));
             }
-            srow[rank] = me;
+            sri[rank] = RowInfo{me, seo[me], seo[me + 1], 0, srin[me], sdr[me]};
         }
         __syncthreads();
         for (;;) {
@@ -1094,35 +1101,46 @@ k_ic_csolve(int n, const int *__restrict__ perm, const double *__restrict__ lev,
             if (lane == 0) i = atomicAdd(&snext, 1);
             i = __builtin_amdgcn_readlane(i, 0);
             if (i >= cnt_rows) break;
-            const int t = cbase + __builtin_amdgcn_readfirstlane(srow[i]);
-            const int o0 = __builtin_amdgcn_readfirstlane(seo[t - cbase]);
-            const int m = __builtin_amdgcn_readfirstlane(seo[t - cbase + 1]) - o0;
-            const double rv = srin[t - cbase];
-            const double dr = sdr[t - cbase];
+            const RowInfo ri = sri[i];
+            const int t = cbase + __builtin_amdgcn_readfirstlane(ri.me);
+            const int o0 = __builtin_amdgcn_readfirstlane(ri.o0);
+            const int m = __builtin_amdgcn_readfirstlane(ri.o1) - o0;
+            const double rv = ri.rv;
+            const double dr = ri.dr;
             double s = MODE == 0 ? rv : 0.0;
             for (int g0 = 0; g0 < m; g0 += WAVE) {
                 const int cnt = min(WAVE, m - g0);
                 const bool act = lane < cnt;
                 int c = 0;
                 bool in = false;
-                double l = 0.0, v = 0.0;
+                double l = 0.0;
+                // a value from HBM and one from LDS in separate registers,
+                // so the LDS read need not wait for the HBM load that would
+                // otherwise write the same register
+                double vg = 0.0, vl = sent();
                 if (act) {
                     const int q = o0 + g0 + lane;
-                    c = staged ? sec[q - e0] : ecol[q];
-                    l = staged ? sev[q - e0] : eval[q];
+                    // LDS reads of their own kind, so the two sources do not
+                    // merge into flat loads (which wait for every memory
+                    // access in flight, the last row's store included)
+                    if (staged) {
+                        c = lds_ld(&sec[q - e0]);
+                        l = lds_ld(&sev[q - e0]);
+                    } else {
+                        c = ecol[q];
+                        l = eval[q];
+                    }
                     in = c >= cbase && c < cbase + cnt_rows;
                     // first load through the caches: y held the sentinel
                     // before this kernel and each value is written once, so
                     // only a sentinel can be stale (polled again past them)
-                    if (!in) v = y[c];
+                    if (!in) vg = y[c];
                 }
-                if (act && in) {
-                    v = sent();
-                    if (lds_ld(&sflag[c - cbase]) == tk) {
-                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-                        v = lds_ld(&sy[c - cbase]);
-                    }
+                if (act && in && lds_ld(&sflag[c - cbase]) == tk) {
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                    vl = lds_ld(&sy[c - cbase]);
                 }
+                double v = in ? vl : vg;
                 int k = 0;
                 long long spins = 0;
                 int idle = 0;
@@ -1148,13 +1166,15 @@ k_ic_csolve(int n, const int *__restrict__ perm, const double *__restrict__ lev,
                     }
                     if (k >= cnt) break;
                     // lane k's value (and maybe later ones) not yet there
-                    if (act && is_sent(v)) {
-                        if (!in) {
-                            v = ld_rlx(&y[c]);
-                        } else if (lds_ld(&sflag[c - cbase]) == tk) {
+                    {
+                        const bool poll = act && is_sent(v);
+                        double pg = v, pl = v;
+                        if (poll && !in) pg = ld_rlx(&y[c]);
+                        if (poll && in && lds_ld(&sflag[c - cbase]) == tk) {
                             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-                            v = lds_ld(&sy[c - cbase]);
+                            pl = lds_ld(&sy[c - cbase]);
                         }
+                        v = in ? pl : pg;
                     }
                     if (!is_sent(readlane_d(v, k))) {
                         spins = 0;
@@ -1172,10 +1192,10 @@ k_ic_csolve(int n, const int *__restrict__ perm, const double *__restrict__ lev,
             }
             const double out = MODE == 0 ? s * dr : (rv - s) / dr;
             if (lane == 0) {
+                st_rlx(&y[t], out);   // first: other chunks wait on it longest
                 lds_st(&sy[t - cbase], out);
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");   // value before flag
                 lds_st(&sflag[t - cbase], tk);
-                st_rlx(&y[t], out);
             }
         }
     }
