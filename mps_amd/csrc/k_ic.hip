@@ -1029,7 +1029,7 @@ k_ic_csolve(int n, const int *__restrict__ perm, const double *__restrict__ lev,
     // the chunk is claimed (chunks of at most ECAP entries; a larger chunk
     // reads its entries from HBM row by row), so that a row's loads are off
     // the chain of rows that wait on one another
-    constexpr int ECAP = CS_CHUNK <= 32 ? 2048 : 4096;
+    constexpr int ECAP = CS_CHUNK <= 32 ? 2048 : CS_CHUNK <= 64 ? 4096 : 8192;
     __shared__ int sec[ECAP];
     __shared__ double sev[ECAP];
     __shared__ double srin[CS_CHUNK], sdr[CS_CHUNK];
@@ -1711,10 +1711,10 @@ static int ic_apply(IcData *d, const double *r, hipStream_t s, int dense, const 
     const double og = mps_hip_get_option("ic_grid");
     const unsigned grid = og >= 1 ? (unsigned)og : cu_count() * 2;
     // chunks of "ic_chunk" positions (32, 64, 128), "ic_waves" wavefronts
-    // (4 or 8, and 16 for 64-position chunks) per workgroup
+    // (4, 8 or 16) per workgroup
     const int ck = (int)mps_hip_get_option("ic_chunk");
     const int ow = (int)mps_hip_get_option("ic_waves");
-    const int nw = ow == 4 ? 4 : (ow == 16 && ck == 64 ? 16 : 8);
+    const int nw = ow == 4 ? 4 : (ow == 16 ? 16 : 8);
     decltype(&k_ic_csolve<0, false, 64, 8>) fwd, bwd;
 #define IC_PICK(C, W)                                                                  \
     do {                                                                               \
@@ -1722,8 +1722,8 @@ static int ic_apply(IcData *d, const double *r, hipStream_t s, int dense, const 
         bwd = dense ? k_ic_csolve<1, true, C, W> : k_ic_csolve<0, true, C, W>;         \
     } while (0)
     if (ck == 64) { if (nw == 4) IC_PICK(64, 4); else if (nw == 16) IC_PICK(64, 16); else IC_PICK(64, 8); }
-    else if (ck == 128) { if (nw == 4) IC_PICK(128, 4); else IC_PICK(128, 8); }
-    else { if (nw == 4) IC_PICK(32, 4); else IC_PICK(32, 8); }
+    else if (ck == 128) { if (nw == 4) IC_PICK(128, 4); else if (nw == 16) IC_PICK(128, 16); else IC_PICK(128, 8); }
+    else { if (nw == 4) IC_PICK(32, 4); else if (nw == 16) IC_PICK(32, 16); else IC_PICK(32, 8); }
 #undef IC_PICK
     prof_begin(PK_IC, s);
     hipLaunchKernelGGL(fwd, dim3(grid), dim3(nw * WAVE), 0, s, d->n_eq, d->perm, d->lev, d->loff, d->lcol,

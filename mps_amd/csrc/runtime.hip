@@ -51,7 +51,7 @@ struct Options {
     double ic_sleep = 1;               // longest backoff of an idle IC(0) wave (0-3)
     double ic_bucket = -1;             // log2 of the levels per bucket of the IC(0) solve order (-1: automatic)
     double ic_chunk = 64;              // positions per workgroup chunk of the IC(0) solves (32, 64, 128)
-    double ic_waves = 16;              // wavefronts per workgroup of the IC(0) solves (4, 8; 16 with 64-position chunks)
+    double ic_waves = 16;              // wavefronts per workgroup of the IC(0) solves (4, 8, 16)
     double ic_factor_form = 2;         // IC(0) factor: 2 two-phase (rows of at most 64 entries), 1 one-phase
     double ic_grid = 0;                // workgroups of the IC(0) solves (0: automatic)
     double ic_fgrid = 0;               // workgroups of the IC(0) factor (0: automatic)
@@ -1225,7 +1225,7 @@ int mps_hip_set_option(const char *key, double value)
         if (value != 1 && value != 2) return MPS_ERR_ARG;
         g_opt.ic_factor_form = value;
     } else if (k == "ic_waves") {
-        if (value != 4 && value != 8 && value != 16) return MPS_ERR_ARG;   // 16: 64-position chunks only
+        if (value != 4 && value != 8 && value != 16) return MPS_ERR_ARG;
         g_opt.ic_waves = value;
     }
     else if (k == "ic_fgrid") g_opt.ic_fgrid = value;
