@@ -943,7 +943,11 @@ k_ic_levels(int n, const int *__restrict__ rp, const int *__restrict__ dpos, con
                     for (int j = 0; j < SB; j++)
                         if (j < bn) bc[j] = col[eb + (size_t)(q + j)];
 #pragma unroll
-                    for (int j = 0; j < SB; j++) bv[j] = (j < bn && bc[j] < base) ? ld_rlx(&lev[bc[j]]) : 0.0;
+                    // first loads through the caches: lev held the
+                    // sentinel before this kernel and each level is written
+                    // once, so only a sentinel can be stale (polled again
+                    // past the caches below)
+                    for (int j = 0; j < SB; j++) bv[j] = (j < bn && bc[j] < base) ? lev[bc[j]] : 0.0;
                     q += bn;
                     bi = 0;
                     prog = true;
